@@ -1,0 +1,89 @@
+"""Build the inputs of a golden case (profiles, source, receivers, medium) with
+the oracle's L2 routines, and run the oracle on it.
+
+Used by the parity tests: the same inputs go to the oracle and to the HIP
+path, and the oracle's outputs are checked against the reference fixtures.
+"""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE / "golden"))
+
+import oracle_lib as orc  # noqa: E402
+from cases import CASES, values  # noqa: E402
+
+GOLDEN = HERE / "golden"
+
+
+def fixture(case: str, prec: str):
+    return np.load(GOLDEN / f"{case}_{prec}.npz")
+
+
+def inputs(case: str, prec: str, nstep: int | None = None) -> dict:
+    """All inputs of ``case`` in precision ``prec``, built the reference's way."""
+    v = values(case, prec)
+    c = CASES[case]
+    dt = np.float64 if prec == "f64" else np.float32
+    nx, ny, npml = c["NX"], c["NY"], c["NPML"]
+    nz = c.get("NZ")
+    nstep = c["NSTEP"] if nstep is None else nstep
+    d = dict(v)
+    d["nstep"] = nstep
+    d["dtype"] = dt
+    # profiles: 2D2:283-465 (x clamps alpha, y does not; 2D4 y origin quirk)
+    d["prof_x"] = orc.profile(nx, v["DELTAX"], v["DELTAT"], npml, v["cp"], v["f0"], prec,
+                              clamp_alpha=True)
+    d["prof_y"] = orc.profile(ny, v["DELTAY"], v["DELTAT"], npml, v["cp"], v["f0"], prec,
+                              origin_max_uses_n=c.get("yorigin_n", False), clamp_alpha=False)
+    if nz is not None:
+        d["prof_z"] = orc.profile(nz, v["DELTAZ"], v["DELTAT"], npml, v["cp"], v["f0"], prec,
+                                  clamp_alpha=False)
+    kind = 2 if c["src"] == "ricker" else 1
+    d["fx"], d["fy"] = orc.stf(nstep, v["DELTAT"], v["f0"], v["t0"], v["factor"], v["ANGLE"],
+                               kind, prec)
+    d["ix_rec"], d["iy_rec"] = orc.receivers(nx, ny, v["DELTAX"], v["DELTAY"], c["NREC"],
+                                             v["xdeb"], v["ydeb"], v["xfin"], v["yfin"], prec)
+    if nz is None:
+        if c.get("model"):
+            f = fixture(case, prec)
+            d["rho"], d["mu"], d["lam"] = f["rho"], f["mu"], f["lam"]
+        else:
+            d["rho"] = np.full((ny, nx), v["density"], dt)
+            d["mu"] = np.full((ny, nx), v["mu"], dt)
+            d["lam"] = np.full((ny, nx), v["lambda"], dt)
+        if c["order"] == 4:  # 2D4:696-704
+            d["ewin"] = (npml, nx - npml + 1, npml, ny - npml + 1)
+        else:  # 2D2:695-703
+            d["ewin"] = (npml + 1, nx - npml, npml + 1, ny - npml)
+    else:
+        shp = (nz, ny, nx)
+        d["rho"] = np.full(shp, v["density"], dt)
+        d["mu"] = np.full(shp, v["mu"], dt)
+        d["lam"] = np.full(shp, v["lambda"], dt)
+        d["lp2m"] = np.full(shp, v["lambdaplustwomu"], dt)
+        d["ksrc"] = nz // 2  # 3D:1074-1081 (cut plane, global k = NZ/2)
+        d["iz_rec"] = np.full(c["NREC"], nz // 2, np.int32)
+        d["ewin"] = (npml + 1, nx - npml, npml + 1, ny - npml, npml + 1, nz - npml)
+    return d
+
+
+def run_oracle(case: str, prec: str, nstep: int | None = None, d: dict | None = None,
+               nthreads: int = 0, energy=True) -> dict:
+    d = inputs(case, prec, nstep) if d is None else d
+    c = CASES[case]
+    if c.get("NZ") is None:
+        return orc.run2d(c["NX"], c["NY"], c["order"], d["DELTAX"], d["DELTAY"], d["DELTAT"],
+                         d["nstep"], d["lam"], d["mu"], d["rho"], d["prof_x"], d["prof_y"],
+                         d["ISOURCE"], d["JSOURCE"], d["fx"], d["fy"], d["ix_rec"], d["iy_rec"],
+                         ewin=d["ewin"] if energy else None, prec=prec, nthreads=nthreads)
+    return orc.run3d(c["NX"], c["NY"], c["NZ"], d["DELTAX"], d["DELTAY"], d["DELTAZ"],
+                     d["DELTAT"], d["nstep"], d["lam"], d["mu"], d["rho"], d["lp2m"],
+                     d["prof_x"], d["prof_y"], d["prof_z"], d["ISOURCE"], d["JSOURCE"],
+                     d["ksrc"], d["fx"], d["fy"], d["ix_rec"], d["iy_rec"], d["iz_rec"],
+                     ewin=d["ewin"] if energy else None, energy_slabs=c.get("nproc", 1),
+                     prec=prec, nthreads=nthreads)

@@ -1,0 +1,89 @@
+"""Generate the golden fixtures in tests/golden/ by running the compiled
+reference programs (built by oracle/build_ref.py from /root/reference).
+
+    python tests/golden/make_golden.py            # all cases, both precisions
+
+Each fixture ``<case>_<prec>.npz`` holds the reference's own outputs at full
+precision: damping profiles, receiver indices, seismograms, energy curve(s),
+the SHA-256 digest of each final wavefield with a regular subsample of it
+(every 4th point per axis in 2D, every 3rd in 3D), and for each C-PML memory array the masks of x/y/z
+lines on which it is non-zero (the arrays themselves are not stored).  For the
+layered cases the medium (rho, mu, lambda) fed to the program is stored too.
+This script needs /root/reference and amdflang; the fixtures it writes do not.
+"""
+from __future__ import annotations
+
+import hashlib
+import sys
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parents[1]
+sys.path.insert(0, str(ROOT / "oracle"))
+sys.path.insert(0, str(HERE))
+
+import build_ref  # noqa: E402
+from cases import CASES, VARIANT_OF, layered_model  # noqa: E402
+
+FIELDS_2D = ("vx", "vy", "sxx", "syy", "sxy")
+MEM_2D = ("m_dvx_dx", "m_dvx_dy", "m_dvy_dx", "m_dvy_dy",
+          "m_dsxx_dx", "m_dsyy_dy", "m_dsxy_dx", "m_dsxy_dy")
+FIELDS_3D = ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
+
+
+def field_digest(a) -> str:
+    """SHA-256 of the array's bytes (C order, little endian) with -0.0 folded to +0.0."""
+    a = np.ascontiguousarray(a + a.dtype.type(0))
+    return hashlib.sha256(a.astype(a.dtype.newbyteorder("<"), copy=False).tobytes()).hexdigest()
+
+
+def make(case: str, prec: str, workroot: Path) -> Path:
+    c = CASES[case]
+    variant = f"{VARIANT_OF[case]}_{prec}"
+    model = None
+    if c.get("model"):
+        model = layered_model(c["seed"], c["NX"], c["NY"], prec)
+    out = build_ref.run(variant, workroot / variant, model=model)
+    save = {}
+    for k in ("prof_x", "prof_y", "prof_z", "sisvx", "sisvy", "ekin", "epot", "etotal",
+              "ix_rec", "iy_rec"):
+        if k in out:
+            save[k] = out[k]
+    if model is not None:
+        save["rho"], save["mu"], save["lam"] = model
+    fields = FIELDS_2D if c.get("NZ") is None else FIELDS_3D
+    sub = (slice(None, None, 4),) * 2 if c.get("NZ") is None else (slice(None, None, 3),) * 3
+    for k in fields:
+        save[f"sha_{k}"] = np.array(field_digest(out[k]))
+        save[f"sub_{k}"] = out[k][sub]
+    mems = [k for k in out if k.startswith("m_")]
+    for k in mems:
+        a = out[k]
+        nz = a != 0
+        if a.ndim == 2:  # (ny, nx)
+            save[f"nzmask_{k}_x"] = nz.any(axis=0)
+            save[f"nzmask_{k}_y"] = nz.any(axis=1)
+        else:  # (nz, ny, nx)
+            save[f"nzmask_{k}_x"] = nz.any(axis=(0, 1))
+            save[f"nzmask_{k}_y"] = nz.any(axis=(0, 2))
+            save[f"nzmask_{k}_z"] = nz.any(axis=(1, 2))
+    path = HERE / f"{case}_{prec}.npz"
+    np.savez_compressed(path, **save)
+    return path
+
+
+def main(argv):
+    names = argv or list(CASES)
+    build_ref.build([f"{VARIANT_OF[n]}_{p}" for n in names for p in ("f64", "f32")])
+    with tempfile.TemporaryDirectory(prefix="cpml_golden_") as tmp:
+        for n in names:
+            for p in ("f64", "f32"):
+                path = make(n, p, Path(tmp))
+                print(f"{path.name}: {path.stat().st_size / 1024:.0f} KiB")
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

@@ -1,0 +1,62 @@
+"""The oracle against the compiled reference's own outputs (tests/golden).
+
+Bar: bit-exact.  The oracle restates 2D2/2D4/3D statement by statement, and
+the fixtures hold what the reference programs compute, so every profile,
+receiver index, seismogram sample, energy value and final field must agree
+exactly (fields through their SHA-256 digests and subsamples).
+"""
+import numpy as np
+import pytest
+
+from case_inputs import fixture, inputs, run_oracle
+from golden.cases import CASES
+from golden.make_golden import FIELDS_2D, FIELDS_3D, field_digest
+
+ALL = [(c, p) for c in CASES for p in ("f64", "f32")]
+
+
+@pytest.mark.parametrize("case,prec", ALL)
+def test_profiles_and_receivers(case, prec):
+    f = fixture(case, prec)
+    d = inputs(case, prec, nstep=1)
+    for k in ("prof_x", "prof_y", "prof_z"):
+        if k in f.files:
+            np.testing.assert_array_equal(d[k], f[k], err_msg=k)
+    np.testing.assert_array_equal(d["ix_rec"], f["ix_rec"])
+    np.testing.assert_array_equal(d["iy_rec"], f["iy_rec"])
+
+
+@pytest.mark.parametrize("case,prec", ALL)
+def test_time_loop_bit_exact(case, prec):
+    f = fixture(case, prec)
+    o = run_oracle(case, prec, nthreads=4)
+    for k in ("sisvx", "sisvy", "ekin", "epot", "etotal"):
+        if k in f.files:
+            np.testing.assert_array_equal(o[k], f[k], err_msg=k)
+    fields = FIELDS_2D if CASES[case].get("NZ") is None else FIELDS_3D
+    step = 4 if CASES[case].get("NZ") is None else 3
+    for k in fields:
+        sub = o[k][(slice(None, None, step),) * o[k].ndim]
+        np.testing.assert_array_equal(sub, f[f"sub_{k}"], err_msg=k)
+        assert field_digest(o[k]) == str(f[f"sha_{k}"]), k
+
+
+@pytest.mark.parametrize("case,prec", [a for a in ALL if a[1] == "f64"])
+def test_memory_variables_zero_outside_slabs(case, prec):
+    """Where a (and a_half) vanish the memory variable must stay exactly 0
+    (2D2:221 'could declare these arrays in PML only'); the reference's own
+    non-zero masks must lie inside {a != 0} of the matching profile."""
+    f = fixture(case, prec)
+    axes = {"x": f["prof_x"], "y": f["prof_y"]}
+    if "prof_z" in f.files:
+        axes["z"] = f["prof_z"]
+    for key in f.files:
+        if not key.startswith("nzmask_"):
+            continue
+        name, axis = key[len("nzmask_m_"):-2], key[-1]
+        deriv_axis = name[-1]
+        if axis != deriv_axis:
+            continue
+        prof = axes[axis]
+        allowed = (prof[0] != 0) | (prof[3] != 0)
+        assert not np.any(f[key] & ~allowed), key
