@@ -1,0 +1,184 @@
+/*
+ * cpml_gpu -- C ABI of the MI355X (gfx950) C-PML time-loop library.
+ *
+ * The reference programs (geodynamics/seismic_cpml) have no library API: the
+ * time loop is inline in each Fortran `program`, configured by compile-time
+ * `parameter`s.  These entry points replace those inner loops; their argument
+ * names are the reference's own (NX, NY, NZ, DELTAX, DELTAT, NPOINTS_PML,
+ * ISOURCE, JSOURCE, ix_rec, iy_rec, sisvx, ...).  The Fortran module
+ * `cpml_gpu` (seismic_cpml_amd/fortran/cpml_gpu.f90) binds them with
+ * ISO_C_BINDING.
+ *
+ * Conventions
+ *  - All functions return a status code (CPML_OK == 0).  No C++ exception
+ *    crosses the ABI.  cpml_gpu_last_error() returns the message.
+ *  - Grid arrays are passed in Fortran order (x fastest), interior points only
+ *    (NX*NY or NX*NY*NZ values), in the handle's precision (prec = 4: float,
+ *    8: double).  Host arrays are borrowed for the duration of the call.
+ *  - Indices (ISOURCE, ix_rec, ...) are 1-based as in the reference.
+ *  - A handle is single-threaded; it owns one HIP stream, its device buffers
+ *    and its captured step graphs.  Distinct handles may run concurrently.
+ */
+#ifndef CPML_GPU_H
+#define CPML_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cpml_solver cpml_solver;
+
+/* status codes */
+#define CPML_OK 0
+#define CPML_ERR_ARG 1      /* invalid argument */
+#define CPML_ERR_CFL 2      /* Courant number > 1 (2D2:513-516, 3D:712-717) */
+#define CPML_ERR_STENCIL 3  /* grid too small for the stencil / slab layout */
+#define CPML_ERR_OOM 4      /* device memory budget exceeded */
+#define CPML_ERR_HIP 5      /* HIP runtime error */
+#define CPML_ERR_BLOWUP 6   /* max |v| above threshold (2D2:716-726, 3D:1183-1195) */
+#define CPML_ERR_STATE 7    /* call out of order (e.g. run before set_medium) */
+#define CPML_ERR_NODEVICE 8 /* no GPU visible */
+
+/* field / memory-variable selectors for get_field / set_field / get_memory */
+enum {
+  CPML_VX = 0, CPML_VY, CPML_VZ, CPML_SIGMAXX, CPML_SIGMAYY, CPML_SIGMAZZ,
+  CPML_SIGMAXY, CPML_SIGMAXZ, CPML_SIGMAYZ
+};
+/* memory variables, named as in the reference (3D:222-240) */
+enum {
+  CPML_MEM_DVX_DX = 0, CPML_MEM_DVX_DY, CPML_MEM_DVX_DZ, CPML_MEM_DVY_DX, CPML_MEM_DVY_DY,
+  CPML_MEM_DVY_DZ, CPML_MEM_DVZ_DX, CPML_MEM_DVZ_DY, CPML_MEM_DVZ_DZ, CPML_MEM_DSIGMAXX_DX,
+  CPML_MEM_DSIGMAYY_DY, CPML_MEM_DSIGMAZZ_DZ, CPML_MEM_DSIGMAXY_DX, CPML_MEM_DSIGMAXY_DY,
+  CPML_MEM_DSIGMAXZ_DX, CPML_MEM_DSIGMAXZ_DZ, CPML_MEM_DSIGMAYZ_DY, CPML_MEM_DSIGMAYZ_DZ
+};
+/* kernels timed by cpml_gpu_kernel_time */
+enum { CPML_KERNEL_STRESS = 0, CPML_KERNEL_VELOCITY = 1 };
+
+/* ---- lifecycle ------------------------------------------------------- */
+
+/* Replaces the L1 parameter block of 2D2:137-218 / 2D4:131-219 / 3D:123-218.
+ * ndim 2 (NZ ignored) or 3; order 2, 4 or 8 in 2D (2D2 / 2D4 / Holberg
+ * operator of ADE8:263-266), 2 in 3D; prec 4 or 8.  NSTEP sizes the source
+ * and seismogram tables.  Validates everything on the host before any HIP
+ * call; returns CPML_ERR_NODEVICE when no GPU is visible. */
+int cpml_gpu_init(cpml_solver **h, int ndim, int order, int prec, int NX, int NY, int NZ,
+                  double DELTAX, double DELTAY, double DELTAZ, double DELTAT, int NPOINTS_PML,
+                  int NSTEP);
+int cpml_gpu_finalize(cpml_solver *h);
+
+/* ---- set-up (replaces L2 uploads) ------------------------------------ */
+
+/* Lame parameters and density per point: 2D2:204, 468-474 (arrays);
+ * 3D:139-145 (scalars in the reference; arrays here).  Checks rho, mu > 0 and
+ * the Courant number with cp = max sqrt((lambda+2mu)/rho) (2D2:513-516). */
+int cpml_gpu_set_medium(cpml_solver *h, const void *lambda, const void *mu, const void *rho);
+/* Same from device arrays (interior-only Fortran order, handle precision). */
+int cpml_gpu_set_medium_device(cpml_solver *h, const void *d_lambda, const void *d_mu,
+                               const void *d_rho);
+
+/* Damping profiles a, b, K at integer and half points per axis, as built by
+ * 2D2:283-465 / 3D:399-667 (z arrays ignored in 2D).  The absorbing slabs are
+ * the points where a or a_half is non-zero or K or K_half differs from 1. */
+int cpml_gpu_set_pml(cpml_solver *h, const void *a_x, const void *b_x, const void *K_x,
+                     const void *a_x_half, const void *b_x_half, const void *K_x_half,
+                     const void *a_y, const void *b_y, const void *K_y, const void *a_y_half,
+                     const void *b_y_half, const void *K_y_half, const void *a_z,
+                     const void *b_z, const void *K_z, const void *a_z_half,
+                     const void *b_z_half, const void *K_z_half);
+
+/* Point force at (ISOURCE, JSOURCE[, KSOURCE]); force_x/force_y[it-1] is the
+ * force at step it (2D2:644-667, 3D:1055-1083).  ISOURCE <= 0 removes it. */
+int cpml_gpu_set_source(cpml_solver *h, int ISOURCE, int JSOURCE, int KSOURCE,
+                        const void *force_x, const void *force_y);
+
+/* Receivers at grid points (2D2:683-686, 3D:1123-1129); iz_rec ignored in 2D. */
+int cpml_gpu_set_receivers(cpml_solver *h, int NREC, const int *ix_rec, const int *iy_rec,
+                           const int *iz_rec);
+
+/* Energy curve every `cadence` steps (0 = off) over the window
+ * window[0..5] = i0,i1,j0,j1,k0,k1 (1-based, inclusive; NULL = the
+ * reference's window: 2D2:695 / 2D4:696 / 3D:1137-1139).  flags bit 0: use the
+ * textbook 3D potential energy instead of the reference formula, which counts
+ * eps_yy*sigma_yy twice and omits eps_zz*sigma_zz (3D:1170-1172). */
+int cpml_gpu_set_energy(cpml_solver *h, int cadence, const int *window, int flags);
+
+/* Stability check inside run(): at it % IT_DISPLAY == 0 and it == 5 the max
+ * norm of v is compared with threshold (reference: 1e25); IT_DISPLAY <= 0
+ * disables it. */
+int cpml_gpu_set_stability(cpml_solver *h, int IT_DISPLAY, double threshold);
+
+/* ---- time loop ------------------------------------------------------- */
+
+/* Runs steps it_begin..it_end (1-based, inclusive) of `do it = 1,NSTEP`
+ * (2D2:550-735, 2D4:551-736, 3D:802-1245): stress + C-PML, velocity +
+ * C-PML, source, Dirichlet edges, receivers, energy.  Blocks until done. */
+int cpml_gpu_run(cpml_solver *h, int it_begin, int it_end);
+
+/* ---- results --------------------------------------------------------- */
+
+/* sisvx(NSTEP,NREC) etc. in Fortran order; sisvz may be NULL. */
+int cpml_gpu_get_seismograms(cpml_solver *h, void *sisvx, void *sisvy, void *sisvz);
+/* total_energy_kinetic(NSTEP), total_energy_potential(NSTEP) (double). */
+int cpml_gpu_get_energy(cpml_solver *h, double *kinetic, double *potential);
+/* One field (CPML_VX ...) as an interior Fortran-order array. */
+int cpml_gpu_get_field(cpml_solver *h, int which, void *out);
+int cpml_gpu_set_field(cpml_solver *h, int which, const void *in);
+/* One memory variable (CPML_MEM_...) expanded to the full grid (zero outside
+ * its absorbing slab), as the reference stores it. */
+int cpml_gpu_get_memory(cpml_solver *h, int which, void *out);
+/* max over the grid of sqrt(vx^2 + vy^2 [+ vz^2]) (2D2:719, 3D:1185). */
+int cpml_gpu_velocity_norm(cpml_solver *h, double *vmax);
+
+/* ---- execution options and measurement ------------------------------- */
+
+/* Steps captured per HIP graph replay (default 16; 0 = launch eagerly). */
+int cpml_gpu_set_graph_steps(cpml_solver *h, int steps);
+/* Planes per block along z for the 3D kernels (default 64). */
+int cpml_gpu_set_kchunk(cpml_solver *h, int kchunk);
+/* When on, run() launches eagerly with a HIP event pair around every stress
+ * and velocity kernel on the handle's stream; see cpml_gpu_kernel_time. */
+int cpml_gpu_set_kernel_timing(cpml_solver *h, int on);
+int cpml_gpu_kernel_time(cpml_solver *h, int kernel, int *launches, double *total_ms);
+/* Device time of the last run() (HIP events around it on the handle's stream). */
+int cpml_gpu_elapsed_ms(cpml_solver *h, double *ms);
+/* Device bytes allocated by the handle. */
+int cpml_gpu_device_bytes(cpml_solver *h, double *bytes);
+/* Widths of the absorbing slabs found by set_pml: w[0..5] = x min/max,
+ * y min/max, z min/max. */
+int cpml_gpu_slab_widths(cpml_solver *h, int *w);
+
+/* Message of the last error on this handle (h == NULL: of the last failed
+ * cpml_gpu_init on this thread).  Always NUL-terminated. */
+int cpml_gpu_last_error(cpml_solver *h, char *buf, int len);
+
+/* ---- host-side set-up helpers (no GPU needed) -------------------------- */
+
+/* C-PML profile along one axis (2D2:283-394 for x; 2D2:396-465 for y;
+ * 3D:399-667): d0 = -(NPOWER+1) cp ln(Rcoef) / (2 L), NPOWER = 2,
+ * K_MAX_PML = 1, ALPHA_MAX_PML = 2 pi (f0/2), Rcoef = 1e-3.
+ * origin_max_uses_n reproduces 2D4:401 (N*DELTA - L), clamp_alpha the
+ * x-only clamp of 2D2:385-386.  Output arrays of length n in precision prec. */
+int cpml_pml_profile(int prec, int n, double delta, double deltat, int npoints_pml, double cp,
+                     double f0, int use_min, int use_max, int origin_max_uses_n,
+                     int clamp_alpha, void *a, void *b, void *K, void *a_half, void *b_half,
+                     void *K_half);
+/* Force tables (2D2:644-657): kind 0 Gaussian, 1 first derivative of a
+ * Gaussian (what the reference programs run), 2 Ricker; ANGLE_FORCE in
+ * degrees clockwise from the vertical y axis. */
+int cpml_source_time_function(int prec, int nstep, double deltat, double f0, double t0,
+                              double factor, double angle_force, int kind, void *force_x,
+                              void *force_y);
+/* Nearest grid point of NREC receivers evenly spaced from (xdeb,ydeb) to
+ * (xfin,yfin) (2D2:483-509).  1-based indices. */
+int cpml_nearest_receivers(int prec, int NX, int NY, double DELTAX, double DELTAY, int NREC,
+                           double xdeb, double ydeb, double xfin, double yfin, int *ix_rec,
+                           int *iy_rec);
+/* Courant number cp*DELTAT*sqrt(1/DELTAX^2 + 1/DELTAY^2 [+ 1/DELTAZ^2])
+ * (2D2:513, 3D:712). */
+int cpml_courant_number(int ndim, double cp, double deltat, double deltax, double deltay,
+                        double deltaz, double *courant);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

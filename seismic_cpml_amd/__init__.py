@@ -1,0 +1,18 @@
+"""seismic_cpml_amd -- the seismic C-PML time loop on AMD Instinct MI355X.
+
+Native core: ``libcpml_gpu.so`` (hand-written HIP kernels for gfx950 behind
+the C ABI in include/cpml_gpu.h), bound from Fortran by the ``cpml_gpu``
+module (fortran/cpml_gpu.f90) and from Python by :mod:`.solver`.
+"""
+from .solver import (  # noqa: F401
+    CpmlError,
+    CpmlGpu,
+    courant_number,
+    library,
+    nearest_receivers,
+    pml_profile,
+    source_time_function,
+)
+
+__all__ = ["CpmlError", "CpmlGpu", "courant_number", "library", "nearest_receivers",
+           "pml_profile", "source_time_function"]

@@ -1,0 +1,116 @@
+"""Build the native parts of seismic_cpml_amd in-tree.
+
+* ``libcpml_gpu.so``  -- HIP kernels + C ABI (include/cpml_gpu.h), gfx950.
+* Fortran module ``cpml_gpu`` and the driver programs (amdflang), linked to it.
+
+Run ``python -m seismic_cpml_amd.build`` (or ``__graft_entry__.build()``).
+Everything is compiled on the host; no GPU is needed to build.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+FSRC = PKG / "fortran"
+BUILD = PKG / "_build"
+LIB = PKG / "libcpml_gpu.so"
+FBIN = FSRC / "bin"
+
+ARCH = os.environ.get("CPML_OFFLOAD_ARCH", "gfx950")
+
+# -ffp-contract=off: no fused multiply-add, so results match the reference
+# statement by statement; packed FP32 instructions disabled (known wrong
+# results on this hardware pool).
+HIPFLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas",
+]
+HIP_SOURCES = ["cpml_kernels.hip", "cpml_api.hip"]
+
+
+def _run(cmd, cwd=None):
+    r = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"command failed ({r.returncode}): {' '.join(map(str, cmd))}\n"
+                           f"{r.stdout}\n{r.stderr}")
+    # surface real warnings but not the host pass' note about the GPU-only feature
+    noise = ("packed-fp32-ops", "is not a recognized feature")
+    lines = [l for l in (r.stderr or "").splitlines() if l and not any(n in l for n in noise)]
+    if lines:
+        print("\n".join(lines), file=sys.stderr)
+    return r
+
+
+def build_hip(verbose: bool = False) -> Path:
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    BUILD.mkdir(exist_ok=True)
+    objs = []
+    for src in HIP_SOURCES:
+        obj = BUILD / (Path(src).stem + ".o")
+        _run([hipcc, *HIPFLAGS, "-c", str(CSRC / src), "-o", str(obj)])
+        objs.append(str(obj))
+        if verbose:
+            print("compiled", src)
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-o", str(tmp), *objs])
+    os.replace(tmp, LIB)
+    if verbose:
+        print("linked", LIB)
+    return LIB
+
+
+# Driver programs: (source, executable, preprocessor macros).  WP is the real
+# kind; the *_small 3D build matches the reduced 3D golden case.
+DRIVERS = [
+    ("seismic_cpml_2d_isotropic_gpu.F90", "seismic_cpml_2d_second_f64", {"ORDER": 2, "WP": 8}),
+    ("seismic_cpml_2d_isotropic_gpu.F90", "seismic_cpml_2d_second_f32", {"ORDER": 2, "WP": 4}),
+    ("seismic_cpml_2d_isotropic_gpu.F90", "seismic_cpml_2d_fourth_f64", {"ORDER": 4, "WP": 8}),
+    ("seismic_cpml_2d_isotropic_gpu.F90", "seismic_cpml_2d_fourth_f32", {"ORDER": 4, "WP": 4}),
+    ("seismic_cpml_3d_isotropic_gpu.F90", "seismic_cpml_3d_f64", {"WP": 8}),
+    ("seismic_cpml_3d_isotropic_gpu.F90", "seismic_cpml_3d_f32", {"WP": 4}),
+    ("seismic_cpml_3d_isotropic_gpu.F90", "seismic_cpml_3d_small_f64",
+     {"WP": 8, "NX_": 40, "NY_": 44, "NZ_": 36, "NSTEP_": 300, "NPML_": 8, "YDEB_": 300,
+      "YFIN_": 100}),
+    ("seismic_cpml_3d_isotropic_gpu.F90", "seismic_cpml_3d_small_f32",
+     {"WP": 4, "NX_": 40, "NY_": 44, "NZ_": 36, "NSTEP_": 300, "NPML_": 8, "YDEB_": 300,
+      "YFIN_": 100}),
+]
+
+
+def build_fortran(verbose: bool = False) -> list[Path]:
+    """Compile the cpml_gpu module and the driver programs (needs amdflang)."""
+    fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+    if not Path(fc).exists():
+        return []
+    BUILD.mkdir(exist_ok=True)
+    FBIN.mkdir(exist_ok=True)
+    mod_obj = BUILD / "cpml_gpu.o"
+    _run([fc, "-O2", "-fPIC", "-J", str(BUILD), "-c", str(FSRC / "cpml_gpu.f90"), "-o",
+          str(mod_obj)])
+    exes = []
+    for src, name, macros in DRIVERS:
+        exe = FBIN / name
+        defs = [f"-D{k}={v}" for k, v in macros.items()]
+        _run([fc, "-O2", "-cpp", *defs, "-J", str(BUILD), "-I", str(BUILD), "-I", str(FSRC),
+              "-o", str(exe), str(FSRC / "drivers" / src), str(mod_obj), f"-L{PKG}",
+              "-lcpml_gpu", "-Wl,-rpath,$ORIGIN/../.."])
+        exes.append(exe)
+        if verbose:
+            print("built", exe)
+    return exes
+
+
+def build(verbose: bool = False):
+    build_hip(verbose)
+    build_fortran(verbose)
+
+
+if __name__ == "__main__":
+    build(verbose=True)

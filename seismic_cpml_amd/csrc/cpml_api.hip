@@ -1,0 +1,1270 @@
+// C ABI and host runtime of libcpml_gpu (see include/cpml_gpu.h).
+//
+// A handle owns one non-blocking HIP stream, the padded wavefield/media
+// arrays, the slab-only C-PML memory variables, the source and receiver
+// tables and a HIP graph that replays a fixed number of time steps.  The
+// current step number lives in device memory and is advanced by the receiver
+// kernel at the end of every step, so one captured graph serves every
+// position in the time loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/cpml_gpu.h"
+#include "cpml_internal.h"
+
+using cpml::Slab;
+
+namespace {
+
+thread_local std::string g_init_error;
+
+// index of each memory variable in the reference naming (include/cpml_gpu.h)
+// and its axis (0 x, 1 y, 2 z)
+constexpr int kMemAxis[18] = {0, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 0, 2, 1, 2};
+// kernel slots: x-slab, y-slab, z-slab memories in the order the kernels use
+constexpr int kSlot3Dx[6] = {CPML_MEM_DVX_DX,      CPML_MEM_DVY_DX,      CPML_MEM_DVZ_DX,
+                             CPML_MEM_DSIGMAXX_DX, CPML_MEM_DSIGMAXY_DX, CPML_MEM_DSIGMAXZ_DX};
+constexpr int kSlot3Dy[6] = {CPML_MEM_DVY_DY,      CPML_MEM_DVX_DY,      CPML_MEM_DVZ_DY,
+                             CPML_MEM_DSIGMAXY_DY, CPML_MEM_DSIGMAYY_DY, CPML_MEM_DSIGMAYZ_DY};
+constexpr int kSlot3Dz[6] = {CPML_MEM_DVZ_DZ,      CPML_MEM_DVX_DZ,      CPML_MEM_DVY_DZ,
+                             CPML_MEM_DSIGMAXZ_DZ, CPML_MEM_DSIGMAYZ_DZ, CPML_MEM_DSIGMAZZ_DZ};
+constexpr int kSlot2Dx[4] = {CPML_MEM_DVX_DX, CPML_MEM_DVY_DX, CPML_MEM_DSIGMAXX_DX,
+                             CPML_MEM_DSIGMAXY_DX};
+constexpr int kSlot2Dy[4] = {CPML_MEM_DVY_DY, CPML_MEM_DVX_DY, CPML_MEM_DSIGMAXY_DY,
+                             CPML_MEM_DSIGMAYY_DY};
+
+bool mem_used(int ndim, int which) {
+  if (ndim == 3) return which >= 0 && which < 18;
+  for (int q = 0; q < 4; ++q)
+    if (kSlot2Dx[q] == which || kSlot2Dy[q] == which) return true;
+  return false;
+}
+
+__global__ void k_coeffs3d(const float *lam, const float *mu, const float *rho, float *cl,
+                           float *cm, float *cb, float dt, int nx, int ny, int nz, long ld,
+                           long ps) {
+  const long n = (long)nx * ny * nz;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n;
+       t += (long)gridDim.x * blockDim.x) {
+    const long q = (t % nx) + ((t / nx) % ny) * ld + (t / ((long)nx * ny)) * ps;
+    cl[q] = dt * lam[q];
+    cm[q] = dt * mu[q];
+    cb[q] = dt / rho[q];
+  }
+}
+
+__global__ void k_coeffs3d_d(const double *lam, const double *mu, const double *rho, double *cl,
+                             double *cm, double *cb, double dt, int nx, int ny, int nz, long ld,
+                             long ps) {
+  const long n = (long)nx * ny * nz;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n;
+       t += (long)gridDim.x * blockDim.x) {
+    const long q = (t % nx) + ((t / nx) % ny) * ld + (t / ((long)nx * ny)) * ps;
+    cl[q] = dt * lam[q];
+    cm[q] = dt * mu[q];
+    cb[q] = dt / rho[q];
+  }
+}
+
+// Scatter an interior-only array (device, Fortran order) into a padded one.
+template <typename T>
+__global__ void k_scatter(const T *src, T *dst, int nx, int ny, int nz, long ld, long ps) {
+  const long n = (long)nx * ny * nz;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n;
+       t += (long)gridDim.x * blockDim.x) {
+    const long q = (t % nx) + ((t / nx) % ny) * ld + (t / ((long)nx * ny)) * ps;
+    dst[q] = src[t];
+  }
+}
+
+// Medium check on the device: flags bad points and reduces max (l+2m)/rho.
+template <typename T>
+__global__ void k_medium_check(const T *lam, const T *mu, const T *rho, int nx, int ny, int nz,
+                               long ld, long ps, unsigned long long *vmax, int *bad) {
+  const long n = (long)nx * ny * nz;
+  double m = 0.0;
+  int b = 0;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n;
+       t += (long)gridDim.x * blockDim.x) {
+    const long q = (t % nx) + ((t / nx) % ny) * ld + (t / ((long)nx * ny)) * ps;
+    const double r = rho[q], u = mu[q], l = lam[q];
+    if (!(r > 0.0) || !(u > 0.0) || !(l + 2.0 * u > 0.0) || !std::isfinite(l)) b = 1;
+    const double v = (l + 2.0 * u) / r;
+    if (v > m) m = v;
+  }
+  if (b) atomicOr(bad, 1);
+  atomicMax(vmax, (unsigned long long)__double_as_longlong(m));
+}
+
+long round_up(long v, long m) { return (v + m - 1) / m * m; }
+
+}  // namespace
+
+struct cpml_solver {
+  int ndim = 2, order = 2, prec = 8, nx = 0, ny = 0, nz = 1, npml = 0, nstep = 0;
+  double dx = 0, dy = 0, dz = 0, dt = 0;
+  int device = 0;
+  size_t esize = 8;
+  int R = 1;
+  long xpad = 0, ld = 0, ps = 0, base = 0, total = 0, mld = 0;
+  Slab sx, sy, sz;
+  hipStream_t stream = nullptr;
+  std::vector<void *> allocs;
+  double bytes = 0;
+  // device arrays (padded arrays point at their allocation start)
+  void *field[9] = {};  // vx vy vz sxx syy szz sxy sxz syz
+  void *media[6] = {};  // lambda mu rho, and in 3D: DELTAT*lambda DELTAT*mu DELTAT/rho
+  void *prof = nullptr; // [3 axes][6][n]
+  void *mem[18] = {};
+  size_t mem_elems[18] = {};
+  void *fx = nullptr, *fy = nullptr;
+  int *step = nullptr;
+  int *recidx = nullptr;
+  void *seis[3] = {};
+  int nrec = 0;
+  double *ekin = nullptr, *epot = nullptr, *epart = nullptr;
+  int eblocks = 0;
+  unsigned long long *scratch_max = nullptr;
+  int *scratch_flag = nullptr;
+  bool have_medium = false, have_pml = false;
+  int isrc = -1, jsrc = -1, ksrc = -1;
+  int cadence = 0, ewin[6] = {}, eflags = 0;
+  int it_display = 0;
+  double threshold = 1e25;
+  int graph_steps = 16;
+  hipGraphExec_t gexec = nullptr;
+  bool graph_dirty = true;
+  int kchunk = 64;
+  bool ktiming = false;
+  std::vector<hipEvent_t> evpool;
+  double ktime[2] = {0, 0};
+  int kcount[2] = {0, 0};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0;
+  double last_vnorm = 0;
+  int host_step = 0;
+  std::string err;
+};
+
+namespace {
+
+int fail(cpml_solver *h, int code, const std::string &msg) {
+  if (h)
+    h->err = msg;
+  else
+    g_init_error = msg;
+  return code;
+}
+
+#define HIPCHK(h, call)                                                                  \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail((h), CPML_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int dev_alloc(cpml_solver *h, void **p, size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(h, e == hipErrorOutOfMemory ? CPML_ERR_OOM : CPML_ERR_HIP,
+                std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+  }
+  h->allocs.push_back(*p);
+  h->bytes += (double)bytes;
+  HIPCHK(h, hipMemsetAsync(*p, 0, bytes, h->stream));
+  return CPML_OK;
+}
+
+void dev_free(cpml_solver *h, void *&p) {
+  if (!p) return;
+  auto it = std::find(h->allocs.begin(), h->allocs.end(), p);
+  if (it != h->allocs.end()) h->allocs.erase(it);
+  (void)hipFree(p);
+  p = nullptr;
+}
+
+int set_device(cpml_solver *h) {
+  HIPCHK(h, hipSetDevice(h->device));
+  return CPML_OK;
+}
+
+template <typename T>
+T *F(cpml_solver *h, int q) {
+  return static_cast<T *>(h->field[q]) + h->base;
+}
+template <typename T>
+T *M(cpml_solver *h, int q) {
+  return static_cast<T *>(h->media[q]) + h->base;
+}
+
+// Copy an interior-only host array into / out of a padded device array.
+int upload_padded(cpml_solver *h, void *dst_alloc, const void *src) {
+  const size_t es = h->esize;
+  char *dst = static_cast<char *>(dst_alloc) + h->base * es;
+  const char *s = static_cast<const char *>(src);
+  const int nzz = h->ndim == 3 ? h->nz : 1;
+  for (int k = 0; k < nzz; ++k)
+    HIPCHK(h, hipMemcpy2DAsync(dst + k * h->ps * es, h->ld * es,
+                               s + (size_t)k * h->nx * h->ny * es, h->nx * es, h->nx * es, h->ny,
+                               hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return CPML_OK;
+}
+
+int download_padded(cpml_solver *h, const void *src_alloc, void *dst) {
+  const size_t es = h->esize;
+  const char *src = static_cast<const char *>(src_alloc) + h->base * es;
+  char *d = static_cast<char *>(dst);
+  const int nzz = h->ndim == 3 ? h->nz : 1;
+  for (int k = 0; k < nzz; ++k)
+    HIPCHK(h, hipMemcpy2DAsync(d + (size_t)k * h->nx * h->ny * es, h->nx * es,
+                               src + k * h->ps * es, h->ld * es, h->nx * es, h->ny,
+                               hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return CPML_OK;
+}
+
+bool field_valid(const cpml_solver *h, int which) {
+  if (which < 0 || which > 8) return false;
+  if (h->ndim == 2)
+    return which == CPML_VX || which == CPML_VY || which == CPML_SIGMAXX ||
+           which == CPML_SIGMAYY || which == CPML_SIGMAXY;
+  return true;
+}
+
+// ----- kernel parameter blocks -----
+
+template <typename T>
+cpml::Prof<T> prof_of(cpml_solver *h, int axis) {
+  const long nmax = std::max(h->nx, std::max(h->ny, h->nz));
+  const T *b = static_cast<const T *>(h->prof) + axis * 6 * nmax;
+  return cpml::Prof<T>{b, b + nmax, b + 2 * nmax, b + 3 * nmax, b + 4 * nmax, b + 5 * nmax};
+}
+
+template <typename T>
+cpml::Step2D<T> step2d(cpml_solver *h) {
+  cpml::Step2D<T> p{};
+  p.vx = F<T>(h, CPML_VX);
+  p.vy = F<T>(h, CPML_VY);
+  p.sxx = F<T>(h, CPML_SIGMAXX);
+  p.syy = F<T>(h, CPML_SIGMAYY);
+  p.sxy = F<T>(h, CPML_SIGMAXY);
+  p.lam = M<T>(h, 0);
+  p.mu = M<T>(h, 1);
+  p.rho = M<T>(h, 2);
+  for (int q = 0; q < 4; ++q) {
+    p.mx[q] = static_cast<T *>(h->mem[kSlot2Dx[q]]);
+    p.my[q] = static_cast<T *>(h->mem[kSlot2Dy[q]]);
+  }
+  p.px = prof_of<T>(h, 0);
+  p.py = prof_of<T>(h, 1);
+  p.sx = h->sx;
+  p.sy = h->sy;
+  p.nx = h->nx;
+  p.ny = h->ny;
+  p.ld = h->ld;
+  p.mld = h->mld;
+  // 2D2:564 divides by DELTAX; 2D4:565 by (24*DELTAX); ADE8:734 by DELTAX
+  p.den_x = h->order == 4 ? T(24) * T(h->dx) : T(h->dx);
+  p.den_y = h->order == 4 ? T(24) * T(h->dy) : T(h->dy);
+  p.dt = T(h->dt);
+  p.isrc = h->isrc;
+  p.jsrc = h->jsrc;
+  p.fx = static_cast<const T *>(h->fx);
+  p.fy = static_cast<const T *>(h->fy);
+  p.step = h->step;
+  return p;
+}
+
+template <typename T>
+cpml::Step3D<T> step3d(cpml_solver *h) {
+  cpml::Step3D<T> p{};
+  for (int q = 0; q < 9; ++q) (&p.vx)[q] = F<T>(h, q);
+  p.clam = M<T>(h, 3);
+  p.cmu = M<T>(h, 4);
+  p.cbuoy = M<T>(h, 5);
+  p.rho = M<T>(h, 2);
+  for (int q = 0; q < 6; ++q) {
+    p.mx[q] = static_cast<T *>(h->mem[kSlot3Dx[q]]);
+    p.my[q] = static_cast<T *>(h->mem[kSlot3Dy[q]]);
+    p.mz[q] = static_cast<T *>(h->mem[kSlot3Dz[q]]);
+  }
+  p.px = prof_of<T>(h, 0);
+  p.py = prof_of<T>(h, 1);
+  p.pz = prof_of<T>(h, 2);
+  p.sx = h->sx;
+  p.sy = h->sy;
+  p.sz = h->sz;
+  p.nx = h->nx;
+  p.ny = h->ny;
+  p.nz = h->nz;
+  p.ld = h->ld;
+  p.ps = h->ps;
+  p.mld = h->mld;
+  p.mps = h->mld * h->ny;
+  // 3D:134-137 ONE_OVER_DELTA = 1/DELTA, applied by multiplication (3D:841)
+  p.odx = T(1) / T(h->dx);
+  p.ody = T(1) / T(h->dy);
+  p.odz = T(1) / T(h->dz);
+  p.dt = T(h->dt);
+  p.isrc = h->isrc;
+  p.jsrc = h->jsrc;
+  p.ksrc = h->ksrc;
+  p.fx = static_cast<const T *>(h->fx);
+  p.fy = static_cast<const T *>(h->fy);
+  p.step = h->step;
+  p.kchunk = h->kchunk;
+  return p;
+}
+
+template <typename T>
+cpml::Record<T> record_of(cpml_solver *h) {
+  cpml::Record<T> r{};
+  r.vx = F<T>(h, CPML_VX);
+  r.vy = F<T>(h, CPML_VY);
+  r.vz = h->ndim == 3 ? F<T>(h, CPML_VZ) : nullptr;
+  r.ld = h->ld;
+  r.ps = h->ps;
+  r.nrec = h->nrec;
+  r.idx = h->recidx;
+  r.sx = static_cast<T *>(h->seis[0]);
+  r.sy = static_cast<T *>(h->seis[1]);
+  r.sz = h->ndim == 3 ? static_cast<T *>(h->seis[2]) : nullptr;
+  r.step = h->step;
+  return r;
+}
+
+template <typename T>
+cpml::Energy<T> energy_of(cpml_solver *h) {
+  cpml::Energy<T> e{};
+  for (int q = 0; q < 9; ++q)
+    (&e.vx)[q] = h->field[q] ? F<T>(h, q) : nullptr;
+  e.lam = M<T>(h, 0);
+  e.mu = M<T>(h, 1);
+  e.rho = M<T>(h, 2);
+  e.ld = h->ld;
+  e.ps = h->ps;
+  e.i0 = h->ewin[0] - 1;
+  e.i1 = h->ewin[1] - 1;
+  e.j0 = h->ewin[2] - 1;
+  e.j1 = h->ewin[3] - 1;
+  e.k0 = h->ewin[4] - 1;
+  e.k1 = h->ewin[5] - 1;
+  e.ndim = h->ndim;
+  e.corrected3d = h->eflags & 1;
+  e.cadence = h->cadence;
+  e.partial = h->epart;
+  e.out_kin = h->ekin;
+  e.out_pot = h->epot;
+  e.step = h->step;
+  return e;
+}
+
+template <typename T>
+hipError_t enqueue_stress(cpml_solver *h) {
+  return h->ndim == 2 ? cpml::launch_stress2d<T>(h->order, step2d<T>(h), h->stream)
+                      : cpml::launch_stress3d<T>(step3d<T>(h), h->stream);
+}
+template <typename T>
+hipError_t enqueue_velocity(cpml_solver *h) {
+  return h->ndim == 2 ? cpml::launch_velocity2d<T>(h->order, step2d<T>(h), h->stream)
+                      : cpml::launch_velocity3d<T>(step3d<T>(h), h->stream);
+}
+template <typename T>
+hipError_t enqueue_tail(cpml_solver *h) {
+  hipError_t e = hipSuccess;
+  if (h->cadence > 0) {
+    e = cpml::launch_energy<T>(energy_of<T>(h), h->eblocks, h->stream);
+    if (e != hipSuccess) return e;
+  }
+  return cpml::launch_record<T>(record_of<T>(h), h->stream);
+}
+
+template <typename T>
+hipError_t enqueue_step(cpml_solver *h) {
+  hipError_t e = enqueue_stress<T>(h);
+  if (e == hipSuccess) e = enqueue_velocity<T>(h);
+  if (e == hipSuccess) e = enqueue_tail<T>(h);
+  return e;
+}
+
+hipError_t enqueue_step_any(cpml_solver *h) {
+  return h->prec == 4 ? enqueue_step<float>(h) : enqueue_step<double>(h);
+}
+
+int build_graph(cpml_solver *h) {
+  if (h->gexec) {
+    (void)hipGraphExecDestroy(h->gexec);
+    h->gexec = nullptr;
+  }
+  hipGraph_t g = nullptr;
+  HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = hipSuccess;
+  for (int s = 0; s < h->graph_steps && e == hipSuccess; ++s) e = enqueue_step_any(h);
+  hipError_t e2 = hipStreamEndCapture(h->stream, &g);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return fail(h, CPML_ERR_HIP,
+                std::string("graph capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+  }
+  e = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(h, CPML_ERR_HIP, std::string("hipGraphInstantiate: ") +
+                                                        hipGetErrorString(e));
+  h->graph_dirty = false;
+  return CPML_OK;
+}
+
+int ensure_events(cpml_solver *h, size_t n) {
+  while (h->evpool.size() < n) {
+    hipEvent_t e;
+    HIPCHK(h, hipEventCreate(&e));
+    h->evpool.push_back(e);
+  }
+  return CPML_OK;
+}
+
+// Enqueue n steps; in timing mode event triples bracket each stress and
+// velocity kernel (evpool[3*s .. 3*s+2] for step s of this segment).
+int enqueue_steps(cpml_solver *h, int n, int *ev_used) {
+  if (h->ktiming) {
+    int rc = ensure_events(h, (size_t)(*ev_used + 3 * n));
+    if (rc) return rc;
+    for (int s = 0; s < n; ++s) {
+      hipEvent_t *e = &h->evpool[*ev_used];
+      HIPCHK(h, hipEventRecord(e[0], h->stream));
+      HIPCHK(h, h->prec == 4 ? enqueue_stress<float>(h) : enqueue_stress<double>(h));
+      HIPCHK(h, hipEventRecord(e[1], h->stream));
+      HIPCHK(h, h->prec == 4 ? enqueue_velocity<float>(h) : enqueue_velocity<double>(h));
+      HIPCHK(h, hipEventRecord(e[2], h->stream));
+      HIPCHK(h, h->prec == 4 ? enqueue_tail<float>(h) : enqueue_tail<double>(h));
+      *ev_used += 3;
+    }
+    return CPML_OK;
+  }
+  if (h->graph_steps > 0 && n >= h->graph_steps) {
+    if (h->graph_dirty || !h->gexec) {
+      int rc = build_graph(h);
+      if (rc) return rc;
+    }
+    while (n >= h->graph_steps) {
+      HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
+      n -= h->graph_steps;
+    }
+  }
+  for (int s = 0; s < n; ++s) HIPCHK(h, enqueue_step_any(h));
+  return CPML_OK;
+}
+
+int velocity_norm(cpml_solver *h, double *out) {
+  HIPCHK(h, hipMemsetAsync(h->scratch_max, 0, sizeof(unsigned long long), h->stream));
+  if (h->prec == 4)
+    HIPCHK(h, cpml::launch_vmax<float>(F<float>(h, 0), F<float>(h, 1),
+                                       h->ndim == 3 ? F<float>(h, 2) : nullptr, h->nx, h->ny,
+                                       h->ndim == 3 ? h->nz : 1, h->ld, h->ps, h->scratch_max,
+                                       h->stream));
+  else
+    HIPCHK(h, cpml::launch_vmax<double>(F<double>(h, 0), F<double>(h, 1),
+                                        h->ndim == 3 ? F<double>(h, 2) : nullptr, h->nx, h->ny,
+                                        h->ndim == 3 ? h->nz : 1, h->ld, h->ps,
+                                        h->scratch_max, h->stream));
+  unsigned long long bits = 0;
+  HIPCHK(h, hipMemcpyAsync(&bits, h->scratch_max, sizeof(bits), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  double v;
+  std::memcpy(&v, &bits, sizeof(v));
+  *out = std::sqrt(v);
+  return CPML_OK;
+}
+
+template <typename T>
+void slab_from_profile(const T *a, const T *ah, const T *K, const T *Kh, int n, Slab &s) {
+  const int half = (n + 1) / 2;
+  int wl = 0, wr = 0;
+  for (int i = 0; i < n; ++i) {
+    const bool nz = a[i] != T(0) || ah[i] != T(0) || K[i] != T(1) || Kh[i] != T(1);
+    if (!nz) continue;
+    if (i < half)
+      wl = std::max(wl, i + 1);
+    else
+      wr = std::max(wr, n - i);
+  }
+  s.wl = wl;
+  s.wr = wr;
+  s.w = wl + wr;
+  s.n = n;
+}
+
+template <typename T>
+int check_medium_host(cpml_solver *h, const T *lam, const T *mu, const T *rho, double *cpmax) {
+  const size_t n = (size_t)h->nx * h->ny * (h->ndim == 3 ? h->nz : 1);
+  double m = 0.0;
+  for (size_t q = 0; q < n; ++q) {
+    const double r = rho[q], u = mu[q], l = lam[q];
+    if (!(r > 0.0) || !(u > 0.0) || !(l + 2.0 * u > 0.0) || !std::isfinite(l))
+      return fail(h, CPML_ERR_ARG,
+                  "medium: rho and mu must be positive and lambda+2mu > 0 at every point "
+                  "(first bad point " + std::to_string(q) + ")");
+    m = std::max(m, (l + 2.0 * u) / r);
+  }
+  *cpmax = std::sqrt(m);
+  return CPML_OK;
+}
+
+int cfl_check(cpml_solver *h, double cpmax) {
+  double c = 0;
+  cpml_courant_number(h->ndim, cpmax, h->dt, h->dx, h->dy, h->dz, &c);
+  if (c > 1.0) {
+    char b[200];
+    std::snprintf(b, sizeof b,
+                  "time step is too large, simulation will be unstable (Courant number %.6g "
+                  "with cp = %.6g)", c, cpmax);
+    return fail(h, CPML_ERR_CFL, b);
+  }
+  return CPML_OK;
+}
+
+int finish_medium(cpml_solver *h) {
+  if (h->ndim == 3) {
+    const int nb = 4096;
+    if (h->prec == 4)
+      hipLaunchKernelGGL(k_coeffs3d, dim3(nb), dim3(256), 0, h->stream, M<float>(h, 0),
+                         M<float>(h, 1), M<float>(h, 2), M<float>(h, 3), M<float>(h, 4),
+                         M<float>(h, 5), (float)h->dt, h->nx, h->ny, h->nz, h->ld, h->ps);
+    else
+      hipLaunchKernelGGL(k_coeffs3d_d, dim3(nb), dim3(256), 0, h->stream, M<double>(h, 0),
+                         M<double>(h, 1), M<double>(h, 2), M<double>(h, 3), M<double>(h, 4),
+                         M<double>(h, 5), h->dt, h->nx, h->ny, h->nz, h->ld, h->ps);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
+  h->have_medium = true;
+  return CPML_OK;
+}
+
+template <typename T>
+int profile_impl(int n, double delta_d, double deltat_d, int npml, double cp_d, double f0_d,
+                 int use_min, int use_max, int origin_max_uses_n, int clamp_alpha, T *a, T *b,
+                 T *K, T *ah, T *bh, T *Kh) {
+  // 2D2:283-394: parameters and the per-point formulas of the damping profile
+  const T delta = T(delta_d), deltat = T(deltat_d), cp = T(cp_d), f0 = T(f0_d);
+  const T pi = std::is_same<T, float>::value ? T(3.141592653589793238462643f)
+                                            : T(3.141592653589793238462643);
+  const T npower = T(2);
+  const T kmax = T(1);
+  const T alpha_max = T(2) * pi * (f0 / T(2));
+  const T L = T(npml) * delta;
+  const T rcoef = std::is_same<T, float>::value ? T(0.001f) : T(0.001);
+  const T d0 = -(((npower + T(1)) * cp * std::log(rcoef)) / (T(2) * L));
+  const T omin = L;
+  const T omax = T(origin_max_uses_n ? n : n - 1) * delta - L;
+  const T eps = std::is_same<T, float>::value ? T(1.0e-6f) : T(1.0e-6);
+  for (int i = 0; i < n; ++i) {
+    T d[2] = {0, 0}, k[2] = {1, 1}, al[2] = {0, 0};
+    const T x[2] = {delta * T(i), delta * T(i) + delta / T(2)};
+    for (int h = 0; h < 2; ++h) {
+      // min side first, max side overwrites (2D2:333-381)
+      T dist[2] = {omin - x[h], x[h] - omax};
+      for (int side = 0; side < 2; ++side) {
+        if (!(side == 0 ? use_min : use_max)) continue;
+        if (dist[side] >= T(0)) {
+          const T xn = dist[side] / L;
+          d[h] = d0 * (xn * xn);
+          k[h] = T(1) + (kmax - T(1)) * (xn * xn);
+          al[h] = alpha_max * (T(1) - xn);
+        }
+      }
+      if (clamp_alpha && al[h] < T(0)) al[h] = T(0);
+    }
+    T bb[2], aa[2];
+    for (int h = 0; h < 2; ++h) {
+      bb[h] = std::exp(-(d[h] / k[h] + al[h]) * deltat);
+      aa[h] = std::fabs(d[h]) > eps ? d[h] * (bb[h] - T(1)) / (k[h] * (d[h] + k[h] * al[h])) : T(0);
+    }
+    a[i] = aa[0];
+    b[i] = bb[0];
+    K[i] = k[0];
+    ah[i] = aa[1];
+    bh[i] = bb[1];
+    Kh[i] = k[1];
+  }
+  return CPML_OK;
+}
+
+template <typename T>
+int stf_impl(int nstep, double deltat_d, double f0_d, double t0_d, double factor_d,
+             double angle_d, int kind, T *fx, T *fy) {
+  // 2D2:644-657
+  const T deltat = T(deltat_d), f0 = T(f0_d), t0 = T(t0_d), factor = T(factor_d);
+  const T pi = std::is_same<T, float>::value ? T(3.141592653589793238462643f)
+                                            : T(3.141592653589793238462643);
+  const T rad = T(angle_d) * (pi / T(180));
+  const T a = pi * pi * f0 * f0;
+  const T sa = std::sin(rad), ca = std::cos(rad);
+  for (int it = 1; it <= nstep; ++it) {
+    const T t = T(it - 1) * deltat;
+    const T u = t - t0;
+    const T g = std::exp(-(a * (u * u)));
+    T s;
+    if (kind == 0)
+      s = factor * g;
+    else if (kind == 1)
+      s = -(factor * T(2) * a * u * g);
+    else
+      s = factor * (T(1) - T(2) * a * (u * u)) * g;
+    fx[it - 1] = sa * s;
+    fy[it - 1] = ca * s;
+  }
+  return CPML_OK;
+}
+
+template <typename T>
+int receivers_impl(int nx, int ny, double dx_d, double dy_d, int nrec, double xdeb_d,
+                   double ydeb_d, double xfin_d, double yfin_d, int *ix, int *iy) {
+  // 2D2:483-509
+  const T dx = T(dx_d), dy = T(dy_d), xdeb = T(xdeb_d), ydeb = T(ydeb_d);
+  const T xsp = nrec > 1 ? (T(xfin_d) - xdeb) / T(nrec - 1) : T(0);
+  const T ysp = nrec > 1 ? (T(yfin_d) - ydeb) / T(nrec - 1) : T(0);
+  for (int r = 0; r < nrec; ++r) {
+    const T xr = xdeb + T(r) * xsp, yr = ydeb + T(r) * ysp;
+    T best = T(1e30);
+    ix[r] = iy[r] = 1;
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const T ex = dx * T(i) - xr, ey = dy * T(j) - yr;
+        const T dist = std::sqrt(ex * ex + ey * ey);
+        if (dist < best) {
+          best = dist;
+          ix[r] = i + 1;
+          iy[r] = j + 1;
+        }
+      }
+  }
+  return CPML_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cpml_gpu_init(cpml_solver **hp, int ndim, int order, int prec, int NX, int NY, int NZ,
+                  double DELTAX, double DELTAY, double DELTAZ, double DELTAT, int NPOINTS_PML,
+                  int NSTEP) {
+  if (!hp) return fail(nullptr, CPML_ERR_ARG, "null handle pointer");
+  *hp = nullptr;
+  if (ndim != 2 && ndim != 3) return fail(nullptr, CPML_ERR_ARG, "ndim must be 2 or 3");
+  if (prec != 4 && prec != 8) return fail(nullptr, CPML_ERR_ARG, "prec must be 4 or 8");
+  if (ndim == 2 && order != 2 && order != 4 && order != 8)
+    return fail(nullptr, CPML_ERR_ARG, "2D order must be 2, 4 or 8");
+  if (ndim == 3 && order != 2) return fail(nullptr, CPML_ERR_ARG, "3D order must be 2");
+  if (NX < 2 || NY < 2 || (ndim == 3 && NZ < 2))
+    return fail(nullptr, CPML_ERR_STENCIL, "grid must have at least 2 points per axis");
+  if (!(DELTAX > 0) || !(DELTAY > 0) || !(DELTAT > 0) || (ndim == 3 && !(DELTAZ > 0)))
+    return fail(nullptr, CPML_ERR_ARG, "DELTAX, DELTAY, DELTAZ and DELTAT must be positive");
+  if (NSTEP < 1) return fail(nullptr, CPML_ERR_ARG, "NSTEP must be >= 1");
+  if (NPOINTS_PML < 0 || 2 * NPOINTS_PML > std::min(NX, NY) ||
+      (ndim == 3 && 2 * NPOINTS_PML > NZ))
+    return fail(nullptr, CPML_ERR_STENCIL, "NPOINTS_PML does not fit the grid");
+  const double cells = (double)NX * NY * (ndim == 3 ? NZ : 1);
+  if (cells > 4.0e10) return fail(nullptr, CPML_ERR_OOM, "grid too large");
+
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(nullptr, CPML_ERR_NODEVICE, "no HIP device visible");
+
+  cpml_solver *h = new cpml_solver();
+  h->ndim = ndim;
+  h->order = order;
+  h->prec = prec;
+  h->nx = NX;
+  h->ny = NY;
+  h->nz = ndim == 3 ? NZ : 1;
+  h->dx = DELTAX;
+  h->dy = DELTAY;
+  h->dz = ndim == 3 ? DELTAZ : 1.0;
+  h->dt = DELTAT;
+  h->npml = NPOINTS_PML;
+  h->nstep = NSTEP;
+  h->esize = prec;
+  h->R = order / 2;
+  const long align = 256 / (long)h->esize;  // elements per 256 bytes
+  h->xpad = align;
+  h->ld = round_up(h->xpad + NX + h->R, align);
+  h->ps = h->ld * (NY + 2 * h->R);
+  h->base = h->xpad + h->R * h->ld + (ndim == 3 ? h->R * h->ps : 0);
+  h->total = h->ps * (ndim == 3 ? NZ + 2 * h->R : 1);
+  h->mld = round_up(NX, align);
+  h->sx.n = NX;
+  h->sy.n = NY;
+  h->sz.n = h->nz;
+  // default energy window: 2D2:695 / 2D4:696 / 3D:1137-1139
+  const int p = NPOINTS_PML;
+  if (ndim == 2 && order == 4) {
+    const int w[6] = {p, NX - p + 1, p, NY - p + 1, 1, 1};
+    std::copy(w, w + 6, h->ewin);
+  } else {
+    const int w[6] = {p + 1, NX - p, p + 1, NY - p, ndim == 3 ? p + 1 : 1, ndim == 3 ? NZ - p : 1};
+    std::copy(w, w + 6, h->ewin);
+  }
+  for (int q = 0; q < 6; ++q) h->ewin[q] = std::max(h->ewin[q], 1);
+
+  int rc = CPML_OK;
+  auto bail = [&](int code) {
+    g_init_error = h->err;
+    cpml_gpu_finalize(h);
+    return code;
+  };
+  if (hipGetDevice(&h->device) != hipSuccess) {
+    h->err = "hipGetDevice failed";
+    return bail(CPML_ERR_HIP);
+  }
+  size_t free_b = 0, tot_b = 0;
+  if (hipMemGetInfo(&free_b, &tot_b) == hipSuccess) {
+    const double need = (double)h->total * h->esize * (ndim == 3 ? 15 : 8);
+    if (need > 0.95 * (double)free_b) {
+      h->err = "device memory budget exceeded: need " + std::to_string(need / 1e9) +
+               " GB, free " + std::to_string(free_b / 1e9) + " GB";
+      return bail(CPML_ERR_OOM);
+    }
+  }
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    h->err = "hipStreamCreate failed";
+    return bail(CPML_ERR_HIP);
+  }
+  if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
+    h->err = "hipEventCreate failed";
+    return bail(CPML_ERR_HIP);
+  }
+  const size_t fbytes = (size_t)h->total * h->esize;
+  for (int q = 0; q < 9 && rc == CPML_OK; ++q)
+    if (field_valid(h, q)) rc = dev_alloc(h, &h->field[q], fbytes);
+  for (int q = 0; q < (ndim == 3 ? 6 : 3) && rc == CPML_OK; ++q)
+    rc = dev_alloc(h, &h->media[q], fbytes);
+  const long nmax = std::max(NX, std::max(NY, h->nz));
+  if (rc == CPML_OK) rc = dev_alloc(h, &h->prof, (size_t)18 * nmax * h->esize);
+  if (rc == CPML_OK) rc = dev_alloc(h, &h->fx, (size_t)NSTEP * h->esize);
+  if (rc == CPML_OK) rc = dev_alloc(h, &h->fy, (size_t)NSTEP * h->esize);
+  if (rc == CPML_OK) rc = dev_alloc(h, (void **)&h->step, sizeof(int) * 64);
+  if (rc == CPML_OK) rc = dev_alloc(h, (void **)&h->scratch_max, 256);
+  if (rc == CPML_OK) rc = dev_alloc(h, (void **)&h->scratch_flag, 256);
+  if (rc == CPML_OK) rc = dev_alloc(h, (void **)&h->recidx, sizeof(int));
+  for (int q = 0; q < 3 && rc == CPML_OK; ++q) rc = dev_alloc(h, &h->seis[q], h->esize);
+  if (rc == CPML_OK && hipStreamSynchronize(h->stream) != hipSuccess) {
+    h->err = "initial clear failed";
+    rc = CPML_ERR_HIP;
+  }
+  if (rc != CPML_OK) return bail(rc);
+  *hp = h;
+  return CPML_OK;
+}
+
+int cpml_gpu_finalize(cpml_solver *h) {
+  if (!h) return CPML_OK;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  for (hipEvent_t e : h->evpool) (void)hipEventDestroy(e);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  for (void *p : h->allocs) (void)hipFree(p);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return CPML_OK;
+}
+
+int cpml_gpu_set_medium(cpml_solver *h, const void *lambda, const void *mu, const void *rho) {
+  if (!h) return CPML_ERR_ARG;
+  if (!lambda || !mu || !rho) return fail(h, CPML_ERR_ARG, "null medium array");
+  int rc = set_device(h);
+  if (rc) return rc;
+  double cpmax = 0;
+  rc = h->prec == 4 ? check_medium_host(h, (const float *)lambda, (const float *)mu,
+                                        (const float *)rho, &cpmax)
+                    : check_medium_host(h, (const double *)lambda, (const double *)mu,
+                                        (const double *)rho, &cpmax);
+  if (rc) return rc;
+  if ((rc = cfl_check(h, cpmax))) return rc;
+  const void *src[3] = {lambda, mu, rho};
+  for (int q = 0; q < 3; ++q)
+    if ((rc = upload_padded(h, h->media[q], src[q]))) return rc;
+  h->graph_dirty = true;
+  return finish_medium(h);
+}
+
+int cpml_gpu_set_medium_device(cpml_solver *h, const void *d_lambda, const void *d_mu,
+                               const void *d_rho) {
+  if (!h) return CPML_ERR_ARG;
+  if (!d_lambda || !d_mu || !d_rho) return fail(h, CPML_ERR_ARG, "null medium array");
+  int rc = set_device(h);
+  if (rc) return rc;
+  const void *src[3] = {d_lambda, d_mu, d_rho};
+  for (int q = 0; q < 3; ++q) {
+    if (h->prec == 4)
+      hipLaunchKernelGGL(k_scatter<float>, dim3(4096), dim3(256), 0, h->stream,
+                         (const float *)src[q], M<float>(h, q), h->nx, h->ny, h->nz, h->ld,
+                         h->ps);
+    else
+      hipLaunchKernelGGL(k_scatter<double>, dim3(4096), dim3(256), 0, h->stream,
+                         (const double *)src[q], M<double>(h, q), h->nx, h->ny, h->nz, h->ld,
+                         h->ps);
+    HIPCHK(h, hipGetLastError());
+  }
+  HIPCHK(h, hipMemsetAsync(h->scratch_max, 0, 256, h->stream));
+  HIPCHK(h, hipMemsetAsync(h->scratch_flag, 0, 256, h->stream));
+  if (h->prec == 4)
+    hipLaunchKernelGGL(k_medium_check<float>, dim3(2048), dim3(256), 0, h->stream,
+                       M<float>(h, 0), M<float>(h, 1), M<float>(h, 2), h->nx, h->ny, h->nz,
+                       h->ld, h->ps, h->scratch_max, h->scratch_flag);
+  else
+    hipLaunchKernelGGL(k_medium_check<double>, dim3(2048), dim3(256), 0, h->stream,
+                       M<double>(h, 0), M<double>(h, 1), M<double>(h, 2), h->nx, h->ny, h->nz,
+                       h->ld, h->ps, h->scratch_max, h->scratch_flag);
+  HIPCHK(h, hipGetLastError());
+  unsigned long long bits = 0;
+  int bad = 0;
+  HIPCHK(h, hipMemcpyAsync(&bits, h->scratch_max, 8, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(&bad, h->scratch_flag, 4, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (bad)
+    return fail(h, CPML_ERR_ARG,
+                "medium: rho and mu must be positive and lambda+2mu > 0 at every point");
+  double v;
+  std::memcpy(&v, &bits, 8);
+  if ((rc = cfl_check(h, std::sqrt(v)))) return rc;
+  h->graph_dirty = true;
+  return finish_medium(h);
+}
+
+int cpml_gpu_set_pml(cpml_solver *h, const void *a_x, const void *b_x, const void *K_x,
+                     const void *a_x_half, const void *b_x_half, const void *K_x_half,
+                     const void *a_y, const void *b_y, const void *K_y, const void *a_y_half,
+                     const void *b_y_half, const void *K_y_half, const void *a_z,
+                     const void *b_z, const void *K_z, const void *a_z_half,
+                     const void *b_z_half, const void *K_z_half) {
+  if (!h) return CPML_ERR_ARG;
+  const void *ax[3][6] = {{a_x, b_x, K_x, a_x_half, b_x_half, K_x_half},
+                          {a_y, b_y, K_y, a_y_half, b_y_half, K_y_half},
+                          {a_z, b_z, K_z, a_z_half, b_z_half, K_z_half}};
+  for (int d = 0; d < h->ndim; ++d)
+    for (int q = 0; q < 6; ++q)
+      if (!ax[d][q]) return fail(h, CPML_ERR_ARG, "null profile array");
+  int rc = set_device(h);
+  if (rc) return rc;
+  const int n[3] = {h->nx, h->ny, h->nz};
+  Slab *sl[3] = {&h->sx, &h->sy, &h->sz};
+  const long nmax = std::max(h->nx, std::max(h->ny, h->nz));
+  std::vector<char> host((size_t)18 * nmax * h->esize, 0);
+  for (int d = 0; d < 3; ++d) {
+    if (d >= h->ndim) {  // unused axis: K = 1, a = 0
+      for (int q = 0; q < 6; ++q)
+        for (long i = 0; i < nmax; ++i) {
+          char *dst = host.data() + ((d * 6 + q) * nmax + i) * h->esize;
+          const double v = (q == 2 || q == 5) ? 1.0 : 0.0;
+          if (h->prec == 4) {
+            float f = (float)v;
+            std::memcpy(dst, &f, 4);
+          } else {
+            std::memcpy(dst, &v, 8);
+          }
+        }
+      *sl[d] = Slab{0, 0, 0, n[d]};
+      continue;
+    }
+    for (int q = 0; q < 6; ++q)
+      std::memcpy(host.data() + (d * 6 + q) * nmax * h->esize, ax[d][q], n[d] * h->esize);
+    if (h->prec == 4)
+      slab_from_profile((const float *)ax[d][0], (const float *)ax[d][3],
+                        (const float *)ax[d][2], (const float *)ax[d][5], n[d], *sl[d]);
+    else
+      slab_from_profile((const double *)ax[d][0], (const double *)ax[d][3],
+                        (const double *)ax[d][2], (const double *)ax[d][5], n[d], *sl[d]);
+    if (sl[d]->w > n[d]) return fail(h, CPML_ERR_STENCIL, "absorbing slabs overlap");
+  }
+  HIPCHK(h, hipMemcpy(h->prof, host.data(), host.size(), hipMemcpyHostToDevice));
+  // (re)allocate the slab-only memory variables, zeroed
+  const size_t ny = h->ny, nzz = h->nz;
+  for (int w = 0; w < 18; ++w) {
+    if (!mem_used(h->ndim, w)) continue;
+    size_t elems = 0;
+    switch (kMemAxis[w]) {
+      case 0: elems = (size_t)h->sx.w * ny * nzz; break;
+      case 1: elems = (size_t)h->mld * h->sy.w * nzz; break;
+      default: elems = (size_t)h->mld * ny * h->sz.w; break;
+    }
+    dev_free(h, h->mem[w]);
+    h->mem_elems[w] = elems;
+    if ((rc = dev_alloc(h, &h->mem[w], elems * h->esize))) return rc;
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->have_pml = true;
+  h->graph_dirty = true;
+  return CPML_OK;
+}
+
+int cpml_gpu_set_source(cpml_solver *h, int ISOURCE, int JSOURCE, int KSOURCE,
+                        const void *force_x, const void *force_y) {
+  if (!h) return CPML_ERR_ARG;
+  int rc = set_device(h);
+  if (rc) return rc;
+  if (ISOURCE <= 0) {
+    h->isrc = h->jsrc = h->ksrc = -1;
+    h->graph_dirty = true;
+    return CPML_OK;
+  }
+  // 2D: the density average at (i+1/2, j+1/2) needs i < NX and j < NY (2D2:663)
+  const int imax = h->ndim == 2 ? h->nx - 1 : h->nx;
+  const int jmax = h->ndim == 2 ? h->ny - 1 : h->ny;
+  if (ISOURCE > imax || JSOURCE < 1 || JSOURCE > jmax ||
+      (h->ndim == 3 && (KSOURCE < 1 || KSOURCE > h->nz)))
+    return fail(h, CPML_ERR_ARG, "source position outside the grid");
+  if (!force_x || !force_y) return fail(h, CPML_ERR_ARG, "null force table");
+  HIPCHK(h, hipMemcpy(h->fx, force_x, (size_t)h->nstep * h->esize, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(h->fy, force_y, (size_t)h->nstep * h->esize, hipMemcpyHostToDevice));
+  h->isrc = ISOURCE - 1;
+  h->jsrc = JSOURCE - 1;
+  h->ksrc = h->ndim == 3 ? KSOURCE - 1 : 0;
+  h->graph_dirty = true;
+  return CPML_OK;
+}
+
+int cpml_gpu_set_receivers(cpml_solver *h, int NREC, const int *ix_rec, const int *iy_rec,
+                           const int *iz_rec) {
+  if (!h) return CPML_ERR_ARG;
+  if (NREC < 0) return fail(h, CPML_ERR_ARG, "NREC must be >= 0");
+  if (NREC > 0 && (!ix_rec || !iy_rec || (h->ndim == 3 && !iz_rec)))
+    return fail(h, CPML_ERR_ARG, "null receiver index array");
+  int rc = set_device(h);
+  if (rc) return rc;
+  std::vector<int> idx(std::max(NREC, 1), 0);
+  for (int r = 0; r < NREC; ++r) {
+    const int i = ix_rec[r], j = iy_rec[r], k = h->ndim == 3 ? iz_rec[r] : 1;
+    if (i < 1 || i > h->nx || j < 1 || j > h->ny || k < 1 || k > h->nz)
+      return fail(h, CPML_ERR_ARG, "receiver " + std::to_string(r + 1) + " outside the grid");
+    const long q = (i - 1) + (long)(j - 1) * h->ld + (long)(k - 1) * h->ps;
+    if (q > 0x7fffffffL) return fail(h, CPML_ERR_ARG, "receiver offset overflow");
+    idx[r] = (int)q;
+  }
+  dev_free(h, *(void **)&h->recidx);
+  for (int q = 0; q < 3; ++q) dev_free(h, h->seis[q]);
+  if ((rc = dev_alloc(h, (void **)&h->recidx, idx.size() * sizeof(int)))) return rc;
+  const size_t sbytes = (size_t)h->nstep * std::max(NREC, 1) * h->esize;
+  for (int q = 0; q < 3; ++q)
+    if ((rc = dev_alloc(h, &h->seis[q], (q == 2 && h->ndim == 2) ? h->esize : sbytes)))
+      return rc;
+  HIPCHK(h, hipMemcpy(h->recidx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
+  h->nrec = NREC;
+  h->graph_dirty = true;
+  return CPML_OK;
+}
+
+int cpml_gpu_set_energy(cpml_solver *h, int cadence, const int *window, int flags) {
+  if (!h) return CPML_ERR_ARG;
+  if (cadence < 0) return fail(h, CPML_ERR_ARG, "cadence must be >= 0");
+  int rc = set_device(h);
+  if (rc) return rc;
+  if (window) {
+    const int n[3] = {h->nx, h->ny, h->nz};
+    for (int d = 0; d < 3; ++d) {
+      const int lo = window[2 * d], hi = window[2 * d + 1];
+      if (d >= h->ndim) continue;
+      if (lo < 1 || hi > n[d] || lo > hi) return fail(h, CPML_ERR_ARG, "bad energy window");
+    }
+    std::copy(window, window + 2 * h->ndim, h->ewin);
+    if (h->ndim == 2) h->ewin[4] = h->ewin[5] = 1;
+  }
+  h->cadence = cadence;
+  h->eflags = flags;
+  dev_free(h, *(void **)&h->ekin);
+  dev_free(h, *(void **)&h->epot);
+  dev_free(h, *(void **)&h->epart);
+  if (cadence > 0) {
+    h->eblocks = h->prec == 4 ? cpml::energy_blocks<float>(energy_of<float>(h))
+                              : cpml::energy_blocks<double>(energy_of<double>(h));
+    if ((rc = dev_alloc(h, (void **)&h->ekin, (size_t)h->nstep * sizeof(double)))) return rc;
+    if ((rc = dev_alloc(h, (void **)&h->epot, (size_t)h->nstep * sizeof(double)))) return rc;
+    if ((rc = dev_alloc(h, (void **)&h->epart, (size_t)h->eblocks * 2 * sizeof(double))))
+      return rc;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
+  h->graph_dirty = true;
+  return CPML_OK;
+}
+
+int cpml_gpu_set_stability(cpml_solver *h, int IT_DISPLAY, double threshold) {
+  if (!h) return CPML_ERR_ARG;
+  h->it_display = IT_DISPLAY;
+  h->threshold = threshold;
+  return CPML_OK;
+}
+
+int cpml_gpu_run(cpml_solver *h, int it_begin, int it_end) {
+  if (!h) return CPML_ERR_ARG;
+  if (!h->have_medium || !h->have_pml)
+    return fail(h, CPML_ERR_STATE, "set_medium and set_pml must be called before run");
+  if (it_begin < 1 || it_end > h->nstep || it_begin > it_end)
+    return fail(h, CPML_ERR_ARG, "steps must satisfy 1 <= it_begin <= it_end <= NSTEP");
+  int rc = set_device(h);
+  if (rc) return rc;
+  h->host_step = it_begin;
+  HIPCHK(h, hipMemcpyAsync(h->step, &h->host_step, sizeof(int), hipMemcpyHostToDevice,
+                           h->stream));
+  HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+  int ev_used = 0;
+  int it = it_begin;
+  while (it <= it_end) {
+    int seg_end = it_end;
+    bool check = false;
+    if (h->it_display > 0) {
+      int c = ((it + h->it_display - 1) / h->it_display) * h->it_display;
+      if (it <= 5) c = std::min(c, 5);
+      if (c <= it_end) {
+        seg_end = c;
+        check = true;
+      }
+    }
+    if ((rc = enqueue_steps(h, seg_end - it + 1, &ev_used))) return rc;
+    it = seg_end + 1;
+    if (check) {
+      double v = 0;
+      if ((rc = velocity_norm(h, &v))) return rc;
+      h->last_vnorm = v;
+      if (!(v <= h->threshold)) {
+        char b[160];
+        std::snprintf(b, sizeof b, "code became unstable and blew up (step %d, max |v| %g)",
+                      seg_end, v);
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        return fail(h, CPML_ERR_BLOWUP, b);
+      }
+    }
+  }
+  HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  float ms = 0;
+  HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->last_ms = ms;
+  for (int s = 0; s + 2 < ev_used; s += 3) {
+    float a = 0, b = 0;
+    HIPCHK(h, hipEventElapsedTime(&a, h->evpool[s], h->evpool[s + 1]));
+    HIPCHK(h, hipEventElapsedTime(&b, h->evpool[s + 1], h->evpool[s + 2]));
+    h->ktime[0] += a;
+    h->ktime[1] += b;
+    h->kcount[0] += 1;
+    h->kcount[1] += 1;
+  }
+  return CPML_OK;
+}
+
+int cpml_gpu_get_seismograms(cpml_solver *h, void *sisvx, void *sisvy, void *sisvz) {
+  if (!h) return CPML_ERR_ARG;
+  int rc = set_device(h);
+  if (rc) return rc;
+  if (h->nrec == 0) return CPML_OK;
+  const size_t n = (size_t)h->nstep * h->nrec;
+  std::vector<char> tmp(n * h->esize);
+  void *outs[3] = {sisvx, sisvy, sisvz};
+  for (int q = 0; q < 3; ++q) {
+    if (!outs[q] || (q == 2 && h->ndim == 2)) continue;
+    HIPCHK(h, hipMemcpy(tmp.data(), h->seis[q], tmp.size(), hipMemcpyDeviceToHost));
+    char *o = static_cast<char *>(outs[q]);
+    for (int it = 0; it < h->nstep; ++it)
+      for (int r = 0; r < h->nrec; ++r)
+        std::memcpy(o + ((size_t)it + (size_t)r * h->nstep) * h->esize,
+                    tmp.data() + ((size_t)it * h->nrec + r) * h->esize, h->esize);
+  }
+  return CPML_OK;
+}
+
+int cpml_gpu_get_energy(cpml_solver *h, double *kinetic, double *potential) {
+  if (!h) return CPML_ERR_ARG;
+  if (h->cadence <= 0) return fail(h, CPML_ERR_STATE, "energy is off (set_energy)");
+  int rc = set_device(h);
+  if (rc) return rc;
+  if (kinetic)
+    HIPCHK(h, hipMemcpy(kinetic, h->ekin, h->nstep * sizeof(double), hipMemcpyDeviceToHost));
+  if (potential)
+    HIPCHK(h, hipMemcpy(potential, h->epot, h->nstep * sizeof(double), hipMemcpyDeviceToHost));
+  return CPML_OK;
+}
+
+int cpml_gpu_get_field(cpml_solver *h, int which, void *out) {
+  if (!h) return CPML_ERR_ARG;
+  if (!field_valid(h, which) || !out) return fail(h, CPML_ERR_ARG, "bad field selector");
+  int rc = set_device(h);
+  if (rc) return rc;
+  return download_padded(h, h->field[which], out);
+}
+
+int cpml_gpu_set_field(cpml_solver *h, int which, const void *in) {
+  if (!h) return CPML_ERR_ARG;
+  if (!field_valid(h, which) || !in) return fail(h, CPML_ERR_ARG, "bad field selector");
+  int rc = set_device(h);
+  if (rc) return rc;
+  return upload_padded(h, h->field[which], in);
+}
+
+int cpml_gpu_get_memory(cpml_solver *h, int which, void *out) {
+  if (!h) return CPML_ERR_ARG;
+  if (!mem_used(h->ndim, which) || !out) return fail(h, CPML_ERR_ARG, "bad memory selector");
+  if (!h->have_pml) return fail(h, CPML_ERR_STATE, "set_pml first");
+  int rc = set_device(h);
+  if (rc) return rc;
+  const size_t es = h->esize;
+  std::vector<char> tmp(std::max<size_t>(h->mem_elems[which], 1) * es);
+  if (h->mem_elems[which])
+    HIPCHK(h, hipMemcpy(tmp.data(), h->mem[which], h->mem_elems[which] * es,
+                        hipMemcpyDeviceToHost));
+  char *o = static_cast<char *>(out);
+  const int nx = h->nx, ny = h->ny, nz = h->nz;
+  std::memset(o, 0, (size_t)nx * ny * nz * es);
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        long src = -1;
+        switch (kMemAxis[which]) {
+          case 0:
+            if (h->sx.in(i)) src = h->sx.s(i) + (long)h->sx.w * (j + (long)ny * k);
+            break;
+          case 1:
+            if (h->sy.in(j)) src = i + h->mld * (h->sy.s(j) + (long)h->sy.w * k);
+            break;
+          default:
+            if (h->sz.in(k)) src = i + h->mld * (j + (long)ny * h->sz.s(k));
+            break;
+        }
+        if (src >= 0)
+          std::memcpy(o + ((size_t)i + (size_t)j * nx + (size_t)k * nx * ny) * es,
+                      tmp.data() + src * es, es);
+      }
+  return CPML_OK;
+}
+
+int cpml_gpu_velocity_norm(cpml_solver *h, double *vmax) {
+  if (!h || !vmax) return CPML_ERR_ARG;
+  int rc = set_device(h);
+  if (rc) return rc;
+  return velocity_norm(h, vmax);
+}
+
+int cpml_gpu_set_graph_steps(cpml_solver *h, int steps) {
+  if (!h) return CPML_ERR_ARG;
+  if (steps < 0 || steps > 4096) return fail(h, CPML_ERR_ARG, "graph steps out of range");
+  h->graph_steps = steps;
+  h->graph_dirty = true;
+  return CPML_OK;
+}
+
+int cpml_gpu_set_kchunk(cpml_solver *h, int kchunk) {
+  if (!h) return CPML_ERR_ARG;
+  if (kchunk < 1) return fail(h, CPML_ERR_ARG, "kchunk must be >= 1");
+  h->kchunk = kchunk;
+  h->graph_dirty = true;
+  return CPML_OK;
+}
+
+int cpml_gpu_set_kernel_timing(cpml_solver *h, int on) {
+  if (!h) return CPML_ERR_ARG;
+  h->ktiming = on != 0;
+  h->ktime[0] = h->ktime[1] = 0;
+  h->kcount[0] = h->kcount[1] = 0;
+  return CPML_OK;
+}
+
+int cpml_gpu_kernel_time(cpml_solver *h, int kernel, int *launches, double *total_ms) {
+  if (!h || kernel < 0 || kernel > 1) return CPML_ERR_ARG;
+  if (launches) *launches = h->kcount[kernel];
+  if (total_ms) *total_ms = h->ktime[kernel];
+  return CPML_OK;
+}
+
+int cpml_gpu_elapsed_ms(cpml_solver *h, double *ms) {
+  if (!h || !ms) return CPML_ERR_ARG;
+  *ms = h->last_ms;
+  return CPML_OK;
+}
+
+int cpml_gpu_device_bytes(cpml_solver *h, double *bytes) {
+  if (!h || !bytes) return CPML_ERR_ARG;
+  *bytes = h->bytes;
+  return CPML_OK;
+}
+
+int cpml_gpu_slab_widths(cpml_solver *h, int *w) {
+  if (!h || !w) return CPML_ERR_ARG;
+  const Slab *s[3] = {&h->sx, &h->sy, &h->sz};
+  for (int d = 0; d < 3; ++d) {
+    w[2 * d] = s[d]->wl;
+    w[2 * d + 1] = s[d]->wr;
+  }
+  return CPML_OK;
+}
+
+int cpml_gpu_last_error(cpml_solver *h, char *buf, int len) {
+  if (!buf || len <= 0) return CPML_ERR_ARG;
+  const std::string &m = h ? h->err : g_init_error;
+  const size_t n = std::min(m.size(), (size_t)len - 1);
+  std::memcpy(buf, m.data(), n);
+  buf[n] = '\0';
+  return CPML_OK;
+}
+
+int cpml_pml_profile(int prec, int n, double delta, double deltat, int npoints_pml, double cp,
+                     double f0, int use_min, int use_max, int origin_max_uses_n,
+                     int clamp_alpha, void *a, void *b, void *K, void *a_half, void *b_half,
+                     void *K_half) {
+  if ((prec != 4 && prec != 8) || n < 1 || npoints_pml < 1 || !(delta > 0) || !a || !b || !K ||
+      !a_half || !b_half || !K_half)
+    return fail(nullptr, CPML_ERR_ARG, "cpml_pml_profile: bad argument");
+  if (prec == 4)
+    return profile_impl<float>(n, delta, deltat, npoints_pml, cp, f0, use_min, use_max,
+                               origin_max_uses_n, clamp_alpha, (float *)a, (float *)b,
+                               (float *)K, (float *)a_half, (float *)b_half, (float *)K_half);
+  return profile_impl<double>(n, delta, deltat, npoints_pml, cp, f0, use_min, use_max,
+                              origin_max_uses_n, clamp_alpha, (double *)a, (double *)b,
+                              (double *)K, (double *)a_half, (double *)b_half, (double *)K_half);
+}
+
+int cpml_source_time_function(int prec, int nstep, double deltat, double f0, double t0,
+                              double factor, double angle_force, int kind, void *force_x,
+                              void *force_y) {
+  if ((prec != 4 && prec != 8) || nstep < 1 || kind < 0 || kind > 2 || !force_x || !force_y)
+    return fail(nullptr, CPML_ERR_ARG, "cpml_source_time_function: bad argument");
+  if (prec == 4)
+    return stf_impl<float>(nstep, deltat, f0, t0, factor, angle_force, kind, (float *)force_x,
+                           (float *)force_y);
+  return stf_impl<double>(nstep, deltat, f0, t0, factor, angle_force, kind, (double *)force_x,
+                          (double *)force_y);
+}
+
+int cpml_nearest_receivers(int prec, int NX, int NY, double DELTAX, double DELTAY, int NREC,
+                           double xdeb, double ydeb, double xfin, double yfin, int *ix_rec,
+                           int *iy_rec) {
+  if ((prec != 4 && prec != 8) || NX < 1 || NY < 1 || NREC < 0 || (NREC && (!ix_rec || !iy_rec)))
+    return fail(nullptr, CPML_ERR_ARG, "cpml_nearest_receivers: bad argument");
+  if (prec == 4)
+    return receivers_impl<float>(NX, NY, DELTAX, DELTAY, NREC, xdeb, ydeb, xfin, yfin, ix_rec,
+                                 iy_rec);
+  return receivers_impl<double>(NX, NY, DELTAX, DELTAY, NREC, xdeb, ydeb, xfin, yfin, ix_rec,
+                                iy_rec);
+}
+
+int cpml_courant_number(int ndim, double cp, double deltat, double deltax, double deltay,
+                        double deltaz, double *courant) {
+  if (!courant || !(deltax > 0) || !(deltay > 0) || (ndim == 3 && !(deltaz > 0)))
+    return fail(nullptr, CPML_ERR_ARG, "cpml_courant_number: bad argument");
+  double s = 1.0 / (deltax * deltax) + 1.0 / (deltay * deltay);
+  if (ndim == 3) s += 1.0 / (deltaz * deltaz);
+  *courant = cp * deltat * std::sqrt(s);
+  return CPML_OK;
+}
+
+}  // extern "C"

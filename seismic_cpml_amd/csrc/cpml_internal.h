@@ -1,0 +1,134 @@
+// Internal declarations shared by the HIP translation units of libcpml_gpu.
+//
+// Device layout (all arrays x-fastest, like the reference's Fortran arrays):
+//
+//  * wavefields and media: one padded array each.  Interior point (i,j,k)
+//    (0-based; Fortran index i+1, j+1, k+1) lives at  base + i + j*ld + k*ps.
+//    A halo of R = order/2 cells on every side is allocated and kept at zero
+//    (the reference's ghost points, 2D4:205 0:NX+1, ADE8:272 -4:NX+4), so the
+//    stencils read neighbours without bounds tests.  The x halo is XPAD
+//    elements wide so that interior rows start 256-byte aligned.
+//  * C-PML memory variables: only inside the absorbing slabs.  Along x the
+//    slab is the columns [0, wl) U [nx-wr, nx) stored compactly as s in
+//    [0, wl+wr); along y and z likewise for rows and planes.  Outside the slab
+//    the reference's memory variable stays exactly 0 (a = 0) and K = 1, so
+//    the update reduces to the plain derivative there.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace cpml {
+
+constexpr int kMaxRecBlock = 256;
+
+// Holberg (1987) 8th-order staggered coefficients as printed in
+// seismic_ADEPML_2D_elastic_RK4_eighth_order.f90:263-266.
+constexpr double kHolberg1 = 1.231666;
+constexpr double kHolberg2 = -1.041182e-1;
+constexpr double kHolberg3 = 2.063707e-2;
+constexpr double kHolberg4 = -3.570998e-3;
+
+struct Slab {
+  int wl = 0, wr = 0, w = 0;  // widths at the min and max side, total
+  int n = 0;                  // grid extent along the axis
+  __host__ __device__ bool in(int i) const { return i < wl || i >= n - wr; }
+  __host__ __device__ int s(int i) const { return i < wl ? i : i - (n - wr) + wl; }
+};
+
+// Profiles along one axis, 6 consecutive arrays of length n:
+// a, b, K, a_half, b_half, K_half (the reference's a_x, b_x, K_x, ...).
+template <typename T>
+struct Prof {
+  const T *a, *b, *K, *ah, *bh, *Kh;
+};
+
+template <typename T>
+struct Step2D {
+  T *vx, *vy, *sxx, *syy, *sxy;  // interior origin (halo at negative offsets)
+  const T *lam, *mu, *rho;
+  // x-slab memories: [0] dvx_dx [1] dvy_dx [2] dsxx_dx [3] dsxy_dx, index s + wx*j
+  T *mx[4];
+  // y-slab memories: [0] dvy_dy [1] dvx_dy [2] dsxy_dy [3] dsyy_dy, index i + mld*s
+  T *my[4];
+  Prof<T> px, py;
+  Slab sx, sy;
+  int nx, ny;
+  long ld;   // row stride of fields
+  long mld;  // row stride of y-slab memories
+  T den_x, den_y, dt;
+  // source (0-based, isrc < 0: none) and step counter
+  int isrc, jsrc;
+  const T *fx, *fy;
+  const int *step;  // current step it (1-based), advanced by the record kernel
+};
+
+template <typename T>
+struct Step3D {
+  T *vx, *vy, *vz, *sxx, *syy, *szz, *sxy, *sxz, *syz;
+  const T *clam, *cmu, *cbuoy;  // DELTAT*lambda, DELTAT*mu, DELTAT/rho
+  const T *rho;                 // raw density (source term, 3D:1081-1082)
+  // x-slab: [0] dvx_dx [1] dvy_dx [2] dvz_dx [3] dsxx_dx [4] dsxy_dx [5] dsxz_dx
+  T *mx[6];
+  // y-slab: [0] dvy_dy [1] dvx_dy [2] dvz_dy [3] dsxy_dy [4] dsyy_dy [5] dsyz_dy
+  T *my[6];
+  // z-slab: [0] dvz_dz [1] dvx_dz [2] dvy_dz [3] dsxz_dz [4] dsyz_dz [5] dszz_dz
+  T *mz[6];
+  Prof<T> px, py, pz;
+  Slab sx, sy, sz;
+  int nx, ny, nz;
+  long ld, ps;    // field strides
+  long mld, mps;  // memory strides: x-slab (w, w*ny), y-slab (mld, mld*wy), z (mld, mld*ny)
+  T odx, ody, odz;
+  T dt;
+  int isrc, jsrc, ksrc;
+  const T *fx, *fy;
+  const int *step;
+  int kchunk;  // planes per block along z
+};
+
+// Receivers / step counter: one block.
+template <typename T>
+struct Record {
+  const T *vx, *vy, *vz;  // interior origin
+  long ld, ps;
+  int nrec;
+  const int *idx;  // nrec offsets (i + j*ld + k*ps)
+  T *sx, *sy, *sz;  // [nstep][nrec]
+  int *step;
+};
+
+// Energy over a window, per step, accumulated in double.
+template <typename T>
+struct Energy {
+  const T *vx, *vy, *vz, *sxx, *syy, *szz, *sxy, *sxz, *syz;
+  const T *lam, *mu, *rho;  // raw media (field layout)
+  long ld, ps;
+  int i0, i1, j0, j1, k0, k1;  // 0-based inclusive
+  int ndim, corrected3d, cadence;
+  double *partial;  // [nblocks][2]
+  double *out_kin, *out_pot;  // [nstep]
+  const int *step;
+};
+
+// Kernel launchers (defined in the .hip files).
+template <typename T>
+hipError_t launch_stress2d(int order, const Step2D<T> &p, hipStream_t s);
+template <typename T>
+hipError_t launch_velocity2d(int order, const Step2D<T> &p, hipStream_t s);
+template <typename T>
+hipError_t launch_stress3d(const Step3D<T> &p, hipStream_t s);
+template <typename T>
+hipError_t launch_velocity3d(const Step3D<T> &p, hipStream_t s);
+template <typename T>
+hipError_t launch_record(const Record<T> &p, hipStream_t s);
+template <typename T>
+hipError_t launch_energy(const Energy<T> &p, int nblocks, hipStream_t s);
+template <typename T>
+int energy_blocks(const Energy<T> &p);
+template <typename T>
+hipError_t launch_vmax(const T *vx, const T *vy, const T *vz, int nx, int ny, int nz, long ld,
+                       long ps, unsigned long long *out, hipStream_t s);
+
+}  // namespace cpml

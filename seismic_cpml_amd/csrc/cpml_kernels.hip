@@ -1,0 +1,528 @@
+// HIP kernels of the C-PML time loop for gfx950 (CDNA4).
+//
+// Arithmetic follows the reference programs statement by statement (the file
+// is compiled with -ffp-contract=off so no multiply-add is fused), which makes
+// the float64 path bit-identical to the reference and the float32 path equal
+// to the reference compiled in single precision:
+//   2D stress       seismic_CPML_2D_isotropic_second_order.f90:556-600
+//                   seismic_CPML_2D_isotropic_fourth_order.f90:557-601
+//   2D velocity     ...second_order.f90:606-641, ...fourth_order.f90:607-642
+//   source          ...second_order.f90:644-667
+//   Dirichlet       ...second_order.f90:670-680
+//   3D stress       seismic_CPML_3D_isotropic_MPI_OpenMP.f90:836-943
+//   3D velocity     ...MPI_OpenMP.f90:976-1052, source 1055-1083, faces 1087-1121
+//   receivers       ...second_order.f90:683-686, ...MPI_OpenMP.f90:1123-1129
+//   energy          ...second_order.f90:694-713, ...MPI_OpenMP.f90:1131-1180
+//
+// Thread mapping: a 64x4 block covers 64 points along x (one wavefront per
+// row, coalesced along the fastest index) and 4 rows along y.  3D kernels
+// march each (i,j) column through a chunk of z planes and carry the z
+// neighbours in registers, so every plane of every field is fetched once per
+// chunk; x/y neighbours come from the same or adjacent rows of the block and
+// hit in L1/L2.
+#include "cpml_internal.h"
+
+namespace cpml {
+
+namespace {
+
+constexpr int BX = 64;
+constexpr int BY = 4;
+
+// Staggered difference between the half points p+s ("+1/2") and p ("-1/2").
+template <typename T, int ORDER>
+struct Op;
+
+template <typename T>
+struct Op<T, 2> {
+  __device__ static T d(const T *f, long p, long s) { return f[p + s] - f[p]; }
+};
+
+// 2D4:565  (27*f(+1/2) - 27*f(-1/2) - f(+3/2) + f(-3/2))
+template <typename T>
+struct Op<T, 4> {
+  __device__ static T d(const T *f, long p, long s) {
+    return T(27) * f[p + s] - T(27) * f[p] - f[p + 2 * s] + f[p - s];
+  }
+};
+
+// ADE8:734-735 association: c1*(..) + c2*(..) + c3*(..) + c4*(..)
+template <typename T>
+struct Op<T, 8> {
+  __device__ static T d(const T *f, long p, long s) {
+    return T(kHolberg1) * (f[p + s] - f[p]) + T(kHolberg2) * (f[p + 2 * s] - f[p - s]) +
+           T(kHolberg3) * (f[p + 3 * s] - f[p - 2 * s]) +
+           T(kHolberg4) * (f[p + 4 * s] - f[p - 3 * s]);
+  }
+};
+
+// C-PML correction of a derivative: psi <- b*psi + a*d ; d <- d/K + psi.
+template <typename T>
+__device__ __forceinline__ T cpml(T d, T *psi, T a, T b, T K) {
+  const T m = b * *psi + a * d;
+  *psi = m;
+  return d / K + m;
+}
+
+// ---------------------------------------------------------------- 2D ----
+
+template <typename T, int ORDER>
+__global__ __launch_bounds__(BX *BY) void k_stress2d(Step2D<T> p) {
+  const int i = blockIdx.x * BX + threadIdx.x;
+  const int j = blockIdx.y * BY + threadIdx.y;
+  if (i >= p.nx || j >= p.ny) return;
+  const long ld = p.ld;
+  const long c = i + j * ld;
+  const bool inx = p.sx.in(i), iny = p.sy.in(j);
+  const long mxo = p.sx.s(i) + (long)p.sx.w * j;
+  const long myo = i + p.mld * p.sy.s(j);
+  // sigma_xx, sigma_yy at (i+1/2, j), Fortran i=1..NX-1, j=2..NY
+  if (i <= p.nx - 2 && j >= 1) {
+    const T lhx = T(0.5) * (p.lam[c + 1] + p.lam[c]);
+    const T mhx = T(0.5) * (p.mu[c + 1] + p.mu[c]);
+    const T lp2m = lhx + T(2) * mhx;
+    T dvx_dx = Op<T, ORDER>::d(p.vx, c, 1) / p.den_x;
+    T dvy_dy = Op<T, ORDER>::d(p.vy, c - ld, ld) / p.den_y;
+    if (inx) dvx_dx = cpml(dvx_dx, p.mx[0] + mxo, p.px.ah[i], p.px.bh[i], p.px.Kh[i]);
+    if (iny) dvy_dy = cpml(dvy_dy, p.my[0] + myo, p.py.a[j], p.py.b[j], p.py.K[j]);
+    p.sxx[c] = p.sxx[c] + (lp2m * dvx_dx + lhx * dvy_dy) * p.dt;
+    p.syy[c] = p.syy[c] + (lhx * dvx_dx + lp2m * dvy_dy) * p.dt;
+  }
+  // sigma_xy at (i, j+1/2), Fortran i=2..NX, j=1..NY-1
+  if (i >= 1 && j <= p.ny - 2) {
+    const T mhy = T(0.5) * (p.mu[c + ld] + p.mu[c]);
+    T dvy_dx = Op<T, ORDER>::d(p.vy, c - 1, 1) / p.den_x;
+    T dvx_dy = Op<T, ORDER>::d(p.vx, c, ld) / p.den_y;
+    if (inx) dvy_dx = cpml(dvy_dx, p.mx[1] + mxo, p.px.a[i], p.px.b[i], p.px.K[i]);
+    // 2D4:596 divides by K_y(j) where 2D2:595 uses K_y_half(j)
+    if (iny)
+      dvx_dy = cpml(dvx_dy, p.my[1] + myo, p.py.ah[j], p.py.bh[j],
+                    ORDER == 4 ? p.py.K[j] : p.py.Kh[j]);
+    p.sxy[c] = p.sxy[c] + mhy * (dvy_dx + dvx_dy) * p.dt;
+  }
+}
+
+template <typename T, int ORDER>
+__global__ __launch_bounds__(BX *BY) void k_velocity2d(Step2D<T> p) {
+  const int i = blockIdx.x * BX + threadIdx.x;
+  const int j = blockIdx.y * BY + threadIdx.y;
+  if (i >= p.nx || j >= p.ny) return;
+  const long ld = p.ld;
+  const long c = i + j * ld;
+  const bool inx = p.sx.in(i), iny = p.sy.in(j);
+  const long mxo = p.sx.s(i) + (long)p.sx.w * j;
+  const long myo = i + p.mld * p.sy.s(j);
+  T vxn = p.vx[c], vyn = p.vy[c];
+  // vx at (i, j), Fortran i=2..NX, j=2..NY
+  if (i >= 1 && j >= 1) {
+    T dsxx_dx = Op<T, ORDER>::d(p.sxx, c - 1, 1) / p.den_x;
+    T dsxy_dy = Op<T, ORDER>::d(p.sxy, c - ld, ld) / p.den_y;
+    if (inx) dsxx_dx = cpml(dsxx_dx, p.mx[2] + mxo, p.px.a[i], p.px.b[i], p.px.K[i]);
+    if (iny) dsxy_dy = cpml(dsxy_dy, p.my[2] + myo, p.py.a[j], p.py.b[j], p.py.K[j]);
+    vxn = vxn + (dsxx_dx + dsxy_dy) * p.dt / p.rho[c];
+  }
+  // vy at (i+1/2, j+1/2), Fortran i=1..NX-1, j=1..NY-1
+  if (i <= p.nx - 2 && j <= p.ny - 2) {
+    const T rhh = T(0.25) * (p.rho[c] + p.rho[c + 1] + p.rho[c + 1 + ld] + p.rho[c + ld]);
+    T dsxy_dx = Op<T, ORDER>::d(p.sxy, c, 1) / p.den_x;
+    T dsyy_dy = Op<T, ORDER>::d(p.syy, c, ld) / p.den_y;
+    if (inx) dsxy_dx = cpml(dsxy_dx, p.mx[3] + mxo, p.px.ah[i], p.px.bh[i], p.px.Kh[i]);
+    if (iny) dsyy_dy = cpml(dsyy_dy, p.my[3] + myo, p.py.ah[j], p.py.bh[j], p.py.Kh[j]);
+    vyn = vyn + (dsxy_dx + dsyy_dy) * p.dt / rhh;
+  }
+  // point force, added after the velocity update (2D2:660-667)
+  if (i == p.isrc && j == p.jsrc) {
+    const int it = *p.step;
+    const T rhh = T(0.25) * (p.rho[c] + p.rho[c + 1] + p.rho[c + 1 + ld] + p.rho[c + ld]);
+    vxn = vxn + p.fx[it - 1] * p.dt / p.rho[c];
+    vyn = vyn + p.fy[it - 1] * p.dt / rhh;
+  }
+  // rigid edges (2D2:670-680)
+  if (i == 0 || i == p.nx - 1 || j == 0 || j == p.ny - 1) {
+    vxn = T(0);
+    vyn = T(0);
+  }
+  p.vx[c] = vxn;
+  p.vy[c] = vyn;
+}
+
+// ---------------------------------------------------------------- 3D ----
+
+template <typename T>
+__global__ __launch_bounds__(BX *BY) void k_stress3d(Step3D<T> p) {
+  const int i = blockIdx.x * BX + threadIdx.x;
+  const int j = blockIdx.y * BY + threadIdx.y;
+  if (i >= p.nx || j >= p.ny) return;
+  const int kb = blockIdx.z * p.kchunk;
+  const int ke = min(kb + p.kchunk, p.nz);
+  const long ld = p.ld, ps = p.ps;
+  const int nx = p.nx, ny = p.ny, nz = p.nz;
+  const bool inx = p.sx.in(i), iny = p.sy.in(j);
+  const long mxo = p.sx.s(i) + (long)p.sx.w * j;  // + k * w*ny
+  const long mxs = (long)p.sx.w * ny;
+  const long myo = i + p.mld * p.sy.s(j);  // + k * mld*wy
+  const long mys = p.mld * p.sy.w;
+  const long mzo = i + p.mld * j;  // + s(k) * mld*ny
+  const long mzs = p.mld * ny;
+  const bool r_n = i <= nx - 2 && j >= 1;  // sigma_ii rows/cols
+  const bool r_xy = i >= 1 && j <= ny - 2;
+  const bool r_xz = i >= 1;
+  const bool r_yz = j <= ny - 2;
+  long c = i + j * ld + kb * ps;
+  T vz_m = p.vz[c - ps];
+  T vx_k = p.vx[c], vy_k = p.vy[c];
+  for (int k = kb; k < ke; ++k, c += ps) {
+    const T vx_p = p.vx[c + ps];
+    const T vy_p = p.vy[c + ps];
+    const T vz_k = p.vz[c];
+    const T cl = p.clam[c], cm = p.cmu[c];
+    const bool inz = p.sz.in(k);
+    const long mzk = mzo + (long)p.sz.s(k) * mzs;
+    // sigma_xx, sigma_yy, sigma_zz: 3D:836-863 (global k >= 2)
+    if (r_n && k >= 1) {
+      T dvx_dx = (p.vx[c + 1] - vx_k) * p.odx;
+      T dvy_dy = (vy_k - p.vy[c - ld]) * p.ody;
+      T dvz_dz = (vz_k - vz_m) * p.odz;
+      if (inx) dvx_dx = cpml(dvx_dx, p.mx[0] + mxo + k * mxs, p.px.ah[i], p.px.bh[i], p.px.Kh[i]);
+      if (iny) dvy_dy = cpml(dvy_dy, p.my[0] + myo + k * mys, p.py.a[j], p.py.b[j], p.py.K[j]);
+      if (inz) dvz_dz = cpml(dvz_dz, p.mz[0] + mzk, p.pz.a[k], p.pz.b[k], p.pz.K[k]);
+      const T lp2m = cl + T(2) * cm;
+      p.sxx[c] = lp2m * dvx_dx + cl * (dvy_dy + dvz_dz) + p.sxx[c];
+      p.syy[c] = cl * (dvx_dx + dvz_dz) + lp2m * dvy_dy + p.syy[c];
+      p.szz[c] = cl * (dvx_dx + dvy_dy) + lp2m * dvz_dz + p.szz[c];
+    }
+    // sigma_xy: 3D:877-894
+    if (r_xy) {
+      T dvy_dx = (vy_k - p.vy[c - 1]) * p.odx;
+      T dvx_dy = (p.vx[c + ld] - vx_k) * p.ody;
+      if (inx) dvy_dx = cpml(dvy_dx, p.mx[1] + mxo + k * mxs, p.px.a[i], p.px.b[i], p.px.K[i]);
+      if (iny) dvx_dy = cpml(dvx_dy, p.my[1] + myo + k * mys, p.py.ah[j], p.py.bh[j], p.py.Kh[j]);
+      p.sxy[c] = cm * (dvy_dx + dvx_dy) + p.sxy[c];
+    }
+    // sigma_xz: 3D:908-925 (global k <= NZ-1)
+    if (r_xz && k <= nz - 2) {
+      T dvz_dx = (vz_k - p.vz[c - 1]) * p.odx;
+      T dvx_dz = (vx_p - vx_k) * p.odz;
+      if (inx) dvz_dx = cpml(dvz_dx, p.mx[2] + mxo + k * mxs, p.px.a[i], p.px.b[i], p.px.K[i]);
+      if (inz) dvx_dz = cpml(dvx_dz, p.mz[1] + mzk, p.pz.ah[k], p.pz.bh[k], p.pz.Kh[k]);
+      p.sxz[c] = cm * (dvz_dx + dvx_dz) + p.sxz[c];
+    }
+    // sigma_yz: 3D:927-943
+    if (r_yz && k <= nz - 2) {
+      T dvz_dy = (p.vz[c + ld] - vz_k) * p.ody;
+      T dvy_dz = (vy_p - vy_k) * p.odz;
+      if (iny) dvz_dy = cpml(dvz_dy, p.my[2] + myo + k * mys, p.py.ah[j], p.py.bh[j], p.py.Kh[j]);
+      if (inz) dvy_dz = cpml(dvy_dz, p.mz[2] + mzk, p.pz.ah[k], p.pz.bh[k], p.pz.Kh[k]);
+      p.syz[c] = cm * (dvz_dy + dvy_dz) + p.syz[c];
+    }
+    vz_m = vz_k;
+    vx_k = vx_p;
+    vy_k = vy_p;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(BX *BY) void k_velocity3d(Step3D<T> p) {
+  const int i = blockIdx.x * BX + threadIdx.x;
+  const int j = blockIdx.y * BY + threadIdx.y;
+  if (i >= p.nx || j >= p.ny) return;
+  const int kb = blockIdx.z * p.kchunk;
+  const int ke = min(kb + p.kchunk, p.nz);
+  const long ld = p.ld, ps = p.ps;
+  const int nx = p.nx, ny = p.ny, nz = p.nz;
+  const bool inx = p.sx.in(i), iny = p.sy.in(j);
+  const long mxo = p.sx.s(i) + (long)p.sx.w * j;
+  const long mxs = (long)p.sx.w * ny;
+  const long myo = i + p.mld * p.sy.s(j);
+  const long mys = p.mld * p.sy.w;
+  const long mzo = i + p.mld * j;
+  const long mzs = p.mld * ny;
+  const bool r_vx = i >= 1 && j >= 1;
+  const bool r_vy = i <= nx - 2 && j <= ny - 2;
+  const bool r_vz = i <= nx - 2 && j >= 1;
+  const bool edge_xy = i == 0 || i == nx - 1 || j == 0 || j == ny - 1;
+  const bool src_col = i == p.isrc && j == p.jsrc;
+  long c = i + j * ld + kb * ps;
+  T sxz_m = p.sxz[c - ps], syz_m = p.syz[c - ps];
+  T szz_k = p.szz[c];
+  for (int k = kb; k < ke; ++k, c += ps) {
+    const T szz_p = p.szz[c + ps];
+    const T sxz_k = p.sxz[c], syz_k = p.syz[c];
+    const T cb = p.cbuoy[c];
+    const bool inz = p.sz.in(k);
+    const long mzk = mzo + (long)p.sz.s(k) * mzs;
+    T vxn = p.vx[c], vyn = p.vy[c], vzn = p.vz[c];
+    // vx: 3D:976-993 (global k >= 2)
+    if (r_vx && k >= 1) {
+      T dsxx_dx = (p.sxx[c] - p.sxx[c - 1]) * p.odx;
+      T dsxy_dy = (p.sxy[c] - p.sxy[c - ld]) * p.ody;
+      T dsxz_dz = (sxz_k - sxz_m) * p.odz;
+      if (inx) dsxx_dx = cpml(dsxx_dx, p.mx[3] + mxo + k * mxs, p.px.a[i], p.px.b[i], p.px.K[i]);
+      if (iny) dsxy_dy = cpml(dsxy_dy, p.my[3] + myo + k * mys, p.py.a[j], p.py.b[j], p.py.K[j]);
+      if (inz) dsxz_dz = cpml(dsxz_dz, p.mz[3] + mzk, p.pz.a[k], p.pz.b[k], p.pz.K[k]);
+      vxn = cb * (dsxx_dx + dsxy_dy + dsxz_dz) + vxn;
+    }
+    // vy: 3D:995-1017
+    if (r_vy && k >= 1) {
+      T dsxy_dx = (p.sxy[c + 1] - p.sxy[c]) * p.odx;
+      T dsyy_dy = (p.syy[c + ld] - p.syy[c]) * p.ody;
+      T dsyz_dz = (syz_k - syz_m) * p.odz;
+      if (inx) dsxy_dx = cpml(dsxy_dx, p.mx[4] + mxo + k * mxs, p.px.ah[i], p.px.bh[i], p.px.Kh[i]);
+      if (iny) dsyy_dy = cpml(dsyy_dy, p.my[4] + myo + k * mys, p.py.ah[j], p.py.bh[j], p.py.Kh[j]);
+      if (inz) dsyz_dz = cpml(dsyz_dz, p.mz[4] + mzk, p.pz.a[k], p.pz.b[k], p.pz.K[k]);
+      vyn = cb * (dsxy_dx + dsyy_dy + dsyz_dz) + vyn;
+    }
+    // vz: 3D:1031-1052 (global k <= NZ-1)
+    if (r_vz && k <= nz - 2) {
+      T dsxz_dx = (p.sxz[c + 1] - sxz_k) * p.odx;
+      T dsyz_dy = (syz_k - p.syz[c - ld]) * p.ody;
+      T dszz_dz = (szz_p - szz_k) * p.odz;
+      if (inx) dsxz_dx = cpml(dsxz_dx, p.mx[5] + mxo + k * mxs, p.px.ah[i], p.px.bh[i], p.px.Kh[i]);
+      if (iny) dsyz_dy = cpml(dsyz_dy, p.my[5] + myo + k * mys, p.py.a[j], p.py.b[j], p.py.K[j]);
+      if (inz) dszz_dz = cpml(dszz_dz, p.mz[5] + mzk, p.pz.ah[k], p.pz.bh[k], p.pz.Kh[k]);
+      vzn = cb * (dsxz_dx + dsyz_dy + dszz_dz) + vzn;
+    }
+    // point force on the cut plane, no density average (3D:1074-1082)
+    if (src_col && k == p.ksrc) {
+      const int it = *p.step;
+      vxn = vxn + p.fx[it - 1] * p.dt / p.rho[c];
+      vyn = vyn + p.fy[it - 1] * p.dt / p.rho[c];
+    }
+    // six rigid faces (3D:1087-1121)
+    if (edge_xy || k == 0 || k == nz - 1) {
+      vxn = T(0);
+      vyn = T(0);
+      vzn = T(0);
+    }
+    p.vx[c] = vxn;
+    p.vy[c] = vyn;
+    p.vz[c] = vzn;
+    sxz_m = sxz_k;
+    syz_m = syz_k;
+    szz_k = szz_p;
+  }
+}
+
+// ------------------------------------------------------ receivers, step --
+
+template <typename T>
+__global__ __launch_bounds__(kMaxRecBlock) void k_record(Record<T> p) {
+  const int it = *p.step;
+  const long row = (long)(it - 1) * p.nrec;
+  for (int r = threadIdx.x; r < p.nrec; r += blockDim.x) {
+    const long q = p.idx[r];
+    p.sx[row + r] = p.vx[q];
+    p.sy[row + r] = p.vy[q];
+    if (p.sz) p.sz[row + r] = p.vz[q];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *p.step = it + 1;
+}
+
+// -------------------------------------------------------------- energy --
+
+constexpr int EB = 256;
+
+template <typename T>
+__device__ void cell_energy(const Energy<T> &p, long q, double &kin, double &pot) {
+  const T r = p.rho[q], l = p.lam[q], m = p.mu[q];
+  if (p.ndim == 2) {
+    // 2D2:694-711 (kinetic summed then halved; potential per cell)
+    const T vx = p.vx[q], vy = p.vy[q];
+    kin += (double)(r * (vx * vx + vy * vy));
+    const T sxx = p.sxx[q], syy = p.syy[q], sxy = p.sxy[q];
+    const T exx = ((l + T(2) * m) * sxx - l * syy) / (T(4) * m * (l + m));
+    const T eyy = ((l + T(2) * m) * syy - l * sxx) / (T(4) * m * (l + m));
+    const T exy = sxy / (T(2) * m);
+    pot += (double)(T(0.5) * (exx * sxx + eyy * syy + T(2) * exy * sxy));
+  } else {
+    // 3D:1149-1176; the reference counts eps_yy*sigma_yy twice and omits
+    // eps_zz*sigma_zz (3D:1170-1172); 'corrected3d' uses the textbook sum.
+    const T vx = p.vx[q], vy = p.vy[q], vz = p.vz[q];
+    kin += (double)(T(0.5) * r * (vx * vx + vy * vy + vz * vz));
+    const T sxx = p.sxx[q], syy = p.syy[q], szz = p.szz[q];
+    const T sxy = p.sxy[q], sxz = p.sxz[q], syz = p.syz[q];
+    const T den = T(2) * m * (T(3) * l + T(2) * m);
+    const T exx = (T(2) * (l + m) * sxx - l * syy - l * szz) / den;
+    const T eyy = (T(2) * (l + m) * syy - l * sxx - l * szz) / den;
+    const T ezz = (T(2) * (l + m) * szz - l * sxx - l * syy) / den;
+    const T exy = sxy / (T(2) * m), exz = sxz / (T(2) * m), eyz = syz / (T(2) * m);
+    const T third = p.corrected3d ? ezz * szz : eyy * syy;
+    pot += (double)(T(0.5) * (exx * sxx + eyy * syy + third + T(2) * exy * sxy +
+                              T(2) * exz * sxz + T(2) * eyz * syz));
+  }
+}
+
+__device__ void block_sum2(double &a, double &b) {
+  __shared__ double sa[EB], sb[EB];
+  sa[threadIdx.x] = a;
+  sb[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = EB / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sa[threadIdx.x] += sa[threadIdx.x + s];
+      sb[threadIdx.x] += sb[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  a = sa[0];
+  b = sb[0];
+}
+
+template <typename T>
+__global__ __launch_bounds__(EB) void k_energy_partial(Energy<T> p) {
+  const int it = *p.step;
+  if (it % p.cadence != 0) return;
+  const int wx = p.i1 - p.i0 + 1, wy = p.j1 - p.j0 + 1, wz = p.k1 - p.k0 + 1;
+  const long n = (long)wx * wy * wz;
+  double kin = 0.0, pot = 0.0;
+  for (long t = blockIdx.x * (long)EB + threadIdx.x; t < n; t += (long)gridDim.x * EB) {
+    const int i = p.i0 + (int)(t % wx);
+    const long r = t / wx;
+    const int j = p.j0 + (int)(r % wy);
+    const int k = p.k0 + (int)(r / wy);
+    cell_energy(p, i + j * p.ld + k * p.ps, kin, pot);
+  }
+  block_sum2(kin, pot);
+  if (threadIdx.x == 0) {
+    p.partial[2 * blockIdx.x] = kin;
+    p.partial[2 * blockIdx.x + 1] = pot;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(EB) void k_energy_final(Energy<T> p, int nblocks) {
+  const int it = *p.step;
+  if (it % p.cadence != 0) return;
+  double kin = 0.0, pot = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += EB) {
+    kin += p.partial[2 * b];
+    pot += p.partial[2 * b + 1];
+  }
+  block_sum2(kin, pot);
+  if (threadIdx.x == 0) {
+    p.out_kin[it - 1] = p.ndim == 2 ? 0.5 * kin : kin;
+    p.out_pot[it - 1] = pot;
+  }
+}
+
+// ------------------------------------------------- max |v| (stability) --
+
+template <typename T>
+__global__ __launch_bounds__(EB) void k_vmax(const T *vx, const T *vy, const T *vz, int nx, int ny,
+                                             int nz, long ld, long ps,
+                                             unsigned long long *out) {
+  const long n = (long)nx * ny * nz;
+  double m = 0.0;
+  for (long t = blockIdx.x * (long)EB + threadIdx.x; t < n; t += (long)gridDim.x * EB) {
+    const int i = (int)(t % nx);
+    const long r = t / nx;
+    const int j = (int)(r % ny);
+    const int k = (int)(r / ny);
+    const long q = i + j * ld + k * ps;
+    T s = vx[q] * vx[q] + vy[q] * vy[q];
+    if (vz) s = s + vz[q] * vz[q];
+    const double d = (double)s;
+    // NaN compares false: force it to the top so a blow-up is reported
+    m = (d > m || d != d) ? d : m;
+  }
+  __shared__ double sm[EB];
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = EB / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      const double o = sm[threadIdx.x + s];
+      if (o > sm[threadIdx.x] || o != o) sm[threadIdx.x] = o;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double v = sm[0];
+    unsigned long long bits = v != v ? 0x7ff8000000000000ull : __double_as_longlong(v);
+    atomicMax(out, bits);
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ launchers --
+
+template <typename T>
+hipError_t launch_stress2d(int order, const Step2D<T> &p, hipStream_t s) {
+  dim3 grid((p.nx + BX - 1) / BX, (p.ny + BY - 1) / BY), block(BX, BY);
+  if (order == 2) hipLaunchKernelGGL((k_stress2d<T, 2>), grid, block, 0, s, p);
+  else if (order == 4) hipLaunchKernelGGL((k_stress2d<T, 4>), grid, block, 0, s, p);
+  else hipLaunchKernelGGL((k_stress2d<T, 8>), grid, block, 0, s, p);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_velocity2d(int order, const Step2D<T> &p, hipStream_t s) {
+  dim3 grid((p.nx + BX - 1) / BX, (p.ny + BY - 1) / BY), block(BX, BY);
+  if (order == 2) hipLaunchKernelGGL((k_velocity2d<T, 2>), grid, block, 0, s, p);
+  else if (order == 4) hipLaunchKernelGGL((k_velocity2d<T, 4>), grid, block, 0, s, p);
+  else hipLaunchKernelGGL((k_velocity2d<T, 8>), grid, block, 0, s, p);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_stress3d(const Step3D<T> &p, hipStream_t s) {
+  dim3 grid((p.nx + BX - 1) / BX, (p.ny + BY - 1) / BY, (p.nz + p.kchunk - 1) / p.kchunk);
+  hipLaunchKernelGGL(k_stress3d<T>, grid, dim3(BX, BY), 0, s, p);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_velocity3d(const Step3D<T> &p, hipStream_t s) {
+  dim3 grid((p.nx + BX - 1) / BX, (p.ny + BY - 1) / BY, (p.nz + p.kchunk - 1) / p.kchunk);
+  hipLaunchKernelGGL(k_velocity3d<T>, grid, dim3(BX, BY), 0, s, p);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_record(const Record<T> &p, hipStream_t s) {
+  hipLaunchKernelGGL(k_record<T>, dim3(1), dim3(kMaxRecBlock), 0, s, p);
+  return hipGetLastError();
+}
+
+template <typename T>
+int energy_blocks(const Energy<T> &p) {
+  const long n = (long)(p.i1 - p.i0 + 1) * (p.j1 - p.j0 + 1) * (p.k1 - p.k0 + 1);
+  long b = (n + EB - 1) / EB;
+  if (b > 2048) b = 2048;
+  return b < 1 ? 1 : (int)b;
+}
+
+template <typename T>
+hipError_t launch_energy(const Energy<T> &p, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_energy_partial<T>, dim3(nblocks), dim3(EB), 0, s, p);
+  hipLaunchKernelGGL(k_energy_final<T>, dim3(1), dim3(EB), 0, s, p, nblocks);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_vmax(const T *vx, const T *vy, const T *vz, int nx, int ny, int nz, long ld,
+                       long ps, unsigned long long *out, hipStream_t s) {
+  const long n = (long)nx * ny * nz;
+  long b = (n + EB - 1) / EB;
+  if (b > 4096) b = 4096;
+  hipLaunchKernelGGL(k_vmax<T>, dim3((unsigned)b), dim3(EB), 0, s, vx, vy, vz, nx, ny, nz, ld, ps,
+                     out);
+  return hipGetLastError();
+}
+
+#define CPML_INSTANTIATE(T)                                                                     \
+  template hipError_t launch_stress2d<T>(int, const Step2D<T> &, hipStream_t);                \
+  template hipError_t launch_velocity2d<T>(int, const Step2D<T> &, hipStream_t);              \
+  template hipError_t launch_stress3d<T>(const Step3D<T> &, hipStream_t);                     \
+  template hipError_t launch_velocity3d<T>(const Step3D<T> &, hipStream_t);                   \
+  template hipError_t launch_record<T>(const Record<T> &, hipStream_t);                       \
+  template hipError_t launch_energy<T>(const Energy<T> &, int, hipStream_t);                  \
+  template int energy_blocks<T>(const Energy<T> &);                                           \
+  template hipError_t launch_vmax<T>(const T *, const T *, const T *, int, int, int, long, long, \
+                                     unsigned long long *, hipStream_t);
+
+CPML_INSTANTIATE(float)
+CPML_INSTANTIATE(double)
+
+}  // namespace cpml

@@ -1,0 +1,302 @@
+! cpml_gpu -- Fortran binding (ISO_C_BINDING) of libcpml_gpu, the MI355X
+! C-PML time-loop library (C declarations: include/cpml_gpu.h).
+!
+! The reference programs (seismic_CPML_2D_isotropic_second_order.f90,
+! seismic_CPML_2D_isotropic_fourth_order.f90,
+! seismic_CPML_3D_isotropic_MPI_OpenMP.f90) keep their parameter block, their
+! set-up (profiles, medium, receivers) and their output writers; the time loop
+!     do it = 1,NSTEP ... enddo        (2D2:550-735, 2D4:551-736, 3D:802-1245)
+! becomes
+!     call cpml_gpu_run(h, it_begin, it_end, ierr)
+! after the arrays have been handed over with the cpml_gpu_set_* calls below.
+!
+! Every routine returns ierr (0 = ok, see the CPML_* constants); the message
+! of a failure is cpml_gpu_error_message(h).  Array arguments take real(4) or
+! real(8) arrays matching the precision given to cpml_gpu_init, in the
+! reference's Fortran layout (x fastest), interior points only.
+module cpml_gpu
+  use, intrinsic :: iso_c_binding
+  implicit none
+  private
+
+  integer, parameter, public :: CPML_OK = 0, CPML_ERR_ARG = 1, CPML_ERR_CFL = 2, &
+       CPML_ERR_STENCIL = 3, CPML_ERR_OOM = 4, CPML_ERR_HIP = 5, CPML_ERR_BLOWUP = 6, &
+       CPML_ERR_STATE = 7, CPML_ERR_NODEVICE = 8
+  integer, parameter, public :: CPML_VX = 0, CPML_VY = 1, CPML_VZ = 2, CPML_SIGMAXX = 3, &
+       CPML_SIGMAYY = 4, CPML_SIGMAZZ = 5, CPML_SIGMAXY = 6, CPML_SIGMAXZ = 7, CPML_SIGMAYZ = 8
+
+  public :: cpml_gpu_init, cpml_gpu_finalize, cpml_gpu_set_medium, cpml_gpu_set_pml_2d, &
+       cpml_gpu_set_pml_3d, cpml_gpu_set_source, cpml_gpu_set_receivers, cpml_gpu_set_energy, &
+       cpml_gpu_set_stability, cpml_gpu_run, cpml_gpu_get_seismograms, cpml_gpu_get_energy, &
+       cpml_gpu_get_field, cpml_gpu_set_field, cpml_gpu_velocity_norm, cpml_gpu_elapsed_ms, &
+       cpml_gpu_error_message
+
+  interface
+    integer(c_int) function c_init(h, ndim, order, prec, nx, ny, nz, dx, dy, dz, dt, npml, &
+                                   nstep) bind(C, name="cpml_gpu_init")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), intent(out) :: h
+      integer(c_int), value :: ndim, order, prec, nx, ny, nz, npml, nstep
+      real(c_double), value :: dx, dy, dz, dt
+    end function
+    integer(c_int) function c_finalize(h) bind(C, name="cpml_gpu_finalize")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function c_set_medium(h, lam, mu, rho) bind(C, name="cpml_gpu_set_medium")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      type(*), dimension(*), intent(in) :: lam, mu, rho
+    end function
+    integer(c_int) function c_set_pml(h, ax, bx, kx, axh, bxh, kxh, ay, by, ky, ayh, byh, kyh, &
+                                      az, bz, kz, azh, bzh, kzh) bind(C, name="cpml_gpu_set_pml")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      type(*), dimension(*), intent(in) :: ax, bx, kx, axh, bxh, kxh, ay, by, ky, ayh, byh, &
+           kyh, az, bz, kz, azh, bzh, kzh
+    end function
+    integer(c_int) function c_set_source(h, is, js, ks, fx, fy) bind(C, name="cpml_gpu_set_source")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      integer(c_int), value :: is, js, ks
+      type(*), dimension(*), intent(in) :: fx, fy
+    end function
+    integer(c_int) function c_set_receivers(h, nrec, ix, iy, iz) &
+         bind(C, name="cpml_gpu_set_receivers")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      integer(c_int), value :: nrec
+      integer(c_int), dimension(*), intent(in) :: ix, iy, iz
+    end function
+    integer(c_int) function c_set_energy(h, cadence, window, flags) &
+         bind(C, name="cpml_gpu_set_energy")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      integer(c_int), value :: cadence, flags
+      integer(c_int), dimension(6), intent(in), optional :: window
+    end function
+    integer(c_int) function c_set_stability(h, it_display, threshold) &
+         bind(C, name="cpml_gpu_set_stability")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: it_display
+      real(c_double), value :: threshold
+    end function
+    integer(c_int) function c_run(h, it_begin, it_end) bind(C, name="cpml_gpu_run")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      integer(c_int), value :: it_begin, it_end
+    end function
+    integer(c_int) function c_get_seismograms(h, sx, sy, sz) &
+         bind(C, name="cpml_gpu_get_seismograms")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      type(*), dimension(*) :: sx, sy
+      type(*), dimension(*), optional :: sz
+    end function
+    integer(c_int) function c_get_energy(h, kin, pot) bind(C, name="cpml_gpu_get_energy")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: h
+      real(c_double), dimension(*) :: kin, pot
+    end function
+    integer(c_int) function c_get_field(h, which, out) bind(C, name="cpml_gpu_get_field")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      integer(c_int), value :: which
+      type(*), dimension(*) :: out
+    end function
+    integer(c_int) function c_set_field(h, which, in) bind(C, name="cpml_gpu_set_field")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      integer(c_int), value :: which
+      type(*), dimension(*), intent(in) :: in
+    end function
+    integer(c_int) function c_velocity_norm(h, v) bind(C, name="cpml_gpu_velocity_norm")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: v
+    end function
+    integer(c_int) function c_elapsed_ms(h, v) bind(C, name="cpml_gpu_elapsed_ms")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: v
+    end function
+    integer(c_int) function c_last_error(h, buf, n) bind(C, name="cpml_gpu_last_error")
+      import :: c_ptr, c_int, c_char
+      type(c_ptr), value :: h
+      character(kind=c_char), dimension(*) :: buf
+      integer(c_int), value :: n
+    end function
+  end interface
+
+contains
+
+  ! Parameter block of the reference (2D2:137-218, 3D:123-218) -> handle.
+  ! prec = 4 or 8 (kind of the arrays passed later); order 2, 4 or 8 (2D).
+  subroutine cpml_gpu_init(h, ndim, order, prec, NX, NY, NZ, DELTAX, DELTAY, DELTAZ, DELTAT, &
+                           NPOINTS_PML, NSTEP, ierr)
+    type(c_ptr), intent(out) :: h
+    integer, intent(in) :: ndim, order, prec, NX, NY, NZ, NPOINTS_PML, NSTEP
+    double precision, intent(in) :: DELTAX, DELTAY, DELTAZ, DELTAT
+    integer, intent(out) :: ierr
+    ierr = c_init(h, ndim, order, prec, NX, NY, NZ, DELTAX, DELTAY, DELTAZ, DELTAT, &
+                  NPOINTS_PML, NSTEP)
+  end subroutine
+
+  subroutine cpml_gpu_finalize(h, ierr)
+    type(c_ptr), intent(inout) :: h
+    integer, intent(out) :: ierr
+    ierr = c_finalize(h)
+    h = c_null_ptr
+  end subroutine
+
+  ! lambda, mu, rho (2D2:204, 468-474); checks the Courant number (2D2:513-516)
+  subroutine cpml_gpu_set_medium(h, lambda, mu, rho, ierr)
+    type(c_ptr), intent(in) :: h
+    type(*), dimension(*), intent(in) :: lambda, mu, rho
+    integer, intent(out) :: ierr
+    ierr = c_set_medium(h, lambda, mu, rho)
+  end subroutine
+
+  ! damping profiles of 2D2:283-465 (x and y)
+  subroutine cpml_gpu_set_pml_2d(h, a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, &
+                                 a_y, b_y, K_y, a_y_half, b_y_half, K_y_half, ierr)
+    type(c_ptr), intent(in) :: h
+    type(*), dimension(*), intent(in) :: a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, &
+         a_y, b_y, K_y, a_y_half, b_y_half, K_y_half
+    integer, intent(out) :: ierr
+    ierr = c_set_pml(h, a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, a_y, b_y, K_y, &
+                     a_y_half, b_y_half, K_y_half, a_x, b_x, K_x, a_x_half, b_x_half, K_x_half)
+  end subroutine
+
+  ! damping profiles of 3D:399-667 (x, y and z)
+  subroutine cpml_gpu_set_pml_3d(h, a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, &
+                                 a_y, b_y, K_y, a_y_half, b_y_half, K_y_half, &
+                                 a_z, b_z, K_z, a_z_half, b_z_half, K_z_half, ierr)
+    type(c_ptr), intent(in) :: h
+    type(*), dimension(*), intent(in) :: a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, &
+         a_y, b_y, K_y, a_y_half, b_y_half, K_y_half, a_z, b_z, K_z, a_z_half, b_z_half, K_z_half
+    integer, intent(out) :: ierr
+    ierr = c_set_pml(h, a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, a_y, b_y, K_y, &
+                     a_y_half, b_y_half, K_y_half, a_z, b_z, K_z, a_z_half, b_z_half, K_z_half)
+  end subroutine
+
+  ! force_x(it), force_y(it) at (ISOURCE, JSOURCE[, KSOURCE]) (2D2:644-667)
+  subroutine cpml_gpu_set_source(h, ISOURCE, JSOURCE, KSOURCE, force_x, force_y, ierr)
+    type(c_ptr), intent(in) :: h
+    integer, intent(in) :: ISOURCE, JSOURCE, KSOURCE
+    type(*), dimension(*), intent(in) :: force_x, force_y
+    integer, intent(out) :: ierr
+    ierr = c_set_source(h, ISOURCE, JSOURCE, KSOURCE, force_x, force_y)
+  end subroutine
+
+  ! ix_rec, iy_rec[, iz_rec] (2D2:494-509)
+  subroutine cpml_gpu_set_receivers(h, NREC, ix_rec, iy_rec, iz_rec, ierr)
+    type(c_ptr), intent(in) :: h
+    integer, intent(in) :: NREC
+    integer, dimension(NREC), intent(in) :: ix_rec, iy_rec
+    integer, dimension(NREC), intent(in), optional :: iz_rec
+    integer, intent(out) :: ierr
+    if (present(iz_rec)) then
+      ierr = c_set_receivers(h, NREC, ix_rec, iy_rec, iz_rec)
+    else
+      ierr = c_set_receivers(h, NREC, ix_rec, iy_rec, ix_rec)
+    endif
+  end subroutine
+
+  ! energy every `cadence` steps over window(1:6) = i0,i1,j0,j1,k0,k1 (absent:
+  ! the reference's window, 2D2:695 / 2D4:696 / 3D:1137-1139)
+  subroutine cpml_gpu_set_energy(h, cadence, ierr, window, corrected3d)
+    type(c_ptr), intent(in) :: h
+    integer, intent(in) :: cadence
+    integer, intent(out) :: ierr
+    integer, dimension(6), intent(in), optional :: window
+    logical, intent(in), optional :: corrected3d
+    integer :: flags
+    flags = 0
+    if (present(corrected3d)) then
+      if (corrected3d) flags = 1
+    endif
+    ierr = c_set_energy(h, cadence, window, flags)
+  end subroutine
+
+  ! max |v| check at it % IT_DISPLAY == 0 and it == 5 (2D2:716-726)
+  subroutine cpml_gpu_set_stability(h, IT_DISPLAY, STABILITY_THRESHOLD, ierr)
+    type(c_ptr), intent(in) :: h
+    integer, intent(in) :: IT_DISPLAY
+    double precision, intent(in) :: STABILITY_THRESHOLD
+    integer, intent(out) :: ierr
+    ierr = c_set_stability(h, IT_DISPLAY, STABILITY_THRESHOLD)
+  end subroutine
+
+  ! steps it_begin..it_end of the time loop
+  subroutine cpml_gpu_run(h, it_begin, it_end, ierr)
+    type(c_ptr), intent(in) :: h
+    integer, intent(in) :: it_begin, it_end
+    integer, intent(out) :: ierr
+    ierr = c_run(h, it_begin, it_end)
+  end subroutine
+
+  ! sisvx(NSTEP,NREC), sisvy(NSTEP,NREC)[, sisvz]
+  subroutine cpml_gpu_get_seismograms(h, sisvx, sisvy, ierr, sisvz)
+    type(c_ptr), intent(in) :: h
+    type(*), dimension(*) :: sisvx, sisvy
+    integer, intent(out) :: ierr
+    type(*), dimension(*), optional :: sisvz
+    ierr = c_get_seismograms(h, sisvx, sisvy, sisvz)
+  end subroutine
+
+  ! total_energy_kinetic(NSTEP), total_energy_potential(NSTEP) (double precision)
+  subroutine cpml_gpu_get_energy(h, total_energy_kinetic, total_energy_potential, ierr)
+    type(c_ptr), intent(in) :: h
+    double precision, dimension(*) :: total_energy_kinetic, total_energy_potential
+    integer, intent(out) :: ierr
+    ierr = c_get_energy(h, total_energy_kinetic, total_energy_potential)
+  end subroutine
+
+  subroutine cpml_gpu_get_field(h, which, field, ierr)
+    type(c_ptr), intent(in) :: h
+    integer, intent(in) :: which
+    type(*), dimension(*) :: field
+    integer, intent(out) :: ierr
+    ierr = c_get_field(h, which, field)
+  end subroutine
+
+  subroutine cpml_gpu_set_field(h, which, field, ierr)
+    type(c_ptr), intent(in) :: h
+    integer, intent(in) :: which
+    type(*), dimension(*), intent(in) :: field
+    integer, intent(out) :: ierr
+    ierr = c_set_field(h, which, field)
+  end subroutine
+
+  ! maxval(sqrt(vx**2 + vy**2 [+ vz**2])) (2D2:719, 3D:1185)
+  subroutine cpml_gpu_velocity_norm(h, velocnorm, ierr)
+    type(c_ptr), intent(in) :: h
+    double precision, intent(out) :: velocnorm
+    integer, intent(out) :: ierr
+    ierr = c_velocity_norm(h, velocnorm)
+  end subroutine
+
+  subroutine cpml_gpu_elapsed_ms(h, ms, ierr)
+    type(c_ptr), intent(in) :: h
+    double precision, intent(out) :: ms
+    integer, intent(out) :: ierr
+    ierr = c_elapsed_ms(h, ms)
+  end subroutine
+
+  function cpml_gpu_error_message(h) result(msg)
+    type(c_ptr), intent(in) :: h
+    character(len=512) :: msg
+    character(kind=c_char), dimension(512) :: buf
+    integer :: i, rc
+    buf = c_null_char
+    rc = c_last_error(h, buf, 512)
+    msg = ' '
+    do i = 1, 512
+      if (buf(i) == c_null_char) exit
+      msg(i:i) = buf(i)
+    enddo
+  end function
+
+end module cpml_gpu

@@ -1,0 +1,312 @@
+"""Python mirror of the ``cpml_gpu`` interface over the C ABI (ctypes).
+
+The reference programs configure the time loop with Fortran ``parameter``s
+and run it inline; ``CpmlGpu`` takes the same names as arguments and runs the
+loop on the GPU through libcpml_gpu.so.  Errors raise :class:`CpmlError`
+carrying the library's status code and message (the reference's ``stop``
+strings for the CFL and blow-up checks).
+
+Arrays are numpy arrays in C order with the *last* axis x, i.e. ``(ny, nx)``
+or ``(nz, ny, nx)`` -- the same memory as the reference's Fortran
+``(NX, NY[, NZ])`` arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libcpml_gpu.so"
+
+CPML_OK = 0
+CPML_ERR_ARG = 1
+CPML_ERR_CFL = 2
+CPML_ERR_STENCIL = 3
+CPML_ERR_OOM = 4
+CPML_ERR_HIP = 5
+CPML_ERR_BLOWUP = 6
+CPML_ERR_STATE = 7
+CPML_ERR_NODEVICE = 8
+
+FIELDS = ("vx", "vy", "vz", "sigmaxx", "sigmayy", "sigmazz", "sigmaxy", "sigmaxz", "sigmayz")
+MEMORY = ("dvx_dx", "dvx_dy", "dvx_dz", "dvy_dx", "dvy_dy", "dvy_dz", "dvz_dx", "dvz_dy",
+          "dvz_dz", "dsigmaxx_dx", "dsigmayy_dy", "dsigmazz_dz", "dsigmaxy_dx", "dsigmaxy_dy",
+          "dsigmaxz_dx", "dsigmaxz_dz", "dsigmayz_dy", "dsigmayz_dz")
+
+# (name, restype, argtypes) of every exported symbol declared in include/cpml_gpu.h
+_VP, _I, _D = C.c_void_p, C.c_int, C.c_double
+_IP, _DP = C.POINTER(C.c_int), C.POINTER(C.c_double)
+SIGNATURES = {
+    "cpml_gpu_init": [C.POINTER(_VP), _I, _I, _I, _I, _I, _I, _D, _D, _D, _D, _I, _I],
+    "cpml_gpu_finalize": [_VP],
+    "cpml_gpu_set_medium": [_VP, _VP, _VP, _VP],
+    "cpml_gpu_set_medium_device": [_VP, _VP, _VP, _VP],
+    "cpml_gpu_set_pml": [_VP] + [_VP] * 18,
+    "cpml_gpu_set_source": [_VP, _I, _I, _I, _VP, _VP],
+    "cpml_gpu_set_receivers": [_VP, _I, _VP, _VP, _VP],
+    "cpml_gpu_set_energy": [_VP, _I, _VP, _I],
+    "cpml_gpu_set_stability": [_VP, _I, _D],
+    "cpml_gpu_run": [_VP, _I, _I],
+    "cpml_gpu_get_seismograms": [_VP, _VP, _VP, _VP],
+    "cpml_gpu_get_energy": [_VP, _VP, _VP],
+    "cpml_gpu_get_field": [_VP, _I, _VP],
+    "cpml_gpu_set_field": [_VP, _I, _VP],
+    "cpml_gpu_get_memory": [_VP, _I, _VP],
+    "cpml_gpu_velocity_norm": [_VP, _DP],
+    "cpml_gpu_set_graph_steps": [_VP, _I],
+    "cpml_gpu_set_kchunk": [_VP, _I],
+    "cpml_gpu_set_kernel_timing": [_VP, _I],
+    "cpml_gpu_kernel_time": [_VP, _I, _IP, _DP],
+    "cpml_gpu_elapsed_ms": [_VP, _DP],
+    "cpml_gpu_device_bytes": [_VP, _DP],
+    "cpml_gpu_slab_widths": [_VP, _IP],
+    "cpml_gpu_last_error": [_VP, C.c_char_p, _I],
+    "cpml_pml_profile": [_I, _I, _D, _D, _I, _D, _D, _I, _I, _I, _I] + [_VP] * 6,
+    "cpml_source_time_function": [_I, _I, _D, _D, _D, _D, _D, _I, _VP, _VP],
+    "cpml_nearest_receivers": [_I, _I, _I, _D, _D, _I, _D, _D, _D, _D, _VP, _VP],
+    "cpml_courant_number": [_I, _D, _D, _D, _D, _D, _DP],
+}
+
+_lib = None
+
+
+class CpmlError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+        self.message = message
+
+
+def library():
+    """Load libcpml_gpu.so; fails loudly when the HIP extension is not built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.is_file():
+            raise RuntimeError(f"{LIB_PATH} not found: build it with "
+                               "`python -m seismic_cpml_amd.build` (hipcc, gfx950)")
+        lib = C.CDLL(str(LIB_PATH))
+        for name, args in SIGNATURES.items():
+            f = getattr(lib, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        _lib = lib
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def _err(h, code):
+    buf = C.create_string_buffer(1024)
+    library().cpml_gpu_last_error(h, buf, 1024)
+    return CpmlError(code, buf.value.decode(errors="replace"))
+
+
+def _check(code, h=None):
+    if code != CPML_OK:
+        raise _err(h, code)
+
+
+def _dtype(prec):
+    return np.float32 if prec == 4 else np.float64
+
+
+# ---------------------------------------------------------------- helpers --
+
+def pml_profile(n, delta, deltat, npoints_pml, cp, f0, prec=8, use_min=True, use_max=True,
+                origin_max_uses_n=False, clamp_alpha=True):
+    """Damping profile along one axis (reference 2D2:283-465).  Returns a
+    (6, n) array: a, b, K, a_half, b_half, K_half."""
+    out = np.zeros((6, n), _dtype(prec))
+    _check(library().cpml_pml_profile(prec, n, float(delta), float(deltat), npoints_pml,
+                                      float(cp), float(f0), int(use_min), int(use_max),
+                                      int(origin_max_uses_n), int(clamp_alpha),
+                                      *[_ptr(out[q]) for q in range(6)]))
+    return out
+
+
+def source_time_function(nstep, deltat, f0, t0, factor, angle_force, kind=1, prec=8):
+    """force_x, force_y per step (reference 2D2:644-657); kind 0 Gaussian,
+    1 first derivative of a Gaussian, 2 Ricker."""
+    fx = np.zeros(nstep, _dtype(prec))
+    fy = np.zeros(nstep, _dtype(prec))
+    _check(library().cpml_source_time_function(prec, nstep, float(deltat), float(f0),
+                                               float(t0), float(factor), float(angle_force),
+                                               kind, _ptr(fx), _ptr(fy)))
+    return fx, fy
+
+
+def nearest_receivers(nx, ny, deltax, deltay, nrec, xdeb, ydeb, xfin, yfin, prec=8):
+    """1-based grid indices of receivers on a line (reference 2D2:483-509)."""
+    ix = np.zeros(max(nrec, 1), np.int32)
+    iy = np.zeros(max(nrec, 1), np.int32)
+    _check(library().cpml_nearest_receivers(prec, nx, ny, float(deltax), float(deltay), nrec,
+                                            float(xdeb), float(ydeb), float(xfin), float(yfin),
+                                            _ptr(ix), _ptr(iy)))
+    return ix[:nrec], iy[:nrec]
+
+
+def courant_number(ndim, cp, deltat, deltax, deltay, deltaz=1.0):
+    out = C.c_double()
+    _check(library().cpml_courant_number(ndim, float(cp), float(deltat), float(deltax),
+                                         float(deltay), float(deltaz), C.byref(out)))
+    return out.value
+
+
+# ----------------------------------------------------------------- solver --
+
+class CpmlGpu:
+    """One C-PML time-loop instance on the current GPU (cpml_gpu_init ...)."""
+
+    def __init__(self, ndim, order, prec, NX, NY, NZ, DELTAX, DELTAY, DELTAZ, DELTAT,
+                 NPOINTS_PML, NSTEP):
+        self.ndim, self.order, self.prec = ndim, order, prec
+        self.nx, self.ny, self.nz = NX, NY, (NZ if ndim == 3 else 1)
+        self.nstep = NSTEP
+        self.dtype = _dtype(prec)
+        self.nrec = 0
+        self._h = C.c_void_p()
+        _check(library().cpml_gpu_init(C.byref(self._h), ndim, order, prec, NX, NY,
+                                       NZ if ndim == 3 else 1, float(DELTAX), float(DELTAY),
+                                       float(DELTAZ), float(DELTAT), NPOINTS_PML, NSTEP))
+
+    # context management
+    def close(self):
+        if self._h:
+            library().cpml_gpu_finalize(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def shape(self):
+        return (self.ny, self.nx) if self.ndim == 2 else (self.nz, self.ny, self.nx)
+
+    def _grid(self, a):
+        a = np.ascontiguousarray(a, self.dtype)
+        if a.shape != self.shape:
+            raise ValueError(f"array shape {a.shape} != grid {self.shape}")
+        return a
+
+    def _vec(self, a, n):
+        a = np.ascontiguousarray(a, self.dtype)
+        if a.shape != (n,):
+            raise ValueError(f"profile length {a.shape} != {n}")
+        return a
+
+    def _call(self, fn, *args):
+        _check(getattr(library(), fn)(self._h, *args), self._h)
+
+    def set_medium(self, lam, mu, rho):
+        lam, mu, rho = self._grid(lam), self._grid(mu), self._grid(rho)
+        self._call("cpml_gpu_set_medium", _ptr(lam), _ptr(mu), _ptr(rho))
+
+    def set_medium_device(self, d_lam, d_mu, d_rho):
+        """Device pointers (ints) to interior-only arrays, e.g. tensor.data_ptr()."""
+        self._call("cpml_gpu_set_medium_device", C.c_void_p(d_lam), C.c_void_p(d_mu),
+                   C.c_void_p(d_rho))
+
+    def set_pml(self, prof_x, prof_y, prof_z=None):
+        """Each profile is (6, n): a, b, K, a_half, b_half, K_half."""
+        px = [self._vec(prof_x[q], self.nx) for q in range(6)]
+        py = [self._vec(prof_y[q], self.ny) for q in range(6)]
+        pz = ([self._vec(prof_z[q], self.nz) for q in range(6)] if self.ndim == 3
+              else [None] * 6)
+        self._call("cpml_gpu_set_pml", *[_ptr(a) for a in px + py + pz])
+
+    def set_source(self, ISOURCE, JSOURCE, KSOURCE, force_x, force_y):
+        fx, fy = self._vec(force_x, self.nstep), self._vec(force_y, self.nstep)
+        self._call("cpml_gpu_set_source", ISOURCE, JSOURCE, KSOURCE, _ptr(fx), _ptr(fy))
+
+    def set_receivers(self, ix_rec, iy_rec, iz_rec=None):
+        ix = np.ascontiguousarray(ix_rec, np.int32)
+        iy = np.ascontiguousarray(iy_rec, np.int32)
+        iz = None if iz_rec is None else np.ascontiguousarray(iz_rec, np.int32)
+        self.nrec = len(ix)
+        self._call("cpml_gpu_set_receivers", len(ix), _ptr(ix), _ptr(iy), _ptr(iz))
+
+    def set_energy(self, cadence=1, window=None, corrected3d=False):
+        w = None if window is None else np.ascontiguousarray(
+            list(window) + [1] * (6 - len(window)), np.int32)
+        self._call("cpml_gpu_set_energy", cadence, _ptr(w), int(corrected3d))
+
+    def set_stability(self, it_display=100, threshold=1e25):
+        self._call("cpml_gpu_set_stability", it_display, float(threshold))
+
+    def set_graph_steps(self, n):
+        self._call("cpml_gpu_set_graph_steps", n)
+
+    def set_kchunk(self, n):
+        self._call("cpml_gpu_set_kchunk", n)
+
+    def set_kernel_timing(self, on=True):
+        self._call("cpml_gpu_set_kernel_timing", int(on))
+
+    def run(self, it_begin=1, it_end=None):
+        self._call("cpml_gpu_run", it_begin, self.nstep if it_end is None else it_end)
+
+    def seismograms(self):
+        """(sisvx, sisvy[, sisvz]) each (NREC, NSTEP) (memory of Fortran (NSTEP, NREC))."""
+        n = max(self.nrec, 1)
+        out = [np.zeros((n, self.nstep), self.dtype) for _ in range(self.ndim)]
+        z = _ptr(out[2]) if self.ndim == 3 else None
+        self._call("cpml_gpu_get_seismograms", _ptr(out[0]), _ptr(out[1]), z)
+        return tuple(o[: self.nrec] for o in out)
+
+    def energy(self):
+        kin = np.zeros(self.nstep)
+        pot = np.zeros(self.nstep)
+        self._call("cpml_gpu_get_energy", _ptr(kin), _ptr(pot))
+        return kin, pot
+
+    def field(self, name):
+        out = np.zeros(self.shape, self.dtype)
+        self._call("cpml_gpu_get_field", FIELDS.index(name), _ptr(out))
+        return out
+
+    def set_field(self, name, a):
+        a = self._grid(a)
+        self._call("cpml_gpu_set_field", FIELDS.index(name), _ptr(a))
+
+    def memory(self, name):
+        out = np.zeros(self.shape, self.dtype)
+        self._call("cpml_gpu_get_memory", MEMORY.index(name), _ptr(out))
+        return out
+
+    def velocity_norm(self):
+        v = C.c_double()
+        self._call("cpml_gpu_velocity_norm", C.byref(v))
+        return v.value
+
+    def kernel_time(self, kernel):
+        n, ms = C.c_int(), C.c_double()
+        self._call("cpml_gpu_kernel_time", kernel, C.byref(n), C.byref(ms))
+        return n.value, ms.value
+
+    def elapsed_ms(self):
+        v = C.c_double()
+        self._call("cpml_gpu_elapsed_ms", C.byref(v))
+        return v.value
+
+    def device_bytes(self):
+        v = C.c_double()
+        self._call("cpml_gpu_device_bytes", C.byref(v))
+        return v.value
+
+    def slab_widths(self):
+        w = (C.c_int * 6)()
+        self._call("cpml_gpu_slab_widths", w)
+        return list(w)

@@ -1,0 +1,200 @@
+"""Synthetic workloads of the benchmark configurations.
+
+* ``shipped_2d(order)``  -- seismic_CPML_2D_isotropic_{second,fourth}_order as
+  shipped (101 x 641, homogeneous, first-derivative-of-Gaussian force).
+* ``layered_2d``         -- 2D 4th-order, 4096^2, 5 m, NPML 20, FP32, 12 seeded
+  horizontal layers (cp ~ U[1500,5500], cs = cp/sqrt(3), rho = 310 cp^0.25),
+  Ricker 10 Hz at the centre, 256 receivers on the top interior row.
+* ``layered_3d``         -- 3D 768^3, 10 m, NPML 20, FP32, layered along z plus
+  a +-5% Gaussian-smoothed (sigma = 8 cells) perturbation, Ricker 8 Hz at the
+  centre, 32 x 32 receivers on the top interior plane, DELTAT for Courant 0.5.
+
+Large media are generated on the GPU with torch (device-memory plumbing only)
+and handed to the library with set_medium_device; the time loop itself runs
+in libcpml_gpu.  The damping profile needs one velocity: cp_max is used.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import solver as S
+
+
+@dataclass
+class Workload:
+    name: str
+    ndim: int
+    order: int
+    prec: int
+    nx: int
+    ny: int
+    nz: int
+    dx: float
+    dt: float
+    npml: int
+    nstep: int
+    f0: float
+    src: tuple
+    rec: tuple            # (ix, iy[, iz]) 1-based int32 arrays
+    prof: tuple           # (prof_x, prof_y[, prof_z])
+    force: tuple          # (force_x, force_y)
+    medium_host: tuple | None = None     # (lam, mu, rho) numpy
+    medium_dev: tuple | None = None      # torch tensors (lam, mu, rho)
+    cp_max: float = 0.0
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def cells(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    def make_solver(self, energy=False, graph_steps=16):
+        s = S.CpmlGpu(self.ndim, self.order, self.prec, self.nx, self.ny, self.nz, self.dx,
+                      self.dx, self.dx, self.dt, self.npml, self.nstep)
+        if self.medium_dev is not None:
+            lam, mu, rho = self.medium_dev
+            s.set_medium_device(lam.data_ptr(), mu.data_ptr(), rho.data_ptr())
+        else:
+            s.set_medium(*self.medium_host)
+        s.set_pml(*self.prof)
+        s.set_source(*self.src, *self.force)
+        s.set_receivers(*self.rec)
+        if energy:
+            s.set_energy(1)
+        s.set_graph_steps(graph_steps)
+        return s
+
+
+def _layers(rng, n, nlayers=12):
+    cuts = np.sort(rng.integers(1, n, size=nlayers - 1))
+    cps = rng.uniform(1500.0, 5500.0, size=nlayers)
+    return cps[np.searchsorted(cuts, np.arange(n), side="right")]
+
+
+def _elastic(cp):
+    cs = cp / math.sqrt(3.0)
+    rho = 310.0 * cp ** 0.25
+    mu = rho * cs * cs
+    lam = rho * (cp * cp - 2.0 * cs * cs)
+    return lam, mu, rho
+
+
+def shipped_2d(order: int = 2, prec: int = 8) -> Workload:
+    """2D2 / 2D4 as shipped (2D2:137-186, 2D4:131-187)."""
+    dt_ = np.float64 if prec == 8 else np.float32
+    nx, ny, npml = 101, 641, 10
+    dx = dt_(10.0)
+    deltat = dt_(2e-3) if order == 2 else dt_(2e-3) / dt_(2)
+    nstep = 2000 if order == 2 else 4000
+    cp, density = dt_(3300.0), dt_(2800.0)
+    cs = cp / dt_(1.732)
+    f0 = dt_(7.0)
+    t0 = dt_(1.2) / f0
+    isrc, jsrc = nx - 2 * npml - 1, 2 * ny // 3 + 1
+    xs = dt_(isrc - 1) * dx
+    px = S.pml_profile(nx, dx, deltat, npml, cp, f0, prec, clamp_alpha=True)
+    py = S.pml_profile(ny, dx, deltat, npml, cp, f0, prec, origin_max_uses_n=(order == 4),
+                       clamp_alpha=False)
+    fx, fy = S.source_time_function(nstep, deltat, f0, t0, dt_(1e7), dt_(135.0), 1, prec)
+    ix, iy = S.nearest_receivers(nx, ny, dx, dx, 2, xs - dt_(100.0), dt_(2300.0), xs,
+                                 dt_(300.0), prec)
+    rho = np.full((ny, nx), density, dt_)
+    mu = np.full((ny, nx), density * cs * cs, dt_)
+    lam = np.full((ny, nx), density * (cp * cp - dt_(2) * cs * cs), dt_)
+    return Workload(f"2d{order}_shipped", 2, order, prec, nx, ny, 1, float(dx), float(deltat),
+                    npml, nstep, float(f0), (isrc, jsrc, 1), (ix, iy), (px, py), (fx, fy),
+                    medium_host=(lam, mu, rho), cp_max=float(cp))
+
+
+def layered_2d(n=4096, nstep=4000, seed=2, order=4, prec=4, courant=0.5, npml=20,
+               dx=5.0, f0=10.0, nrec=256) -> Workload:
+    dt_ = np.float64 if prec == 8 else np.float32
+    rng = np.random.default_rng(seed)
+    cp = np.repeat(_layers(rng, n)[:, None], n, axis=1)  # layers along y (depth)
+    lam, mu, rho = (a.astype(dt_) for a in _elastic(cp))
+    cp_max = float(cp.max())
+    deltat = courant * dx / (cp_max * math.sqrt(2.0))
+    px = S.pml_profile(n, dx, deltat, npml, cp_max, f0, prec, clamp_alpha=True)
+    py = S.pml_profile(n, dx, deltat, npml, cp_max, f0, prec, clamp_alpha=False)
+    fx, fy = S.source_time_function(nstep, deltat, f0, 1.2 / f0, 1e7, 135.0, 2, prec)
+    ix = np.linspace(npml + 1, n - npml, nrec).round().astype(np.int32)
+    iy = np.full(nrec, n - npml, np.int32)
+    return Workload(f"2d{order}_layered_{n}", 2, order, prec, n, n, 1, dx, deltat, npml, nstep,
+                    f0, (n // 2, n // 2, 1), (ix, iy), (px, py), (fx, fy),
+                    medium_host=(lam, mu, rho), cp_max=cp_max)
+
+
+def layered_3d(n=768, nstep=2000, seed=3, prec=4, courant=0.5, npml=20, dx=10.0, f0=8.0,
+               nrec_side=32, sigma=8.0, amp=0.05, device="cuda") -> Workload:
+    """3D layered + smoothed perturbation, generated on the GPU with torch."""
+    import torch
+
+    tdt = torch.float32 if prec == 4 else torch.float64
+    rng = np.random.default_rng(seed)
+    cpz = torch.tensor(_layers(rng, n), dtype=torch.float64, device=device)
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    noise = torch.randn((n, n, n), generator=g, device=device, dtype=torch.float32)
+    # Gaussian smoothing with sigma cells, applied in the Fourier domain
+    spec = torch.fft.rfftn(noise)
+    del noise
+    w_full = 2.0 * math.pi * torch.fft.fftfreq(n, device=device)
+    w_half = 2.0 * math.pi * torch.fft.rfftfreq(n, device=device)
+    gz = torch.exp(-0.5 * (sigma * w_full) ** 2).view(n, 1, 1)
+    gy = torch.exp(-0.5 * (sigma * w_full) ** 2).view(1, n, 1)
+    gx = torch.exp(-0.5 * (sigma * w_half) ** 2).view(1, 1, -1)
+    spec *= gz
+    spec *= gy
+    spec *= gx
+    noise = torch.fft.irfftn(spec, s=(n, n, n))
+    del spec
+    noise = noise / noise.abs().max()
+    cp = (cpz.view(n, 1, 1).float() * (1.0 + amp * noise))  # (z, y, x) layers along z
+    del noise
+    cp_max = float(cp.max())
+    cs = cp / math.sqrt(3.0)
+    rho = 310.0 * cp ** 0.25
+    mu = (rho * cs * cs).to(tdt).contiguous()
+    lam = (rho * (cp * cp - 2.0 * cs * cs)).to(tdt).contiguous()
+    rho = rho.to(tdt).contiguous()
+    del cp, cs
+    torch.cuda.synchronize()
+    deltat = courant * dx / (cp_max * math.sqrt(3.0))
+    px = S.pml_profile(n, dx, deltat, npml, cp_max, f0, prec, clamp_alpha=True)
+    py = S.pml_profile(n, dx, deltat, npml, cp_max, f0, prec, clamp_alpha=False)
+    pz = S.pml_profile(n, dx, deltat, npml, cp_max, f0, prec, clamp_alpha=False)
+    fx, fy = S.source_time_function(nstep, deltat, f0, 1.2 / f0, 1e7, 135.0, 2, prec)
+    line = np.linspace(npml + 1, n - npml, nrec_side).round().astype(np.int32)
+    ix, iy = np.meshgrid(line, line, indexing="xy")
+    ix = ix.ravel().astype(np.int32)
+    iy = iy.ravel().astype(np.int32)
+    iz = np.full(ix.size, n - npml, np.int32)
+    return Workload(f"3d_layered_{n}", 3, 2, prec, n, n, n, dx, deltat, npml, nstep, f0,
+                    (n // 2, n // 2, n // 2), (ix, iy, iz), (px, py, pz), (fx, fy),
+                    medium_dev=(lam, mu, rho), cp_max=cp_max)
+
+
+def bytes_per_step(w: Workload, slabs) -> dict:
+    """Algorithmic HBM bytes of one time step, per kernel: every array touched
+    once per phase.  3D stress: read v(3), sigma(6), DELTAT*lambda, DELTAT*mu,
+    write sigma(6) = 17 words; velocity: read sigma(6), v(3), DELTAT/rho, write
+    v(3) = 13 words; each phase reads and writes 3 memory variables per
+    absorbing slab point (6 words).  2D: stress 5 reads (vx, vy, sigma x3) +
+    lambda, mu + 3 writes = 10 words; velocity 5 + rho + 2 writes = 8 words;
+    2 memory variables per slab point per phase (4 words)."""
+    es = 4 if w.prec == 4 else 8
+    wx = slabs[0] + slabs[1]
+    wy = slabs[2] + slabs[3]
+    wz = slabs[4] + slabs[5]
+    n = w.cells
+    if w.ndim == 3:
+        slab_pts = wx * w.ny * w.nz + w.nx * wy * w.nz + w.nx * w.ny * wz
+        stress = es * (17 * n + 6 * slab_pts)
+        vel = es * (13 * n + 6 * slab_pts)
+    else:
+        slab_pts = wx * w.ny + w.nx * wy
+        stress = es * (10 * n + 4 * slab_pts)
+        vel = es * (8 * n + 4 * slab_pts)
+    return {"stress": stress, "velocity": vel, "step": stress + vel}

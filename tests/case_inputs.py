@@ -73,7 +73,10 @@ def inputs(case: str, prec: str, nstep: int | None = None) -> dict:
 
 
 def run_oracle(case: str, prec: str, nstep: int | None = None, d: dict | None = None,
-               nthreads: int = 0, energy=True) -> dict:
+               nthreads: int = 0, energy=True, derived_lp2m: bool = False) -> dict:
+    """derived_lp2m: 3D only -- use DELTAT*lambda + 2*(DELTAT*mu) for the
+    (lambda+2mu) coefficient, as the HIP kernels do, instead of the reference's
+    DELTAT*(rho*cp*cp) (3D:145, 298)."""
     d = inputs(case, prec, nstep) if d is None else d
     c = CASES[case]
     if c.get("NZ") is None:
@@ -82,7 +85,8 @@ def run_oracle(case: str, prec: str, nstep: int | None = None, d: dict | None = 
                          d["ISOURCE"], d["JSOURCE"], d["fx"], d["fy"], d["ix_rec"], d["iy_rec"],
                          ewin=d["ewin"] if energy else None, prec=prec, nthreads=nthreads)
     return orc.run3d(c["NX"], c["NY"], c["NZ"], d["DELTAX"], d["DELTAY"], d["DELTAZ"],
-                     d["DELTAT"], d["nstep"], d["lam"], d["mu"], d["rho"], d["lp2m"],
+                     d["DELTAT"], d["nstep"], d["lam"], d["mu"], d["rho"],
+                     None if derived_lp2m else d["lp2m"],
                      d["prof_x"], d["prof_y"], d["prof_z"], d["ISOURCE"], d["JSOURCE"],
                      d["ksrc"], d["fx"], d["fy"], d["ix_rec"], d["iy_rec"], d["iz_rec"],
                      ewin=d["ewin"] if energy else None, energy_slabs=c.get("nproc", 1),

@@ -1,0 +1,118 @@
+"""C-ABI checks that need no GPU: the library loads, exports every symbol the
+header declares, validates arguments on the host before touching HIP, and its
+host-side set-up helpers reproduce the reference (through the oracle, which
+is pinned bit-exactly to the compiled reference)."""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import seismic_cpml_amd as sc
+from seismic_cpml_amd import solver
+
+import oracle_lib as orc
+from case_inputs import fixture
+from golden.cases import CASES, values
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def header_symbols():
+    text = (ROOT / "include" / "cpml_gpu.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\bint\s+(cpml_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = sc.library()
+    syms = header_symbols()
+    assert len(syms) >= 25
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(solver.SIGNATURES), set(syms) ^ set(solver.SIGNATURES)
+
+
+def _init(*args):
+    import ctypes as C
+
+    h = C.c_void_p()
+    rc = sc.library().cpml_gpu_init(C.byref(h), *args)
+    return rc, h
+
+
+@pytest.mark.parametrize("args,code", [
+    ((4, 2, 8, 10, 10, 1, 1.0, 1.0, 1.0, 1e-3, 2, 10), solver.CPML_ERR_ARG),   # ndim
+    ((2, 6, 8, 10, 10, 1, 1.0, 1.0, 1.0, 1e-3, 2, 10), solver.CPML_ERR_ARG),   # order
+    ((3, 4, 8, 10, 10, 10, 1.0, 1.0, 1.0, 1e-3, 2, 10), solver.CPML_ERR_ARG),  # 3D order 4
+    ((2, 2, 2, 10, 10, 1, 1.0, 1.0, 1.0, 1e-3, 2, 10), solver.CPML_ERR_ARG),   # prec
+    ((2, 2, 8, 1, 10, 1, 1.0, 1.0, 1.0, 1e-3, 2, 10), solver.CPML_ERR_STENCIL),
+    ((2, 2, 8, 10, 10, 1, -1.0, 1.0, 1.0, 1e-3, 2, 10), solver.CPML_ERR_ARG),
+    ((2, 2, 8, 10, 10, 1, 1.0, 1.0, 1.0, 0.0, 2, 10), solver.CPML_ERR_ARG),
+    ((2, 2, 8, 10, 10, 1, 1.0, 1.0, 1.0, 1e-3, 2, 0), solver.CPML_ERR_ARG),    # NSTEP
+    ((2, 2, 8, 10, 10, 1, 1.0, 1.0, 1.0, 1e-3, 6, 10), solver.CPML_ERR_STENCIL),  # PML too wide
+])
+def test_init_validates_on_host(args, code):
+    rc, h = _init(*args)
+    assert rc == code
+    assert not h.value
+    msg = sc.solver._err(None, rc).message
+    assert msg
+
+
+def test_init_without_gpu_reports_nodevice():
+    import torch
+
+    if torch.cuda.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(sc.CpmlError) as e:
+        sc.CpmlGpu(2, 2, 8, 32, 32, 1, 10.0, 10.0, 10.0, 1e-3, 4, 10)
+    assert e.value.code == solver.CPML_ERR_NODEVICE
+
+
+def test_courant_number_matches_reference_formula():
+    # 2D2:513 with the shipped parameters: 3300 * 2e-3 * sqrt(2)/10
+    c = sc.courant_number(2, 3300.0, 2e-3, 10.0, 10.0)
+    assert c == 3300.0 * 2e-3 * np.sqrt(1.0 / 10.0 ** 2 + 1.0 / 10.0 ** 2)
+    c3 = sc.courant_number(3, 3300.0, 1.6e-3, 10.0, 10.0, 10.0)
+    assert abs(c3 - 3300 * 1.6e-3 * np.sqrt(3) / 10) < 1e-15
+
+
+@pytest.mark.parametrize("case", list(CASES))
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_host_profile_helper_matches_reference(case, prec):
+    """The library's host profile builder against the reference's own profiles."""
+    v = values(case, prec)
+    c = CASES[case]
+    f = fixture(case, prec)
+    p = 8 if prec == "f64" else 4
+    px = sc.pml_profile(c["NX"], v["DELTAX"], v["DELTAT"], c["NPML"], v["cp"], v["f0"], p,
+                        clamp_alpha=True)
+    py = sc.pml_profile(c["NY"], v["DELTAY"], v["DELTAT"], c["NPML"], v["cp"], v["f0"], p,
+                        origin_max_uses_n=c.get("yorigin_n", False), clamp_alpha=False)
+    np.testing.assert_array_equal(px, f["prof_x"])
+    np.testing.assert_array_equal(py, f["prof_y"])
+    if "prof_z" in f.files:
+        pz = sc.pml_profile(c["NZ"], v["DELTAZ"], v["DELTAT"], c["NPML"], v["cp"], v["f0"], p,
+                            clamp_alpha=False)
+        np.testing.assert_array_equal(pz, f["prof_z"])
+
+
+@pytest.mark.parametrize("case", list(CASES))
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_host_stf_and_receivers_match_oracle(case, prec):
+    v = values(case, prec)
+    c = CASES[case]
+    p = 8 if prec == "f64" else 4
+    kind = 2 if c["src"] == "ricker" else 1
+    fx, fy = sc.source_time_function(c["NSTEP"], v["DELTAT"], v["f0"], v["t0"], v["factor"],
+                                     v["ANGLE"], kind, p)
+    ox, oy = orc.stf(c["NSTEP"], v["DELTAT"], v["f0"], v["t0"], v["factor"], v["ANGLE"], kind,
+                     prec)
+    np.testing.assert_array_equal(fx, ox)
+    np.testing.assert_array_equal(fy, oy)
+    ix, iy = sc.nearest_receivers(c["NX"], c["NY"], v["DELTAX"], v["DELTAY"], c["NREC"],
+                                  v["xdeb"], v["ydeb"], v["xfin"], v["yfin"], p)
+    f = fixture(case, prec)
+    np.testing.assert_array_equal(ix, f["ix_rec"])
+    np.testing.assert_array_equal(iy, f["iy_rec"])

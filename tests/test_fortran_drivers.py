@@ -1,0 +1,85 @@
+"""The Fortran driver programs (cpml_gpu module over the C ABI) against the
+compiled reference's outputs.
+
+The 2D drivers run the shipped configurations of 2D2 and 2D4; their
+seismograms must equal the reference's bit-for-bit in both precisions and
+their ASCII Vx_file_001.dat must match the reference's single-precision
+values.  The reduced 3D build matches the 3D golden case within the 3D
+tolerances (1e-12 float64, 1e-5 float32 rel-L2).
+"""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from case_inputs import fixture
+from gpu_runner import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+BIN = Path(__file__).resolve().parents[1] / "seismic_cpml_amd" / "fortran" / "bin"
+
+
+def run_driver(name, tmp_path):
+    exe = BIN / name
+    assert exe.is_file(), f"{exe} not built (python -m seismic_cpml_amd.build)"
+    r = subprocess.run([str(exe)], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "End of the simulation" in r.stdout
+    return r.stdout
+
+
+def read_dump(path, dt, nstep, nrec, nplane):
+    raw = path.read_bytes()
+    es = np.dtype(dt).itemsize
+    off = 0
+
+    def take(n, t):
+        nonlocal off
+        a = np.frombuffer(raw, dtype=t, count=n, offset=off)
+        off += n * np.dtype(t).itemsize
+        return a
+
+    out = {"sisvx": take(nstep * nrec, dt).reshape(nrec, nstep),
+           "sisvy": take(nstep * nrec, dt).reshape(nrec, nstep),
+           "ekin": take(nstep, np.float64), "epot": take(nstep, np.float64),
+           "vx": take(nplane, dt)}
+    assert es in (4, 8)
+    return out
+
+
+@pytest.mark.parametrize("order,prec", [(2, "f64"), (2, "f32"), (4, "f64"), (4, "f32")])
+def test_2d_driver_matches_reference(order, prec, tmp_path):
+    name = f"seismic_cpml_2d_{'second' if order == 2 else 'fourth'}_{prec}"
+    case = f"2d{order}_shipped"
+    out = run_driver(name, tmp_path)
+    assert "Max norm velocity vector" in out
+    f = fixture(case, prec)
+    dt = np.float64 if prec == "f64" else np.float32
+    nstep = 2000 if order == 2 else 4000
+    d = read_dump(tmp_path / "gpu_dump.bin", dt, nstep, 2, 101 * 641)
+    np.testing.assert_array_equal(d["sisvx"], f["sisvx"])
+    np.testing.assert_array_equal(d["sisvy"], f["sisvy"])
+    etol = 1e-12 if prec == "f64" else 1e-4
+    assert rel_l2(d["ekin"], f["ekin"]) < etol
+    assert rel_l2(d["epot"], f["epot"]) < etol
+    np.testing.assert_array_equal(d["vx"].reshape(641, 101)[::4, ::4], f["sub_vx"])
+    # the ASCII seismogram in the reference's format
+    txt = np.loadtxt(tmp_path / "Vx_file_001.dat")
+    assert txt.shape == (nstep, 2)
+    np.testing.assert_array_equal(txt[:, 1].astype(np.float32),
+                                  f["sisvx"][0].astype(np.float32))
+    assert (tmp_path / "energy.dat").is_file()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_3d_driver_matches_reference(prec, tmp_path):
+    run_driver(f"seismic_cpml_3d_small_{prec}", tmp_path)
+    f = fixture("3d_small", prec)
+    dt = np.float64 if prec == "f64" else np.float32
+    d = read_dump(tmp_path / "gpu_dump.bin", dt, 300, 2, 40 * 44)
+    tol = 1e-12 if prec == "f64" else 1e-5
+    assert rel_l2(d["sisvx"], f["sisvx"]) < tol
+    assert rel_l2(d["sisvy"], f["sisvy"]) < tol
+    assert rel_l2(d["ekin"] + d["epot"], f["etotal"]) < (1e-12 if prec == "f64" else 1e-4)
