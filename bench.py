@@ -53,10 +53,11 @@ def cpu_baseline(w, lam, mu, rho, nzs=128, seconds=15.0, threads=None):
                          py, pzs, n // 2, n // 2, nzs // 2, fx[:nst], fy[:nst], ixr, ixr,
                          np.array([nzs // 2], np.int32), state=st, **args)
 
-    t = time.perf_counter()
     out = run(1, state)  # warm-up step (also pages the arrays in)
-    t1 = time.perf_counter() - t
-    steps = max(1, min(200, int(seconds / max(t1, 1e-3))))
+    t = time.perf_counter()
+    out = run(2, out["state"])
+    t1 = (time.perf_counter() - t) / 2
+    steps = max(1, min(400, int(seconds / max(t1, 1e-3))))
     t = time.perf_counter()
     run(steps, out["state"])
     el = time.perf_counter() - t
@@ -76,6 +77,8 @@ def main():
     ap.add_argument("--kchunk", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a device copy of 2 GiB (reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; runs on several GPUs are not part of it "
@@ -87,6 +90,12 @@ def main():
     from seismic_cpml_amd import workloads
 
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        a = torch.ones(2 ** 29, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     nstep = args.warmup + args.steps
     w = workloads.layered_3d(n=args.n, nstep=nstep)
     s = w.make_solver()
