@@ -311,6 +311,8 @@ cpml::Step3D<T> step3d(cpml_solver *h) {
   p.ps = h->ps;
   p.mld = h->mld;
   p.mps = h->mld * h->ny;
+  p.base = h->base;
+  p.xpad = h->xpad;
   // 3D:134-137 ONE_OVER_DELTA = 1/DELTA, applied by multiplication (3D:841)
   p.odx = T(1) / T(h->dx);
   p.ody = T(1) / T(h->dy);
@@ -677,6 +679,9 @@ int cpml_gpu_init(cpml_solver **hp, int ndim, int order, int prec, int NX, int N
     return fail(nullptr, CPML_ERR_STENCIL, "NPOINTS_PML does not fit the grid");
   const double cells = (double)NX * NY * (ndim == 3 ? NZ : 1);
   if (cells > 4.0e10) return fail(nullptr, CPML_ERR_OOM, "grid too large");
+  // the 3D kernels index every array with unsigned 32-bit offsets
+  if (ndim == 3 && ((double)NX + 80) * (NY + 2) * (NZ + 2) > 4.0e9)
+    return fail(nullptr, CPML_ERR_OOM, "3D grid too large for 32-bit offsets (about 1500^3)");
 
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)

@@ -79,6 +79,8 @@ struct Step3D {
   Slab sx, sy, sz;
   int nx, ny, nz;
   long ld, ps;    // field strides
+  long base;      // offset of interior point (0,0,0) from the allocation start
+  long xpad;      // x halo width (interior rows start 256-byte aligned)
   long mld, mps;  // memory strides: x-slab (w, w*ny), y-slab (mld, mld*wy), z (mld, mld*ny)
   T odx, ody, odz;
   T dt;

@@ -147,116 +147,270 @@ __global__ __launch_bounds__(BX *BY) void k_velocity2d(Step2D<T> p) {
 }
 
 // ---------------------------------------------------------------- 3D ----
+//
+// A 3D tile is 64 points along x (one wavefront per row) by TY3 rows, and
+// marches through a chunk of z planes carrying the z neighbours in
+// registers.  All offsets are unsigned 32-bit from the start of the
+// allocation (the host checks that every array fits), added to uniform base
+// pointers, and the loads of plane k+1 are issued before the arithmetic and
+// stores of plane k (register double buffering).
+
+constexpr int TY3 = 4;
+
+struct Tile3 {
+  int i, j, kb, ke;
+  bool ok;
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD one
+// contiguous band of tiles (x fastest, then y, then z chunk) so the tiles that
+// read each other's halo rows run on the same XCD and share its L2.
+__device__ __forceinline__ Tile3 tile3(int nx, int ny, int nz, int kchunk) {
+  const int ntx = (nx + BX - 1) / BX, nty = (ny + TY3 - 1) / TY3;
+  const int ntz = (nz + kchunk - 1) / kchunk;
+  const int nt = ntx * nty * ntz;
+  const int h = blockIdx.x;
+  const int xcd = h & 7, loc = h >> 3;
+  const int q = nt >> 3, r = nt & 7;
+  const int t = xcd * q + min(xcd, r) + loc;
+  const int tx = t % ntx, rest = t / ntx;
+  const int ty = rest % nty, tz = rest / nty;
+  Tile3 o;
+  o.i = tx * BX + threadIdx.x;
+  o.j = ty * TY3 + threadIdx.y;
+  o.kb = tz * kchunk;
+  o.ke = min(o.kb + kchunk, nz);
+  o.ok = o.i < nx && o.j < ny;
+  return o;
+}
+
+// Buffer-resource access to one padded 3D array, rebased at the plane kb-1
+// of the block's z chunk so every byte offset fits 32 bits.  A thread keeps
+// one VGPR byte offset (the point (i-1, j-1) of that plane); the row and plane
+// shifts are uniform (SGPR soffset) and the x shift an instruction immediate.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes,
+                                           0x00020000);
+}
 
 template <typename T>
-__global__ __launch_bounds__(BX *BY) void k_stress3d(Step3D<T> p) {
-  const int i = blockIdx.x * BX + threadIdx.x;
-  const int j = blockIdx.y * BY + threadIdx.y;
-  if (i >= p.nx || j >= p.ny) return;
-  const int kb = blockIdx.z * p.kchunk;
-  const int ke = min(kb + p.kchunk, p.nz);
-  const long ld = p.ld, ps = p.ps;
+struct BufOp;
+
+template <>
+struct BufOp<float> {
+  __device__ static float ld(__amdgpu_buffer_rsrc_t r, unsigned v, unsigned s) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, v, s, 0));
+  }
+  __device__ static void st(float x, __amdgpu_buffer_rsrc_t r, unsigned v, unsigned s) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, v, s, 0);
+  }
+};
+
+template <>
+struct BufOp<double> {
+  __device__ static double ld(__amdgpu_buffer_rsrc_t r, unsigned v, unsigned s) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, v, s, 0));
+  }
+  __device__ static void st(double x, __amdgpu_buffer_rsrc_t r, unsigned v, unsigned s) {
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, v, s, 0);
+  }
+};
+
+// Geometry of the rebased chunk buffers of one tile.
+template <typename T>
+struct Chunk3 {
+  unsigned v0;   // byte offset of (i-1, j-1) in buffer plane 0
+  unsigned ldb;  // row stride in bytes
+  unsigned psb;  // plane stride in bytes
+  unsigned bytes;
+  long origin;   // element offset of buffer plane 0 from the allocation start
+  __device__ Chunk3(const Step3D<T> &p, const Tile3 &t) {
+    ldb = (unsigned)(p.ld * sizeof(T));
+    psb = (unsigned)(p.ps * sizeof(T));
+    bytes = (unsigned)((t.ke - t.kb + 2) * p.ps * sizeof(T));
+    origin = (long)t.kb * p.ps;  // allocation plane kb == interior plane kb-1
+    v0 = (unsigned)((t.j * p.ld + p.xpad + t.i - 1) * sizeof(T));
+  }
+  // SGPR part of the offset of (i+dx, j+dy, k+dz) for buffer plane kk = k-kb+1
+  __device__ unsigned so(int kk, int dy, int dz) const {
+    return (unsigned)(kk + dz) * psb + (unsigned)(1 + dy) * ldb;
+  }
+  __device__ __amdgpu_buffer_rsrc_t rsrc(const T *interior_origin, long base) const {
+    return make_rsrc(interior_origin - base + origin, bytes);
+  }
+};
+
+// load / store of (i+DX, j+dy, k+dz): the x shift goes to the immediate
+#define CPML_LD(r, DX, dy, dz) BufOp<T>::ld(r, g.v0 + (1 + (DX)) * (unsigned)sizeof(T), g.so(kk, dy, dz))
+#define CPML_ST(x, r) BufOp<T>::st(x, r, g.v0 + (unsigned)sizeof(T), g.so(kk, 0, 0))
+
+// Values the stress kernel reads at one plane.
+template <typename T>
+struct SPlane {
+  T vx_p, vy_p, vz_k;                              // vx, vy at k+1; vz at k
+  T vx_xp, vx_yp, vy_ym, vy_xm, vz_xm, vz_yp;      // x/y neighbours at k
+  T cl, cm, sxx, syy, szz, sxy, sxz, syz;
+};
+
+template <typename T>
+__global__ __launch_bounds__(BX *TY3) void k_stress3d(Step3D<T> p) {
+  const Tile3 t = tile3(p.nx, p.ny, p.nz, p.kchunk);
+  if (!t.ok) return;
+  const int i = t.i, j = t.j;
   const int nx = p.nx, ny = p.ny, nz = p.nz;
+  const Chunk3<T> g(p, t);
+  const auto rvx = g.rsrc(p.vx, p.base), rvy = g.rsrc(p.vy, p.base), rvz = g.rsrc(p.vz, p.base);
+  const auto rcl = g.rsrc(p.clam, p.base), rcm = g.rsrc(p.cmu, p.base);
+  const auto rxx = g.rsrc(p.sxx, p.base), ryy = g.rsrc(p.syy, p.base);
+  const auto rzz = g.rsrc(p.szz, p.base), rxy = g.rsrc(p.sxy, p.base);
+  const auto rxz = g.rsrc(p.sxz, p.base), ryz = g.rsrc(p.syz, p.base);
   const bool inx = p.sx.in(i), iny = p.sy.in(j);
-  const long mxo = p.sx.s(i) + (long)p.sx.w * j;  // + k * w*ny
-  const long mxs = (long)p.sx.w * ny;
-  const long myo = i + p.mld * p.sy.s(j);  // + k * mld*wy
-  const long mys = p.mld * p.sy.w;
-  const long mzo = i + p.mld * j;  // + s(k) * mld*ny
-  const long mzs = p.mld * ny;
-  const bool r_n = i <= nx - 2 && j >= 1;  // sigma_ii rows/cols
+  const unsigned mxo = p.sx.s(i) + (unsigned)p.sx.w * j, mxs = (unsigned)p.sx.w * ny;
+  const unsigned myo = i + (unsigned)p.mld * p.sy.s(j), mys = (unsigned)p.mld * p.sy.w;
+  const unsigned mzo = i + (unsigned)p.mld * j, mzs = (unsigned)p.mld * ny;
+  const bool r_n = i <= nx - 2 && j >= 1;
   const bool r_xy = i >= 1 && j <= ny - 2;
   const bool r_xz = i >= 1;
   const bool r_yz = j <= ny - 2;
-  long c = i + j * ld + kb * ps;
-  T vz_m = p.vz[c - ps];
-  T vx_k = p.vx[c], vy_k = p.vy[c];
-  for (int k = kb; k < ke; ++k, c += ps) {
-    const T vx_p = p.vx[c + ps];
-    const T vy_p = p.vy[c + ps];
-    const T vz_k = p.vz[c];
-    const T cl = p.clam[c], cm = p.cmu[c];
+  auto load = [&](SPlane<T> &L, int kk) {
+    L.vx_p = CPML_LD(rvx, 0, 0, 1);
+    L.vy_p = CPML_LD(rvy, 0, 0, 1);
+    L.vz_k = CPML_LD(rvz, 0, 0, 0);
+    L.vx_xp = CPML_LD(rvx, 1, 0, 0);
+    L.vx_yp = CPML_LD(rvx, 0, 1, 0);
+    L.vy_ym = CPML_LD(rvy, 0, -1, 0);
+    L.vy_xm = CPML_LD(rvy, -1, 0, 0);
+    L.vz_xm = CPML_LD(rvz, -1, 0, 0);
+    L.vz_yp = CPML_LD(rvz, 0, 1, 0);
+    L.cl = CPML_LD(rcl, 0, 0, 0);
+    L.cm = CPML_LD(rcm, 0, 0, 0);
+    L.sxx = CPML_LD(rxx, 0, 0, 0);
+    L.syy = CPML_LD(ryy, 0, 0, 0);
+    L.szz = CPML_LD(rzz, 0, 0, 0);
+    L.sxy = CPML_LD(rxy, 0, 0, 0);
+    L.sxz = CPML_LD(rxz, 0, 0, 0);
+    L.syz = CPML_LD(ryz, 0, 0, 0);
+  };
+  int kk = 1;  // buffer plane of k = kb
+  T vz_m = CPML_LD(rvz, 0, 0, -1);
+  T vx_k = CPML_LD(rvx, 0, 0, 0), vy_k = CPML_LD(rvy, 0, 0, 0);
+  SPlane<T> L;
+  load(L, kk);
+  for (int k = t.kb; k < t.ke; ++k, ++kk) {
+    SPlane<T> N = L;
+    if (k + 1 < t.ke) load(N, kk + 1);
     const bool inz = p.sz.in(k);
-    const long mzk = mzo + (long)p.sz.s(k) * mzs;
+    const unsigned mzk = mzo + (unsigned)p.sz.s(k) * mzs;
+    const T cl = L.cl, cm = L.cm;
     // sigma_xx, sigma_yy, sigma_zz: 3D:836-863 (global k >= 2)
     if (r_n && k >= 1) {
-      T dvx_dx = (p.vx[c + 1] - vx_k) * p.odx;
-      T dvy_dy = (vy_k - p.vy[c - ld]) * p.ody;
-      T dvz_dz = (vz_k - vz_m) * p.odz;
+      T dvx_dx = (L.vx_xp - vx_k) * p.odx;
+      T dvy_dy = (vy_k - L.vy_ym) * p.ody;
+      T dvz_dz = (L.vz_k - vz_m) * p.odz;
       if (inx) dvx_dx = cpml(dvx_dx, p.mx[0] + mxo + k * mxs, p.px.ah[i], p.px.bh[i], p.px.Kh[i]);
       if (iny) dvy_dy = cpml(dvy_dy, p.my[0] + myo + k * mys, p.py.a[j], p.py.b[j], p.py.K[j]);
       if (inz) dvz_dz = cpml(dvz_dz, p.mz[0] + mzk, p.pz.a[k], p.pz.b[k], p.pz.K[k]);
       const T lp2m = cl + T(2) * cm;
-      p.sxx[c] = lp2m * dvx_dx + cl * (dvy_dy + dvz_dz) + p.sxx[c];
-      p.syy[c] = cl * (dvx_dx + dvz_dz) + lp2m * dvy_dy + p.syy[c];
-      p.szz[c] = cl * (dvx_dx + dvy_dy) + lp2m * dvz_dz + p.szz[c];
+      CPML_ST(lp2m * dvx_dx + cl * (dvy_dy + dvz_dz) + L.sxx, rxx);
+      CPML_ST(cl * (dvx_dx + dvz_dz) + lp2m * dvy_dy + L.syy, ryy);
+      CPML_ST(cl * (dvx_dx + dvy_dy) + lp2m * dvz_dz + L.szz, rzz);
     }
     // sigma_xy: 3D:877-894
     if (r_xy) {
-      T dvy_dx = (vy_k - p.vy[c - 1]) * p.odx;
-      T dvx_dy = (p.vx[c + ld] - vx_k) * p.ody;
+      T dvy_dx = (vy_k - L.vy_xm) * p.odx;
+      T dvx_dy = (L.vx_yp - vx_k) * p.ody;
       if (inx) dvy_dx = cpml(dvy_dx, p.mx[1] + mxo + k * mxs, p.px.a[i], p.px.b[i], p.px.K[i]);
       if (iny) dvx_dy = cpml(dvx_dy, p.my[1] + myo + k * mys, p.py.ah[j], p.py.bh[j], p.py.Kh[j]);
-      p.sxy[c] = cm * (dvy_dx + dvx_dy) + p.sxy[c];
+      CPML_ST(cm * (dvy_dx + dvx_dy) + L.sxy, rxy);
     }
     // sigma_xz: 3D:908-925 (global k <= NZ-1)
     if (r_xz && k <= nz - 2) {
-      T dvz_dx = (vz_k - p.vz[c - 1]) * p.odx;
-      T dvx_dz = (vx_p - vx_k) * p.odz;
+      T dvz_dx = (L.vz_k - L.vz_xm) * p.odx;
+      T dvx_dz = (L.vx_p - vx_k) * p.odz;
       if (inx) dvz_dx = cpml(dvz_dx, p.mx[2] + mxo + k * mxs, p.px.a[i], p.px.b[i], p.px.K[i]);
       if (inz) dvx_dz = cpml(dvx_dz, p.mz[1] + mzk, p.pz.ah[k], p.pz.bh[k], p.pz.Kh[k]);
-      p.sxz[c] = cm * (dvz_dx + dvx_dz) + p.sxz[c];
+      CPML_ST(cm * (dvz_dx + dvx_dz) + L.sxz, rxz);
     }
     // sigma_yz: 3D:927-943
     if (r_yz && k <= nz - 2) {
-      T dvz_dy = (p.vz[c + ld] - vz_k) * p.ody;
-      T dvy_dz = (vy_p - vy_k) * p.odz;
+      T dvz_dy = (L.vz_yp - L.vz_k) * p.ody;
+      T dvy_dz = (L.vy_p - vy_k) * p.odz;
       if (iny) dvz_dy = cpml(dvz_dy, p.my[2] + myo + k * mys, p.py.ah[j], p.py.bh[j], p.py.Kh[j]);
       if (inz) dvy_dz = cpml(dvy_dz, p.mz[2] + mzk, p.pz.ah[k], p.pz.bh[k], p.pz.Kh[k]);
-      p.syz[c] = cm * (dvz_dy + dvy_dz) + p.syz[c];
+      CPML_ST(cm * (dvz_dy + dvy_dz) + L.syz, ryz);
     }
-    vz_m = vz_k;
-    vx_k = vx_p;
-    vy_k = vy_p;
+    vz_m = L.vz_k;
+    vx_k = L.vx_p;
+    vy_k = L.vy_p;
+    L = N;
   }
 }
 
+// Values the velocity kernel reads at one plane.
 template <typename T>
-__global__ __launch_bounds__(BX *BY) void k_velocity3d(Step3D<T> p) {
-  const int i = blockIdx.x * BX + threadIdx.x;
-  const int j = blockIdx.y * BY + threadIdx.y;
-  if (i >= p.nx || j >= p.ny) return;
-  const int kb = blockIdx.z * p.kchunk;
-  const int ke = min(kb + p.kchunk, p.nz);
-  const long ld = p.ld, ps = p.ps;
+struct VPlane {
+  T szz_p, sxz_k, syz_k;                           // szz at k+1; sxz, syz at k
+  T sxx_k, sxx_xm, sxy_k, sxy_ym, sxy_xp, syy_k, syy_yp, sxz_xp, syz_ym;
+  T cb, vx, vy, vz;
+};
+
+template <typename T>
+__global__ __launch_bounds__(BX *TY3) void k_velocity3d(Step3D<T> p) {
+  const Tile3 t = tile3(p.nx, p.ny, p.nz, p.kchunk);
+  if (!t.ok) return;
+  const int i = t.i, j = t.j;
   const int nx = p.nx, ny = p.ny, nz = p.nz;
+  const Chunk3<T> g(p, t);
+  const auto rxx = g.rsrc(p.sxx, p.base), ryy = g.rsrc(p.syy, p.base);
+  const auto rzz = g.rsrc(p.szz, p.base), rxy = g.rsrc(p.sxy, p.base);
+  const auto rxz = g.rsrc(p.sxz, p.base), ryz = g.rsrc(p.syz, p.base);
+  const auto rcb = g.rsrc(p.cbuoy, p.base);
+  const auto rvx = g.rsrc(p.vx, p.base), rvy = g.rsrc(p.vy, p.base), rvz = g.rsrc(p.vz, p.base);
   const bool inx = p.sx.in(i), iny = p.sy.in(j);
-  const long mxo = p.sx.s(i) + (long)p.sx.w * j;
-  const long mxs = (long)p.sx.w * ny;
-  const long myo = i + p.mld * p.sy.s(j);
-  const long mys = p.mld * p.sy.w;
-  const long mzo = i + p.mld * j;
-  const long mzs = p.mld * ny;
+  const unsigned mxo = p.sx.s(i) + (unsigned)p.sx.w * j, mxs = (unsigned)p.sx.w * ny;
+  const unsigned myo = i + (unsigned)p.mld * p.sy.s(j), mys = (unsigned)p.mld * p.sy.w;
+  const unsigned mzo = i + (unsigned)p.mld * j, mzs = (unsigned)p.mld * ny;
   const bool r_vx = i >= 1 && j >= 1;
   const bool r_vy = i <= nx - 2 && j <= ny - 2;
   const bool r_vz = i <= nx - 2 && j >= 1;
   const bool edge_xy = i == 0 || i == nx - 1 || j == 0 || j == ny - 1;
   const bool src_col = i == p.isrc && j == p.jsrc;
-  long c = i + j * ld + kb * ps;
-  T sxz_m = p.sxz[c - ps], syz_m = p.syz[c - ps];
-  T szz_k = p.szz[c];
-  for (int k = kb; k < ke; ++k, c += ps) {
-    const T szz_p = p.szz[c + ps];
-    const T sxz_k = p.sxz[c], syz_k = p.syz[c];
-    const T cb = p.cbuoy[c];
+  auto load = [&](VPlane<T> &L, int kk) {
+    L.szz_p = CPML_LD(rzz, 0, 0, 1);
+    L.sxz_k = CPML_LD(rxz, 0, 0, 0);
+    L.syz_k = CPML_LD(ryz, 0, 0, 0);
+    L.sxx_k = CPML_LD(rxx, 0, 0, 0);
+    L.sxx_xm = CPML_LD(rxx, -1, 0, 0);
+    L.sxy_k = CPML_LD(rxy, 0, 0, 0);
+    L.sxy_ym = CPML_LD(rxy, 0, -1, 0);
+    L.sxy_xp = CPML_LD(rxy, 1, 0, 0);
+    L.syy_k = CPML_LD(ryy, 0, 0, 0);
+    L.syy_yp = CPML_LD(ryy, 0, 1, 0);
+    L.sxz_xp = CPML_LD(rxz, 1, 0, 0);
+    L.syz_ym = CPML_LD(ryz, 0, -1, 0);
+    L.cb = CPML_LD(rcb, 0, 0, 0);
+    L.vx = CPML_LD(rvx, 0, 0, 0);
+    L.vy = CPML_LD(rvy, 0, 0, 0);
+    L.vz = CPML_LD(rvz, 0, 0, 0);
+  };
+  int kk = 1;
+  T sxz_m = CPML_LD(rxz, 0, 0, -1), syz_m = CPML_LD(ryz, 0, 0, -1);
+  T szz_k = CPML_LD(rzz, 0, 0, 0);
+  VPlane<T> L;
+  load(L, kk);
+  for (int k = t.kb; k < t.ke; ++k, ++kk) {
+    VPlane<T> N = L;
+    if (k + 1 < t.ke) load(N, kk + 1);
     const bool inz = p.sz.in(k);
-    const long mzk = mzo + (long)p.sz.s(k) * mzs;
-    T vxn = p.vx[c], vyn = p.vy[c], vzn = p.vz[c];
+    const unsigned mzk = mzo + (unsigned)p.sz.s(k) * mzs;
+    const T cb = L.cb;
+    T vxn = L.vx, vyn = L.vy, vzn = L.vz;
     // vx: 3D:976-993 (global k >= 2)
     if (r_vx && k >= 1) {
-      T dsxx_dx = (p.sxx[c] - p.sxx[c - 1]) * p.odx;
-      T dsxy_dy = (p.sxy[c] - p.sxy[c - ld]) * p.ody;
-      T dsxz_dz = (sxz_k - sxz_m) * p.odz;
+      T dsxx_dx = (L.sxx_k - L.sxx_xm) * p.odx;
+      T dsxy_dy = (L.sxy_k - L.sxy_ym) * p.ody;
+      T dsxz_dz = (L.sxz_k - sxz_m) * p.odz;
       if (inx) dsxx_dx = cpml(dsxx_dx, p.mx[3] + mxo + k * mxs, p.px.a[i], p.px.b[i], p.px.K[i]);
       if (iny) dsxy_dy = cpml(dsxy_dy, p.my[3] + myo + k * mys, p.py.a[j], p.py.b[j], p.py.K[j]);
       if (inz) dsxz_dz = cpml(dsxz_dz, p.mz[3] + mzk, p.pz.a[k], p.pz.b[k], p.pz.K[k]);
@@ -264,9 +418,9 @@ __global__ __launch_bounds__(BX *BY) void k_velocity3d(Step3D<T> p) {
     }
     // vy: 3D:995-1017
     if (r_vy && k >= 1) {
-      T dsxy_dx = (p.sxy[c + 1] - p.sxy[c]) * p.odx;
-      T dsyy_dy = (p.syy[c + ld] - p.syy[c]) * p.ody;
-      T dsyz_dz = (syz_k - syz_m) * p.odz;
+      T dsxy_dx = (L.sxy_xp - L.sxy_k) * p.odx;
+      T dsyy_dy = (L.syy_yp - L.syy_k) * p.ody;
+      T dsyz_dz = (L.syz_k - syz_m) * p.odz;
       if (inx) dsxy_dx = cpml(dsxy_dx, p.mx[4] + mxo + k * mxs, p.px.ah[i], p.px.bh[i], p.px.Kh[i]);
       if (iny) dsyy_dy = cpml(dsyy_dy, p.my[4] + myo + k * mys, p.py.ah[j], p.py.bh[j], p.py.Kh[j]);
       if (inz) dsyz_dz = cpml(dsyz_dz, p.mz[4] + mzk, p.pz.a[k], p.pz.b[k], p.pz.K[k]);
@@ -274,9 +428,9 @@ __global__ __launch_bounds__(BX *BY) void k_velocity3d(Step3D<T> p) {
     }
     // vz: 3D:1031-1052 (global k <= NZ-1)
     if (r_vz && k <= nz - 2) {
-      T dsxz_dx = (p.sxz[c + 1] - sxz_k) * p.odx;
-      T dsyz_dy = (syz_k - p.syz[c - ld]) * p.ody;
-      T dszz_dz = (szz_p - szz_k) * p.odz;
+      T dsxz_dx = (L.sxz_xp - L.sxz_k) * p.odx;
+      T dsyz_dy = (L.syz_k - L.syz_ym) * p.ody;
+      T dszz_dz = (L.szz_p - szz_k) * p.odz;
       if (inx) dsxz_dx = cpml(dsxz_dx, p.mx[5] + mxo + k * mxs, p.px.ah[i], p.px.bh[i], p.px.Kh[i]);
       if (iny) dsyz_dy = cpml(dsyz_dy, p.my[5] + myo + k * mys, p.py.a[j], p.py.b[j], p.py.K[j]);
       if (inz) dszz_dz = cpml(dszz_dz, p.mz[5] + mzk, p.pz.ah[k], p.pz.bh[k], p.pz.Kh[k]);
@@ -285,8 +439,9 @@ __global__ __launch_bounds__(BX *BY) void k_velocity3d(Step3D<T> p) {
     // point force on the cut plane, no density average (3D:1074-1082)
     if (src_col && k == p.ksrc) {
       const int it = *p.step;
-      vxn = vxn + p.fx[it - 1] * p.dt / p.rho[c];
-      vyn = vyn + p.fy[it - 1] * p.dt / p.rho[c];
+      const T rs = p.rho[i + (long)j * p.ld + (long)k * p.ps];
+      vxn = vxn + p.fx[it - 1] * p.dt / rs;
+      vyn = vyn + p.fy[it - 1] * p.dt / rs;
     }
     // six rigid faces (3D:1087-1121)
     if (edge_xy || k == 0 || k == nz - 1) {
@@ -294,14 +449,18 @@ __global__ __launch_bounds__(BX *BY) void k_velocity3d(Step3D<T> p) {
       vyn = T(0);
       vzn = T(0);
     }
-    p.vx[c] = vxn;
-    p.vy[c] = vyn;
-    p.vz[c] = vzn;
-    sxz_m = sxz_k;
-    syz_m = syz_k;
-    szz_k = szz_p;
+    CPML_ST(vxn, rvx);
+    CPML_ST(vyn, rvy);
+    CPML_ST(vzn, rvz);
+    sxz_m = L.sxz_k;
+    syz_m = L.syz_k;
+    szz_k = L.szz_p;
+    L = N;
   }
 }
+
+#undef CPML_LD
+#undef CPML_ST
 
 // ------------------------------------------------------ receivers, step --
 
@@ -466,16 +625,20 @@ hipError_t launch_velocity2d(int order, const Step2D<T> &p, hipStream_t s) {
 }
 
 template <typename T>
+unsigned ntiles3(const Step3D<T> &p) {
+  return (unsigned)((p.nx + BX - 1) / BX) * ((p.ny + TY3 - 1) / TY3) *
+         ((p.nz + p.kchunk - 1) / p.kchunk);
+}
+
+template <typename T>
 hipError_t launch_stress3d(const Step3D<T> &p, hipStream_t s) {
-  dim3 grid((p.nx + BX - 1) / BX, (p.ny + BY - 1) / BY, (p.nz + p.kchunk - 1) / p.kchunk);
-  hipLaunchKernelGGL(k_stress3d<T>, grid, dim3(BX, BY), 0, s, p);
+  hipLaunchKernelGGL(k_stress3d<T>, dim3(ntiles3(p)), dim3(BX, TY3), 0, s, p);
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_velocity3d(const Step3D<T> &p, hipStream_t s) {
-  dim3 grid((p.nx + BX - 1) / BX, (p.ny + BY - 1) / BY, (p.nz + p.kchunk - 1) / p.kchunk);
-  hipLaunchKernelGGL(k_velocity3d<T>, grid, dim3(BX, BY), 0, s, p);
+  hipLaunchKernelGGL(k_velocity3d<T>, dim3(ntiles3(p)), dim3(BX, TY3), 0, s, p);
   return hipGetLastError();
 }
 
