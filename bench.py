@@ -75,6 +75,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=768, help="grid points per axis")
     ap.add_argument("--kchunk", type=int, default=0)
+    ap.add_argument("--tile", type=int, default=0, help="rows per 3D tile (4, 8, 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -101,6 +102,8 @@ def main():
     s = w.make_solver()
     if args.kchunk:
         s.set_kchunk(args.kchunk)
+    if args.tile:
+        s.set_tile(args.tile)
     slabs = s.slab_widths()
     if args.warmup > 0:
         s.run(1, args.warmup)

@@ -133,8 +133,10 @@ int cpml_gpu_velocity_norm(cpml_solver *h, double *vmax);
 
 /* Steps captured per HIP graph replay (default 16; 0 = launch eagerly). */
 int cpml_gpu_set_graph_steps(cpml_solver *h, int steps);
-/* Planes per block along z for the 3D kernels (default 64). */
+/* 3D tiling: planes per block along z (default 8) and rows of 64 points per
+ * block (4, 8 or 16; default 4).  Results do not depend on either. */
 int cpml_gpu_set_kchunk(cpml_solver *h, int kchunk);
+int cpml_gpu_set_tile(cpml_solver *h, int tile_y);
 /* When on, run() launches eagerly with a HIP event pair around every stress
  * and velocity kernel on the handle's stream; see cpml_gpu_kernel_time. */
 int cpml_gpu_set_kernel_timing(cpml_solver *h, int on);

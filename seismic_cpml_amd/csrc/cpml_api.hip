@@ -141,7 +141,8 @@ struct cpml_solver {
   int graph_steps = 16;
   hipGraphExec_t gexec = nullptr;
   bool graph_dirty = true;
-  int kchunk = 64;
+  int kchunk = 8;
+  int tile_y = 4;
   bool ktiming = false;
   std::vector<hipEvent_t> evpool;
   double ktime[2] = {0, 0};
@@ -325,6 +326,7 @@ cpml::Step3D<T> step3d(cpml_solver *h) {
   p.fy = static_cast<const T *>(h->fy);
   p.step = h->step;
   p.kchunk = h->kchunk;
+  p.tile_y = h->tile_y;
   return p;
 }
 
@@ -1172,6 +1174,15 @@ int cpml_gpu_set_kchunk(cpml_solver *h, int kchunk) {
   if (!h) return CPML_ERR_ARG;
   if (kchunk < 1) return fail(h, CPML_ERR_ARG, "kchunk must be >= 1");
   h->kchunk = kchunk;
+  h->graph_dirty = true;
+  return CPML_OK;
+}
+
+int cpml_gpu_set_tile(cpml_solver *h, int tile_y) {
+  if (!h) return CPML_ERR_ARG;
+  if (tile_y != 4 && tile_y != 8 && tile_y != 16)
+    return fail(h, CPML_ERR_ARG, "tile rows must be 4, 8 or 16");
+  h->tile_y = tile_y;
   h->graph_dirty = true;
   return CPML_OK;
 }

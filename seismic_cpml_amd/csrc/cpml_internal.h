@@ -88,6 +88,7 @@ struct Step3D {
   const T *fx, *fy;
   const int *step;
   int kchunk;  // planes per block along z
+  int tile_y;  // rows per block (4, 8 or 16)
 };
 
 // Receivers / step counter: one block.

@@ -155,8 +155,6 @@ __global__ __launch_bounds__(BX *BY) void k_velocity2d(Step2D<T> p) {
 // pointers, and the loads of plane k+1 are issued before the arithmetic and
 // stores of plane k (register double buffering).
 
-constexpr int TY3 = 4;
-
 struct Tile3 {
   int i, j, kb, ke;
   bool ok;
@@ -165,6 +163,7 @@ struct Tile3 {
 // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD one
 // contiguous band of tiles (x fastest, then y, then z chunk) so the tiles that
 // read each other's halo rows run on the same XCD and share its L2.
+template <int TY3>
 __device__ __forceinline__ Tile3 tile3(int nx, int ny, int nz, int kchunk) {
   const int ntx = (nx + BX - 1) / BX, nty = (ny + TY3 - 1) / TY3;
   const int ntz = (nz + kchunk - 1) / kchunk;
@@ -253,9 +252,9 @@ struct SPlane {
   T cl, cm, sxx, syy, szz, sxy, sxz, syz;
 };
 
-template <typename T>
+template <typename T, int TY3>
 __global__ __launch_bounds__(BX *TY3) void k_stress3d(Step3D<T> p) {
-  const Tile3 t = tile3(p.nx, p.ny, p.nz, p.kchunk);
+  const Tile3 t = tile3<TY3>(p.nx, p.ny, p.nz, p.kchunk);
   if (!t.ok) return;
   const int i = t.i, j = t.j;
   const int nx = p.nx, ny = p.ny, nz = p.nz;
@@ -355,9 +354,9 @@ struct VPlane {
   T cb, vx, vy, vz;
 };
 
-template <typename T>
+template <typename T, int TY3>
 __global__ __launch_bounds__(BX *TY3) void k_velocity3d(Step3D<T> p) {
-  const Tile3 t = tile3(p.nx, p.ny, p.nz, p.kchunk);
+  const Tile3 t = tile3<TY3>(p.nx, p.ny, p.nz, p.kchunk);
   if (!t.ok) return;
   const int i = t.i, j = t.j;
   const int nx = p.nx, ny = p.ny, nz = p.nz;
@@ -625,20 +624,31 @@ hipError_t launch_velocity2d(int order, const Step2D<T> &p, hipStream_t s) {
 }
 
 template <typename T>
-unsigned ntiles3(const Step3D<T> &p) {
-  return (unsigned)((p.nx + BX - 1) / BX) * ((p.ny + TY3 - 1) / TY3) *
+unsigned ntiles3(const Step3D<T> &p, int ty) {
+  return (unsigned)((p.nx + BX - 1) / BX) * ((p.ny + ty - 1) / ty) *
          ((p.nz + p.kchunk - 1) / p.kchunk);
 }
 
+// p.tile_y selects the tile height (4, 8 or 16 rows of 64 points)
 template <typename T>
 hipError_t launch_stress3d(const Step3D<T> &p, hipStream_t s) {
-  hipLaunchKernelGGL(k_stress3d<T>, dim3(ntiles3(p)), dim3(BX, TY3), 0, s, p);
+  if (p.tile_y == 16)
+    hipLaunchKernelGGL((k_stress3d<T, 16>), dim3(ntiles3(p, 16)), dim3(BX, 16), 0, s, p);
+  else if (p.tile_y == 8)
+    hipLaunchKernelGGL((k_stress3d<T, 8>), dim3(ntiles3(p, 8)), dim3(BX, 8), 0, s, p);
+  else
+    hipLaunchKernelGGL((k_stress3d<T, 4>), dim3(ntiles3(p, 4)), dim3(BX, 4), 0, s, p);
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_velocity3d(const Step3D<T> &p, hipStream_t s) {
-  hipLaunchKernelGGL(k_velocity3d<T>, dim3(ntiles3(p)), dim3(BX, TY3), 0, s, p);
+  if (p.tile_y == 16)
+    hipLaunchKernelGGL((k_velocity3d<T, 16>), dim3(ntiles3(p, 16)), dim3(BX, 16), 0, s, p);
+  else if (p.tile_y == 8)
+    hipLaunchKernelGGL((k_velocity3d<T, 8>), dim3(ntiles3(p, 8)), dim3(BX, 8), 0, s, p);
+  else
+    hipLaunchKernelGGL((k_velocity3d<T, 4>), dim3(ntiles3(p, 4)), dim3(BX, 4), 0, s, p);
   return hipGetLastError();
 }
 

@@ -127,12 +127,12 @@ def test_run_in_chunks_equals_one_run():
     np.testing.assert_array_equal(whole, parts)
 
 
-@pytest.mark.parametrize("kchunk", [1, 7, 64])
-def test_3d_kchunk_does_not_change_results(kchunk):
+@pytest.mark.parametrize("kchunk,tile", [(1, 4), (7, 8), (64, 16), (5, 16)])
+def test_3d_tiling_does_not_change_results(kchunk, tile):
     case, prec = "3d_small", "f32"
     d = inputs(case, prec, 80)
     ref = run_gpu(case, prec, d=d, energy=False)
-    g = run_gpu(case, prec, d=d, energy=False, kchunk=kchunk)
+    g = run_gpu(case, prec, d=d, energy=False, kchunk=kchunk, tile=tile)
     for k in _fields(case):
         np.testing.assert_array_equal(g[k], ref[k])
 
