@@ -76,6 +76,8 @@ def main():
     ap.add_argument("--n", type=int, default=768, help="grid points per axis")
     ap.add_argument("--kchunk", type=int, default=0)
     ap.add_argument("--tile", type=int, default=0, help="rows per 3D tile (4, 8, 16)")
+    ap.add_argument("--two-phase", action="store_true",
+                    help="separate stress and velocity kernels instead of the fused step")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -102,6 +104,8 @@ def main():
     s = w.make_solver()
     if args.kchunk:
         s.set_kchunk(args.kchunk)
+    if not args.two_phase:
+        s.set_fused(True)
     if args.tile:
         s.set_tile(args.tile)
     slabs = s.slab_widths()

@@ -51,7 +51,7 @@ enum {
   CPML_MEM_DSIGMAYY_DY, CPML_MEM_DSIGMAZZ_DZ, CPML_MEM_DSIGMAXY_DX, CPML_MEM_DSIGMAXY_DY,
   CPML_MEM_DSIGMAXZ_DX, CPML_MEM_DSIGMAXZ_DZ, CPML_MEM_DSIGMAYZ_DY, CPML_MEM_DSIGMAYZ_DZ
 };
-/* kernels timed by cpml_gpu_kernel_time */
+/* kernels timed by cpml_gpu_kernel_time (the fused 3D step counts as STRESS) */
 enum { CPML_KERNEL_STRESS = 0, CPML_KERNEL_VELOCITY = 1 };
 
 /* ---- lifecycle ------------------------------------------------------- */
@@ -137,6 +137,11 @@ int cpml_gpu_set_graph_steps(cpml_solver *h, int steps);
  * block (4, 8 or 16; default 4).  Results do not depend on either. */
 int cpml_gpu_set_kchunk(cpml_solver *h, int kchunk);
 int cpml_gpu_set_tile(cpml_solver *h, int tile_y);
+/* 3D only: run each time step as ONE fused kernel (stress and velocity in a
+ * single pass over HBM, state kept in two ping-pong buffers: 9 more field
+ * arrays).  Results are identical to the two-kernel path.  Tile rows 8 or 16
+ * (set_tile; 4 selects 8). */
+int cpml_gpu_set_fused(cpml_solver *h, int on);
 /* When on, run() launches eagerly with a HIP event pair around every stress
  * and velocity kernel on the handle's stream; see cpml_gpu_kernel_time. */
 int cpml_gpu_set_kernel_timing(cpml_solver *h, int on);

@@ -120,9 +120,13 @@ struct cpml_solver {
   double bytes = 0;
   // device arrays (padded arrays point at their allocation start)
   void *field[9] = {};  // vx vy vz sxx syy szz sxy sxz syz
+  void *field2[9] = {}; // second state buffer of the fused 3D step
+  int cur = 0;          // 1: the state lives in field2 / mem2
+  bool fused = false;   // 3D: one fused kernel per step (ping-pong state)
   void *media[6] = {};  // lambda mu rho, and in 3D: DELTAT*lambda DELTAT*mu DELTAT/rho
   void *prof = nullptr; // [3 axes][6][n]
   void *mem[18] = {};
+  void *mem2[18] = {};
   size_t mem_elems[18] = {};
   void *fx = nullptr, *fy = nullptr;
   int *step = nullptr;
@@ -139,7 +143,7 @@ struct cpml_solver {
   int it_display = 0;
   double threshold = 1e25;
   int graph_steps = 16;
-  hipGraphExec_t gexec = nullptr;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};  // per starting state buffer
   bool graph_dirty = true;
   int kchunk = 8;
   int tile_y = 4;
@@ -198,9 +202,12 @@ int set_device(cpml_solver *h) {
   return CPML_OK;
 }
 
+void *fbuf(cpml_solver *h, int q) { return (h->cur ? h->field2 : h->field)[q]; }
+void *mbuf(cpml_solver *h, int w) { return (h->cur ? h->mem2 : h->mem)[w]; }
+
 template <typename T>
 T *F(cpml_solver *h, int q) {
-  return static_cast<T *>(h->field[q]) + h->base;
+  return static_cast<T *>(fbuf(h, q)) + h->base;
 }
 template <typename T>
 T *M(cpml_solver *h, int q) {
@@ -263,8 +270,8 @@ cpml::Step2D<T> step2d(cpml_solver *h) {
   p.mu = M<T>(h, 1);
   p.rho = M<T>(h, 2);
   for (int q = 0; q < 4; ++q) {
-    p.mx[q] = static_cast<T *>(h->mem[kSlot2Dx[q]]);
-    p.my[q] = static_cast<T *>(h->mem[kSlot2Dy[q]]);
+    p.mx[q] = static_cast<T *>(mbuf(h, kSlot2Dx[q]));
+    p.my[q] = static_cast<T *>(mbuf(h, kSlot2Dy[q]));
   }
   p.px = prof_of<T>(h, 0);
   p.py = prof_of<T>(h, 1);
@@ -295,9 +302,9 @@ cpml::Step3D<T> step3d(cpml_solver *h) {
   p.cbuoy = M<T>(h, 5);
   p.rho = M<T>(h, 2);
   for (int q = 0; q < 6; ++q) {
-    p.mx[q] = static_cast<T *>(h->mem[kSlot3Dx[q]]);
-    p.my[q] = static_cast<T *>(h->mem[kSlot3Dy[q]]);
-    p.mz[q] = static_cast<T *>(h->mem[kSlot3Dz[q]]);
+    p.mx[q] = static_cast<T *>(mbuf(h, kSlot3Dx[q]));
+    p.my[q] = static_cast<T *>(mbuf(h, kSlot3Dy[q]));
+    p.mz[q] = static_cast<T *>(mbuf(h, kSlot3Dz[q]));
   }
   p.px = prof_of<T>(h, 0);
   p.py = prof_of<T>(h, 1);
@@ -314,6 +321,7 @@ cpml::Step3D<T> step3d(cpml_solver *h) {
   p.mps = h->mld * h->ny;
   p.base = h->base;
   p.xpad = h->xpad;
+  p.R = h->R;
   // 3D:134-137 ONE_OVER_DELTA = 1/DELTA, applied by multiplication (3D:841)
   p.odx = T(1) / T(h->dx);
   p.ody = T(1) / T(h->dy);
@@ -351,7 +359,7 @@ template <typename T>
 cpml::Energy<T> energy_of(cpml_solver *h) {
   cpml::Energy<T> e{};
   for (int q = 0; q < 9; ++q)
-    (&e.vx)[q] = h->field[q] ? F<T>(h, q) : nullptr;
+    (&e.vx)[q] = fbuf(h, q) ? F<T>(h, q) : nullptr;
   e.lam = M<T>(h, 0);
   e.mu = M<T>(h, 1);
   e.rho = M<T>(h, 2);
@@ -370,6 +378,30 @@ cpml::Energy<T> energy_of(cpml_solver *h) {
   e.out_kin = h->ekin;
   e.out_pot = h->epot;
   e.step = h->step;
+  return e;
+}
+
+template <typename T>
+cpml::Fused3D<T> fused_of(cpml_solver *h) {
+  cpml::Fused3D<T> P{};
+  P.a = step3d<T>(h);
+  void **nf = h->cur ? h->field : h->field2;
+  void **nm = h->cur ? h->mem : h->mem2;
+  for (int q = 0; q < 9; ++q) (&P.vx)[q] = static_cast<T *>(nf[q]) + h->base;
+  for (int q = 0; q < 6; ++q) {
+    P.mx[q] = static_cast<T *>(nm[kSlot3Dx[q]]);
+    P.my[q] = static_cast<T *>(nm[kSlot3Dy[q]]);
+    P.mz[q] = static_cast<T *>(nm[kSlot3Dz[q]]);
+  }
+  return P;
+}
+
+// The fused step writes the other state buffer; the host flips `cur` as the
+// launch is enqueued, so later launches (and captures) read the new state.
+template <typename T>
+hipError_t enqueue_fused(cpml_solver *h) {
+  hipError_t e = cpml::launch_fused3d<T>(fused_of<T>(h), h->stream);
+  h->cur ^= 1;
   return e;
 }
 
@@ -395,36 +427,52 @@ hipError_t enqueue_tail(cpml_solver *h) {
 
 template <typename T>
 hipError_t enqueue_step(cpml_solver *h) {
-  hipError_t e = enqueue_stress<T>(h);
-  if (e == hipSuccess) e = enqueue_velocity<T>(h);
+  hipError_t e;
+  if (h->fused) {
+    e = enqueue_fused<T>(h);
+  } else {
+    e = enqueue_stress<T>(h);
+    if (e == hipSuccess) e = enqueue_velocity<T>(h);
+  }
   if (e == hipSuccess) e = enqueue_tail<T>(h);
   return e;
+}
+
+int graph_len(const cpml_solver *h) {
+  // an even number of fused steps returns the state to the buffer it started in
+  return h->fused ? (h->graph_steps + 1) / 2 * 2 : h->graph_steps;
 }
 
 hipError_t enqueue_step_any(cpml_solver *h) {
   return h->prec == 4 ? enqueue_step<float>(h) : enqueue_step<double>(h);
 }
 
+void drop_graphs(cpml_solver *h) {
+  for (auto &g : h->gexec)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+// Capture graph_len() steps starting from the current state buffer.
 int build_graph(cpml_solver *h) {
-  if (h->gexec) {
-    (void)hipGraphExecDestroy(h->gexec);
-    h->gexec = nullptr;
-  }
+  const int slot = h->cur;
   hipGraph_t g = nullptr;
   HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   hipError_t e = hipSuccess;
-  for (int s = 0; s < h->graph_steps && e == hipSuccess; ++s) e = enqueue_step_any(h);
+  for (int s = 0; s < graph_len(h) && e == hipSuccess; ++s) e = enqueue_step_any(h);
+  h->cur = slot;  // capture does not execute: the state is still where it was
   hipError_t e2 = hipStreamEndCapture(h->stream, &g);
   if (e != hipSuccess || e2 != hipSuccess) {
     if (g) (void)hipGraphDestroy(g);
     return fail(h, CPML_ERR_HIP,
                 std::string("graph capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
   }
-  e = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(&h->gexec[slot], g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) return fail(h, CPML_ERR_HIP, std::string("hipGraphInstantiate: ") +
                                                         hipGetErrorString(e));
-  h->graph_dirty = false;
   return CPML_OK;
 }
 
@@ -446,23 +494,33 @@ int enqueue_steps(cpml_solver *h, int n, int *ev_used) {
     for (int s = 0; s < n; ++s) {
       hipEvent_t *e = &h->evpool[*ev_used];
       HIPCHK(h, hipEventRecord(e[0], h->stream));
-      HIPCHK(h, h->prec == 4 ? enqueue_stress<float>(h) : enqueue_stress<double>(h));
-      HIPCHK(h, hipEventRecord(e[1], h->stream));
-      HIPCHK(h, h->prec == 4 ? enqueue_velocity<float>(h) : enqueue_velocity<double>(h));
+      if (h->fused) {
+        HIPCHK(h, h->prec == 4 ? enqueue_fused<float>(h) : enqueue_fused<double>(h));
+        HIPCHK(h, hipEventRecord(e[1], h->stream));
+      } else {
+        HIPCHK(h, h->prec == 4 ? enqueue_stress<float>(h) : enqueue_stress<double>(h));
+        HIPCHK(h, hipEventRecord(e[1], h->stream));
+        HIPCHK(h, h->prec == 4 ? enqueue_velocity<float>(h) : enqueue_velocity<double>(h));
+      }
       HIPCHK(h, hipEventRecord(e[2], h->stream));
       HIPCHK(h, h->prec == 4 ? enqueue_tail<float>(h) : enqueue_tail<double>(h));
       *ev_used += 3;
     }
     return CPML_OK;
   }
-  if (h->graph_steps > 0 && n >= h->graph_steps) {
-    if (h->graph_dirty || !h->gexec) {
+  const int gl = graph_len(h);
+  if (gl > 0 && n >= gl) {
+    if (h->graph_dirty) {
+      drop_graphs(h);
+      h->graph_dirty = false;
+    }
+    if (!h->gexec[h->cur]) {
       int rc = build_graph(h);
       if (rc) return rc;
     }
-    while (n >= h->graph_steps) {
-      HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
-      n -= h->graph_steps;
+    while (n >= gl) {
+      HIPCHK(h, hipGraphLaunch(h->gexec[h->cur], h->stream));
+      n -= gl;  // gl is even when fused: the state returns to the same buffer
     }
   }
   for (int s = 0; s < n; ++s) HIPCHK(h, enqueue_step_any(h));
@@ -703,7 +761,7 @@ int cpml_gpu_init(cpml_solver **hp, int ndim, int order, int prec, int NX, int N
   h->npml = NPOINTS_PML;
   h->nstep = NSTEP;
   h->esize = prec;
-  h->R = order / 2;
+  h->R = ndim == 3 ? 2 : order / 2;  // 3D: two halo cells (the fused step reads i-2, j-2, k-2)
   const long align = 256 / (long)h->esize;  // elements per 256 bytes
   h->xpad = align;
   h->ld = round_up(h->xpad + NX + h->R, align);
@@ -779,7 +837,8 @@ int cpml_gpu_finalize(cpml_solver *h) {
   if (!h) return CPML_OK;
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  for (auto g : h->gexec)
+    if (g) (void)hipGraphExecDestroy(g);
   for (hipEvent_t e : h->evpool) (void)hipEventDestroy(e);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -909,8 +968,10 @@ int cpml_gpu_set_pml(cpml_solver *h, const void *a_x, const void *b_x, const voi
       default: elems = (size_t)h->mld * ny * h->sz.w; break;
     }
     dev_free(h, h->mem[w]);
+    dev_free(h, h->mem2[w]);
     h->mem_elems[w] = elems;
     if ((rc = dev_alloc(h, &h->mem[w], elems * h->esize))) return rc;
+    if (h->fused && (rc = dev_alloc(h, &h->mem2[w], elems * h->esize))) return rc;
   }
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->have_pml = true;
@@ -1062,11 +1123,13 @@ int cpml_gpu_run(cpml_solver *h, int it_begin, int it_end) {
   for (int s = 0; s + 2 < ev_used; s += 3) {
     float a = 0, b = 0;
     HIPCHK(h, hipEventElapsedTime(&a, h->evpool[s], h->evpool[s + 1]));
-    HIPCHK(h, hipEventElapsedTime(&b, h->evpool[s + 1], h->evpool[s + 2]));
     h->ktime[0] += a;
-    h->ktime[1] += b;
     h->kcount[0] += 1;
-    h->kcount[1] += 1;
+    if (!h->fused) {
+      HIPCHK(h, hipEventElapsedTime(&b, h->evpool[s + 1], h->evpool[s + 2]));
+      h->ktime[1] += b;
+      h->kcount[1] += 1;
+    }
   }
   return CPML_OK;
 }
@@ -1108,7 +1171,7 @@ int cpml_gpu_get_field(cpml_solver *h, int which, void *out) {
   if (!field_valid(h, which) || !out) return fail(h, CPML_ERR_ARG, "bad field selector");
   int rc = set_device(h);
   if (rc) return rc;
-  return download_padded(h, h->field[which], out);
+  return download_padded(h, fbuf(h, which), out);
 }
 
 int cpml_gpu_set_field(cpml_solver *h, int which, const void *in) {
@@ -1116,7 +1179,7 @@ int cpml_gpu_set_field(cpml_solver *h, int which, const void *in) {
   if (!field_valid(h, which) || !in) return fail(h, CPML_ERR_ARG, "bad field selector");
   int rc = set_device(h);
   if (rc) return rc;
-  return upload_padded(h, h->field[which], in);
+  return upload_padded(h, fbuf(h, which), in);
 }
 
 int cpml_gpu_get_memory(cpml_solver *h, int which, void *out) {
@@ -1128,7 +1191,7 @@ int cpml_gpu_get_memory(cpml_solver *h, int which, void *out) {
   const size_t es = h->esize;
   std::vector<char> tmp(std::max<size_t>(h->mem_elems[which], 1) * es);
   if (h->mem_elems[which])
-    HIPCHK(h, hipMemcpy(tmp.data(), h->mem[which], h->mem_elems[which] * es,
+    HIPCHK(h, hipMemcpy(tmp.data(), mbuf(h, which), h->mem_elems[which] * es,
                         hipMemcpyDeviceToHost));
   char *o = static_cast<char *>(out);
   const int nx = h->nx, ny = h->ny, nz = h->nz;
@@ -1174,6 +1237,42 @@ int cpml_gpu_set_kchunk(cpml_solver *h, int kchunk) {
   if (!h) return CPML_ERR_ARG;
   if (kchunk < 1) return fail(h, CPML_ERR_ARG, "kchunk must be >= 1");
   h->kchunk = kchunk;
+  h->graph_dirty = true;
+  return CPML_OK;
+}
+
+int cpml_gpu_set_fused(cpml_solver *h, int on) {
+  if (!h) return CPML_ERR_ARG;
+  if (on && h->ndim != 3) return fail(h, CPML_ERR_ARG, "the fused step is 3D only");
+  int rc = set_device(h);
+  if (rc) return rc;
+  if (on && !h->fused) {
+    size_t free_b = 0, tot_b = 0;
+    if (hipMemGetInfo(&free_b, &tot_b) == hipSuccess &&
+        (double)h->total * h->esize * 9 > 0.95 * (double)free_b)
+      return fail(h, CPML_ERR_OOM, "not enough device memory for the second state buffer");
+    for (int q = 0; q < 9; ++q)
+      if (!h->field2[q] && (rc = dev_alloc(h, &h->field2[q], (size_t)h->total * h->esize)))
+        return rc;
+    for (int w = 0; w < 18 && h->have_pml; ++w)
+      if (mem_used(h->ndim, w) && !h->mem2[w] &&
+          (rc = dev_alloc(h, &h->mem2[w], h->mem_elems[w] * h->esize)))
+        return rc;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
+  if (!on && h->cur) {  // bring the state back to the first buffer
+    for (int q = 0; q < 9; ++q)
+      if (h->field[q])
+        HIPCHK(h, hipMemcpyAsync(h->field[q], h->field2[q], (size_t)h->total * h->esize,
+                                 hipMemcpyDeviceToDevice, h->stream));
+    for (int w = 0; w < 18; ++w)
+      if (h->mem[w] && h->mem_elems[w])
+        HIPCHK(h, hipMemcpyAsync(h->mem[w], h->mem2[w], h->mem_elems[w] * h->esize,
+                                 hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->cur = 0;
+  }
+  h->fused = on != 0;
   h->graph_dirty = true;
   return CPML_OK;
 }

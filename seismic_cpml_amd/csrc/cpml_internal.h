@@ -33,7 +33,9 @@ constexpr double kHolberg4 = -3.570998e-3;
 struct Slab {
   int wl = 0, wr = 0, w = 0;  // widths at the min and max side, total
   int n = 0;                  // grid extent along the axis
-  __host__ __device__ bool in(int i) const { return i < wl || i >= n - wr; }
+  __host__ __device__ bool in(int i) const {
+    return i >= 0 && i < n && (i < wl || i >= n - wr);
+  }
   __host__ __device__ int s(int i) const { return i < wl ? i : i - (n - wr) + wl; }
 };
 
@@ -81,6 +83,7 @@ struct Step3D {
   long ld, ps;    // field strides
   long base;      // offset of interior point (0,0,0) from the allocation start
   long xpad;      // x halo width (interior rows start 256-byte aligned)
+  int R;          // halo rows / planes on each side
   long mld, mps;  // memory strides: x-slab (w, w*ny), y-slab (mld, mld*wy), z (mld, mld*ny)
   T odx, ody, odz;
   T dt;
@@ -89,6 +92,14 @@ struct Step3D {
   const int *step;
   int kchunk;  // planes per block along z
   int tile_y;  // rows per block (4, 8 or 16)
+};
+
+// Fused 3D step: reads the state in `a`, writes the next state here.
+template <typename T>
+struct Fused3D {
+  Step3D<T> a;
+  T *vx, *vy, *vz, *sxx, *syy, *szz, *sxy, *sxz, *syz;  // next state, interior origin
+  T *mx[6], *my[6], *mz[6];                             // next memory variables
 };
 
 // Receivers / step counter: one block.
@@ -124,6 +135,8 @@ template <typename T>
 hipError_t launch_stress3d(const Step3D<T> &p, hipStream_t s);
 template <typename T>
 hipError_t launch_velocity3d(const Step3D<T> &p, hipStream_t s);
+template <typename T>
+hipError_t launch_fused3d(const Fused3D<T> &p, hipStream_t s);
 template <typename T>
 hipError_t launch_record(const Record<T> &p, hipStream_t s);
 template <typename T>

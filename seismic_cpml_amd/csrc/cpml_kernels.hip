@@ -20,13 +20,14 @@
 // neighbours in registers, so every plane of every field is fetched once per
 // chunk; x/y neighbours come from the same or adjacent rows of the block and
 // hit in L1/L2.
+#include "cpml_device.h"
 #include "cpml_internal.h"
 
 namespace cpml {
 
 namespace {
 
-constexpr int BX = 64;
+
 constexpr int BY = 4;
 
 // Staggered difference between the half points p+s ("+1/2") and p ("-1/2").
@@ -55,14 +56,6 @@ struct Op<T, 8> {
            T(kHolberg4) * (f[p + 4 * s] - f[p - 3 * s]);
   }
 };
-
-// C-PML correction of a derivative: psi <- b*psi + a*d ; d <- d/K + psi.
-template <typename T>
-__device__ __forceinline__ T cpml(T d, T *psi, T a, T b, T K) {
-  const T m = b * *psi + a * d;
-  *psi = m;
-  return d / K + m;
-}
 
 // ---------------------------------------------------------------- 2D ----
 
@@ -183,39 +176,6 @@ __device__ __forceinline__ Tile3 tile3(int nx, int ny, int nz, int kchunk) {
   return o;
 }
 
-// Buffer-resource access to one padded 3D array, rebased at the plane kb-1
-// of the block's z chunk so every byte offset fits 32 bits.  A thread keeps
-// one VGPR byte offset (the point (i-1, j-1) of that plane); the row and plane
-// shifts are uniform (SGPR soffset) and the x shift an instruction immediate.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, unsigned bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes,
-                                           0x00020000);
-}
-
-template <typename T>
-struct BufOp;
-
-template <>
-struct BufOp<float> {
-  __device__ static float ld(__amdgpu_buffer_rsrc_t r, unsigned v, unsigned s) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, v, s, 0));
-  }
-  __device__ static void st(float x, __amdgpu_buffer_rsrc_t r, unsigned v, unsigned s) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, v, s, 0);
-  }
-};
-
-template <>
-struct BufOp<double> {
-  __device__ static double ld(__amdgpu_buffer_rsrc_t r, unsigned v, unsigned s) {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, v, s, 0));
-  }
-  __device__ static void st(double x, __amdgpu_buffer_rsrc_t r, unsigned v, unsigned s) {
-    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, v, s, 0);
-  }
-};
-
 // Geometry of the rebased chunk buffers of one tile.
 template <typename T>
 struct Chunk3 {
@@ -228,8 +188,8 @@ struct Chunk3 {
     ldb = (unsigned)(p.ld * sizeof(T));
     psb = (unsigned)(p.ps * sizeof(T));
     bytes = (unsigned)((t.ke - t.kb + 2) * p.ps * sizeof(T));
-    origin = (long)t.kb * p.ps;  // allocation plane kb == interior plane kb-1
-    v0 = (unsigned)((t.j * p.ld + p.xpad + t.i - 1) * sizeof(T));
+    origin = (long)(t.kb - 1 + p.R) * p.ps;  // allocation plane of interior plane kb-1
+    v0 = (unsigned)(((t.j - 1 + p.R) * p.ld + p.xpad + t.i - 1) * sizeof(T));
   }
   // SGPR part of the offset of (i+dx, j+dy, k+dz) for buffer plane kk = k-kb+1
   __device__ unsigned so(int kk, int dy, int dz) const {

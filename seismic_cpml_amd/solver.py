@@ -58,6 +58,7 @@ SIGNATURES = {
     "cpml_gpu_set_graph_steps": [_VP, _I],
     "cpml_gpu_set_kchunk": [_VP, _I],
     "cpml_gpu_set_tile": [_VP, _I],
+    "cpml_gpu_set_fused": [_VP, _I],
     "cpml_gpu_set_kernel_timing": [_VP, _I],
     "cpml_gpu_kernel_time": [_VP, _I, _IP, _DP],
     "cpml_gpu_elapsed_ms": [_VP, _DP],
@@ -255,6 +256,9 @@ class CpmlGpu:
 
     def set_tile(self, rows):
         self._call("cpml_gpu_set_tile", rows)
+
+    def set_fused(self, on=True):
+        self._call("cpml_gpu_set_fused", int(on))
 
     def set_kernel_timing(self, on=True):
         self._call("cpml_gpu_set_kernel_timing", int(on))

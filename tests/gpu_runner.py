@@ -25,7 +25,7 @@ def oracle_mem_name(name: str) -> str:
 
 
 def make_solver(case: str, prec: str, d: dict, energy=True, graph_steps=16, kchunk=None,
-                tile=None):
+                tile=None, fused=False):
     c = CASES[case]
     p = 8 if prec == "f64" else 4
     ndim = 3 if c.get("NZ") else 2
@@ -42,14 +42,16 @@ def make_solver(case: str, prec: str, d: dict, energy=True, graph_steps=16, kchu
         s.set_kchunk(kchunk)
     if tile:
         s.set_tile(tile)
+    if fused:
+        s.set_fused(True)
     return s
 
 
 def run_gpu(case: str, prec: str, nstep: int | None = None, d: dict | None = None,
             energy=True, graph_steps=16, kchunk=None, memory=False, timing=False,
-            tile=None) -> dict:
+            tile=None, fused=False) -> dict:
     d = inputs(case, prec, nstep) if d is None else d
-    with make_solver(case, prec, d, energy, graph_steps, kchunk, tile) as s:
+    with make_solver(case, prec, d, energy, graph_steps, kchunk, tile, fused) as s:
         if timing:
             s.set_kernel_timing(True)
         s.run()
