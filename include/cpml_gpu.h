@@ -158,6 +158,52 @@ int cpml_gpu_slab_widths(cpml_solver *h, int *w);
  * cpml_gpu_init on this thread).  Always NUL-terminated. */
 int cpml_gpu_last_error(cpml_solver *h, char *buf, int len);
 
+/* ---- batched multi-shot 2D ------------------------------------------- */
+
+/* The reference runs one shot per program execution (2D2:137-218 fixes one
+ * source and one model); a survey of many shots is a loop of such runs.
+ * cpml_batch advances nshots independent 2D shots together on one GPU, one
+ * launch per phase for all of them; each shot has its own grid size, spacing,
+ * time step, NPOINTS_PML, medium, profiles, source and receivers, and its
+ * results equal a cpml_solver run of that shot alone, bit for bit.  Per-shot
+ * arrays are indexed 0..nshots-1; every other convention is cpml_solver's.
+ * Energy and the stability check of cpml_gpu_run are not computed here. */
+typedef struct cpml_batch cpml_batch;
+
+int cpml_batch_init(cpml_batch **b, int order, int prec, int nshots, const int *NX,
+                    const int *NY, const double *DELTAX, const double *DELTAY,
+                    const double *DELTAT, const int *NPOINTS_PML, int NSTEP);
+int cpml_batch_finalize(cpml_batch *b);
+/* as cpml_gpu_set_medium / set_pml / set_source / set_receivers for one shot */
+int cpml_batch_set_medium(cpml_batch *b, int shot, const void *lambda, const void *mu,
+                          const void *rho);
+int cpml_batch_set_pml(cpml_batch *b, int shot, const void *a_x, const void *b_x,
+                       const void *K_x, const void *a_x_half, const void *b_x_half,
+                       const void *K_x_half, const void *a_y, const void *b_y, const void *K_y,
+                       const void *a_y_half, const void *b_y_half, const void *K_y_half);
+int cpml_batch_set_source(cpml_batch *b, int shot, int ISOURCE, int JSOURCE,
+                          const void *force_x, const void *force_y);
+int cpml_batch_set_receivers(cpml_batch *b, int shot, int NREC, const int *ix_rec,
+                             const int *iy_rec);
+/* Launch shape: rows per tile (4, 8, 16); blocks_per_cu 0 = one workgroup
+ * per tile, else a persistent grid of blocks_per_cu * CUs workgroups; steps
+ * per captured graph (0: no graph). */
+int cpml_batch_set_tiling(cpml_batch *b, int tile_y, int blocks_per_cu, int graph_steps);
+/* Times every launch shape over trial_steps steps of this batch, keeps the
+ * fastest, then returns the batch to rest (fields, memory variables and
+ * seismograms zero).  cache_path (may be NULL) is a text file of earlier
+ * choices keyed by device, order, precision and batch shape; a hit skips the
+ * trials (best_ms = -1). */
+int cpml_batch_autotune(cpml_batch *b, int trial_steps, const char *cache_path, int *tile_y,
+                        int *blocks_per_cu, double *best_ms);
+/* Steps it_begin..it_end of every shot (2D2:545-735 per shot). */
+int cpml_batch_run(cpml_batch *b, int it_begin, int it_end);
+int cpml_batch_get_seismograms(cpml_batch *b, int shot, void *sisvx, void *sisvy);
+/* which: CPML_VX, CPML_VY, CPML_SIGMAXX, CPML_SIGMAYY or CPML_SIGMAXY */
+int cpml_batch_get_field(cpml_batch *b, int shot, int which, void *out);
+int cpml_batch_elapsed_ms(cpml_batch *b, double *ms);
+int cpml_batch_last_error(cpml_batch *b, char *buf, int len);
+
 /* ---- host-side set-up helpers (no GPU needed) -------------------------- */
 
 /* C-PML profile along one axis (2D2:283-394 for x; 2D2:396-465 for y;

@@ -5,6 +5,7 @@ the C ABI in include/cpml_gpu.h), bound from Fortran by the ``cpml_gpu``
 module (fortran/cpml_gpu.f90) and from Python by :mod:`.solver`.
 """
 from .solver import (  # noqa: F401
+    CpmlBatch,
     CpmlError,
     CpmlGpu,
     courant_number,
@@ -14,5 +15,5 @@ from .solver import (  # noqa: F401
     source_time_function,
 )
 
-__all__ = ["CpmlError", "CpmlGpu", "courant_number", "library", "nearest_receivers",
+__all__ = ["CpmlBatch", "CpmlError", "CpmlGpu", "courant_number", "library", "nearest_receivers",
            "pml_profile", "source_time_function"]

@@ -290,6 +290,7 @@ cpml::Step2D<T> step2d(cpml_solver *h) {
   p.fx = static_cast<const T *>(h->fx);
   p.fy = static_cast<const T *>(h->fy);
   p.step = h->step;
+  p.tile_y = h->tile_y;
   return p;
 }
 
@@ -547,24 +548,6 @@ int velocity_norm(cpml_solver *h, double *out) {
   std::memcpy(&v, &bits, sizeof(v));
   *out = std::sqrt(v);
   return CPML_OK;
-}
-
-template <typename T>
-void slab_from_profile(const T *a, const T *ah, const T *K, const T *Kh, int n, Slab &s) {
-  const int half = (n + 1) / 2;
-  int wl = 0, wr = 0;
-  for (int i = 0; i < n; ++i) {
-    const bool nz = a[i] != T(0) || ah[i] != T(0) || K[i] != T(1) || Kh[i] != T(1);
-    if (!nz) continue;
-    if (i < half)
-      wl = std::max(wl, i + 1);
-    else
-      wr = std::max(wr, n - i);
-  }
-  s.wl = wl;
-  s.wr = wr;
-  s.w = wl + wr;
-  s.n = n;
 }
 
 template <typename T>
@@ -949,10 +932,10 @@ int cpml_gpu_set_pml(cpml_solver *h, const void *a_x, const void *b_x, const voi
     for (int q = 0; q < 6; ++q)
       std::memcpy(host.data() + (d * 6 + q) * nmax * h->esize, ax[d][q], n[d] * h->esize);
     if (h->prec == 4)
-      slab_from_profile((const float *)ax[d][0], (const float *)ax[d][3],
+      cpml::slab_from_profile((const float *)ax[d][0], (const float *)ax[d][3],
                         (const float *)ax[d][2], (const float *)ax[d][5], n[d], *sl[d]);
     else
-      slab_from_profile((const double *)ax[d][0], (const double *)ax[d][3],
+      cpml::slab_from_profile((const double *)ax[d][0], (const double *)ax[d][3],
                         (const double *)ax[d][2], (const double *)ax[d][5], n[d], *sl[d]);
     if (sl[d]->w > n[d]) return fail(h, CPML_ERR_STENCIL, "absorbing slabs overlap");
   }

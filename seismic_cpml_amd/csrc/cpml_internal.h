@@ -39,6 +39,25 @@ struct Slab {
   __host__ __device__ int s(int i) const { return i < wl ? i : i - (n - wr) + wl; }
 };
 
+// Slab widths from one axis's profiles: a point belongs to the slab when any
+// of a, a_half differs from 0 or K, K_half from 1; the widths are the extents
+// of such points from the min side (first half) and the max side.
+template <typename T>
+inline void slab_from_profile(const T *a, const T *ah, const T *K, const T *Kh, int n, Slab &s) {
+  const int half = (n + 1) / 2;
+  int wl = 0, wr = 0;
+  for (int i = 0; i < n; ++i) {
+    const bool nz = a[i] != T(0) || ah[i] != T(0) || K[i] != T(1) || Kh[i] != T(1);
+    if (!nz) continue;
+    if (i < half) wl = i + 1 > wl ? i + 1 : wl;
+    else wr = n - i > wr ? n - i : wr;
+  }
+  s.wl = wl;
+  s.wr = wr;
+  s.w = wl + wr;
+  s.n = n;
+}
+
 // Profiles along one axis, 6 consecutive arrays of length n:
 // a, b, K, a_half, b_half, K_half (the reference's a_x, b_x, K_x, ...).
 template <typename T>
@@ -64,6 +83,7 @@ struct Step2D {
   int isrc, jsrc;
   const T *fx, *fy;
   const int *step;  // current step it (1-based), advanced by the record kernel
+  int tile_y;       // rows per block (4, 8 or 16)
 };
 
 template <typename T>

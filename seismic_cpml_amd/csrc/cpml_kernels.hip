@@ -22,121 +22,27 @@
 // hit in L1/L2.
 #include "cpml_device.h"
 #include "cpml_internal.h"
+#include "cpml_points2d.h"
 
 namespace cpml {
 
 namespace {
 
 
-constexpr int BY = 4;
-
-// Staggered difference between the half points p+s ("+1/2") and p ("-1/2").
-template <typename T, int ORDER>
-struct Op;
-
-template <typename T>
-struct Op<T, 2> {
-  __device__ static T d(const T *f, long p, long s) { return f[p + s] - f[p]; }
-};
-
-// 2D4:565  (27*f(+1/2) - 27*f(-1/2) - f(+3/2) + f(-3/2))
-template <typename T>
-struct Op<T, 4> {
-  __device__ static T d(const T *f, long p, long s) {
-    return T(27) * f[p + s] - T(27) * f[p] - f[p + 2 * s] + f[p - s];
-  }
-};
-
-// ADE8:734-735 association: c1*(..) + c2*(..) + c3*(..) + c4*(..)
-template <typename T>
-struct Op<T, 8> {
-  __device__ static T d(const T *f, long p, long s) {
-    return T(kHolberg1) * (f[p + s] - f[p]) + T(kHolberg2) * (f[p + 2 * s] - f[p - s]) +
-           T(kHolberg3) * (f[p + 3 * s] - f[p - 2 * s]) +
-           T(kHolberg4) * (f[p + 4 * s] - f[p - 3 * s]);
-  }
-};
-
 // ---------------------------------------------------------------- 2D ----
 
-template <typename T, int ORDER>
-__global__ __launch_bounds__(BX *BY) void k_stress2d(Step2D<T> p) {
+template <typename T, int ORDER, int TY>
+__global__ __launch_bounds__(BX *TY) void k_stress2d(Step2D<T> p) {
   const int i = blockIdx.x * BX + threadIdx.x;
-  const int j = blockIdx.y * BY + threadIdx.y;
-  if (i >= p.nx || j >= p.ny) return;
-  const long ld = p.ld;
-  const long c = i + j * ld;
-  const bool inx = p.sx.in(i), iny = p.sy.in(j);
-  const long mxo = p.sx.s(i) + (long)p.sx.w * j;
-  const long myo = i + p.mld * p.sy.s(j);
-  // sigma_xx, sigma_yy at (i+1/2, j), Fortran i=1..NX-1, j=2..NY
-  if (i <= p.nx - 2 && j >= 1) {
-    const T lhx = T(0.5) * (p.lam[c + 1] + p.lam[c]);
-    const T mhx = T(0.5) * (p.mu[c + 1] + p.mu[c]);
-    const T lp2m = lhx + T(2) * mhx;
-    T dvx_dx = Op<T, ORDER>::d(p.vx, c, 1) / p.den_x;
-    T dvy_dy = Op<T, ORDER>::d(p.vy, c - ld, ld) / p.den_y;
-    if (inx) dvx_dx = cpml(dvx_dx, p.mx[0] + mxo, p.px.ah[i], p.px.bh[i], p.px.Kh[i]);
-    if (iny) dvy_dy = cpml(dvy_dy, p.my[0] + myo, p.py.a[j], p.py.b[j], p.py.K[j]);
-    p.sxx[c] = p.sxx[c] + (lp2m * dvx_dx + lhx * dvy_dy) * p.dt;
-    p.syy[c] = p.syy[c] + (lhx * dvx_dx + lp2m * dvy_dy) * p.dt;
-  }
-  // sigma_xy at (i, j+1/2), Fortran i=2..NX, j=1..NY-1
-  if (i >= 1 && j <= p.ny - 2) {
-    const T mhy = T(0.5) * (p.mu[c + ld] + p.mu[c]);
-    T dvy_dx = Op<T, ORDER>::d(p.vy, c - 1, 1) / p.den_x;
-    T dvx_dy = Op<T, ORDER>::d(p.vx, c, ld) / p.den_y;
-    if (inx) dvy_dx = cpml(dvy_dx, p.mx[1] + mxo, p.px.a[i], p.px.b[i], p.px.K[i]);
-    // 2D4:596 divides by K_y(j) where 2D2:595 uses K_y_half(j)
-    if (iny)
-      dvx_dy = cpml(dvx_dy, p.my[1] + myo, p.py.ah[j], p.py.bh[j],
-                    ORDER == 4 ? p.py.K[j] : p.py.Kh[j]);
-    p.sxy[c] = p.sxy[c] + mhy * (dvy_dx + dvx_dy) * p.dt;
-  }
+  const int j = blockIdx.y * TY + threadIdx.y;
+  if (i < p.nx && j < p.ny) stress2d_point<T, ORDER>(p, i, j);
 }
 
-template <typename T, int ORDER>
-__global__ __launch_bounds__(BX *BY) void k_velocity2d(Step2D<T> p) {
+template <typename T, int ORDER, int TY>
+__global__ __launch_bounds__(BX *TY) void k_velocity2d(Step2D<T> p) {
   const int i = blockIdx.x * BX + threadIdx.x;
-  const int j = blockIdx.y * BY + threadIdx.y;
-  if (i >= p.nx || j >= p.ny) return;
-  const long ld = p.ld;
-  const long c = i + j * ld;
-  const bool inx = p.sx.in(i), iny = p.sy.in(j);
-  const long mxo = p.sx.s(i) + (long)p.sx.w * j;
-  const long myo = i + p.mld * p.sy.s(j);
-  T vxn = p.vx[c], vyn = p.vy[c];
-  // vx at (i, j), Fortran i=2..NX, j=2..NY
-  if (i >= 1 && j >= 1) {
-    T dsxx_dx = Op<T, ORDER>::d(p.sxx, c - 1, 1) / p.den_x;
-    T dsxy_dy = Op<T, ORDER>::d(p.sxy, c - ld, ld) / p.den_y;
-    if (inx) dsxx_dx = cpml(dsxx_dx, p.mx[2] + mxo, p.px.a[i], p.px.b[i], p.px.K[i]);
-    if (iny) dsxy_dy = cpml(dsxy_dy, p.my[2] + myo, p.py.a[j], p.py.b[j], p.py.K[j]);
-    vxn = vxn + (dsxx_dx + dsxy_dy) * p.dt / p.rho[c];
-  }
-  // vy at (i+1/2, j+1/2), Fortran i=1..NX-1, j=1..NY-1
-  if (i <= p.nx - 2 && j <= p.ny - 2) {
-    const T rhh = T(0.25) * (p.rho[c] + p.rho[c + 1] + p.rho[c + 1 + ld] + p.rho[c + ld]);
-    T dsxy_dx = Op<T, ORDER>::d(p.sxy, c, 1) / p.den_x;
-    T dsyy_dy = Op<T, ORDER>::d(p.syy, c, ld) / p.den_y;
-    if (inx) dsxy_dx = cpml(dsxy_dx, p.mx[3] + mxo, p.px.ah[i], p.px.bh[i], p.px.Kh[i]);
-    if (iny) dsyy_dy = cpml(dsyy_dy, p.my[3] + myo, p.py.ah[j], p.py.bh[j], p.py.Kh[j]);
-    vyn = vyn + (dsxy_dx + dsyy_dy) * p.dt / rhh;
-  }
-  // point force, added after the velocity update (2D2:660-667)
-  if (i == p.isrc && j == p.jsrc) {
-    const int it = *p.step;
-    const T rhh = T(0.25) * (p.rho[c] + p.rho[c + 1] + p.rho[c + 1 + ld] + p.rho[c + ld]);
-    vxn = vxn + p.fx[it - 1] * p.dt / p.rho[c];
-    vyn = vyn + p.fy[it - 1] * p.dt / rhh;
-  }
-  // rigid edges (2D2:670-680)
-  if (i == 0 || i == p.nx - 1 || j == 0 || j == p.ny - 1) {
-    vxn = T(0);
-    vyn = T(0);
-  }
-  p.vx[c] = vxn;
-  p.vy[c] = vyn;
+  const int j = blockIdx.y * TY + threadIdx.y;
+  if (i < p.nx && j < p.ny) velocity2d_point<T, ORDER>(p, i, j);
 }
 
 // ---------------------------------------------------------------- 3D ----
@@ -565,22 +471,36 @@ __global__ __launch_bounds__(EB) void k_vmax(const T *vx, const T *vy, const T *
 
 // ------------------------------------------------------------ launchers --
 
-template <typename T>
-hipError_t launch_stress2d(int order, const Step2D<T> &p, hipStream_t s) {
-  dim3 grid((p.nx + BX - 1) / BX, (p.ny + BY - 1) / BY), block(BX, BY);
-  if (order == 2) hipLaunchKernelGGL((k_stress2d<T, 2>), grid, block, 0, s, p);
-  else if (order == 4) hipLaunchKernelGGL((k_stress2d<T, 4>), grid, block, 0, s, p);
-  else hipLaunchKernelGGL((k_stress2d<T, 8>), grid, block, 0, s, p);
+template <typename T, int TY>
+static hipError_t stress2d_ty(int order, const Step2D<T> &p, hipStream_t s) {
+  dim3 grid((p.nx + BX - 1) / BX, (p.ny + TY - 1) / TY), block(BX, TY);
+  if (order == 2) hipLaunchKernelGGL((k_stress2d<T, 2, TY>), grid, block, 0, s, p);
+  else if (order == 4) hipLaunchKernelGGL((k_stress2d<T, 4, TY>), grid, block, 0, s, p);
+  else hipLaunchKernelGGL((k_stress2d<T, 8, TY>), grid, block, 0, s, p);
+  return hipGetLastError();
+}
+
+template <typename T, int TY>
+static hipError_t velocity2d_ty(int order, const Step2D<T> &p, hipStream_t s) {
+  dim3 grid((p.nx + BX - 1) / BX, (p.ny + TY - 1) / TY), block(BX, TY);
+  if (order == 2) hipLaunchKernelGGL((k_velocity2d<T, 2, TY>), grid, block, 0, s, p);
+  else if (order == 4) hipLaunchKernelGGL((k_velocity2d<T, 4, TY>), grid, block, 0, s, p);
+  else hipLaunchKernelGGL((k_velocity2d<T, 8, TY>), grid, block, 0, s, p);
   return hipGetLastError();
 }
 
 template <typename T>
+hipError_t launch_stress2d(int order, const Step2D<T> &p, hipStream_t s) {
+  if (p.tile_y == 16) return stress2d_ty<T, 16>(order, p, s);
+  if (p.tile_y == 8) return stress2d_ty<T, 8>(order, p, s);
+  return stress2d_ty<T, 4>(order, p, s);
+}
+
+template <typename T>
 hipError_t launch_velocity2d(int order, const Step2D<T> &p, hipStream_t s) {
-  dim3 grid((p.nx + BX - 1) / BX, (p.ny + BY - 1) / BY), block(BX, BY);
-  if (order == 2) hipLaunchKernelGGL((k_velocity2d<T, 2>), grid, block, 0, s, p);
-  else if (order == 4) hipLaunchKernelGGL((k_velocity2d<T, 4>), grid, block, 0, s, p);
-  else hipLaunchKernelGGL((k_velocity2d<T, 8>), grid, block, 0, s, p);
-  return hipGetLastError();
+  if (p.tile_y == 16) return velocity2d_ty<T, 16>(order, p, s);
+  if (p.tile_y == 8) return velocity2d_ty<T, 8>(order, p, s);
+  return velocity2d_ty<T, 4>(order, p, s);
 }
 
 template <typename T>

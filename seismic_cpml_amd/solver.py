@@ -65,6 +65,19 @@ SIGNATURES = {
     "cpml_gpu_device_bytes": [_VP, _DP],
     "cpml_gpu_slab_widths": [_VP, _IP],
     "cpml_gpu_last_error": [_VP, C.c_char_p, _I],
+    "cpml_batch_init": [C.POINTER(_VP), _I, _I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _I],
+    "cpml_batch_finalize": [_VP],
+    "cpml_batch_set_medium": [_VP, _I, _VP, _VP, _VP],
+    "cpml_batch_set_pml": [_VP, _I] + [_VP] * 12,
+    "cpml_batch_set_source": [_VP, _I, _I, _I, _VP, _VP],
+    "cpml_batch_set_receivers": [_VP, _I, _I, _VP, _VP],
+    "cpml_batch_set_tiling": [_VP, _I, _I, _I],
+    "cpml_batch_autotune": [_VP, _I, C.c_char_p, _IP, _IP, _DP],
+    "cpml_batch_run": [_VP, _I, _I],
+    "cpml_batch_get_seismograms": [_VP, _I, _VP, _VP],
+    "cpml_batch_get_field": [_VP, _I, _I, _VP],
+    "cpml_batch_elapsed_ms": [_VP, _DP],
+    "cpml_batch_last_error": [_VP, C.c_char_p, _I],
     "cpml_pml_profile": [_I, _I, _D, _D, _I, _D, _D, _I, _I, _I, _I] + [_VP] * 6,
     "cpml_source_time_function": [_I, _I, _D, _D, _D, _D, _D, _I, _VP, _VP],
     "cpml_nearest_receivers": [_I, _I, _I, _D, _D, _I, _D, _D, _D, _D, _VP, _VP],
@@ -101,15 +114,15 @@ def _ptr(a):
     return None if a is None else C.c_void_p(a.ctypes.data)
 
 
-def _err(h, code):
+def _err(h, code, getter="cpml_gpu_last_error"):
     buf = C.create_string_buffer(1024)
-    library().cpml_gpu_last_error(h, buf, 1024)
+    getattr(library(), getter)(h, buf, 1024)
     return CpmlError(code, buf.value.decode(errors="replace"))
 
 
-def _check(code, h=None):
+def _check(code, h=None, getter="cpml_gpu_last_error"):
     if code != CPML_OK:
-        raise _err(h, code)
+        raise _err(h, code, getter)
 
 
 def _dtype(prec):
@@ -318,3 +331,116 @@ class CpmlGpu:
         w = (C.c_int * 6)()
         self._call("cpml_gpu_slab_widths", w)
         return list(w)
+
+
+# ------------------------------------------------------------------ batch --
+
+class CpmlBatch:
+    """Many independent 2D shots advanced together (cpml_batch_init ...).
+
+    ``shots`` is a list of dicts with keys NX, NY, DELTAX, DELTAY, DELTAT,
+    NPOINTS_PML; every shot is then configured with the same calls as
+    :class:`CpmlGpu` plus a leading shot index.  A batched shot's results
+    equal a :class:`CpmlGpu` run of that shot alone, bit for bit."""
+
+    BATCH_FIELDS = ("vx", "vy", "sigmaxx", "sigmayy", "sigmaxy")
+
+    def __init__(self, order, prec, shots, NSTEP):
+        self.order, self.prec, self.nstep = order, prec, NSTEP
+        self.dtype = _dtype(prec)
+        self.shots = [dict(s) for s in shots]
+        self.nrec = [0] * len(shots)
+        n = len(shots)
+        col = lambda k, t: np.ascontiguousarray([s[k] for s in shots], t)  # noqa: E731
+        nx, ny = col("NX", np.int32), col("NY", np.int32)
+        dx, dy, dt = col("DELTAX", np.float64), col("DELTAY", np.float64), col("DELTAT",
+                                                                                np.float64)
+        npml = col("NPOINTS_PML", np.int32)
+        self._h = C.c_void_p()
+        _check(library().cpml_batch_init(C.byref(self._h), order, prec, n, _ptr(nx), _ptr(ny),
+                                         _ptr(dx), _ptr(dy), _ptr(dt), _ptr(npml), NSTEP),
+               None, "cpml_batch_last_error")
+
+    def close(self):
+        if self._h:
+            library().cpml_batch_finalize(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.shots)
+
+    def _call(self, fn, *args):
+        _check(getattr(library(), fn)(self._h, *args), self._h, "cpml_batch_last_error")
+
+    def _shape(self, q):
+        return (self.shots[q]["NY"], self.shots[q]["NX"])
+
+    def _arr(self, a, shape):
+        a = np.ascontiguousarray(a, self.dtype)
+        if a.shape != shape:
+            raise ValueError(f"array shape {a.shape} != {shape}")
+        return a
+
+    def set_medium(self, q, lam, mu, rho):
+        sh = self._shape(q)
+        lam, mu, rho = (self._arr(x, sh) for x in (lam, mu, rho))
+        self._call("cpml_batch_set_medium", q, _ptr(lam), _ptr(mu), _ptr(rho))
+
+    def set_pml(self, q, prof_x, prof_y):
+        ny, nx = self._shape(q)
+        px = [self._arr(prof_x[k], (nx,)) for k in range(6)]
+        py = [self._arr(prof_y[k], (ny,)) for k in range(6)]
+        self._call("cpml_batch_set_pml", q, *[_ptr(a) for a in px + py])
+
+    def set_source(self, q, ISOURCE, JSOURCE, force_x, force_y):
+        fx = self._arr(force_x, (self.nstep,))
+        fy = self._arr(force_y, (self.nstep,))
+        self._call("cpml_batch_set_source", q, ISOURCE, JSOURCE, _ptr(fx), _ptr(fy))
+
+    def set_receivers(self, q, ix_rec, iy_rec):
+        ix = np.ascontiguousarray(ix_rec, np.int32)
+        iy = np.ascontiguousarray(iy_rec, np.int32)
+        self.nrec[q] = len(ix)
+        self._call("cpml_batch_set_receivers", q, len(ix), _ptr(ix), _ptr(iy))
+
+    def set_tiling(self, tile_y=4, blocks_per_cu=0, graph_steps=16):
+        self._call("cpml_batch_set_tiling", tile_y, blocks_per_cu, graph_steps)
+
+    def autotune(self, trial_steps=20, cache_path=None):
+        """Returns (tile_y, blocks_per_cu, best_ms); best_ms is -1 on a cache hit."""
+        ty, bpc, ms = C.c_int(), C.c_int(), C.c_double()
+        path = None if cache_path is None else str(cache_path).encode()
+        self._call("cpml_batch_autotune", trial_steps, path, C.byref(ty), C.byref(bpc),
+                   C.byref(ms))
+        return ty.value, bpc.value, ms.value
+
+    def run(self, it_begin=1, it_end=None):
+        self._call("cpml_batch_run", it_begin, self.nstep if it_end is None else it_end)
+
+    def seismograms(self, q):
+        n = max(self.nrec[q], 1)
+        out = [np.zeros((n, self.nstep), self.dtype) for _ in range(2)]
+        self._call("cpml_batch_get_seismograms", q, _ptr(out[0]), _ptr(out[1]))
+        return tuple(o[: self.nrec[q]] for o in out)
+
+    def field(self, q, name):
+        out = np.zeros(self._shape(q), self.dtype)
+        self._call("cpml_batch_get_field", q, FIELDS.index(name), _ptr(out))
+        return out
+
+    def elapsed_ms(self):
+        v = C.c_double()
+        self._call("cpml_batch_elapsed_ms", C.byref(v))
+        return v.value

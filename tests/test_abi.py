@@ -116,3 +116,45 @@ def test_host_stf_and_receivers_match_oracle(case, prec):
     f = fixture(case, prec)
     np.testing.assert_array_equal(ix, f["ix_rec"])
     np.testing.assert_array_equal(iy, f["iy_rec"])
+
+
+def _batch_init(order, prec, nx, ny, dx, dt, npml, nstep):
+    import ctypes as C
+
+    n = len(nx)
+    arr = lambda v, t: np.ascontiguousarray(v, t)  # noqa: E731
+    NX, NY, NP = arr(nx, np.int32), arr(ny, np.int32), arr(npml, np.int32)
+    DX, DT = arr(dx, np.float64), arr(dt, np.float64)
+    h = C.c_void_p()
+    rc = sc.library().cpml_batch_init(C.byref(h), order, prec, n, solver._ptr(NX),
+                                      solver._ptr(NY), solver._ptr(DX), solver._ptr(DX),
+                                      solver._ptr(DT), solver._ptr(NP), nstep)
+    return rc, h
+
+
+@pytest.mark.parametrize("args,code", [
+    ((6, 8, [32], [32], [1.0], [1e-3], [4], 10), solver.CPML_ERR_ARG),        # order
+    ((2, 2, [32], [32], [1.0], [1e-3], [4], 10), solver.CPML_ERR_ARG),        # prec
+    ((2, 8, [], [], [], [], [], 10), solver.CPML_ERR_ARG),                    # no shots
+    ((2, 8, [32, 1], [32, 9], [1.0, 1.0], [1e-3] * 2, [4, 0], 10), solver.CPML_ERR_STENCIL),
+    ((2, 8, [32, 32], [32, 32], [1.0, -1.0], [1e-3] * 2, [4, 4], 10), solver.CPML_ERR_ARG),
+    ((4, 4, [32, 32], [32, 32], [1.0, 1.0], [1e-3, 0.0], [4, 4], 10), solver.CPML_ERR_ARG),
+    ((2, 8, [32, 32], [32, 20], [1.0, 1.0], [1e-3] * 2, [4, 11], 10), solver.CPML_ERR_STENCIL),
+    ((2, 8, [32], [32], [1.0], [1e-3], [4], 0), solver.CPML_ERR_ARG),         # NSTEP
+])
+def test_batch_init_validates_on_host(args, code):
+    rc, h = _batch_init(*args)
+    assert rc == code
+    assert not h.value
+    assert solver._err(None, rc, "cpml_batch_last_error").message
+
+
+def test_batch_without_gpu_reports_nodevice():
+    import torch
+
+    if torch.cuda.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(sc.CpmlError) as e:
+        sc.CpmlBatch(2, 8, [dict(NX=32, NY=32, DELTAX=10.0, DELTAY=10.0, DELTAT=1e-3,
+                                 NPOINTS_PML=4)], 10)
+    assert e.value.code == solver.CPML_ERR_NODEVICE
