@@ -12,6 +12,13 @@ whole grid.  Inputs are resident in HBM before the timed region.
 Prints ONE JSON line: Gcell-updates/s, HIP-event kernel durations and the
 HBM roofline of the dominant kernel, and a CPU baseline (the C oracle, the
 reference loop structure with OpenMP) timed on a bounded sub-block.
+
+Other workloads (--workload): ``2d`` -- 2D 4th order 4096^2, 5 m, NPML 20,
+FP32, 12 seeded layers, Ricker 10 Hz at the centre, 256 receivers on the top
+interior row, 4000 steps; ``batch`` -- 128 2D 4th-order shots with seeded
+uneven grids 512..2048 per side, 3000 steps, batched solver against 128
+sequential plain-solver runs; ``shipped`` -- 2D2 as shipped (101 x 641, FP64,
+2000 steps).
 """
 from __future__ import annotations
 
@@ -26,14 +33,49 @@ ROOT = Path(__file__).resolve().parent
 HBM_PEAK_GBS = 8000.0
 
 
+def _orc():
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as orc
+
+    return orc
+
+
+def cpu_baseline_2d(nx, ny, order, dx, dt, lam, mu, rho, px, py, fx, fy, seconds=15.0,
+                    threads=None, what=""):
+    """Time the oracle's 2D loop (reference structure, full-grid memory
+    variables, OpenMP) on a sample grid for about `seconds`."""
+    import numpy as np
+
+    orc = _orc()
+    threads = threads or min(16, os.cpu_count() or 1)
+    prec = "f32" if lam.dtype == np.float32 else "f64"
+    ix = np.array([nx // 2], np.int32)
+    iy = np.array([ny // 2], np.int32)
+
+    def run(nst, st):
+        return orc.run2d(nx, ny, order, dx, dx, dt, nst, lam, mu, rho, px, py, nx // 2, ny // 2,
+                         fx[:nst], fy[:nst], ix, iy, prec=prec, state=st, nthreads=threads)
+
+    out = run(1, None)
+    t = time.perf_counter()
+    out = run(2, out["state"])
+    t1 = (time.perf_counter() - t) / 2
+    steps = max(1, min(len(fx), int(seconds / max(t1, 1e-4))))
+    t = time.perf_counter()
+    run(steps, out["state"])
+    el = time.perf_counter() - t
+    return {"value": round(nx * ny * steps / el / 1e9, 4), "unit": "Gcell/s", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle C 2D loop (reference structure, OpenMP) on {what}{nx}x{ny}, "
+                      f"{steps} steps, {el:.1f} s"}
+
+
 def cpu_baseline(w, lam, mu, rho, nzs=128, seconds=15.0, threads=None):
     """Time the oracle's 3D loop (reference structure, full-grid memory
     variables) on the top nzs planes of the same medium."""
     import numpy as np
 
-    sys.path.insert(0, str(ROOT / "tests"))
-    import oracle_lib as orc
-
+    orc = _orc()
     threads = threads or min(16, os.cpu_count() or 1)
     n = w.nx
     z0 = w.nz - nzs
@@ -68,16 +110,236 @@ def cpu_baseline(w, lam, mu, rho, nzs=128, seconds=15.0, threads=None):
                       f"planes of the same medium, {steps} steps, {el:.1f} s"}
 
 
+def _timed_run(s, warmup, nstep):
+    import torch
+
+    if warmup > 0:
+        s.run(1, warmup)
+    s.set_kernel_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.run(warmup + 1, nstep)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def _line(metric, value, args, steps, ms_per_step, dtype, config, roofline, cpu):
+    return {
+        "metric": metric, "value": round(value, 3), "unit": "Gcell/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+        "data": "synthetic", "config": config, "roofline": roofline, "cpu_baseline": cpu,
+    }
+
+
+def _roofline(nb, avg):
+    dom = max(avg, key=avg.get)
+    achieved = nb[dom] / (avg[dom] * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None}
+
+
+def bench_3d(args):
+    import torch
+
+    from seismic_cpml_amd import workloads
+
+    steps = args.steps or 2000
+    nstep = args.warmup + steps
+    w = workloads.layered_3d(n=args.n or 768, nstep=nstep)
+    s = w.make_solver()
+    if args.kchunk:
+        s.set_kchunk(args.kchunk)
+    if not args.two_phase:
+        s.set_fused(True)
+    if args.tile:
+        s.set_tile(args.tile)
+    slabs = s.slab_widths()
+    wall = _timed_run(s, args.warmup, nstep)
+    torch.cuda.synchronize()
+
+    cells = w.cells
+    value = cells * steps / wall / 1e9
+    fused = not args.two_phase
+    nb = workloads.bytes_per_step(w, slabs, fused=fused)
+    names = ((0, "fused"),) if fused else ((0, "stress"), (1, "velocity"))
+    kt = {name: s.kernel_time(k) for k, name in names}
+    avg = {name: ms / max(n, 1) for name, (n, ms) in kt.items()}
+    step_gbs = nb["step"] / (wall / steps) / 1e9
+    nb2 = workloads.bytes_per_step(w, slabs, fused=False)
+    vnorm = s.velocity_norm()
+    s.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        lam, mu, rho = w.medium_dev
+        cpu = cpu_baseline(w, lam, mu, rho, seconds=args.cpu_seconds)
+    config = {
+        "workload": f"3D isotropic elastic C-PML {w.nx}^3, NPML 20, 2nd order, layered "
+                    "+-5% smoothed medium, Ricker 8 Hz, 1024 receivers",
+        "grid": [w.nx, w.ny, w.nz], "npml": w.npml, "deltat": w.dt,
+        "kernel_path": "fused" if fused else "two-phase",
+        "slab_widths": slabs,
+        "bytes_per_cell_update": round(nb["step"] / cells, 3),
+        "step_hbm_gbs": round(step_gbs, 1),
+        "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+        # the same rate priced with the two-phase byte count (each array once per phase)
+        "two_phase_bytes_per_cell_update": round(nb2["step"] / cells, 3),
+        "two_phase_model_frac": round(nb2["step"] / (wall / steps) / 1e9 / HBM_PEAK_GBS, 4),
+        "kernel_ms": {k: round(v, 4) for k, v in avg.items()},
+        "max_velocity_norm": vnorm,
+    }
+    return _line("Gcell-updates/s (3D 768^3 FP32 C-PML time loop)", value, args, steps,
+                 wall / steps * 1e3, "f32", config, _roofline(nb, avg), cpu)
+
+
+def bench_2d(args):
+    import torch
+
+    from seismic_cpml_amd import workloads
+
+    steps = args.steps or 4000
+    nstep = args.warmup + steps
+    n = args.n or 4096
+    w = workloads.layered_2d(n=n, nstep=nstep)
+    s = w.make_solver()
+    if args.tile:
+        s.set_tile(args.tile)
+    slabs = s.slab_widths()
+    wall = _timed_run(s, args.warmup, nstep)
+    torch.cuda.synchronize()
+    value = w.cells * steps / wall / 1e9
+    nb = workloads.bytes_per_step(w, slabs)
+    kt = {name: s.kernel_time(k) for k, name in ((0, "stress"), (1, "velocity"))}
+    avg = {name: ms / max(c, 1) for name, (c, ms) in kt.items()}
+    step_gbs = nb["step"] / (wall / steps) / 1e9
+    s.close()
+    cpu = None
+    if not args.no_cpu_baseline:
+        lam, mu, rho = w.medium_host
+        rows = 1024
+        px, py = w.prof
+        cpu = cpu_baseline_2d(n, rows, 4, w.dx, w.dt, lam[:rows], mu[:rows], rho[:rows], px,
+                              py[:, :rows], *w.force, seconds=args.cpu_seconds,
+                              what="the first rows of the same medium, ")
+    config = {
+        "workload": f"2D isotropic elastic C-PML {n}^2, 4th order, NPML 20, 12 seeded layers, "
+                    "Ricker 10 Hz at the centre, 256 receivers on the top interior row",
+        "grid": [w.nx, w.ny], "npml": w.npml, "deltat": w.dt, "slab_widths": slabs,
+        "tile_rows": args.tile or 4,
+        "bytes_per_cell_update": round(nb["step"] / w.cells, 3),
+        "step_hbm_gbs": round(step_gbs, 1), "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+        "kernel_ms": {k: round(v, 4) for k, v in avg.items()},
+    }
+    return _line("Gcell-updates/s (2D 4096^2 4th-order FP32 C-PML time loop)", value, args,
+                  steps, wall / steps * 1e3, "f32", config, _roofline(nb, avg), cpu)
+
+
+def bench_batch(args):
+    import numpy as np
+    import torch
+
+    from seismic_cpml_amd import workloads
+
+    steps = args.steps or 3000
+    w = workloads.batch_2d(nshots=args.shots, nstep=steps)
+    t = time.perf_counter()
+    b = w.make_batch()
+    setup = time.perf_counter() - t
+    cache = ROOT / "gpurun_out" / "batch_tune.txt" if args.tune_cache is None else args.tune_cache
+    Path(cache).parent.mkdir(parents=True, exist_ok=True)
+    # the autotuner leaves the batch at rest with the fastest launch shape set
+    ty, bpc, tune_ms = b.autotune(trial_steps=min(30, steps), cache_path=str(cache))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.run(1, steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    cells = w.cells
+    value = cells * steps / wall / 1e9
+    seis = [b.seismograms(q) for q in range(len(w.shots))]
+    b.close()
+    # the same shots one after another with the plain solver
+    seq_run, seq_total, same = 0.0, 0.0, 0
+    nseq = len(w.shots) if args.seq_shots is None else min(args.seq_shots, len(w.shots))
+    for q in range(nseq):
+        t0 = time.perf_counter()
+        h = w.make_solver(q)
+        t1 = time.perf_counter()
+        h.run()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        sx, sy = h.seismograms()
+        h.close()
+        seq_run += t2 - t1
+        seq_total += t2 - t0
+        same += int(np.array_equal(sx, seis[q][0]) and np.array_equal(sy, seis[q][1]))
+    seq_cells = sum(s["NX"] * s["NY"] for s in w.shots[:nseq])
+    seq_value = seq_cells * steps / seq_run / 1e9
+    es = 4 if w.prec == 4 else 8
+    bytes_per = es * 18  # two-phase 2D interior words; slabs add under 1%
+    config = {
+        "workload": f"{len(w.shots)} shots of 2D 4th-order C-PML, NX, NY ~ U{{512..2048}} "
+                    "(seeded), NPML 20, per-shot layered model, source and Ricker f0 in "
+                    "[5, 15] Hz, 64 receivers each",
+        "cells_per_step": cells, "setup_s": round(setup, 2),
+        "autotune": {"tile_rows": ty, "blocks_per_cu": bpc, "trial_ms": tune_ms},
+        "sequential": {"shots": nseq, "gcell_s": round(seq_value, 3),
+                       "run_s": round(seq_run, 3), "with_setup_s": round(seq_total, 3),
+                       "bitwise_equal_seismograms": same},
+        "batched_vs_sequential": round(value / seq_value, 3),
+        "step_hbm_frac": round(bytes_per * value / HBM_PEAK_GBS, 4),
+    }
+    roof = {"bound": "hbm", "kernel": "batch step", "achieved": round(bytes_per * value, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(bytes_per * value / HBM_PEAK_GBS, 4), "traffic": None}
+    cpu = None
+    if not args.no_cpu_baseline:
+        s0 = w.shots[0]
+        cpu = cpu_baseline_2d(s0["NX"], s0["NY"], 4, s0["DELTAX"], s0["DELTAT"], s0["lam"],
+                              s0["mu"], s0["rho"], s0["prof_x"], s0["prof_y"], s0["fx"],
+                              s0["fy"], seconds=args.cpu_seconds, what="shot 0, ")
+    return _line("Gcell-updates/s (128-shot batched 2D 4th-order FP32 C-PML)", value, args,
+                 steps, wall / steps * 1e3, "f32", config, roof, cpu)
+
+
+def bench_shipped(args):
+    import torch
+
+    from seismic_cpml_amd import workloads
+
+    w = workloads.shipped_2d(2, 8)
+    s = w.make_solver()
+    wall = _timed_run(s, 0, w.nstep)
+    torch.cuda.synchronize()
+    value = w.cells * w.nstep / wall / 1e9
+    s.close()
+    cpu = None
+    if not args.no_cpu_baseline:
+        lam, mu, rho = w.medium_host
+        cpu = cpu_baseline_2d(w.nx, w.ny, 2, w.dx, w.dt, lam, mu, rho, *w.prof, *w.force,
+                              seconds=args.cpu_seconds, threads=1, what="the shipped grid ")
+    config = {"workload": "seismic_CPML_2D_isotropic_second_order as shipped: 101 x 641, "
+                          "FP64, 2000 steps"}
+    return _line("Gcell-updates/s (2D2 as shipped, FP64)", value, args, w.nstep,
+                 wall / w.nstep * 1e3, "f64", config, None, cpu)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--workload", choices=("3d", "2d", "batch", "shipped"), default="3d")
+    ap.add_argument("--steps", type=int, default=0, help="timed steps (0: the workload's own)")
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--n", type=int, default=768, help="grid points per axis")
+    ap.add_argument("--n", type=int, default=0, help="grid points per axis (3d, 2d)")
     ap.add_argument("--kchunk", type=int, default=0)
-    ap.add_argument("--tile", type=int, default=0, help="rows per 3D tile (4, 8, 16)")
+    ap.add_argument("--tile", type=int, default=0, help="rows per tile (4, 8, 16)")
     ap.add_argument("--two-phase", action="store_true",
-                    help="separate stress and velocity kernels instead of the fused step")
+                    help="3d: separate stress and velocity kernels instead of the fused step")
+    ap.add_argument("--shots", type=int, default=128, help="batch: number of shots")
+    ap.add_argument("--seq-shots", type=int, default=None,
+                    help="batch: shots to rerun sequentially (default all)")
+    ap.add_argument("--tune-cache", default=None, help="batch: autotuner cache file")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -90,8 +352,6 @@ def main():
 
     import torch
 
-    from seismic_cpml_amd import workloads
-
     torch.cuda.set_device(0)
     if args.copy_calibration:
         a = torch.ones(2 ** 29, dtype=torch.float32, device="cuda")
@@ -99,73 +359,8 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    nstep = args.warmup + args.steps
-    w = workloads.layered_3d(n=args.n, nstep=nstep)
-    s = w.make_solver()
-    if args.kchunk:
-        s.set_kchunk(args.kchunk)
-    if not args.two_phase:
-        s.set_fused(True)
-    if args.tile:
-        s.set_tile(args.tile)
-    slabs = s.slab_widths()
-    if args.warmup > 0:
-        s.run(1, args.warmup)
-    s.set_kernel_timing(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    s.run(args.warmup + 1, nstep)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-
-    cells = w.cells
-    value = cells * args.steps / wall / 1e9
-    nb = workloads.bytes_per_step(w, slabs)
-    kt = {name: s.kernel_time(k) for k, name in ((0, "stress"), (1, "velocity"))}
-    avg = {name: ms / max(n, 1) for name, (n, ms) in kt.items()}
-    dom = max(avg, key=avg.get)
-    achieved = nb[dom] / (avg[dom] * 1e-3) / 1e9
-    step_gbs = nb["step"] / (wall / args.steps) / 1e9
-    vnorm = s.velocity_norm()
-    s.close()
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        lam, mu, rho = w.medium_dev
-        cpu = cpu_baseline(w, lam, mu, rho, seconds=args.cpu_seconds)
-
-    line = {
-        "metric": "Gcell-updates/s (3D 768^3 FP32 C-PML time loop)",
-        "value": round(value, 3),
-        "unit": "Gcell/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": f"3D isotropic elastic C-PML {args.n}^3, NPML 20, 2nd order, layered "
-                        "+-5% smoothed medium, Ricker 8 Hz, 1024 receivers",
-            "grid": [w.nx, w.ny, w.nz], "npml": w.npml, "deltat": w.dt,
-            "slab_widths": slabs,
-            "bytes_per_cell_update": round(nb["step"] / cells, 3),
-            "step_hbm_gbs": round(step_gbs, 1),
-            "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 4),
-            "kernel_ms": {k: round(v, 4) for k, v in avg.items()},
-            "max_velocity_norm": vnorm,
-        },
-        "roofline": {
-            "bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-        },
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(line), flush=True)
+    fn = {"3d": bench_3d, "2d": bench_2d, "batch": bench_batch, "shipped": bench_shipped}
+    print(json.dumps(fn[args.workload](args)), flush=True)
 
 
 if __name__ == "__main__":

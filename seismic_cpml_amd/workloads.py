@@ -176,14 +176,16 @@ def layered_3d(n=768, nstep=2000, seed=3, prec=4, courant=0.5, npml=20, dx=10.0,
                     medium_dev=(lam, mu, rho), cp_max=cp_max)
 
 
-def bytes_per_step(w: Workload, slabs) -> dict:
+def bytes_per_step(w: Workload, slabs, fused: bool = False) -> dict:
     """Algorithmic HBM bytes of one time step, per kernel: every array touched
-    once per phase.  3D stress: read v(3), sigma(6), DELTAT*lambda, DELTAT*mu,
+    once per kernel.  3D stress: read v(3), sigma(6), DELTAT*lambda, DELTAT*mu,
     write sigma(6) = 17 words; velocity: read sigma(6), v(3), DELTAT/rho, write
     v(3) = 13 words; each phase reads and writes 3 memory variables per
-    absorbing slab point (6 words).  2D: stress 5 reads (vx, vy, sigma x3) +
-    lambda, mu + 3 writes = 10 words; velocity 5 + rho + 2 writes = 8 words;
-    2 memory variables per slab point per phase (4 words)."""
+    absorbing slab point (6 words).  The fused 3D step reads v(3), sigma(6)
+    and the 3 coefficients and writes v(3), sigma(6): 21 words, plus 12 words
+    per slab point.  2D: stress 5 reads (vx, vy, sigma x3) + lambda, mu + 3
+    writes = 10 words; velocity 5 + rho + 2 writes = 8 words; 2 memory
+    variables per slab point per phase (4 words)."""
     es = 4 if w.prec == 4 else 8
     wx = slabs[0] + slabs[1]
     wy = slabs[2] + slabs[3]
@@ -191,6 +193,9 @@ def bytes_per_step(w: Workload, slabs) -> dict:
     n = w.cells
     if w.ndim == 3:
         slab_pts = wx * w.ny * w.nz + w.nx * wy * w.nz + w.nx * w.ny * wz
+        if fused:
+            f = es * (21 * n + 12 * slab_pts)
+            return {"fused": f, "step": f}
         stress = es * (17 * n + 6 * slab_pts)
         vel = es * (13 * n + 6 * slab_pts)
     else:
@@ -198,3 +203,69 @@ def bytes_per_step(w: Workload, slabs) -> dict:
         stress = es * (10 * n + 4 * slab_pts)
         vel = es * (8 * n + 4 * slab_pts)
     return {"stress": stress, "velocity": vel, "step": stress + vel}
+
+
+@dataclass
+class BatchWorkload:
+    """Per-shot inputs of a batched 2D run (host arrays)."""
+    order: int
+    prec: int
+    nstep: int
+    shots: list  # dicts: NX NY DELTAX DELTAY DELTAT NPOINTS_PML lam mu rho prof_x prof_y
+    #              ISOURCE JSOURCE fx fy ix_rec iy_rec f0
+
+    @property
+    def cells(self) -> int:
+        return sum(s["NX"] * s["NY"] for s in self.shots)
+
+    def make_batch(self, graph_steps=16):
+        b = S.CpmlBatch(self.order, self.prec, self.shots, self.nstep)
+        for q, s in enumerate(self.shots):
+            b.set_medium(q, s["lam"], s["mu"], s["rho"])
+            b.set_pml(q, s["prof_x"], s["prof_y"])
+            b.set_source(q, s["ISOURCE"], s["JSOURCE"], s["fx"], s["fy"])
+            b.set_receivers(q, s["ix_rec"], s["iy_rec"])
+        b.set_tiling(4, 0, graph_steps)
+        return b
+
+    def make_solver(self, q, graph_steps=16):
+        """The plain single-shot solver of shot q (the sequential comparator)."""
+        s = self.shots[q]
+        h = S.CpmlGpu(2, self.order, self.prec, s["NX"], s["NY"], 1, s["DELTAX"], s["DELTAY"],
+                      1.0, s["DELTAT"], s["NPOINTS_PML"], self.nstep)
+        h.set_medium(s["lam"], s["mu"], s["rho"])
+        h.set_pml(s["prof_x"], s["prof_y"])
+        h.set_source(s["ISOURCE"], s["JSOURCE"], 1, s["fx"], s["fy"])
+        h.set_receivers(s["ix_rec"], s["iy_rec"])
+        h.set_graph_steps(graph_steps)
+        return h
+
+
+def batch_2d(nshots=128, nmin=512, nmax=2048, nstep=3000, seed=4, order=4, prec=4,
+             courant=0.5, npml=20, dx=5.0, nrec=64) -> BatchWorkload:
+    """Multi-shot 2D survey: per shot a seeded grid size NX, NY ~ U{nmin..nmax}
+    (uneven, non-square), its own 12-layer model, source position (at least
+    NPML + 10 cells inside) and Ricker f0 ~ U[5, 15] Hz; DELTAT for Courant
+    0.5 of the shot's cp_max; nrec receivers on the top interior row."""
+    dt_ = np.float64 if prec == 8 else np.float32
+    rng = np.random.default_rng(seed)
+    shots = []
+    for _ in range(nshots):
+        nx, ny = (int(v) for v in rng.integers(nmin, nmax + 1, size=2))
+        cpy = _layers(rng, ny)
+        cp = np.broadcast_to(cpy[:, None], (ny, nx))
+        lam, mu, rho = (np.ascontiguousarray(a, dt_) for a in _elastic(cp))
+        cp_max = float(cpy.max())
+        f0 = float(rng.uniform(5.0, 15.0))
+        deltat = courant * dx / (cp_max * math.sqrt(2.0))
+        m = npml + 10
+        isrc, jsrc = int(rng.integers(m, nx - m)), int(rng.integers(m, ny - m))
+        px = S.pml_profile(nx, dx, deltat, npml, cp_max, f0, prec, clamp_alpha=True)
+        py = S.pml_profile(ny, dx, deltat, npml, cp_max, f0, prec, clamp_alpha=False)
+        fx, fy = S.source_time_function(nstep, deltat, f0, 1.2 / f0, 1e7, 135.0, 2, prec)
+        ix = np.linspace(npml + 1, nx - npml, nrec).round().astype(np.int32)
+        iy = np.full(nrec, ny - npml, np.int32)
+        shots.append(dict(NX=nx, NY=ny, DELTAX=dx, DELTAY=dx, DELTAT=deltat, NPOINTS_PML=npml,
+                          lam=lam, mu=mu, rho=rho, prof_x=px, prof_y=py, ISOURCE=isrc,
+                          JSOURCE=jsrc, fx=fx, fy=fy, ix_rec=ix, iy_rec=iy, f0=f0))
+    return BatchWorkload(order, prec, nstep, shots)
