@@ -204,6 +204,63 @@ int cpml_batch_get_field(cpml_batch *b, int shot, int which, void *out);
 int cpml_batch_elapsed_ms(cpml_batch *b, double *ms);
 int cpml_batch_last_error(cpml_batch *b, char *buf, int len);
 
+/* ---- learned thin absorbing boundary (3D, FP32) ------------------------ */
+
+/* The reference absorbs outgoing waves with a C-PML of NPOINTS_PML cells
+ * (3D:399-667 profiles, 3D:836-1052 memory variables).  The learned form
+ * replaces it by a 4-cell strip on every face: the grid is the interior plus
+ * the strip, updated without C-PML (profiles with a = 0, K = 1), and after
+ * each step (after the source and the rigid edges, before the receivers) a
+ * per-face corrector adds a correction to the strip cells the face owns
+ * (the nearest face; ties to the lowest of x-, x+, y-, y+, z-, z+).
+ *
+ * Corrector of one face: three linear 3x3x3 convolutions 9 -> 16 -> 16 -> 9
+ * channels (no bias, no activation) over a 7-cell-deep box along the face,
+ * inputs = the 9 fields divided by scales[9]; its output times the scales is
+ * added to the fields.  Weights: float[6 faces][3 layers][27 taps][16 in]
+ * [16 out], tap = (dd+1)*9 + (da+1)*3 + (db+1) for offsets along the inward
+ * depth and the face's two tangential axes in increasing axis order; input
+ * channels 9..15 of layer 1 and output channels 9..15 of layer 3 are padding
+ * (keep them 0).  Inference uses BF16 weights and activations with FP32
+ * accumulation (MFMA); training runs in FP32. */
+typedef struct cpml_abc_net cpml_abc_net;
+#define CPML_BOUNDARY_CPML 0
+#define CPML_BOUNDARY_LEARNED 1
+#define CPML_ABC_WEIGHTS (6 * 3 * 27 * 16 * 16)
+
+/* weights may be NULL (all zero: the corrector then leaves the fields as the
+ * plain step made them). */
+int cpml_abc_net_create(cpml_abc_net **net, const float *weights, const float *scales);
+int cpml_abc_net_destroy(cpml_abc_net *net);
+int cpml_abc_net_get_weights(cpml_abc_net *net, float *weights);
+int cpml_abc_net_set_weights(cpml_abc_net *net, const float *weights);
+/* mode CPML_BOUNDARY_LEARNED attaches net (not owned; keep it alive while
+ * attached) to a 3D FP32 handle whose profiles have no absorption;
+ * CPML_BOUNDARY_CPML detaches it. */
+int cpml_gpu_set_boundary(cpml_solver *h, int mode, cpml_abc_net *net);
+/* Adam optimiser state (zeroed). */
+int cpml_abc_train_init(cpml_abc_net *net, double lr, double beta1, double beta2, double eps);
+/* One training step on one time step of a teacher run: the student grid
+ * (no C-PML, no corrector, no source) is the window of the teacher starting
+ * at 0-based cell (i0, j0, k0).  Loads the teacher's current state into the
+ * student, steps the student once (P), steps the teacher once (step `it`,
+ * T), regresses the corrector's output on (T - P) over the owned strip cells
+ * and takes one Adam step.  loss6 (may be NULL): per-face loss
+ * sum (y - target)^2 / sum x^2 in scaled units. */
+int cpml_abc_train_step(cpml_abc_net *net, cpml_solver *student, cpml_solver *teacher, int i0,
+                        int j0, int k0, int it, double *loss6);
+/* Loss and gradient (float[CPML_ABC_WEIGHTS]) of the current student state
+ * (taken as P) against the teacher's current state (taken as T); no step, no
+ * update.  Needs cpml_abc_train_init. */
+int cpml_abc_grad(cpml_abc_net *net, cpml_solver *student, cpml_solver *teacher, int i0, int j0,
+                  int k0, double *loss6, float *grad);
+/* Corrector output for the current state of h, before ownership and scaling:
+ * float[face][field 9][depth 4][b][a] (each face's block 9*4*na*nb),
+ * computed by the FP32 training path (use_mfma = 0) or the BF16 MFMA
+ * inference kernel (use_mfma = 1). */
+int cpml_abc_correction(cpml_abc_net *net, cpml_solver *h, int use_mfma, float *out);
+int cpml_abc_last_error(cpml_abc_net *net, char *buf, int len);
+
 /* ---- host-side set-up helpers (no GPU needed) -------------------------- */
 
 /* C-PML profile along one axis (2D2:283-394 for x; 2D2:396-465 for y;

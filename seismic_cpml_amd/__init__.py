@@ -5,15 +5,17 @@ the C ABI in include/cpml_gpu.h), bound from Fortran by the ``cpml_gpu``
 module (fortran/cpml_gpu.f90) and from Python by :mod:`.solver`.
 """
 from .solver import (  # noqa: F401
+    CpmlAbcNet,
     CpmlBatch,
     CpmlError,
     CpmlGpu,
     courant_number,
     library,
+    no_absorption,
     nearest_receivers,
     pml_profile,
     source_time_function,
 )
 
-__all__ = ["CpmlBatch", "CpmlError", "CpmlGpu", "courant_number", "library", "nearest_receivers",
+__all__ = ["CpmlAbcNet", "CpmlBatch", "CpmlError", "CpmlGpu", "courant_number", "library", "nearest_receivers", "no_absorption",
            "pml_profile", "source_time_function"]

@@ -155,6 +155,8 @@ struct cpml_solver {
   double last_ms = 0;
   double last_vnorm = 0;
   int host_step = 0;
+  cpml_abc_net *abc = nullptr;  // learned boundary corrector (not owned)
+  void *abc_corr = nullptr;     // its per-step correction buffer
   std::string err;
 };
 
@@ -426,6 +428,36 @@ hipError_t enqueue_tail(cpml_solver *h) {
   return cpml::launch_record<T>(record_of<T>(h), h->stream);
 }
 
+cpml::SolverView view_of(cpml_solver *h) {
+  cpml::SolverView v{};
+  v.ndim = h->ndim;
+  v.order = h->order;
+  v.prec = h->prec;
+  v.nx = h->nx;
+  v.ny = h->ny;
+  v.nz = h->nz;
+  v.ld = h->ld;
+  v.ps = h->ps;
+  for (int q = 0; q < 9; ++q)
+    v.f[q] = fbuf(h, q) ? static_cast<char *>(fbuf(h, q)) + h->base * h->esize : nullptr;
+  const cpml::Slab *sl[3] = {&h->sx, &h->sy, &h->sz};
+  for (int a = 0; a < 3; ++a) {
+    v.slab_w[2 * a] = sl[a]->wl;
+    v.slab_w[2 * a + 1] = sl[a]->wr;
+  }
+  v.learned = h->abc != nullptr;
+  v.stream = h->stream;
+  v.device = h->device;
+  return v;
+}
+
+// learned boundary: correct the strip after the velocity phase (source and
+// rigid edges included), before receivers and energy see the state
+hipError_t enqueue_boundary(cpml_solver *h) {
+  if (!h->abc) return hipSuccess;
+  return cpml::abc_enqueue(h->abc, static_cast<float *>(h->abc_corr), view_of(h));
+}
+
 template <typename T>
 hipError_t enqueue_step(cpml_solver *h) {
   hipError_t e;
@@ -435,6 +467,7 @@ hipError_t enqueue_step(cpml_solver *h) {
     e = enqueue_stress<T>(h);
     if (e == hipSuccess) e = enqueue_velocity<T>(h);
   }
+  if (e == hipSuccess) e = enqueue_boundary(h);
   if (e == hipSuccess) e = enqueue_tail<T>(h);
   return e;
 }
@@ -504,6 +537,7 @@ int enqueue_steps(cpml_solver *h, int n, int *ev_used) {
         HIPCHK(h, h->prec == 4 ? enqueue_velocity<float>(h) : enqueue_velocity<double>(h));
       }
       HIPCHK(h, hipEventRecord(e[2], h->stream));
+      HIPCHK(h, enqueue_boundary(h));
       HIPCHK(h, h->prec == 4 ? enqueue_tail<float>(h) : enqueue_tail<double>(h));
       *ev_used += 3;
     }
@@ -1066,6 +1100,10 @@ int cpml_gpu_run(cpml_solver *h, int it_begin, int it_end) {
     return fail(h, CPML_ERR_ARG, "steps must satisfy 1 <= it_begin <= it_end <= NSTEP");
   int rc = set_device(h);
   if (rc) return rc;
+  if (h->abc) {
+    std::string m;
+    if ((rc = cpml::abc_prepare(h->abc, view_of(h), &m))) return fail(h, rc, m);
+  }
   h->host_step = it_begin;
   HIPCHK(h, hipMemcpyAsync(h->step, &h->host_step, sizeof(int), hipMemcpyHostToDevice,
                            h->stream));
@@ -1224,6 +1262,35 @@ int cpml_gpu_set_kchunk(cpml_solver *h, int kchunk) {
   return CPML_OK;
 }
 
+int cpml_gpu_set_boundary(cpml_solver *h, int mode, cpml_abc_net *net) {
+  if (!h) return CPML_ERR_ARG;
+  int rc = set_device(h);
+  if (rc) return rc;
+  if (mode == CPML_BOUNDARY_CPML) {
+    h->abc = nullptr;
+    dev_free(h, h->abc_corr);
+    h->graph_dirty = true;
+    return CPML_OK;
+  }
+  if (mode != CPML_BOUNDARY_LEARNED || !net)
+    return fail(h, CPML_ERR_ARG, "mode must be CPML_BOUNDARY_CPML or CPML_BOUNDARY_LEARNED "
+                                 "with a corrector");
+  if (!h->have_pml) return fail(h, CPML_ERR_STATE, "call set_pml (no absorption) first");
+  if (h->sx.w || h->sy.w || h->sz.w)
+    return fail(h, CPML_ERR_ARG, "the learned boundary replaces the C-PML: set profiles with "
+                                 "a = 0 and K = 1 everywhere");
+  std::string m;
+  if ((rc = cpml::abc_prepare(net, view_of(h), &m))) return fail(h, rc, m);
+  dev_free(h, h->abc_corr);
+  if ((rc = dev_alloc(h, &h->abc_corr,
+                      (size_t)cpml::abc_corr_floats(view_of(h)) * sizeof(float))))
+    return rc;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->abc = net;
+  h->graph_dirty = true;
+  return CPML_OK;
+}
+
 int cpml_gpu_set_fused(cpml_solver *h, int on) {
   if (!h) return CPML_ERR_ARG;
   if (on && h->ndim != 3) return fail(h, CPML_ERR_ARG, "the fused step is 3D only");
@@ -1366,3 +1433,9 @@ int cpml_courant_number(int ndim, double cp, double deltat, double deltax, doubl
 }
 
 }  // extern "C"
+
+int cpml::solver_view(cpml_solver *h, cpml::SolverView *v) {
+  if (!h || !v) return CPML_ERR_ARG;
+  *v = view_of(h);
+  return CPML_OK;
+}

@@ -18,6 +18,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <string>
+
+#include "../../include/cpml_gpu.h"
 
 namespace cpml {
 
@@ -145,6 +148,25 @@ struct Energy {
   double *out_kin, *out_pot;  // [nstep]
   const int *step;
 };
+
+// What the learned-boundary code needs of a solver handle: sizes, strides and
+// the current state (interior origin of each field), its stream and device.
+struct SolverView {
+  int ndim, order, prec, nx, ny, nz;
+  long ld, ps;
+  void *f[9];  // vx vy vz sxx syy szz sxy sxz syz
+  int slab_w[6];
+  int learned;  // a boundary corrector is attached
+  hipStream_t stream;
+  int device;
+};
+
+// defined in cpml_api.hip
+int solver_view(cpml_solver *h, SolverView *v);
+// defined in cpml_abc.hip: correction buffer size, weight upload, per-step launch
+long abc_corr_floats(const SolverView &v);
+int abc_prepare(cpml_abc_net *n, const SolverView &v, std::string *err);
+hipError_t abc_enqueue(cpml_abc_net *n, float *corr, const SolverView &v);
 
 // Kernel launchers (defined in the .hip files).
 template <typename T>

@@ -1,0 +1,232 @@
+"""Learned thin absorbing boundary: training and evaluation on one GPU.
+
+The corrector (libcpml_gpu, cpml_abc_*) replaces a 20-cell C-PML by a 4-cell
+strip.  This module drives it through the C ABI:
+
+* ``layered_model``   -- seeded random layered medium (cp ~ U[1500, 5500],
+  cs = cp / sqrt(3), rho = 310 cp^0.25, layers along z), generated on the GPU.
+* ``train``           -- online training on wide-C-PML teacher runs of this
+  solver (interior n^3 + 20-cell C-PML; the student is the interior + 4 cells,
+  i.e. the teacher window starting 16 cells in).
+* ``reflection``      -- reflected energy of a boundary in dB: the test run
+  and a reference run in a doubled domain (2n interior + 20-cell C-PML, the
+  same medium extended by its edge values) record the three velocity
+  components on an interior lattice; reflected energy = sum (u_test -
+  u_ref)^2 over all receivers and steps, relative to sum u_ref^2.
+* ``time_to_solution`` -- wall time of the learned form ((n + 8)^3 plus the
+  corrector) against the plain 20-cell C-PML ((n + 40)^3), same steps.
+
+Torch is used only to build media on the device (plumbing); every step and
+every training kernel is in libcpml_gpu.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import solver as S
+
+NPML_WIDE = 20
+STRIP = 4
+OFFSET = NPML_WIDE - STRIP  # student/teacher window offset (cells)
+
+
+def field_scales(z_ref: float = 8.0e6):
+    """Velocities unscaled, stresses divided by a typical impedance rho*cp."""
+    return np.array([1, 1, 1] + [z_ref] * 6, np.float32)
+
+
+def init_weights(seed: int = 0, std: float = 0.05):
+    """Random FP32 weights, padding channels zero."""
+    rng = np.random.default_rng(seed)
+    w = (rng.standard_normal((6, 3, 27, 16, 16)) * std).astype(np.float32)
+    w[:, 0, :, 9:, :] = 0  # layer 1 inputs 9..15
+    w[:, 2, :, :, 9:] = 0  # layer 3 outputs 9..15
+    return w
+
+
+@dataclass
+class Model:
+    cpz: np.ndarray      # cp per z index of the interior
+    dx: float
+    dt: float
+    f0: float
+    src: tuple           # 0-based interior cell of the source
+
+    @property
+    def n(self):
+        return len(self.cpz)
+
+
+def layered_model(n: int, seed: int, dx=10.0, courant=0.5, f0=None, nlayers=12,
+                  src_margin=12) -> Model:
+    rng = np.random.default_rng(seed)
+    cuts = np.sort(rng.integers(1, n, size=nlayers - 1))
+    cps = rng.uniform(1500.0, 5500.0, size=nlayers)
+    cpz = cps[np.searchsorted(cuts, np.arange(n), side="right")]
+    f0 = float(rng.uniform(5.0, 15.0)) if f0 is None else float(f0)
+    src = tuple(int(v) for v in rng.integers(src_margin, n - src_margin, size=3))
+    dt = courant * dx / (5500.0 * math.sqrt(3.0))  # Courant 0.5 of the fastest possible cp
+    return Model(cpz, dx, dt, f0, src)
+
+
+def _medium(cpz_ext, nxy, device="cuda"):
+    """lam, mu, rho torch tensors (nz, ny, nx) for a z-layered cp profile."""
+    import torch
+
+    cp = torch.tensor(cpz_ext, dtype=torch.float64, device=device)
+    cs = cp / math.sqrt(3.0)
+    rho = 310.0 * cp ** 0.25
+    mu = rho * cs * cs
+    lam = rho * (cp * cp - 2.0 * cs * cs)
+    out = []
+    for a in (lam, mu, rho):
+        out.append(a.float().view(-1, 1, 1).expand(-1, nxy[1], nxy[0]).contiguous())
+    return out
+
+
+def make_run(model: Model, pad: int, absorb: int, nstep: int, ext: int = 0,
+             receivers=None, source=True):
+    """A solver over the interior grown by ``ext`` cells of extended medium
+    on every side, then ``pad`` boundary cells, of which the outer ``absorb``
+    are a C-PML (0: no absorption).  Returns (solver, offset of interior
+    cell 0, media tensors kept alive)."""
+    n_int = model.n + 2 * ext
+    n = n_int + 2 * pad
+    off = pad + ext
+    z = np.clip(np.arange(n) - off, 0, model.n - 1)
+    lam, mu, rho = _medium(model.cpz[z], (n, n))
+    s = S.CpmlGpu(3, 2, 4, n, n, n, model.dx, model.dx, model.dx, model.dt,
+                  max(absorb, 1), nstep)
+    s.set_medium_device(lam.data_ptr(), mu.data_ptr(), rho.data_ptr())
+    cp_max = float(model.cpz.max())
+    if absorb > 0:
+        # the C-PML of exactly `absorb` cells at the outer edge of the grid
+        prof = S.pml_profile(n, model.dx, model.dt, absorb, cp_max, model.f0, 4,
+                             clamp_alpha=False)
+        s.set_pml(prof, prof, prof)
+    else:
+        p0 = S.no_absorption(n)
+        s.set_pml(p0, p0, p0)
+    if source:
+        fx, fy = S.source_time_function(nstep, model.dt, model.f0, 1.2 / model.f0, 1e7, 135.0,
+                                        2, 4)
+        i, j, k = (c + off + 1 for c in model.src)
+        s.set_source(i, j, k, fx, fy)
+    else:
+        s.set_source(0, 0, 0, np.zeros(nstep, np.float32), np.zeros(nstep, np.float32))
+    if receivers is not None:
+        ix, iy, iz = (np.asarray(r, np.int32) + off + 1 for r in receivers)
+        s.set_receivers(ix, iy, iz)
+    else:
+        s.set_receivers(np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.int32))
+    return s, off, (lam, mu, rho)
+
+
+def train(net: S.CpmlAbcNet, n: int = 128, runs: int = 16, nstep: int = 600,
+          first_step: int = 30, stride: int = 1, seed: int = 1000, log=None):
+    """Online training: for each seeded model a wide-C-PML teacher run
+    (interior n^3 + 20 cells) and a plain student grid (n + 8)^3.  Returns the
+    mean loss per run."""
+    history = []
+    for r in range(runs):
+        m = layered_model(n, seed + r)
+        teacher, _, keep_t = make_run(m, NPML_WIDE, NPML_WIDE, nstep)
+        teacher.set_fused(True)
+        student, _, keep_s = make_run(m, STRIP, 0, 1, source=False)
+        teacher.run(1, first_step - 1)
+        losses = []
+        for it in range(first_step, nstep + 1):
+            if (it - first_step) % stride:
+                teacher.run(it, it)
+                continue
+            losses.append(float(net.train_step(student, teacher, (OFFSET,) * 3, it).sum()))
+        teacher.close()
+        student.close()
+        del keep_t, keep_s
+        history.append(float(np.mean(losses)))
+        if log:
+            log(f"run {r}: f0 {m.f0:.1f} Hz, src {m.src}, mean loss {history[-1]:.4e}")
+    return history
+
+
+def lattice(n: int, per_axis: int = 8, margin: int = 4):
+    """Receiver lattice over the interior (0-based cells)."""
+    c = np.linspace(margin, n - 1 - margin, per_axis).round().astype(np.int32)
+    z, y, x = np.meshgrid(c, c, c, indexing="ij")
+    return x.ravel(), y.ravel(), z.ravel()
+
+
+def _seis(s):
+    return np.stack(s.seismograms())  # (3, nrec, nstep)
+
+
+def reflection(model: Model, nstep: int, net: S.CpmlAbcNet | None = None, per_axis: int = 8,
+               reference=None):
+    """Reflected energy (dB) and seismogram rel-L2 of the learned strip, the
+    plain 20-cell C-PML and the rigid (no absorption) strip, each against the
+    doubled-domain reference.  Returns a dict (and the reference seismograms
+    for reuse)."""
+    rec = lattice(model.n, per_axis)
+    if reference is None:
+        ref, _, keep = make_run(model, NPML_WIDE, NPML_WIDE, nstep, ext=model.n // 2,
+                                receivers=rec)
+        ref.set_fused(True)
+        ref.run()
+        reference = _seis(ref)
+        ref.close()
+        del keep
+    e_ref = float((reference.astype(np.float64) ** 2).sum())
+    out = {"reference_energy": e_ref}
+
+    def score(u):
+        err = float(((u.astype(np.float64) - reference) ** 2).sum())
+        return {"db": 10.0 * math.log10(max(err, 1e-300) / e_ref),
+                "rel_l2": math.sqrt(err / e_ref)}
+
+    runs = [("cpml20", NPML_WIDE, NPML_WIDE, None), ("rigid4", STRIP, 0, None)]
+    if net is not None:
+        runs.append(("learned4", STRIP, 0, net))
+    for name, pad, absorb, nn in runs:
+        s, _, keep = make_run(model, pad, absorb, nstep, receivers=rec)
+        if nn is not None:
+            s.set_boundary(nn)
+        s.run()
+        u = _seis(s)
+        out[name] = score(u)
+        out[name]["finite"] = bool(np.isfinite(u).all())
+        s.close()
+        del keep
+    return out, reference
+
+
+def time_to_solution(model: Model, nstep: int, net: S.CpmlAbcNet, repeats: int = 1):
+    """Wall time of nstep steps: learned strip ((n+8)^3 + corrector) against
+    the 20-cell C-PML ((n+40)^3), both with the fused step kernel, graphs,
+    no receivers."""
+    import torch
+
+    res = {}
+    for name, pad, absorb, nn in (("cpml20", NPML_WIDE, NPML_WIDE, None),
+                                  ("learned4", STRIP, 0, net)):
+        s, _, keep = make_run(model, pad, absorb, nstep)
+        s.set_fused(True)
+        if nn is not None:
+            s.set_boundary(nn)
+        s.run(1, min(16, nstep))  # build graphs
+        best = None
+        for _ in range(repeats):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            s.run(1, nstep)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            best = el if best is None else min(best, el)
+        res[name] = {"seconds": best, "cells": (model.n + 2 * pad) ** 3}
+        s.close()
+        del keep
+    res["speedup"] = res["cpml20"]["seconds"] / res["learned4"]["seconds"]
+    return res

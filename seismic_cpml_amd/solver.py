@@ -78,6 +78,16 @@ SIGNATURES = {
     "cpml_batch_get_field": [_VP, _I, _I, _VP],
     "cpml_batch_elapsed_ms": [_VP, _DP],
     "cpml_batch_last_error": [_VP, C.c_char_p, _I],
+    "cpml_abc_net_create": [C.POINTER(_VP), _VP, _VP],
+    "cpml_abc_net_destroy": [_VP],
+    "cpml_abc_net_get_weights": [_VP, _VP],
+    "cpml_abc_net_set_weights": [_VP, _VP],
+    "cpml_gpu_set_boundary": [_VP, _I, _VP],
+    "cpml_abc_train_init": [_VP, _D, _D, _D, _D],
+    "cpml_abc_train_step": [_VP, _VP, _VP, _I, _I, _I, _I, _VP],
+    "cpml_abc_grad": [_VP, _VP, _VP, _I, _I, _I, _VP, _VP],
+    "cpml_abc_correction": [_VP, _VP, _I, _VP],
+    "cpml_abc_last_error": [_VP, C.c_char_p, _I],
     "cpml_pml_profile": [_I, _I, _D, _D, _I, _D, _D, _I, _I, _I, _I] + [_VP] * 6,
     "cpml_source_time_function": [_I, _I, _D, _D, _D, _D, _D, _I, _VP, _VP],
     "cpml_nearest_receivers": [_I, _I, _I, _D, _D, _I, _D, _D, _D, _D, _VP, _VP],
@@ -273,6 +283,16 @@ class CpmlGpu:
     def set_fused(self, on=True):
         self._call("cpml_gpu_set_fused", int(on))
 
+    def set_boundary(self, net=None):
+        """Attach a learned strip corrector (:class:`CpmlAbcNet`), or detach
+        with None.  The handle must have been given profiles without
+        absorption (see :func:`no_absorption`)."""
+        if net is None:
+            self._call("cpml_gpu_set_boundary", 0, None)
+        else:
+            self._call("cpml_gpu_set_boundary", 1, net._h)
+        self._abc = net
+
     def set_kernel_timing(self, on=True):
         self._call("cpml_gpu_set_kernel_timing", int(on))
 
@@ -331,6 +351,109 @@ class CpmlGpu:
         w = (C.c_int * 6)()
         self._call("cpml_gpu_slab_widths", w)
         return list(w)
+
+
+def no_absorption(n, prec=4):
+    """(6, n) profile without damping: a = b = 0, K = 1 (the grid is then
+    updated by the plain staggered-grid step)."""
+    p = np.zeros((6, n), _dtype(prec))
+    p[2] = 1
+    p[5] = 1
+    return p
+
+
+# ------------------------------------------------------- learned boundary --
+
+ABC_WEIGHTS = 6 * 3 * 27 * 16 * 16
+
+
+class CpmlAbcNet:
+    """Per-face strip corrector of the learned thin absorbing boundary
+    (cpml_abc_*).  weights: float32 (6, 3, 27, 16, 16) [face][layer][tap]
+    [in][out]; scales: 9 positive field scales (vx vy vz sxx syy szz sxy sxz
+    syz)."""
+
+    def __init__(self, scales, weights=None):
+        sc = np.ascontiguousarray(scales, np.float32)
+        if sc.shape != (9,):
+            raise ValueError("scales must have 9 values")
+        w = None if weights is None else self._w(weights)
+        self._h = C.c_void_p()
+        self.scales = sc
+        _check(library().cpml_abc_net_create(C.byref(self._h), _ptr(w), _ptr(sc)), None,
+               "cpml_abc_last_error")
+
+    @staticmethod
+    def _w(weights):
+        w = np.ascontiguousarray(weights, np.float32)
+        if w.size != ABC_WEIGHTS:
+            raise ValueError(f"weights must have {ABC_WEIGHTS} values")
+        return w
+
+    def _call(self, fn, *args):
+        _check(getattr(library(), fn)(self._h, *args), self._h, "cpml_abc_last_error")
+
+    def close(self):
+        if self._h:
+            library().cpml_abc_net_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def weights(self):
+        w = np.zeros((6, 3, 27, 16, 16), np.float32)
+        self._call("cpml_abc_net_get_weights", _ptr(w))
+        return w
+
+    @weights.setter
+    def weights(self, w):
+        w = self._w(w)
+        self._call("cpml_abc_net_set_weights", _ptr(w))
+
+    def train_init(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        self._call("cpml_abc_train_init", float(lr), float(beta1), float(beta2), float(eps))
+
+    def train_step(self, student, teacher, offset, it):
+        loss = np.zeros(6)
+        self._call("cpml_abc_train_step", student._h, teacher._h, *[int(o) for o in offset],
+                   int(it), _ptr(loss))
+        return loss
+
+    def grad(self, student, teacher, offset):
+        loss = np.zeros(6)
+        g = np.zeros((6, 3, 27, 16, 16), np.float32)
+        self._call("cpml_abc_grad", student._h, teacher._h, *[int(o) for o in offset],
+                   _ptr(loss), _ptr(g))
+        return loss, g
+
+    def correction(self, h, use_mfma=True):
+        """Corrector output for h's current state: list of 6 arrays
+        (9, 4, nb, na) in scaled units."""
+        n = (h.nx, h.ny, h.nz)
+        shapes = []
+        for f in range(6):
+            ax = f // 2
+            ta, tb = (1, 2) if ax == 0 else ((0, 2) if ax == 1 else (0, 1))
+            shapes.append((9, 4, n[tb], n[ta]))
+        out = np.zeros(sum(int(np.prod(s)) for s in shapes), np.float32)
+        self._call("cpml_abc_correction", h._h, int(bool(use_mfma)), _ptr(out))
+        res, o = [], 0
+        for s in shapes:
+            k = int(np.prod(s))
+            res.append(out[o:o + k].reshape(s))
+            o += k
+        return res
 
 
 # ------------------------------------------------------------------ batch --

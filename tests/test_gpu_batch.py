@@ -41,6 +41,10 @@ def shots_for(order, prec, nstep=NSTEP):
     s0 = _shot_from_case(lay, prec, nstep)
     s1 = dict(s0, ISOURCE=30, JSOURCE=70)
     s2 = _shot_from_case(shp, prec, nstep)
+    if order == 8:
+        # the shipped DELTAT is beyond the 8th-order operator's stability
+        # limit (Courant 0.93 times Holberg's sum |c_k| = 1.36)
+        s2["DELTAT"] = s2["DELTAT"] / 2
     v = values(lay, prec)
     nx, ny, npml = 70, 50, 8
     s3 = dict(s0, NX=nx, NY=ny, NPOINTS_PML=npml,
