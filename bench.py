@@ -325,10 +325,72 @@ def bench_shipped(args):
                  wall / w.nstep * 1e3, "f64", config, None, cpu)
 
 
+def bench_abc(args):
+    """Learned thin boundary against the 20-cell C-PML: time to solution of
+    the same steps on the 3D interior n^3, reflected energy against a
+    doubled-domain reference, and the corrector's MFMA rate."""
+    from seismic_cpml_amd import learned_abc as L
+
+    n = args.n or 384
+    if args.abc_weights and Path(args.abc_weights).is_file():
+        net = L.load_net(args.abc_weights)
+        trained = f"weights from {Path(args.abc_weights).name}"
+    else:
+        net = L.S.CpmlAbcNet(L.field_scales(), L.init_weights(args.abc_seed, std=0.02))
+        net.train_init(lr=2e-3, batch=8)
+        t = time.perf_counter()
+        L.train(net, n=128, runs=args.abc_runs, nstep=600, seed=args.abc_seed)
+        trained = f"trained here: {args.abc_runs} teacher runs of 128^3, {time.perf_counter() - t:.0f} s"
+    m = L.layered_model(n, seed=args.abc_seed + 7777, f0=10.0)
+    nstep = args.steps or 3 * n
+    tts = L.time_to_solution(m, nstep, net)
+    # corrector alone: the learned-size grid without the corrector
+    rig, _, keep = L.make_run(m, L.STRIP, 0, nstep)
+    rig.set_fused(True)
+    rig.run(1, min(16, nstep))
+    import torch
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rig.run(1, nstep)
+    torch.cuda.synchronize()
+    t_rigid = time.perf_counter() - t0
+    rig.close()
+    del keep
+    corr_s = (tts["learned4"]["seconds"] - t_rigid) / nstep
+    flops = L.corrector_flops(n + 2 * L.STRIP)
+    refl, _ = L.reflection(m, nstep, net, per_axis=args.abc_lattice)
+    tflops = flops / max(corr_s, 1e-12) / 1e12
+    config = {
+        "workload": f"3D {n}^3 interior FP32, layered seeded medium, Ricker 10 Hz, {nstep} steps:"
+                    f" learned 4-cell strip ({n + 8}^3 + corrector) vs 20-cell C-PML "
+                    f"({n + 40}^3); reflection against a doubled-domain reference",
+        "corrector": trained,
+        "time_s": {"cpml20": round(tts["cpml20"]["seconds"], 4),
+                   "learned4": round(tts["learned4"]["seconds"], 4),
+                   "plain_strip_grid_without_corrector": round(t_rigid, 4)},
+        "corrector_ms_per_step": round(corr_s * 1e3, 4),
+        "reflection_db": {k: round(v["db"], 2) for k, v in refl.items() if isinstance(v, dict)},
+        "seismogram_rel_l2": {k: round(v["rel_l2"], 5) for k, v in refl.items()
+                              if isinstance(v, dict)},
+    }
+    roof = {"bound": "mfma", "kernel": "strip corrector (bf16 MFMA, fp32 accumulate)",
+            "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
+            "frac": round(tflops / 2500.0, 4), "traffic": None}
+    net.close()
+    return {
+        "metric": "learned-ABC vs 20-cell C-PML time-to-solution (3D 384^3 FP32)",
+        "value": round(tts["speedup"], 4), "unit": "x", "n_gpus": 1, "steps": nstep,
+        "warmup": 16, "ms_per_step": round(tts["learned4"]["seconds"] / nstep * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic", "config": config, "roofline": roof, "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("3d", "2d", "batch", "shipped"), default="3d")
+    ap.add_argument("--workload", choices=("3d", "2d", "batch", "shipped", "abc"), default="3d")
     ap.add_argument("--steps", type=int, default=0, help="timed steps (0: the workload's own)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=0, help="grid points per axis (3d, 2d)")
@@ -340,6 +402,12 @@ def main():
     ap.add_argument("--seq-shots", type=int, default=None,
                     help="batch: shots to rerun sequentially (default all)")
     ap.add_argument("--tune-cache", default=None, help="batch: autotuner cache file")
+    ap.add_argument("--abc-weights", default=str(ROOT / "seismic_cpml_amd" / "data" /
+                                                 "abc_weights.npz"),
+                    help="abc: trained corrector (trains a short one when missing)")
+    ap.add_argument("--abc-runs", type=int, default=8, help="abc: teacher runs if training")
+    ap.add_argument("--abc-seed", type=int, default=1000)
+    ap.add_argument("--abc-lattice", type=int, default=8, help="abc: receivers per axis")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -359,7 +427,8 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    fn = {"3d": bench_3d, "2d": bench_2d, "batch": bench_batch, "shipped": bench_shipped}
+    fn = {"3d": bench_3d, "2d": bench_2d, "batch": bench_batch, "shipped": bench_shipped,
+          "abc": bench_abc}
     print(json.dumps(fn[args.workload](args)), flush=True)
 
 

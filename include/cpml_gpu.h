@@ -238,20 +238,23 @@ int cpml_abc_net_set_weights(cpml_abc_net *net, const float *weights);
  * attached) to a 3D FP32 handle whose profiles have no absorption;
  * CPML_BOUNDARY_CPML detaches it. */
 int cpml_gpu_set_boundary(cpml_solver *h, int mode, cpml_abc_net *net);
-/* Adam optimiser state (zeroed). */
-int cpml_abc_train_init(cpml_abc_net *net, double lr, double beta1, double beta2, double eps);
+/* Adam optimiser state (zeroed); one Adam step every `batch` training steps. */
+int cpml_abc_train_init(cpml_abc_net *net, double lr, double beta1, double beta2, double eps,
+                        int batch);
 /* One training step on one time step of a teacher run: the student grid
  * (no C-PML, no corrector, no source) is the window of the teacher starting
  * at 0-based cell (i0, j0, k0).  Loads the teacher's current state into the
  * student, steps the student once (P), steps the teacher once (step `it`,
- * T), regresses the corrector's output on (T - P) over the owned strip cells
- * and takes one Adam step.  loss6 (may be NULL): per-face loss
- * sum (y - target)^2 / sum x^2 in scaled units. */
+ * T) and adds the gradient of this sample to the mini-batch; the batch's
+ * Adam step minimises, per face, sum (y - target)^2 / sum target^2 over the
+ * owned strip cells of its samples (target = (T - P) / scales).  loss6 (may
+ * be NULL): this sample's per-face relative error before the update. */
 int cpml_abc_train_step(cpml_abc_net *net, cpml_solver *student, cpml_solver *teacher, int i0,
                         int j0, int k0, int it, double *loss6);
-/* Loss and gradient (float[CPML_ABC_WEIGHTS]) of the current student state
- * (taken as P) against the teacher's current state (taken as T); no step, no
- * update.  Needs cpml_abc_train_init. */
+/* Per-face relative error and its gradient (float[CPML_ABC_WEIGHTS]) for the
+ * current student state (taken as P) against the teacher's current state
+ * (taken as T); no step, no update (a partial mini-batch is dropped).  Needs
+ * cpml_abc_train_init. */
 int cpml_abc_grad(cpml_abc_net *net, cpml_solver *student, cpml_solver *teacher, int i0, int j0,
                   int k0, double *loss6, float *grad);
 /* Corrector output for the current state of h, before ownership and scaling:

@@ -154,11 +154,11 @@ __device__ __forceinline__ double block_sum(double v) {
 // per-face normaliser sum(x^2).
 __global__ __launch_bounds__(TB) void k_gather(Geo g, Fields s, Fields t, long tld, long tps,
                                                long toff, Scales inv, float *X, float *T,
-                                               float *M, double *norm) {
+                                               float *M, double *norm, double *tnorm) {
   const int f = blockIdx.y;
   const Face &F = g.f[f];
   const int q = blockIdx.x * TB + threadIdx.x;
-  double acc = 0;
+  double acc = 0, tacc = 0;
   if (q < D * F.na * F.nb) {
     const int a = q % F.na, b = (q / F.na) % F.nb, d = q / (F.na * F.nb);
     int c[3];
@@ -177,15 +177,24 @@ __global__ __launch_bounds__(TB) void k_gather(Geo g, Fields s, Fields t, long t
     if (d < S) {
       const long to = c[0] + c[1] * tld + c[2] * tps + toff;
       float *tt = T + (F.off4 + q) * C;
+      const bool own = owner(g, c) == f;
 #pragma unroll
-      for (int k = 0; k < NF; ++k) tt[k] = (t.f[k][to] - u[k]) * inv.s[k];
+      for (int k = 0; k < NF; ++k) {
+        tt[k] = (t.f[k][to] - u[k]) * inv.s[k];
+        if (own) tacc += (double)tt[k] * tt[k];
+      }
 #pragma unroll
       for (int k = NF; k < C; ++k) tt[k] = 0.f;
-      M[F.off4 + q] = owner(g, c) == f ? 1.f : 0.f;
+      M[F.off4 + q] = own ? 1.f : 0.f;
     }
   }
   const double bs = block_sum(acc);
-  if (threadIdx.x == 0) atomicAdd(norm + f, bs);
+  __syncthreads();
+  const double bt = block_sum(tacc);
+  if (threadIdx.x == 0) {
+    atomicAdd(norm + f, bs);
+    atomicAdd(tnorm + f, bt);
+  }
 }
 
 // 3x3x3 convolution over every face box, outputs at depth d < dmax (zero
@@ -270,17 +279,22 @@ __global__ __launch_bounds__(TB) void k_wgrad(Geo g, const float *in, const floa
   part[(((long)f * TAPS + t) * NCHUNK + chunk) * (C * C) + threadIdx.x] = acc;
 }
 
-__global__ __launch_bounds__(TB) void k_wreduce(const float *part, float *G, int layer) {
+// G (+)= sum of the chunk partials; accumulate over the samples of a mini-batch
+__global__ __launch_bounds__(TB) void k_wreduce(const float *part, float *G, int layer,
+                                                int accumulate) {
   const int ft = blockIdx.x;  // face * 27 + tap
   const int f = ft / TAPS, t = ft % TAPS;
   float s = 0.f;
   for (int k = 0; k < NCHUNK; ++k) s += part[((long)ft * NCHUNK + k) * (C * C) + threadIdx.x];
-  G[(long)f * WF + layer * WL + t * C * C + threadIdx.x] = s;
+  float *gp = G + (long)f * WF + layer * WL + t * C * C + threadIdx.x;
+  *gp = accumulate ? *gp + s : s;
 }
 
-// Loss of every face and the gradient of the last layer's output.
+// Squared error of every face over its owned strip cells and the gradient of
+// the last layer's output (both unnormalised; the host divides by the target
+// energy of the mini-batch).
 __global__ __launch_bounds__(TB) void k_loss(Geo g, const float *Y, const float *T, const float *M,
-                                             const double *norm, float *G3, double *loss) {
+                                             float *G3, double *err) {
   const int f = blockIdx.y;
   const Face &F = g.f[f];
   const int q = blockIdx.x * TB + threadIdx.x;
@@ -290,32 +304,28 @@ __global__ __launch_bounds__(TB) void k_loss(Geo g, const float *Y, const float 
     float *gp = G3 + (F.off7 + q) * C;
     if (d < S) {
       const float m = M[F.off4 + q];
-      // a box still at rest (the wave has not arrived) contributes nothing
-      const double nf = norm[f] > 0 ? 1.0 / norm[f] : 0.0;
-      const float inv = (float)nf;
       const float *y = Y + (F.off7 + q) * C;
       const float *tt = T + (F.off4 + q) * C;
 #pragma unroll
       for (int k = 0; k < C; ++k) {
         const float diff = k < NF ? (y[k] - tt[k]) * m : 0.f;
-        gp[k] = 2.f * diff * inv;
+        gp[k] = 2.f * diff;
         l += (double)diff * diff;
       }
-      l *= nf;
     } else {
 #pragma unroll
       for (int k = 0; k < C; ++k) gp[k] = 0.f;
     }
   }
   const double bs = block_sum(l);
-  if (threadIdx.x == 0) atomicAdd(loss + f, bs);
+  if (threadIdx.x == 0) atomicAdd(err + f, bs);
 }
 
-__global__ void k_adam(float *W, const float *G, float *m, float *v, int n, float lr, float b1,
-                       float b2, float eps, float bc1, float bc2) {
+__global__ void k_adam(float *W, const float *G, Scales gscale, float *m, float *v, int n,
+                       float lr, float b1, float b2, float eps, float bc1, float bc2) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float gr = G[i];
+  const float gr = G[i] * gscale.s[i / WF];
   const float mi = b1 * m[i] + (1.f - b1) * gr;
   const float vi = b2 * v[i] + (1.f - b2) * gr * gr;
   m[i] = mi;
@@ -501,7 +511,9 @@ struct cpml_abc_net {
   long cap7 = 0, cap4 = 0;
   float *X = nullptr, *H1 = nullptr, *H2 = nullptr, *Y = nullptr, *G1 = nullptr, *G2 = nullptr,
         *G3 = nullptr, *T = nullptr, *M = nullptr, *part = nullptr, *corr = nullptr;
-  double *red = nullptr;  // norm[6], loss[6]
+  double *red = nullptr;  // per face: box energy [0,6), squared error [6,12), target energy [12,18)
+  int batch = 1, acc_count = 0;      // mini-batch: samples per Adam step, samples so far
+  double acc_tn[6] = {0, 0, 0, 0, 0, 0};
   std::vector<void *> allocs;
   std::string err;
 };
@@ -592,9 +604,10 @@ hipError_t forward32(cpml_abc_net *n, const Geo &g, hipStream_t s) {
   return hipGetLastError();
 }
 
-// gather + forward + loss + backward into n->G; per-face losses into loss6
+// gather + forward + loss + backward of one sample into n->G (added to it when
+// accumulate); per-face squared error and owned target energy into err6, tn6
 int grad32(cpml_abc_net *n, const cpml::SolverView &st, const cpml::SolverView &te, int i0,
-           int j0, int k0, double *loss6) {
+           int j0, int k0, int accumulate, double *err6, double *tn6) {
   const Geo g = make_geo(st);
   int rc = ensure_buffers(n, g);
   if (rc) return rc;
@@ -603,27 +616,29 @@ int grad32(cpml_abc_net *n, const cpml::SolverView &st, const cpml::SolverView &
       k0 + st.nz > te.nz)
     return afail(n, CPML_ERR_ARG, "student window outside the teacher grid");
   hipStream_t s = st.stream;
-  ACHK(n, hipMemsetAsync(n->red, 0, 12 * sizeof(double), s));
+  ACHK(n, hipMemsetAsync(n->red, 0, 18 * sizeof(double), s));
   const dim3 grid((g.max7 + TB - 1) / TB, 6);
   const long toff = i0 + j0 * te.ld + k0 * te.ps;
   hipLaunchKernelGGL(k_gather, grid, dim3(TB), 0, s, g, fields_of(st), fields_of(te), te.ld,
-                     te.ps, toff, n->inv, n->X, n->T, n->M, n->red);
+                     te.ps, toff, n->inv, n->X, n->T, n->M, n->red, n->red + 12);
   ACHK(n, forward32(n, g, s));
-  hipLaunchKernelGGL(k_loss, grid, dim3(TB), 0, s, g, n->Y, n->T, n->M, n->red, n->G3,
-                     n->red + 6);
+  hipLaunchKernelGGL(k_loss, grid, dim3(TB), 0, s, g, n->Y, n->T, n->M, n->G3, n->red + 6);
   const dim3 wg(NCHUNK, TAPS, 6);
   hipLaunchKernelGGL(k_wgrad, wg, dim3(TB), 0, s, g, n->H2, n->G3, n->part, DMAX3);
-  hipLaunchKernelGGL(k_wreduce, dim3(6 * TAPS), dim3(TB), 0, s, n->part, n->G, 2);
+  hipLaunchKernelGGL(k_wreduce, dim3(6 * TAPS), dim3(TB), 0, s, n->part, n->G, 2, accumulate);
   hipLaunchKernelGGL(k_conv<true>, grid, dim3(TB), 0, s, g, n->G3, n->G2, n->W + 2 * WL, DMAX2);
   hipLaunchKernelGGL(k_wgrad, wg, dim3(TB), 0, s, g, n->H1, n->G2, n->part, DMAX2);
-  hipLaunchKernelGGL(k_wreduce, dim3(6 * TAPS), dim3(TB), 0, s, n->part, n->G, 1);
+  hipLaunchKernelGGL(k_wreduce, dim3(6 * TAPS), dim3(TB), 0, s, n->part, n->G, 1, accumulate);
   hipLaunchKernelGGL(k_conv<true>, grid, dim3(TB), 0, s, g, n->G2, n->G1, n->W + WL, DMAX1);
   hipLaunchKernelGGL(k_wgrad, wg, dim3(TB), 0, s, g, n->X, n->G1, n->part, DMAX1);
-  hipLaunchKernelGGL(k_wreduce, dim3(6 * TAPS), dim3(TB), 0, s, n->part, n->G, 0);
+  hipLaunchKernelGGL(k_wreduce, dim3(6 * TAPS), dim3(TB), 0, s, n->part, n->G, 0, accumulate);
   ACHK(n, hipGetLastError());
-  if (loss6) {
-    ACHK(n, hipMemcpyAsync(loss6, n->red + 6, 6 * sizeof(double), hipMemcpyDeviceToHost, s));
-    ACHK(n, hipStreamSynchronize(s));
+  double red[18];
+  ACHK(n, hipMemcpyAsync(red, n->red, sizeof(red), hipMemcpyDeviceToHost, s));
+  ACHK(n, hipStreamSynchronize(s));
+  for (int f = 0; f < 6; ++f) {
+    err6[f] = red[6 + f];
+    tn6[f] = red[12 + f];
   }
   return CPML_OK;
 }
@@ -688,7 +703,7 @@ int cpml_abc_net_create(cpml_abc_net **np, const float *weights, const float *sc
   }
   int rc = aalloc(n, (void **)&n->W, WN * sizeof(float));
   if (!rc) rc = aalloc(n, (void **)&n->Wb, WN * sizeof(unsigned short));
-  if (!rc) rc = aalloc(n, (void **)&n->red, 12 * sizeof(double));
+  if (!rc) rc = aalloc(n, (void **)&n->red, 18 * sizeof(double));
   if (!rc) rc = aalloc(n, (void **)&n->part, (size_t)6 * TAPS * NCHUNK * C * C * sizeof(float));
   if (!rc && weights && hipMemcpy(n->W, weights, WN * sizeof(float), hipMemcpyHostToDevice))
     rc = afail(n, CPML_ERR_HIP, "weight upload failed");
@@ -723,9 +738,11 @@ int cpml_abc_net_set_weights(cpml_abc_net *n, const float *weights) {
   return CPML_OK;
 }
 
-int cpml_abc_train_init(cpml_abc_net *n, double lr, double beta1, double beta2, double eps) {
+int cpml_abc_train_init(cpml_abc_net *n, double lr, double beta1, double beta2, double eps,
+                        int batch) {
   if (!n) return afail(n, CPML_ERR_ARG, "null net");
-  if (!(lr > 0) || !(beta1 >= 0 && beta1 < 1) || !(beta2 >= 0 && beta2 < 1) || !(eps > 0))
+  if (!(lr > 0) || !(beta1 >= 0 && beta1 < 1) || !(beta2 >= 0 && beta2 < 1) || !(eps > 0) ||
+      batch < 1 || batch > 4096)
     return afail(n, CPML_ERR_ARG, "bad optimiser parameters");
   ACHK(n, hipSetDevice(n->device));
   int rc;
@@ -734,6 +751,9 @@ int cpml_abc_train_init(cpml_abc_net *n, double lr, double beta1, double beta2, 
     ACHK(n, hipMemset(*p, 0, WN * sizeof(float)));
   }
   n->t = 0;
+  n->batch = batch;
+  n->acc_count = 0;
+  for (double &v : n->acc_tn) v = 0;
   n->lr = lr;
   n->b1 = beta1;
   n->b2 = beta2;
@@ -745,13 +765,21 @@ int cpml_abc_grad(cpml_abc_net *n, cpml_solver *student, cpml_solver *teacher, i
                   int k0, double *loss6, float *grad) {
   if (!n) return afail(n, CPML_ERR_ARG, "null net");
   cpml::SolverView st{}, te{};
+  double err[6], tn[6];
   int rc = views(n, student, &st, "student");
   if (!rc) rc = views(n, teacher, &te, "teacher");
-  if (!rc) rc = grad32(n, st, te, i0, j0, k0, loss6);
+  if (!rc) rc = grad32(n, st, te, i0, j0, k0, 0, err, tn);
   if (rc) return rc;
+  n->acc_count = 0;  // the gradient buffer no longer holds a partial mini-batch
+  for (double &v : n->acc_tn) v = 0;
+  if (loss6)
+    for (int f = 0; f < 6; ++f) loss6[f] = tn[f] > 0 ? err[f] / tn[f] : 0.0;
   if (grad) {
-    ACHK(n, hipStreamSynchronize(st.stream));
     ACHK(n, hipMemcpy(grad, n->G, WN * sizeof(float), hipMemcpyDeviceToHost));
+    for (int f = 0; f < 6; ++f) {
+      const float sc = tn[f] > 0 ? (float)(1.0 / tn[f]) : 0.f;
+      for (int e = 0; e < WF; ++e) grad[f * WF + e] *= sc;
+    }
   }
   return CPML_OK;
 }
@@ -783,18 +811,25 @@ int cpml_abc_train_step(cpml_abc_net *n, cpml_solver *student, cpml_solver *teac
   // the runs may have switched state buffers: take fresh views
   if ((rc = views(n, student, &st, "student")) || (rc = views(n, teacher, &te, "teacher")))
     return rc;
-  if ((rc = grad32(n, st, te, i0, j0, k0, nullptr))) return rc;
+  double err[6], tn[6];
+  if ((rc = grad32(n, st, te, i0, j0, k0, n->acc_count > 0, err, tn))) return rc;
+  for (int f = 0; f < 6; ++f) n->acc_tn[f] += tn[f];
+  if (loss6)
+    for (int f = 0; f < 6; ++f) loss6[f] = tn[f] > 0 ? err[f] / tn[f] : 0.0;
+  if (++n->acc_count < n->batch) return CPML_OK;
+  // Adam step on the mini-batch: gradient of sum(err) / sum(target energy), per face
+  Scales gs{};
+  for (int f = 0; f < 6; ++f) gs.s[f] = n->acc_tn[f] > 0 ? (float)(1.0 / n->acc_tn[f]) : 0.f;
   n->t += 1;
   const float bc1 = (float)(1.0 - std::pow(n->b1, n->t));
   const float bc2 = (float)(1.0 - std::pow(n->b2, n->t));
-  hipLaunchKernelGGL(k_adam, dim3((WN + 255) / 256), dim3(256), 0, st.stream, n->W, n->G, n->m,
-                     n->v, WN, (float)n->lr, (float)n->b1, (float)n->b2, (float)n->eps, bc1, bc2);
+  hipLaunchKernelGGL(k_adam, dim3((WN + 255) / 256), dim3(256), 0, st.stream, n->W, n->G, gs,
+                     n->m, n->v, WN, (float)n->lr, (float)n->b1, (float)n->b2, (float)n->eps,
+                     bc1, bc2);
   ACHK(n, hipGetLastError());
   n->wb_dirty = true;
-  if (loss6) {
-    ACHK(n, hipMemcpyAsync(loss6, n->red + 6, 6 * sizeof(double), hipMemcpyDeviceToHost,
-                           st.stream));
-  }
+  n->acc_count = 0;
+  for (double &v : n->acc_tn) v = 0;
   ACHK(n, hipStreamSynchronize(st.stream));
   return CPML_OK;
 }
@@ -813,10 +848,10 @@ int cpml_abc_correction(cpml_abc_net *n, cpml_solver *h, int use_mfma, float *ou
     hipLaunchKernelGGL(k_infer, dim3(g.tiles), dim3(TB), 0, s, g, fields_of(v), n->inv, n->Wb,
                        n->corr);
   } else {
-    ACHK(n, hipMemsetAsync(n->red, 0, 12 * sizeof(double), s));
+    ACHK(n, hipMemsetAsync(n->red, 0, 18 * sizeof(double), s));
     const dim3 grid((g.max7 + TB - 1) / TB, 6);
     hipLaunchKernelGGL(k_gather, grid, dim3(TB), 0, s, g, fields_of(v), fields_of(v), v.ld, v.ps,
-                       0L, n->inv, n->X, n->T, n->M, n->red);
+                       0L, n->inv, n->X, n->T, n->M, n->red, n->red + 12);
     ACHK(n, forward32(n, g, s));
     hipLaunchKernelGGL(k_y_to_corr, grid, dim3(TB), 0, s, g, n->Y, n->corr);
   }

@@ -30,6 +30,14 @@ module cpml_gpu
        cpml_gpu_set_stability, cpml_gpu_run, cpml_gpu_get_seismograms, cpml_gpu_get_energy, &
        cpml_gpu_get_field, cpml_gpu_set_field, cpml_gpu_velocity_norm, cpml_gpu_elapsed_ms, &
        cpml_gpu_error_message
+  ! many 2D shots in one launch per phase (no reference counterpart: a survey
+  ! is a loop of reference runs)
+  public :: cpml_gpu_batch_init, cpml_gpu_batch_finalize, cpml_gpu_batch_set_medium, &
+       cpml_gpu_batch_set_pml, cpml_gpu_batch_set_source, cpml_gpu_batch_set_receivers, &
+       cpml_gpu_batch_autotune, cpml_gpu_batch_run, cpml_gpu_batch_get_seismograms
+  ! learned thin absorbing boundary (replaces the C-PML of 3D:399-667)
+  integer, parameter, public :: CPML_BOUNDARY_CPML = 0, CPML_BOUNDARY_LEARNED = 1
+  public :: cpml_gpu_abc_create, cpml_gpu_abc_destroy, cpml_gpu_set_boundary
 
   interface
     integer(c_int) function c_init(h, ndim, order, prec, nx, ny, nz, dx, dy, dz, dt, npml, &
@@ -120,6 +128,81 @@ module cpml_gpu
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: h
       real(c_double), intent(out) :: v
+    end function
+    integer(c_int) function c_batch_init(b, order, prec, nshots, nx, ny, dx, dy, dt, npml, &
+                                         nstep) bind(C, name="cpml_batch_init")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), intent(out) :: b
+      integer(c_int), value :: order, prec, nshots, nstep
+      integer(c_int), dimension(*), intent(in) :: nx, ny, npml
+      real(c_double), dimension(*), intent(in) :: dx, dy, dt
+    end function
+    integer(c_int) function c_batch_finalize(b) bind(C, name="cpml_batch_finalize")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: b
+    end function
+    integer(c_int) function c_batch_set_medium(b, shot, lam, mu, rho) &
+         bind(C, name="cpml_batch_set_medium")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: b
+      integer(c_int), value :: shot
+      type(*), dimension(*), intent(in) :: lam, mu, rho
+    end function
+    integer(c_int) function c_batch_set_pml(b, shot, ax, bx, kx, axh, bxh, kxh, ay, by, ky, &
+                                            ayh, byh, kyh) bind(C, name="cpml_batch_set_pml")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: b
+      integer(c_int), value :: shot
+      type(*), dimension(*), intent(in) :: ax, bx, kx, axh, bxh, kxh, ay, by, ky, ayh, byh, kyh
+    end function
+    integer(c_int) function c_batch_set_source(b, shot, is, js, fx, fy) &
+         bind(C, name="cpml_batch_set_source")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: b
+      integer(c_int), value :: shot, is, js
+      type(*), dimension(*), intent(in) :: fx, fy
+    end function
+    integer(c_int) function c_batch_set_receivers(b, shot, nrec, ix, iy) &
+         bind(C, name="cpml_batch_set_receivers")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: b
+      integer(c_int), value :: shot, nrec
+      integer(c_int), dimension(*), intent(in) :: ix, iy
+    end function
+    integer(c_int) function c_batch_autotune(b, trial, path, ty, bpc, ms) &
+         bind(C, name="cpml_batch_autotune")
+      import :: c_ptr, c_int, c_char, c_double
+      type(c_ptr), value :: b
+      integer(c_int), value :: trial
+      character(kind=c_char), dimension(*), intent(in) :: path
+      integer(c_int), intent(out) :: ty, bpc
+      real(c_double), intent(out) :: ms
+    end function
+    integer(c_int) function c_batch_run(b, it_begin, it_end) bind(C, name="cpml_batch_run")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: b
+      integer(c_int), value :: it_begin, it_end
+    end function
+    integer(c_int) function c_batch_get_seismograms(b, shot, sx, sy) &
+         bind(C, name="cpml_batch_get_seismograms")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: b
+      integer(c_int), value :: shot
+      type(*), dimension(*) :: sx, sy
+    end function
+    integer(c_int) function c_abc_create(net, w, scales) bind(C, name="cpml_abc_net_create")
+      import :: c_ptr, c_int, c_float
+      type(c_ptr), intent(out) :: net
+      real(c_float), dimension(*), intent(in) :: w, scales
+    end function
+    integer(c_int) function c_abc_destroy(net) bind(C, name="cpml_abc_net_destroy")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: net
+    end function
+    integer(c_int) function c_set_boundary(h, mode, net) bind(C, name="cpml_gpu_set_boundary")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h, net
+      integer(c_int), value :: mode
     end function
     integer(c_int) function c_last_error(h, buf, n) bind(C, name="cpml_gpu_last_error")
       import :: c_ptr, c_int, c_char
@@ -283,6 +366,109 @@ contains
     double precision, intent(out) :: ms
     integer, intent(out) :: ierr
     ierr = c_elapsed_ms(h, ms)
+  end subroutine
+
+  ! ---- batched 2D shots: per-shot arrays NX(nshots), ... (shot = 1..nshots)
+  subroutine cpml_gpu_batch_init(b, order, prec, nshots, NX, NY, DELTAX, DELTAY, DELTAT, &
+                                 NPOINTS_PML, NSTEP, ierr)
+    type(c_ptr), intent(out) :: b
+    integer, intent(in) :: order, prec, nshots, NSTEP
+    integer, dimension(nshots), intent(in) :: NX, NY, NPOINTS_PML
+    double precision, dimension(nshots), intent(in) :: DELTAX, DELTAY, DELTAT
+    integer, intent(out) :: ierr
+    ierr = c_batch_init(b, order, prec, nshots, NX, NY, DELTAX, DELTAY, DELTAT, NPOINTS_PML, &
+                        NSTEP)
+  end subroutine
+
+  subroutine cpml_gpu_batch_finalize(b, ierr)
+    type(c_ptr), intent(inout) :: b
+    integer, intent(out) :: ierr
+    ierr = c_batch_finalize(b)
+    b = c_null_ptr
+  end subroutine
+
+  subroutine cpml_gpu_batch_set_medium(b, shot, lambda, mu, rho, ierr)
+    type(c_ptr), intent(in) :: b
+    integer, intent(in) :: shot
+    type(*), dimension(*), intent(in) :: lambda, mu, rho
+    integer, intent(out) :: ierr
+    ierr = c_batch_set_medium(b, shot - 1, lambda, mu, rho)
+  end subroutine
+
+  subroutine cpml_gpu_batch_set_pml(b, shot, a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, &
+                                    a_y, b_y, K_y, a_y_half, b_y_half, K_y_half, ierr)
+    type(c_ptr), intent(in) :: b
+    integer, intent(in) :: shot
+    type(*), dimension(*), intent(in) :: a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, &
+         a_y, b_y, K_y, a_y_half, b_y_half, K_y_half
+    integer, intent(out) :: ierr
+    ierr = c_batch_set_pml(b, shot - 1, a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, a_y, b_y, &
+                           K_y, a_y_half, b_y_half, K_y_half)
+  end subroutine
+
+  subroutine cpml_gpu_batch_set_source(b, shot, ISOURCE, JSOURCE, force_x, force_y, ierr)
+    type(c_ptr), intent(in) :: b
+    integer, intent(in) :: shot, ISOURCE, JSOURCE
+    type(*), dimension(*), intent(in) :: force_x, force_y
+    integer, intent(out) :: ierr
+    ierr = c_batch_set_source(b, shot - 1, ISOURCE, JSOURCE, force_x, force_y)
+  end subroutine
+
+  subroutine cpml_gpu_batch_set_receivers(b, shot, NREC, ix_rec, iy_rec, ierr)
+    type(c_ptr), intent(in) :: b
+    integer, intent(in) :: shot, NREC
+    integer, dimension(NREC), intent(in) :: ix_rec, iy_rec
+    integer, intent(out) :: ierr
+    ierr = c_batch_set_receivers(b, shot - 1, NREC, ix_rec, iy_rec)
+  end subroutine
+
+  ! picks rows per tile and the persistent grid size; cache_path may be ''
+  subroutine cpml_gpu_batch_autotune(b, trial_steps, cache_path, ierr)
+    type(c_ptr), intent(in) :: b
+    integer, intent(in) :: trial_steps
+    character(len=*), intent(in) :: cache_path
+    integer, intent(out) :: ierr
+    integer(c_int) :: ty, bpc
+    real(c_double) :: ms
+    ierr = c_batch_autotune(b, trial_steps, trim(cache_path) // c_null_char, ty, bpc, ms)
+  end subroutine
+
+  subroutine cpml_gpu_batch_run(b, it_begin, it_end, ierr)
+    type(c_ptr), intent(in) :: b
+    integer, intent(in) :: it_begin, it_end
+    integer, intent(out) :: ierr
+    ierr = c_batch_run(b, it_begin, it_end)
+  end subroutine
+
+  subroutine cpml_gpu_batch_get_seismograms(b, shot, sisvx, sisvy, ierr)
+    type(c_ptr), intent(in) :: b
+    integer, intent(in) :: shot
+    type(*), dimension(*) :: sisvx, sisvy
+    integer, intent(out) :: ierr
+    ierr = c_batch_get_seismograms(b, shot - 1, sisvx, sisvy)
+  end subroutine
+
+  ! ---- learned thin boundary: weights(6*3*27*16*16), scales(9), real(4)
+  subroutine cpml_gpu_abc_create(net, weights, scales, ierr)
+    type(c_ptr), intent(out) :: net
+    real(c_float), dimension(*), intent(in) :: weights, scales
+    integer, intent(out) :: ierr
+    ierr = c_abc_create(net, weights, scales)
+  end subroutine
+
+  subroutine cpml_gpu_abc_destroy(net, ierr)
+    type(c_ptr), intent(inout) :: net
+    integer, intent(out) :: ierr
+    ierr = c_abc_destroy(net)
+    net = c_null_ptr
+  end subroutine
+
+  ! mode CPML_BOUNDARY_LEARNED with a corrector, or CPML_BOUNDARY_CPML
+  subroutine cpml_gpu_set_boundary(h, mode, net, ierr)
+    type(c_ptr), intent(in) :: h, net
+    integer, intent(in) :: mode
+    integer, intent(out) :: ierr
+    ierr = c_set_boundary(h, mode, net)
   end subroutine
 
   function cpml_gpu_error_message(h) result(msg)

@@ -24,6 +24,7 @@ from __future__ import annotations
 import math
 import time
 from dataclasses import dataclass
+from pathlib import Path
 
 import numpy as np
 
@@ -68,7 +69,8 @@ def layered_model(n: int, seed: int, dx=10.0, courant=0.5, f0=None, nlayers=12,
     cps = rng.uniform(1500.0, 5500.0, size=nlayers)
     cpz = cps[np.searchsorted(cuts, np.arange(n), side="right")]
     f0 = float(rng.uniform(5.0, 15.0)) if f0 is None else float(f0)
-    src = tuple(int(v) for v in rng.integers(src_margin, n - src_margin, size=3))
+    margin = min(src_margin, (n - 1) // 2)
+    src = tuple(int(v) for v in rng.integers(margin, n - margin, size=3))
     dt = courant * dx / (5500.0 * math.sqrt(3.0))  # Courant 0.5 of the fastest possible cp
     return Model(cpz, dx, dt, f0, src)
 
@@ -230,3 +232,52 @@ def time_to_solution(model: Model, nstep: int, net: S.CpmlAbcNet, repeats: int =
         del keep
     res["speedup"] = res["cpml20"]["seconds"] / res["learned4"]["seconds"]
     return res
+
+
+def corrector_flops(n_grid: int) -> float:
+    """Algorithmic FLOPs of one corrector pass on an n_grid^3 grid: per face
+    column, layer 1 at 6 depths (27*9*16 MACs each), layer 2 at 5 depths
+    (27*16*16), layer 3 at 4 depths (27*16*9); 2 FLOPs per MAC."""
+    macs = 6 * 27 * 9 * 16 + 5 * 27 * 16 * 16 + 4 * 27 * 16 * 9
+    return 2.0 * macs * 6 * n_grid * n_grid
+
+
+WEIGHTS_FILE = Path(__file__).resolve().parent / "data" / "abc_weights.npz"
+
+
+def save_weights(net: S.CpmlAbcNet, path=WEIGHTS_FILE, **meta):
+    path = Path(path)
+    path.parent.mkdir(parents=True, exist_ok=True)
+    np.savez_compressed(path, weights=net.weights, scales=net.scales,
+                        meta=np.array(repr(meta)))
+
+
+def load_net(path=WEIGHTS_FILE):
+    d = np.load(path, allow_pickle=False)
+    return S.CpmlAbcNet(d["scales"], d["weights"])
+
+
+def main(argv=None):
+    import argparse
+
+    ap = argparse.ArgumentParser(description="train the learned thin absorbing boundary")
+    ap.add_argument("--n", type=int, default=128, help="teacher interior cells per axis")
+    ap.add_argument("--runs", type=int, default=16)
+    ap.add_argument("--nstep", type=int, default=600)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--seed", type=int, default=1000)
+    ap.add_argument("--out", default=str(WEIGHTS_FILE))
+    args = ap.parse_args(argv)
+    net = S.CpmlAbcNet(field_scales(), init_weights(args.seed, std=0.02))
+    net.train_init(lr=args.lr, batch=args.batch)
+    t = time.perf_counter()
+    hist = train(net, n=args.n, runs=args.runs, nstep=args.nstep, seed=args.seed,
+                 log=lambda m: print(m, flush=True))
+    save_weights(net, args.out, n=args.n, runs=args.runs, nstep=args.nstep, lr=args.lr,
+                 seed=args.seed, loss=hist, seconds=time.perf_counter() - t)
+    print(f"saved {args.out} after {time.perf_counter() - t:.1f} s", flush=True)
+
+
+if __name__ == "__main__":
+    main()

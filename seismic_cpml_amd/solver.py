@@ -83,7 +83,7 @@ SIGNATURES = {
     "cpml_abc_net_get_weights": [_VP, _VP],
     "cpml_abc_net_set_weights": [_VP, _VP],
     "cpml_gpu_set_boundary": [_VP, _I, _VP],
-    "cpml_abc_train_init": [_VP, _D, _D, _D, _D],
+    "cpml_abc_train_init": [_VP, _D, _D, _D, _D, _I],
     "cpml_abc_train_step": [_VP, _VP, _VP, _I, _I, _I, _I, _VP],
     "cpml_abc_grad": [_VP, _VP, _VP, _I, _I, _I, _VP, _VP],
     "cpml_abc_correction": [_VP, _VP, _I, _VP],
@@ -421,8 +421,9 @@ class CpmlAbcNet:
         w = self._w(w)
         self._call("cpml_abc_net_set_weights", _ptr(w))
 
-    def train_init(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
-        self._call("cpml_abc_train_init", float(lr), float(beta1), float(beta2), float(eps))
+    def train_init(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8, batch=8):
+        self._call("cpml_abc_train_init", float(lr), float(beta1), float(beta2), float(eps),
+                   int(batch))
 
     def train_step(self, student, teacher, offset, it):
         loss = np.zeros(6)
@@ -508,6 +509,8 @@ class CpmlBatch:
         _check(getattr(library(), fn)(self._h, *args), self._h, "cpml_batch_last_error")
 
     def _shape(self, q):
+        if not 0 <= q < len(self.shots):
+            raise CpmlError(CPML_ERR_ARG, f"shot index {q} out of range")
         return (self.shots[q]["NY"], self.shots[q]["NX"])
 
     def _arr(self, a, shape):

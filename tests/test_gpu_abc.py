@@ -7,7 +7,8 @@ documented network (per face: the 9 scaled fields in a 7-deep box, three
 autograd in float64).  Tolerances:
   * FP32 training forward vs torch: rel-L2 1e-5;
   * BF16 MFMA inference vs torch: rel-L2 3e-2 (bf16 weights and activations);
-  * FP32 gradients vs torch float64: rel-L2 1e-4 per face and layer.
+  * FP32 gradients vs torch float64: rel-L2 1e-4 per face and layer (loss of
+    a face: squared error over its owned strip cells / target energy there).
 A zero corrector must leave the plain step bit-identical.
 """
 import numpy as np
@@ -46,7 +47,9 @@ def face_axes(f):
 
 def take(arr, ax, g):
     """Slice of a (nz, ny, nx) array at index g along grid axis ax -> (nb, na)."""
-    return [arr[:, :, g], arr[:, g, :], arr[g, :, :]][ax]
+    if ax == 0:
+        return arr[:, :, g]
+    return arr[:, g, :] if ax == 1 else arr[g, :, :]
 
 
 def boxes(state, n, depth=7):
@@ -131,7 +134,7 @@ def test_corrector_gradient_matches_torch():
         y = h[0, :9, :4]
         target = torch.from_numpy(tb[f][:9] - xb[f][:9])
         m = torch.from_numpy(owner_mask(n, f).astype(np.float64))[None]
-        lf = (((y - target) * m) ** 2).sum() / float((xs[f] ** 2).sum())
+        lf = (((y - target) * m) ** 2).sum() / float(((target * m) ** 2).sum())
         lf.backward()
         assert abs(loss[f] - lf.item()) <= 1e-4 * abs(lf.item()), f
         for l in range(3):
@@ -158,15 +161,16 @@ def test_zero_corrector_leaves_plain_step_unchanged():
 def test_training_reduces_loss_and_learned_run_is_finite():
     torch.manual_seed(0)
     with CpmlAbcNet(learned_abc.field_scales(), learned_abc.init_weights(4, std=0.02)) as net:
-        net.train_init(lr=2e-3)
+        net.train_init(lr=2e-3, batch=4)
         m = learned_abc.layered_model(24, seed=7, f0=12.0, src_margin=10)
-        teacher, _, kt = learned_abc.make_run(m, 20, 20, 220)
+        # the Ricker pulse peaks at step 1.2 / f0 / DELTAT = 190
+        teacher, _, kt = learned_abc.make_run(m, 20, 20, 420)
         student, _, ks = learned_abc.make_run(m, 4, 0, 1, source=False)
-        teacher.run(1, 19)
+        teacher.run(1, 149)
         losses = [net.train_step(student, teacher, (16, 16, 16), it).sum()
-                  for it in range(20, 221)]
+                  for it in range(150, 421)]
         assert np.isfinite(losses).all()
-        assert np.mean(losses[-30:]) < 0.5 * np.mean(losses[:30])
+        assert np.mean(losses[-40:]) < 0.5 * np.mean(losses[:40])
         teacher.close()
         student.close()
         s, _, k = learned_abc.make_run(m, 4, 0, 200)
