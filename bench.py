@@ -132,11 +132,22 @@ def _line(metric, value, args, steps, ms_per_step, dtype, config, roofline, cpu)
     }
 
 
-def _roofline(nb, avg):
+def _traffic(key, kernel):
+    """Measured HBM bytes per dispatch of `kernel` in configuration `key`
+    (profiles/traffic.json, from rocprofv3 FETCH_SIZE / WRITE_SIZE runs)."""
+    p = ROOT / "profiles" / "traffic.json"
+    if not p.is_file():
+        return None
+    d = json.loads(p.read_text()).get(key, {}).get(kernel)
+    return None if d is None else int(d["read_bytes"] + d["write_bytes"])
+
+
+def _roofline(nb, avg, key=None):
     dom = max(avg, key=avg.get)
     achieved = nb[dom] / (avg[dom] * 1e-3) / 1e9
     return {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None}
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": _traffic(key, dom) if key else None, "algorithmic_bytes": int(nb[dom])}
 
 
 def bench_3d(args):
@@ -150,7 +161,7 @@ def bench_3d(args):
     s = w.make_solver()
     if args.kchunk:
         s.set_kchunk(args.kchunk)
-    if not args.two_phase:
+    if args.fused:
         s.set_fused(True)
     if args.tile:
         s.set_tile(args.tile)
@@ -160,7 +171,7 @@ def bench_3d(args):
 
     cells = w.cells
     value = cells * steps / wall / 1e9
-    fused = not args.two_phase
+    fused = args.fused
     nb = workloads.bytes_per_step(w, slabs, fused=fused)
     names = ((0, "fused"),) if fused else ((0, "stress"), (1, "velocity"))
     kt = {name: s.kernel_time(k) for k, name in names}
@@ -189,8 +200,10 @@ def bench_3d(args):
         "kernel_ms": {k: round(v, 4) for k, v in avg.items()},
         "max_velocity_norm": vnorm,
     }
+    key = (f"3d_{w.nx}_{'fused' if fused else 'two-phase'}_tile{args.tile or 4}_"
+           f"kchunk{args.kchunk or 8}")
     return _line("Gcell-updates/s (3D 768^3 FP32 C-PML time loop)", value, args, steps,
-                 wall / steps * 1e3, "f32", config, _roofline(nb, avg), cpu)
+                 wall / steps * 1e3, "f32", config, _roofline(nb, avg, key), cpu)
 
 
 def bench_2d(args):
@@ -232,7 +245,8 @@ def bench_2d(args):
         "kernel_ms": {k: round(v, 4) for k, v in avg.items()},
     }
     return _line("Gcell-updates/s (2D 4096^2 4th-order FP32 C-PML time loop)", value, args,
-                  steps, wall / steps * 1e3, "f32", config, _roofline(nb, avg), cpu)
+                  steps, wall / steps * 1e3, "f32", config,
+                  _roofline(nb, avg, f"2d_{n}_tile{args.tile or 4}"), cpu)
 
 
 def bench_batch(args):
@@ -346,7 +360,6 @@ def bench_abc(args):
     tts = L.time_to_solution(m, nstep, net)
     # corrector alone: the learned-size grid without the corrector
     rig, _, keep = L.make_run(m, L.STRIP, 0, nstep)
-    rig.set_fused(True)
     rig.run(1, min(16, nstep))
     import torch
 
@@ -396,8 +409,9 @@ def main():
     ap.add_argument("--n", type=int, default=0, help="grid points per axis (3d, 2d)")
     ap.add_argument("--kchunk", type=int, default=0)
     ap.add_argument("--tile", type=int, default=0, help="rows per tile (4, 8, 16)")
-    ap.add_argument("--two-phase", action="store_true",
-                    help="3d: separate stress and velocity kernels instead of the fused step")
+    ap.add_argument("--fused", action="store_true",
+                    help="3d: one fused kernel per step instead of the stress and velocity "
+                         "kernels")
     ap.add_argument("--shots", type=int, default=128, help="batch: number of shots")
     ap.add_argument("--seq-shots", type=int, default=None,
                     help="batch: shots to rerun sequentially (default all)")

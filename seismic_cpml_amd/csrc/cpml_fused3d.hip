@@ -29,14 +29,18 @@ namespace {
 
 constexpr int FTX = 64;
 
+// One thread per sigma point of the tile and its 1-point ring (tile points
+// first, so tile waves load whole rows), rounded up to whole waves; the v
+// points of the 2-point shell go to the last threads.
 template <int TY>
 struct FG {
   static constexpr int RW = FTX + 2, RH = TY + 2;   // sigma ring: x in [-1,64], y in [-1,TY]
   static constexpr int HW = FTX + 4, HH = TY + 4;   // v region:   x in [-2,65], y in [-2,TY+1]
-  static constexpr int NT = FTX * TY;               // threads, one tile point each
-  static constexpr int NRING = RW * RH - FTX * TY;  // ring points outside the tile
+  static constexpr int NTILE = FTX * TY;            // tile points
+  static constexpr int NSIG = RW * RH;              // tile + ring points
+  static constexpr int NT = (NSIG + 63) / 64 * 64;  // threads
   static constexpr int NSHELL = HW * HH - RW * RH;  // v points outside the ring
-  static_assert(NRING <= NT && NSHELL <= NT, "one ring and one shell point per thread");
+  static_assert(NT <= 1024 && NSHELL <= NT, "block too large");
 };
 
 // ring point r -> tile-relative (x, y): row -1, row TY, then columns -1 and 64
@@ -117,7 +121,7 @@ struct SP {
 };
 
 template <typename T, int TY>
-__global__ __launch_bounds__(FTX *TY) void k_fused3d(Fused3D<T> P) {
+__global__ __launch_bounds__(FG<TY>::NT) void k_fused3d(Fused3D<T> P) {
   using G = FG<TY>;
   __shared__ T vH[3][G::HH * G::HW];
   __shared__ T sR[2][5][G::RH * G::RW];
@@ -141,21 +145,25 @@ __global__ __launch_bounds__(FTX *TY) void k_fused3d(Fused3D<T> P) {
   auto soff = [&](int k) -> unsigned { return (unsigned)(k - ft.kb + 2) * psb; };
   auto LD = [&](__amdgpu_buffer_rsrc_t r, unsigned v, int k) { return BufOp<T>::ld(r, v, soff(k)); };
 
-  // ---- points of this thread: its tile point (a = 0), a ring point for the
-  // last NRING threads (a = 1), a shell point for the first NSHELL threads
-  constexpr int NP = 2;
-  constexpr int NTP = 1;  // tile points per thread
+  // ---- points of this thread: one sigma point (tile points for the first
+  // NTILE threads, ring points after them), a shell v point for the last
+  // NSHELL threads
+  constexpr int NP = 1;
+  constexpr int NTP = 1;
+  const bool is_tile = tid < G::NTILE;
   int px[NP], py[NP];
   bool has[NP], ld_ok[NP];
   unsigned vo[NP];
-  px[0] = tid % FTX;
-  py[0] = tid / FTX;
-  has[0] = true;
-  has[1] = tid >= G::NT - G::NRING;
-  ring_xy(has[1] ? tid - (G::NT - G::NRING) : 0, TY, px[1], py[1]);
+  has[0] = tid < G::NSIG;
+  if (is_tile) {
+    px[0] = tid % FTX;
+    py[0] = tid / FTX;
+  } else {
+    ring_xy(has[0] ? tid - G::NTILE : 0, TY, px[0], py[0]);
+  }
   int shx, shy;
-  const bool has_sh = tid < G::NSHELL;
-  shell_xy(has_sh ? tid : 0, TY, shx, shy);
+  const bool has_sh = tid >= G::NT - G::NSHELL;
+  shell_xy(has_sh ? tid - (G::NT - G::NSHELL) : 0, TY, shx, shy);
   // points addressable in the allocation: i <= nx+1 and j <= ny+1
 #pragma unroll
   for (int a = 0; a < NP; ++a) {
@@ -189,7 +197,7 @@ __global__ __launch_bounds__(FTX *TY) void k_fused3d(Fused3D<T> P) {
     }
 #pragma unroll
     for (int a = 0; a < NTP; ++a)
-      if (ld_ok[a]) pcb[a] = LD(acb, vo[a], k - 1);
+      if (is_tile && ld_ok[a]) pcb[a] = LD(acb, vo[a], k - 1);
     if (sh_ok) {
       pshv[0] = LD(avx, sho, k);
       pshv[1] = LD(avy, sho, k);
@@ -295,7 +303,7 @@ __global__ __launch_bounds__(FTX *TY) void k_fused3d(Fused3D<T> P) {
       if (inside) {
         const int h = (py[a] + 2) * G::HW + px[a] + 2;
         const T vx_c = S[a].xo1, vy_c = S[a].yo1, vz_c = S[a].zo1;
-        const bool st = a < NTP && store_k;
+        const bool st = is_tile && store_k;
         const bool inx = p.sx.in(i), iny = p.sy.in(j);
         const unsigned ox = p.sx.s(i) + (unsigned)p.sx.w * (j + (unsigned)ny * k);
         const unsigned oy = i + (unsigned)p.mld * (p.sy.s(j) + (unsigned)p.sy.w * k);
@@ -369,16 +377,14 @@ __global__ __launch_bounds__(FTX *TY) void k_fused3d(Fused3D<T> P) {
       sR[k & 1][2][r] = sn[3];
       sR[k & 1][3][r] = sn[4];
       sR[k & 1][4][r] = sn[5];
-      if (a < NTP) {
 #pragma unroll
-        for (int q = 0; q < 6; ++q) snew[a][q] = sn[q];
-      }
+      for (int q = 0; q < 6; ++q) snew[a][q] = sn[q];
     }
     __syncthreads();
 
     // 3. v(k-1) on the tile
     const int kv = k - 1;
-    if (kv >= ft.kb && kv < ft.ke) {
+    if (is_tile && kv >= ft.kb && kv < ft.ke) {
       const int it = *p.step;
       const bool inzv = p.sz.in(kv);
 #pragma unroll
@@ -474,12 +480,17 @@ unsigned fused_tiles3d(const Fused3D<T> &P, int ty) {
          ((p.nz + p.kchunk - 1) / p.kchunk);
 }
 
+// tile rows 4 (448 threads), 8 (704) or 12 (960)
 template <typename T>
 hipError_t launch_fused3d(const Fused3D<T> &P, hipStream_t s) {
-  if (P.a.tile_y == 16)
-    hipLaunchKernelGGL((k_fused3d<T, 16>), dim3(fused_tiles3d(P, 16)), dim3(FTX * 16), 0, s, P);
+  const int ty = P.a.tile_y;
+  if (ty >= 12)
+    hipLaunchKernelGGL((k_fused3d<T, 12>), dim3(fused_tiles3d(P, 12)), dim3(FG<12>::NT), 0, s,
+                       P);
+  else if (ty >= 8)
+    hipLaunchKernelGGL((k_fused3d<T, 8>), dim3(fused_tiles3d(P, 8)), dim3(FG<8>::NT), 0, s, P);
   else
-    hipLaunchKernelGGL((k_fused3d<T, 8>), dim3(fused_tiles3d(P, 8)), dim3(FTX * 8), 0, s, P);
+    hipLaunchKernelGGL((k_fused3d<T, 4>), dim3(fused_tiles3d(P, 4)), dim3(FG<4>::NT), 0, s, P);
   return hipGetLastError();
 }
 

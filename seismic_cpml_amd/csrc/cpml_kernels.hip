@@ -31,17 +31,31 @@ namespace {
 
 // ---------------------------------------------------------------- 2D ----
 
+// XCD-aware tile order: the hardware hands workgroup h to XCD h % 8; each XCD
+// gets one contiguous band of tiles (x fastest), so the rows and columns that
+// neighbouring tiles share are read through the same L2.  Sets this thread's
+// grid point (i, j).
+template <int TY>
+__device__ __forceinline__ void tile2d(int nx, int ny, int &i, int &j) {
+  const int ntx = (nx + BX - 1) / BX, nt = ntx * ((ny + TY - 1) / TY);
+  const int h = blockIdx.x, xcd = h & 7, loc = h >> 3;
+  const int q = nt >> 3, r = nt & 7;
+  const int t = xcd * q + min(xcd, r) + loc;
+  i = (t % ntx) * BX + threadIdx.x;
+  j = (t / ntx) * TY + threadIdx.y;
+}
+
 template <typename T, int ORDER, int TY>
 __global__ __launch_bounds__(BX *TY) void k_stress2d(Step2D<T> p) {
-  const int i = blockIdx.x * BX + threadIdx.x;
-  const int j = blockIdx.y * TY + threadIdx.y;
+  int i, j;
+  tile2d<TY>(p.nx, p.ny, i, j);
   if (i < p.nx && j < p.ny) stress2d_point<T, ORDER>(p, i, j);
 }
 
 template <typename T, int ORDER, int TY>
 __global__ __launch_bounds__(BX *TY) void k_velocity2d(Step2D<T> p) {
-  const int i = blockIdx.x * BX + threadIdx.x;
-  const int j = blockIdx.y * TY + threadIdx.y;
+  int i, j;
+  tile2d<TY>(p.nx, p.ny, i, j);
   if (i < p.nx && j < p.ny) velocity2d_point<T, ORDER>(p, i, j);
 }
 
@@ -473,7 +487,7 @@ __global__ __launch_bounds__(EB) void k_vmax(const T *vx, const T *vy, const T *
 
 template <typename T, int TY>
 static hipError_t stress2d_ty(int order, const Step2D<T> &p, hipStream_t s) {
-  dim3 grid((p.nx + BX - 1) / BX, (p.ny + TY - 1) / TY), block(BX, TY);
+  dim3 grid(((p.nx + BX - 1) / BX) * ((p.ny + TY - 1) / TY)), block(BX, TY);
   if (order == 2) hipLaunchKernelGGL((k_stress2d<T, 2, TY>), grid, block, 0, s, p);
   else if (order == 4) hipLaunchKernelGGL((k_stress2d<T, 4, TY>), grid, block, 0, s, p);
   else hipLaunchKernelGGL((k_stress2d<T, 8, TY>), grid, block, 0, s, p);
@@ -482,7 +496,7 @@ static hipError_t stress2d_ty(int order, const Step2D<T> &p, hipStream_t s) {
 
 template <typename T, int TY>
 static hipError_t velocity2d_ty(int order, const Step2D<T> &p, hipStream_t s) {
-  dim3 grid((p.nx + BX - 1) / BX, (p.ny + TY - 1) / TY), block(BX, TY);
+  dim3 grid(((p.nx + BX - 1) / BX) * ((p.ny + TY - 1) / TY)), block(BX, TY);
   if (order == 2) hipLaunchKernelGGL((k_velocity2d<T, 2, TY>), grid, block, 0, s, p);
   else if (order == 4) hipLaunchKernelGGL((k_velocity2d<T, 4, TY>), grid, block, 0, s, p);
   else hipLaunchKernelGGL((k_velocity2d<T, 8, TY>), grid, block, 0, s, p);

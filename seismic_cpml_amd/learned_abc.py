@@ -137,7 +137,6 @@ def train(net: S.CpmlAbcNet, n: int = 128, runs: int = 16, nstep: int = 600,
     for r in range(runs):
         m = layered_model(n, seed + r)
         teacher, _, keep_t = make_run(m, NPML_WIDE, NPML_WIDE, nstep)
-        teacher.set_fused(True)
         student, _, keep_s = make_run(m, STRIP, 0, 1, source=False)
         teacher.run(1, first_step - 1)
         losses = []
@@ -176,7 +175,6 @@ def reflection(model: Model, nstep: int, net: S.CpmlAbcNet | None = None, per_ax
     if reference is None:
         ref, _, keep = make_run(model, NPML_WIDE, NPML_WIDE, nstep, ext=model.n // 2,
                                 receivers=rec)
-        ref.set_fused(True)
         ref.run()
         reference = _seis(ref)
         ref.close()
@@ -207,15 +205,14 @@ def reflection(model: Model, nstep: int, net: S.CpmlAbcNet | None = None, per_ax
 
 def time_to_solution(model: Model, nstep: int, net: S.CpmlAbcNet, repeats: int = 1):
     """Wall time of nstep steps: learned strip ((n+8)^3 + corrector) against
-    the 20-cell C-PML ((n+40)^3), both with the fused step kernel, graphs,
-    no receivers."""
+    the 20-cell C-PML ((n+40)^3), both with the same step kernels and
+    graphs, no receivers."""
     import torch
 
     res = {}
     for name, pad, absorb, nn in (("cpml20", NPML_WIDE, NPML_WIDE, None),
                                   ("learned4", STRIP, 0, net)):
         s, _, keep = make_run(model, pad, absorb, nstep)
-        s.set_fused(True)
         if nn is not None:
             s.set_boundary(nn)
         s.run(1, min(16, nstep))  # build graphs
