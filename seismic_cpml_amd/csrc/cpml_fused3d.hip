@@ -104,9 +104,10 @@ __device__ __forceinline__ FTile ftile(int nx, int ny, int nz, int kchunk) {
 // derivative is updated at this point; otherwise the old value is carried
 // over (when a new array is given).
 template <typename T>
-__device__ __forceinline__ T pml(bool on, bool slab, T d, const T *mo, T *mn, T a, T b, T K) {
+__device__ __forceinline__ T pml(bool on, bool slab, T d, const T *mo, T *mn, const T *a,
+                                 const T *b, const T *K, int x) {
   if (!slab) return d;
-  if (on) return cpml2(d, mo, mn, a, b, K);
+  if (on) return cpml2(d, mo, mn, a[x], b[x], K[x]);
   if (mn) *mn = *mo;
   return d;
 }
@@ -135,10 +136,18 @@ __global__ __launch_bounds__(FG<TY>::NT) void k_fused3d(Fused3D<T> P) {
   const unsigned bytes = (unsigned)((long)(ft.ke - ft.kb + 4) * p.ps * es);
   const long origin = (long)(ft.kb - 2 + p.R) * p.ps - p.base;
   auto rs = [&](const T *f) { return make_rsrc(f + origin, bytes); };
-  const auto avx = rs(p.vx), avy = rs(p.vy), avz = rs(p.vz);
-  const auto axx = rs(p.sxx), ayy = rs(p.syy), azz = rs(p.szz);
-  const auto axy = rs(p.sxy), axz = rs(p.sxz), ayz = rs(p.syz);
-  const auto acl = rs(p.clam), acm = rs(p.cmu), acb = rs(p.cbuoy);
+  // The ~60 pointers of the step (fields, coefficients, 36 memory arrays, 18
+  // profiles, next state) do not fit in scalar registers across the z loop.
+  // They are re-read from the kernel-argument segment where they are used
+  // (scalar loads, cached), through a pointer the compiler cannot prove
+  // loop-invariant, so it neither hoists them nor spills them.
+  using KArgs = const __attribute__((address_space(4))) Fused3D<T> *;
+  auto args = [&]() {
+    // P is the kernel's only argument: it sits at offset 0 of the segment
+    KArgs q = (KArgs)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(q));
+    return q;
+  };
   auto voff = [&](int x, int y) -> unsigned {
     return (unsigned)(((long)(ft.j0 + y + p.R) * p.ld + p.xpad + ft.i0 + x) * es);
   };
@@ -180,6 +189,11 @@ __global__ __launch_bounds__(FG<TY>::NT) void k_fused3d(Fused3D<T> P) {
   T shv[3], pshv[3];
 
   auto prefetch = [&](int k) {  // loads used at iteration k (sigma plane k)
+    const KArgs Q = args();
+    const auto avx = rs(Q->a.vx), avy = rs(Q->a.vy), avz = rs(Q->a.vz);
+    const auto axx = rs(Q->a.sxx), ayy = rs(Q->a.syy), azz = rs(Q->a.szz);
+    const auto axy = rs(Q->a.sxy), axz = rs(Q->a.sxz), ayz = rs(Q->a.syz);
+    const auto acl = rs(Q->a.clam), acm = rs(Q->a.cmu), acb = rs(Q->a.cbuoy);
 #pragma unroll
     for (int a = 0; a < NP; ++a) {
       if (!ld_ok[a]) continue;
@@ -234,9 +248,9 @@ __global__ __launch_bounds__(FG<TY>::NT) void k_fused3d(Fused3D<T> P) {
     S[a].xo1 = S[a].yo1 = S[a].zo0 = zero;
     S[a].xo2 = S[a].yo2 = S[a].zo1 = zero;
     if (ld_ok[a]) {
-      S[a].xo2 = LD(avx, vo[a], ft.kb - 1);  // becomes xo1 at iteration kb-1
-      S[a].yo2 = LD(avy, vo[a], ft.kb - 1);
-      S[a].zo1 = LD(avz, vo[a], ft.kb - 2);  // becomes zo0 at iteration kb-1
+      S[a].xo2 = LD(rs(p.vx), vo[a], ft.kb - 1);  // becomes xo1 at iteration kb-1
+      S[a].yo2 = LD(rs(p.vy), vo[a], ft.kb - 1);
+      S[a].zo1 = LD(rs(p.vz), vo[a], ft.kb - 2);  // becomes zo0 at iteration kb-1
     }
   }
 #pragma unroll
@@ -249,6 +263,7 @@ __global__ __launch_bounds__(FG<TY>::NT) void k_fused3d(Fused3D<T> P) {
   prefetch(ft.kb - 1);
 
   for (int k = ft.kb - 1; k <= ft.ke; ++k) {
+    const KArgs Q = args();
     // history for the velocity phase: v(k-1) at the tile points (xo1 still
     // holds plane k-1 until take() shifts the planes)
 #pragma unroll
@@ -277,15 +292,6 @@ __global__ __launch_bounds__(FG<TY>::NT) void k_fused3d(Fused3D<T> P) {
     __syncthreads();
 
     // 2. sigma(k) on the tile and the ring
-    // The profile values are loop invariant; reading them through an index the
-    // compiler cannot see through keeps them out of registers across the z loop
-    // (they are only needed in the absorbing slabs, from L1).
-    int oz0 = 0;
-    asm volatile("" : "+v"(oz0));
-    const T *pxa = p.px.a + oz0, *pxb = p.px.b + oz0, *pxK = p.px.K + oz0;
-    const T *pxah = p.px.ah + oz0, *pxbh = p.px.bh + oz0, *pxKh = p.px.Kh + oz0;
-    const T *pya = p.py.a + oz0, *pyb = p.py.b + oz0, *pyK = p.py.K + oz0;
-    const T *pyah = p.py.ah + oz0, *pybh = p.py.bh + oz0, *pyKh = p.py.Kh + oz0;
     const bool kin = k >= 0 && k < nz;
     const bool inz = kin && p.sz.in(k);
     const bool store_k = k >= ft.kb && k < ft.ke;
@@ -315,12 +321,10 @@ __global__ __launch_bounds__(FG<TY>::NT) void k_fused3d(Fused3D<T> P) {
           T dvx_dx = (vH[0][h + 1] - vx_c) * p.odx;
           T dvy_dy = (vy_c - vH[1][h - G::HW]) * p.ody;
           T dvz_dz = (vz_c - S[a].zo0) * p.odz;
-          dvx_dx = pml(r_n, inx, dvx_dx, p.mx[0] + ox, st ? P.mx[0] + ox : nullptr,
-                       pxah[i], pxbh[i], pxKh[i]);
-          dvy_dy = pml(r_n, iny, dvy_dy, p.my[0] + oy, st ? P.my[0] + oy : nullptr, pya[j],
-                       pyb[j], pyK[j]);
-          dvz_dz = pml(r_n, inz, dvz_dz, p.mz[0] + oz, st ? P.mz[0] + oz : nullptr, p.pz.a[k],
-                       p.pz.b[k], p.pz.K[k]);
+          dvx_dx = pml(r_n, inx, dvx_dx, Q->a.mx[0] + ox, st ? Q->mx[0] + ox : nullptr,
+                       Q->a.px.ah, Q->a.px.bh, Q->a.px.Kh, i);
+          dvy_dy = pml(r_n, iny, dvy_dy, Q->a.my[0] + oy, st ? Q->my[0] + oy : nullptr, Q->a.py.a, Q->a.py.b, Q->a.py.K, j);
+          dvz_dz = pml(r_n, inz, dvz_dz, Q->a.mz[0] + oz, st ? Q->mz[0] + oz : nullptr, Q->a.pz.a, Q->a.pz.b, Q->a.pz.K, k);
           if (r_n) {
             const T lp2m = cl + T(2) * cm;
             sn[0] = lp2m * dvx_dx + cl * (dvy_dy + dvz_dz) + S[a].so[0];
@@ -333,10 +337,9 @@ __global__ __launch_bounds__(FG<TY>::NT) void k_fused3d(Fused3D<T> P) {
         {
           T dvy_dx = (vy_c - vH[1][h - 1]) * p.odx;
           T dvx_dy = (vH[0][h + G::HW] - vx_c) * p.ody;
-          dvy_dx = pml(r_xy, inx, dvy_dx, p.mx[1] + ox, st ? P.mx[1] + ox : nullptr, pxa[i],
-                       pxb[i], pxK[i]);
-          dvx_dy = pml(r_xy, iny, dvx_dy, p.my[1] + oy, st ? P.my[1] + oy : nullptr,
-                       pyah[j], pybh[j], pyKh[j]);
+          dvy_dx = pml(r_xy, inx, dvy_dx, Q->a.mx[1] + ox, st ? Q->mx[1] + ox : nullptr, Q->a.px.a, Q->a.px.b, Q->a.px.K, i);
+          dvx_dy = pml(r_xy, iny, dvx_dy, Q->a.my[1] + oy, st ? Q->my[1] + oy : nullptr,
+                       Q->a.py.ah, Q->a.py.bh, Q->a.py.Kh, j);
           if (r_xy) sn[3] = cm * (dvy_dx + dvx_dy) + S[a].so[3];
         }
         // sigma_xz: 3D:908-925
@@ -344,10 +347,9 @@ __global__ __launch_bounds__(FG<TY>::NT) void k_fused3d(Fused3D<T> P) {
         {
           T dvz_dx = (vz_c - vH[2][h - 1]) * p.odx;
           T dvx_dz = (S[a].xo2 - vx_c) * p.odz;
-          dvz_dx = pml(r_xz, inx, dvz_dx, p.mx[2] + ox, st ? P.mx[2] + ox : nullptr, pxa[i],
-                       pxb[i], pxK[i]);
-          dvx_dz = pml(r_xz, inz, dvx_dz, p.mz[1] + oz, st ? P.mz[1] + oz : nullptr,
-                       p.pz.ah[k], p.pz.bh[k], p.pz.Kh[k]);
+          dvz_dx = pml(r_xz, inx, dvz_dx, Q->a.mx[2] + ox, st ? Q->mx[2] + ox : nullptr, Q->a.px.a, Q->a.px.b, Q->a.px.K, i);
+          dvx_dz = pml(r_xz, inz, dvx_dz, Q->a.mz[1] + oz, st ? Q->mz[1] + oz : nullptr,
+                       Q->a.pz.ah, Q->a.pz.bh, Q->a.pz.Kh, k);
           if (r_xz) sn[4] = cm * (dvz_dx + dvx_dz) + S[a].so[4];
         }
         // sigma_yz: 3D:927-943
@@ -355,20 +357,20 @@ __global__ __launch_bounds__(FG<TY>::NT) void k_fused3d(Fused3D<T> P) {
         {
           T dvz_dy = (vH[2][h + G::HW] - vz_c) * p.ody;
           T dvy_dz = (S[a].yo2 - vy_c) * p.odz;
-          dvz_dy = pml(r_yz, iny, dvz_dy, p.my[2] + oy, st ? P.my[2] + oy : nullptr,
-                       pyah[j], pybh[j], pyKh[j]);
-          dvy_dz = pml(r_yz, inz, dvy_dz, p.mz[2] + oz, st ? P.mz[2] + oz : nullptr,
-                       p.pz.ah[k], p.pz.bh[k], p.pz.Kh[k]);
+          dvz_dy = pml(r_yz, iny, dvz_dy, Q->a.my[2] + oy, st ? Q->my[2] + oy : nullptr,
+                       Q->a.py.ah, Q->a.py.bh, Q->a.py.Kh, j);
+          dvy_dz = pml(r_yz, inz, dvy_dz, Q->a.mz[2] + oz, st ? Q->mz[2] + oz : nullptr,
+                       Q->a.pz.ah, Q->a.pz.bh, Q->a.pz.Kh, k);
           if (r_yz) sn[5] = cm * (dvz_dy + dvy_dz) + S[a].so[5];
         }
         if (st) {
           const unsigned so_k = soff(k);
-          BufOp<T>::st(sn[0], rs(P.sxx), vo[a], so_k);
-          BufOp<T>::st(sn[1], rs(P.syy), vo[a], so_k);
-          BufOp<T>::st(sn[2], rs(P.szz), vo[a], so_k);
-          BufOp<T>::st(sn[3], rs(P.sxy), vo[a], so_k);
-          BufOp<T>::st(sn[4], rs(P.sxz), vo[a], so_k);
-          BufOp<T>::st(sn[5], rs(P.syz), vo[a], so_k);
+          BufOp<T>::st(sn[0], rs(Q->sxx), vo[a], so_k);
+          BufOp<T>::st(sn[1], rs(Q->syy), vo[a], so_k);
+          BufOp<T>::st(sn[2], rs(Q->szz), vo[a], so_k);
+          BufOp<T>::st(sn[3], rs(Q->sxy), vo[a], so_k);
+          BufOp<T>::st(sn[4], rs(Q->sxz), vo[a], so_k);
+          BufOp<T>::st(sn[5], rs(Q->syz), vo[a], so_k);
         }
       }
       const int r = (py[a] + 1) * G::RW + px[a] + 1;
@@ -406,12 +408,9 @@ __global__ __launch_bounds__(FG<TY>::NT) void k_fused3d(Fused3D<T> P) {
           T dsxx_dx = (s0[r] - s0[r - 1]) * p.odx;
           T dsxy_dy = (s2[r] - s2[r - G::RW]) * p.ody;
           T dsxz_dz = (sxz1[a] - sxz2[a]) * p.odz;
-          dsxx_dx = pml(r_vx, inx, dsxx_dx, p.mx[3] + ox, P.mx[3] + ox, pxa[i], pxb[i],
-                        pxK[i]);
-          dsxy_dy = pml(r_vx, iny, dsxy_dy, p.my[3] + oy, P.my[3] + oy, pya[j], pyb[j],
-                        pyK[j]);
-          dsxz_dz = pml(r_vx, inzv, dsxz_dz, p.mz[3] + oz, P.mz[3] + oz, p.pz.a[kv],
-                        p.pz.b[kv], p.pz.K[kv]);
+          dsxx_dx = pml(r_vx, inx, dsxx_dx, Q->a.mx[3] + ox, Q->mx[3] + ox, Q->a.px.a, Q->a.px.b, Q->a.px.K, i);
+          dsxy_dy = pml(r_vx, iny, dsxy_dy, Q->a.my[3] + oy, Q->my[3] + oy, Q->a.py.a, Q->a.py.b, Q->a.py.K, j);
+          dsxz_dz = pml(r_vx, inzv, dsxz_dz, Q->a.mz[3] + oz, Q->mz[3] + oz, Q->a.pz.a, Q->a.pz.b, Q->a.pz.K, kv);
           if (r_vx) vxn = c * (dsxx_dx + dsxy_dy + dsxz_dz) + vxn;
         }
         // vy: 3D:995-1017
@@ -420,12 +419,9 @@ __global__ __launch_bounds__(FG<TY>::NT) void k_fused3d(Fused3D<T> P) {
           T dsxy_dx = (s2[r + 1] - s2[r]) * p.odx;
           T dsyy_dy = (s1[r + G::RW] - s1[r]) * p.ody;
           T dsyz_dz = (syz1[a] - syz2[a]) * p.odz;
-          dsxy_dx = pml(r_vy, inx, dsxy_dx, p.mx[4] + ox, P.mx[4] + ox, pxah[i],
-                        pxbh[i], pxKh[i]);
-          dsyy_dy = pml(r_vy, iny, dsyy_dy, p.my[4] + oy, P.my[4] + oy, pyah[j],
-                        pybh[j], pyKh[j]);
-          dsyz_dz = pml(r_vy, inzv, dsyz_dz, p.mz[4] + oz, P.mz[4] + oz, p.pz.a[kv],
-                        p.pz.b[kv], p.pz.K[kv]);
+          dsxy_dx = pml(r_vy, inx, dsxy_dx, Q->a.mx[4] + ox, Q->mx[4] + ox, Q->a.px.ah, Q->a.px.bh, Q->a.px.Kh, i);
+          dsyy_dy = pml(r_vy, iny, dsyy_dy, Q->a.my[4] + oy, Q->my[4] + oy, Q->a.py.ah, Q->a.py.bh, Q->a.py.Kh, j);
+          dsyz_dz = pml(r_vy, inzv, dsyz_dz, Q->a.mz[4] + oz, Q->mz[4] + oz, Q->a.pz.a, Q->a.pz.b, Q->a.pz.K, kv);
           if (r_vy) vyn = c * (dsxy_dx + dsyy_dy + dsyz_dz) + vyn;
         }
         // vz: 3D:1031-1052
@@ -434,12 +430,9 @@ __global__ __launch_bounds__(FG<TY>::NT) void k_fused3d(Fused3D<T> P) {
           T dsxz_dx = (s3[r + 1] - s3[r]) * p.odx;
           T dsyz_dy = (s4[r] - s4[r - G::RW]) * p.ody;
           T dszz_dz = (snew[a][2] - szz1[a]) * p.odz;
-          dsxz_dx = pml(r_vz, inx, dsxz_dx, p.mx[5] + ox, P.mx[5] + ox, pxah[i],
-                        pxbh[i], pxKh[i]);
-          dsyz_dy = pml(r_vz, iny, dsyz_dy, p.my[5] + oy, P.my[5] + oy, pya[j], pyb[j],
-                        pyK[j]);
-          dszz_dz = pml(r_vz, inzv, dszz_dz, p.mz[5] + oz, P.mz[5] + oz, p.pz.ah[kv],
-                        p.pz.bh[kv], p.pz.Kh[kv]);
+          dsxz_dx = pml(r_vz, inx, dsxz_dx, Q->a.mx[5] + ox, Q->mx[5] + ox, Q->a.px.ah, Q->a.px.bh, Q->a.px.Kh, i);
+          dsyz_dy = pml(r_vz, iny, dsyz_dy, Q->a.my[5] + oy, Q->my[5] + oy, Q->a.py.a, Q->a.py.b, Q->a.py.K, j);
+          dszz_dz = pml(r_vz, inzv, dszz_dz, Q->a.mz[5] + oz, Q->mz[5] + oz, Q->a.pz.ah, Q->a.pz.bh, Q->a.pz.Kh, kv);
           if (r_vz) vzn = c * (dsxz_dx + dsyz_dy + dszz_dz) + vzn;
         }
         // point force (3D:1074-1082) and rigid faces (3D:1087-1121)
@@ -454,9 +447,9 @@ __global__ __launch_bounds__(FG<TY>::NT) void k_fused3d(Fused3D<T> P) {
           vzn = T(0);
         }
         const unsigned so_v = soff(kv);
-        BufOp<T>::st(vxn, rs(P.vx), vo[a], so_v);
-        BufOp<T>::st(vyn, rs(P.vy), vo[a], so_v);
-        BufOp<T>::st(vzn, rs(P.vz), vo[a], so_v);
+        BufOp<T>::st(vxn, rs(Q->vx), vo[a], so_v);
+        BufOp<T>::st(vyn, rs(Q->vy), vo[a], so_v);
+        BufOp<T>::st(vzn, rs(Q->vz), vo[a], so_v);
       }
     }
     // sigma history at the tile points
