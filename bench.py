@@ -289,7 +289,7 @@ def bench_batch(args):
         seq_total += t2 - t0
         same += int(np.array_equal(sx, seis[q][0]) and np.array_equal(sy, seis[q][1]))
     seq_cells = sum(s["NX"] * s["NY"] for s in w.shots[:nseq])
-    seq_value = seq_cells * steps / seq_run / 1e9
+    seq_value = seq_cells * steps / seq_run / 1e9 if nseq else None
     es = 4 if w.prec == 4 else 8
     bytes_per = es * 18  # two-phase 2D interior words; slabs add under 1%
     config = {
@@ -298,10 +298,10 @@ def bench_batch(args):
                     "[5, 15] Hz, 64 receivers each",
         "cells_per_step": cells, "setup_s": round(setup, 2),
         "autotune": {"tile_rows": ty, "blocks_per_cu": bpc, "trial_ms": tune_ms},
-        "sequential": {"shots": nseq, "gcell_s": round(seq_value, 3),
+        "sequential": {"shots": nseq, "gcell_s": seq_value and round(seq_value, 3),
                        "run_s": round(seq_run, 3), "with_setup_s": round(seq_total, 3),
                        "bitwise_equal_seismograms": same},
-        "batched_vs_sequential": round(value / seq_value, 3),
+        "batched_vs_sequential": seq_value and round(value / seq_value, 3),
         "step_hbm_frac": round(bytes_per * value / HBM_PEAK_GBS, 4),
     }
     roof = {"bound": "hbm", "kernel": "batch step", "achieved": round(bytes_per * value, 1),

@@ -133,8 +133,9 @@ int cpml_gpu_velocity_norm(cpml_solver *h, double *vmax);
 
 /* Steps captured per HIP graph replay (default 16; 0 = launch eagerly). */
 int cpml_gpu_set_graph_steps(cpml_solver *h, int steps);
-/* 3D tiling: planes per block along z (default 8) and rows of 64 points per
- * block (4, 8 or 16; default 4).  Results do not depend on either. */
+/* Tiling: planes per block along z (3D, default 8) and rows of 64 points per
+ * block (4, 8 or 16; default 4; the fused 3D step uses 12 rows for 16, its
+ * block-size limit).  Results do not depend on either. */
 int cpml_gpu_set_kchunk(cpml_solver *h, int kchunk);
 int cpml_gpu_set_tile(cpml_solver *h, int tile_y);
 /* 3D only: run each time step as ONE fused kernel (stress and velocity in a
@@ -185,9 +186,10 @@ int cpml_batch_set_source(cpml_batch *b, int shot, int ISOURCE, int JSOURCE,
                           const void *force_x, const void *force_y);
 int cpml_batch_set_receivers(cpml_batch *b, int shot, int NREC, const int *ix_rec,
                              const int *iy_rec);
-/* Launch shape: rows per tile (4, 8, 16); blocks_per_cu 0 = one workgroup
- * per tile, else a persistent grid of blocks_per_cu * CUs workgroups; steps
- * per captured graph (0: no graph). */
+/* Launch shape: rows per tile (4, 8, 16, or 32 / 64 walked by 16-row
+ * workgroups); blocks_per_cu 0 = one workgroup per tile, else a persistent
+ * grid of blocks_per_cu * CUs workgroups; steps per captured graph (0: no
+ * graph). */
 int cpml_batch_set_tiling(cpml_batch *b, int tile_y, int blocks_per_cu, int graph_steps);
 /* Times every launch shape over trial_steps steps of this batch, keeps the
  * fastest, then returns the batch to rest (fields, memory variables and

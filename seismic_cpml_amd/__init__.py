@@ -4,7 +4,15 @@ Native core: ``libcpml_gpu.so`` (hand-written HIP kernels for gfx950 behind
 the C ABI in include/cpml_gpu.h), bound from Fortran by the ``cpml_gpu``
 module (fortran/cpml_gpu.f90) and from Python by :mod:`.solver`.
 """
-from .solver import (  # noqa: F401
+# PyTorch (used here only for device-memory plumbing) ships its own HIP
+# runtime; it must be loaded before libcpml_gpu initialises the system one,
+# or torch later finds no device.  Import it first when it is installed.
+try:  # noqa: SIM105
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
+from .solver import (  # noqa: E402,F401
     CpmlAbcNet,
     CpmlBatch,
     CpmlError,

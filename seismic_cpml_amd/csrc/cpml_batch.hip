@@ -59,7 +59,10 @@ __device__ __forceinline__ int shot_of(const int *__restrict__ tstart, int nshot
   return lo;
 }
 
-template <typename T, int ORDER, int TY, bool STRESS>
+// A tile is 64 points by TY*RG rows; the block (64 x TY threads) walks its RG
+// row groups one after another, so the shot's descriptor is read once per
+// TY*RG rows.
+template <typename T, int ORDER, int TY, int RG, bool STRESS>
 __global__ __launch_bounds__(BX *TY) void k_batch2d(const Step2D<T> *__restrict__ shots,
                                                    const int *__restrict__ tstart, int nshots,
                                                    int ntiles) {
@@ -71,10 +74,14 @@ __global__ __launch_bounds__(BX *TY) void k_batch2d(const Step2D<T> *__restrict_
     const int lt = t - tstart[s];
     const int ntx = (p.nx + BX - 1) / BX;
     const int i = (lt % ntx) * BX + threadIdx.x;
-    const int j = (lt / ntx) * TY + threadIdx.y;
-    if (i < p.nx && j < p.ny) {
-      if (STRESS) stress2d_point<T, ORDER>(p, i, j);
-      else velocity2d_point<T, ORDER>(p, i, j);
+    const int j0 = (lt / ntx) * (TY * RG) + threadIdx.y;
+#pragma unroll
+    for (int r = 0; r < RG; ++r) {
+      const int j = j0 + r * TY;
+      if (i < p.nx && j < p.ny) {
+        if (STRESS) stress2d_point<T, ORDER>(p, i, j);
+        else velocity2d_point<T, ORDER>(p, i, j);
+      }
     }
   }
 }
@@ -94,18 +101,18 @@ __global__ __launch_bounds__(kMaxRecBlock) void k_batch_record(const Record<T> *
   if (threadIdx.x == 0) *p.step = it + 1;
 }
 
-template <typename T, int TY>
+template <typename T, int TY, int RG>
 hipError_t batch_phase_ty(int order, bool stress, const Step2D<T> *shots, const int *tstart,
                           int nshots, int ntiles, int grid, hipStream_t s) {
   const dim3 g(grid), blk(BX, TY);
-#define CPML_B(O)                                                                              \
-  do {                                                                                         \
-    if (stress)                                                                                \
-      hipLaunchKernelGGL((k_batch2d<T, O, TY, true>), g, blk, 0, s, shots, tstart, nshots,    \
-                         ntiles);                                                              \
-    else                                                                                       \
-      hipLaunchKernelGGL((k_batch2d<T, O, TY, false>), g, blk, 0, s, shots, tstart, nshots,   \
-                         ntiles);                                                              \
+#define CPML_B(O)                                                                            \
+  do {                                                                                       \
+    if (stress)                                                                              \
+      hipLaunchKernelGGL((k_batch2d<T, O, TY, RG, true>), g, blk, 0, s, shots, tstart,      \
+                         nshots, ntiles);                                                    \
+    else                                                                                     \
+      hipLaunchKernelGGL((k_batch2d<T, O, TY, RG, false>), g, blk, 0, s, shots, tstart,     \
+                         nshots, ntiles);                                                    \
   } while (0)
   if (order == 2) CPML_B(2);
   else if (order == 4) CPML_B(4);
@@ -114,13 +121,18 @@ hipError_t batch_phase_ty(int order, bool stress, const Step2D<T> *shots, const 
   return hipGetLastError();
 }
 
+// rows per tile: 4, 8, 16 (one row group) or 32, 64 (2 or 4 groups of 16)
 template <typename T>
-hipError_t batch_phase(int order, int ty, bool stress, const void *shots, const int *tstart,
+hipError_t batch_phase(int order, int rows, bool stress, const void *shots, const int *tstart,
                        int nshots, int ntiles, int grid, hipStream_t s) {
   const auto *sh = static_cast<const Step2D<T> *>(shots);
-  if (ty == 16) return batch_phase_ty<T, 16>(order, stress, sh, tstart, nshots, ntiles, grid, s);
-  if (ty == 8) return batch_phase_ty<T, 8>(order, stress, sh, tstart, nshots, ntiles, grid, s);
-  return batch_phase_ty<T, 4>(order, stress, sh, tstart, nshots, ntiles, grid, s);
+  switch (rows) {
+    case 64: return batch_phase_ty<T, 16, 4>(order, stress, sh, tstart, nshots, ntiles, grid, s);
+    case 32: return batch_phase_ty<T, 16, 2>(order, stress, sh, tstart, nshots, ntiles, grid, s);
+    case 16: return batch_phase_ty<T, 16, 1>(order, stress, sh, tstart, nshots, ntiles, grid, s);
+    case 8: return batch_phase_ty<T, 8, 1>(order, stress, sh, tstart, nshots, ntiles, grid, s);
+    default: return batch_phase_ty<T, 4, 1>(order, stress, sh, tstart, nshots, ntiles, grid, s);
+  }
 }
 
 template <typename T>
@@ -672,8 +684,8 @@ int cpml_batch_set_receivers(cpml_batch *b, int shot, int NREC, const int *ix_re
 
 int cpml_batch_set_tiling(cpml_batch *b, int tile_y, int blocks_per_cu, int graph_steps) {
   if (!b) return CPML_ERR_ARG;
-  if (tile_y != 4 && tile_y != 8 && tile_y != 16)
-    return bfail(b, CPML_ERR_ARG, "tile rows must be 4, 8 or 16");
+  if (tile_y != 4 && tile_y != 8 && tile_y != 16 && tile_y != 32 && tile_y != 64)
+    return bfail(b, CPML_ERR_ARG, "tile rows must be 4, 8, 16, 32 or 64");
   if (blocks_per_cu < 0 || blocks_per_cu > 64 || graph_steps < 0 || graph_steps > 4096)
     return bfail(b, CPML_ERR_ARG, "bad launch parameters");
   b->tile_y = tile_y;
@@ -694,7 +706,7 @@ int cpml_batch_autotune(cpml_batch *b, int trial_steps, const char *cache_path, 
   int ty = 4, bpc = 0;
   double best = -1;
   if (!cache_lookup(cache_path, key, &ty, &bpc)) {
-    const int tys[3] = {4, 8, 16};
+    const int tys[5] = {4, 8, 16, 32, 64};
     const int bpcs[4] = {0, 2, 4, 8};
     for (int a : tys)
       for (int c : bpcs) {

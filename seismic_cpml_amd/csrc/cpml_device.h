@@ -7,13 +7,31 @@ namespace cpml {
 
 constexpr int BX = 64;  // points per wavefront row along x
 
+// Position of a tile's z chunk in its XCD band -> z chunk.  When the chunks
+// divide into 8 equal groups, band b holds the chunks c with c % 8 == b, so
+// every XCD gets the same share of the (slower) absorbing z-slab planes
+// instead of the first and last bands holding all of them.
+__device__ __forceinline__ int zchunk_of(int band_pos, int ntz) {
+  if (ntz & 7) return band_pos;
+  const int m = ntz >> 3;
+  return band_pos / m + 8 * (band_pos % m);
+}
+
+// d / K.  The reference profiles use K_MAX_PML = 1 (2D2:215), so K is 1 at
+// every point and x / 1 == x exactly in IEEE arithmetic: the correctly
+// rounded division (a dozen instructions) is only executed where K != 1.
+template <typename T>
+__device__ __forceinline__ T over_k(T d, T K) {
+  return K == T(1) ? d : d / K;
+}
+
 // C-PML correction of a derivative: psi <- b*psi + a*d ; d <- d/K + psi
 // (e.g. seismic_CPML_2D_isotropic_second_order.f90:567-571).
 template <typename T>
 __device__ __forceinline__ T cpml(T d, T *psi, T a, T b, T K) {
   const T m = b * *psi + a * d;
   *psi = m;
-  return d / K + m;
+  return over_k(d, K) + m;
 }
 
 // Same recursion with the old value read from one array and the new one
@@ -22,7 +40,19 @@ template <typename T>
 __device__ __forceinline__ T cpml2(T d, const T *psi_old, T *psi_new, T a, T b, T K) {
   const T m = b * *psi_old + a * d;
   if (psi_new) *psi_new = m;
-  return d / K + m;
+  return over_k(d, K) + m;
+}
+
+// The kernel's only argument, read through the kernel-argument segment with a
+// pointer the compiler cannot prove loop-invariant.  Kernels whose argument
+// block holds more pointers than fit in scalar registers re-read the ones
+// they need inside their z loop (scalar loads, cached) instead of keeping
+// them live across it (which spills scalar registers into vector lanes).
+template <typename A>
+__device__ __forceinline__ const __attribute__((address_space(4))) A *kernarg() {
+  auto q = (const __attribute__((address_space(4))) A *)__builtin_amdgcn_kernarg_segment_ptr();
+  asm volatile("" : "+s"(q));
+  return q;
 }
 
 // Raw buffer resource over `bytes` bytes at p (gfx9 family word 3).

@@ -91,7 +91,7 @@ __device__ __forceinline__ FTile ftile(int nx, int ny, int nz, int kchunk) {
   const int q = nt >> 3, r = nt & 7;
   const int t = xcd * q + min(xcd, r) + loc;
   const int tx = t % ntx, rest = t / ntx;
-  const int ty = rest % nty, tz = rest / nty;
+  const int ty = rest % nty, tz = zchunk_of(rest / nty, ntz);
   FTile f;
   f.i0 = tx * FTX;
   f.j0 = ty * TY;
@@ -137,17 +137,10 @@ __global__ __launch_bounds__(FG<TY>::NT) void k_fused3d(Fused3D<T> P) {
   const long origin = (long)(ft.kb - 2 + p.R) * p.ps - p.base;
   auto rs = [&](const T *f) { return make_rsrc(f + origin, bytes); };
   // The ~60 pointers of the step (fields, coefficients, 36 memory arrays, 18
-  // profiles, next state) do not fit in scalar registers across the z loop.
-  // They are re-read from the kernel-argument segment where they are used
-  // (scalar loads, cached), through a pointer the compiler cannot prove
-  // loop-invariant, so it neither hoists them nor spills them.
+  // profiles, next state) do not fit in scalar registers across the z loop:
+  // they are re-read where they are used (kernarg()).
   using KArgs = const __attribute__((address_space(4))) Fused3D<T> *;
-  auto args = [&]() {
-    // P is the kernel's only argument: it sits at offset 0 of the segment
-    KArgs q = (KArgs)__builtin_amdgcn_kernarg_segment_ptr();
-    asm volatile("" : "+s"(q));
-    return q;
-  };
+  auto args = [&]() -> KArgs { return kernarg<Fused3D<T>>(); };
   auto voff = [&](int x, int y) -> unsigned {
     return (unsigned)(((long)(ft.j0 + y + p.R) * p.ld + p.xpad + ft.i0 + x) * es);
   };

@@ -75,7 +75,8 @@ struct Tile3 {
 
 // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD one
 // contiguous band of tiles (x fastest, then y, then z chunk) so the tiles that
-// read each other's halo rows run on the same XCD and share its L2.
+// read each other's halo rows run on the same XCD and share its L2; the z
+// chunks are interleaved over the bands (zchunk_of).
 template <int TY3>
 __device__ __forceinline__ Tile3 tile3(int nx, int ny, int nz, int kchunk) {
   const int ntx = (nx + BX - 1) / BX, nty = (ny + TY3 - 1) / TY3;
@@ -86,7 +87,7 @@ __device__ __forceinline__ Tile3 tile3(int nx, int ny, int nz, int kchunk) {
   const int q = nt >> 3, r = nt & 7;
   const int t = xcd * q + min(xcd, r) + loc;
   const int tx = t % ntx, rest = t / ntx;
-  const int ty = rest % nty, tz = rest / nty;
+  const int ty = rest % nty, tz = zchunk_of(rest / nty, ntz);
   Tile3 o;
   o.i = tx * BX + threadIdx.x;
   o.j = ty * TY3 + threadIdx.y;
@@ -171,12 +172,25 @@ __global__ __launch_bounds__(BX *TY3) void k_stress3d(Step3D<T> p) {
     L.sxz = CPML_LD(rxz, 0, 0, 0);
     L.syz = CPML_LD(ryz, 0, 0, 0);
   };
+  // x / y profile values of this column (loop invariant; only read in the
+  // absorbing slabs)
+  T xa = 0, xb = 0, xK = 1, xah = 0, xbh = 0, xKh = 1;
+  T ya = 0, yb = 0, yK = 1, yah = 0, ybh = 0, yKh = 1;
+  if (inx) {
+    xa = p.px.a[i]; xb = p.px.b[i]; xK = p.px.K[i];
+    xah = p.px.ah[i]; xbh = p.px.bh[i]; xKh = p.px.Kh[i];
+  }
+  if (iny) {
+    ya = p.py.a[j]; yb = p.py.b[j]; yK = p.py.K[j];
+    yah = p.py.ah[j]; ybh = p.py.bh[j]; yKh = p.py.Kh[j];
+  }
   int kk = 1;  // buffer plane of k = kb
   T vz_m = CPML_LD(rvz, 0, 0, -1);
   T vx_k = CPML_LD(rvx, 0, 0, 0), vy_k = CPML_LD(rvy, 0, 0, 0);
   SPlane<T> L;
   load(L, kk);
   for (int k = t.kb; k < t.ke; ++k, ++kk) {
+    const auto Q = kernarg<Step3D<T>>();
     SPlane<T> N = L;
     if (k + 1 < t.ke) load(N, kk + 1);
     const bool inz = p.sz.in(k);
@@ -187,9 +201,9 @@ __global__ __launch_bounds__(BX *TY3) void k_stress3d(Step3D<T> p) {
       T dvx_dx = (L.vx_xp - vx_k) * p.odx;
       T dvy_dy = (vy_k - L.vy_ym) * p.ody;
       T dvz_dz = (L.vz_k - vz_m) * p.odz;
-      if (inx) dvx_dx = cpml(dvx_dx, p.mx[0] + mxo + k * mxs, p.px.ah[i], p.px.bh[i], p.px.Kh[i]);
-      if (iny) dvy_dy = cpml(dvy_dy, p.my[0] + myo + k * mys, p.py.a[j], p.py.b[j], p.py.K[j]);
-      if (inz) dvz_dz = cpml(dvz_dz, p.mz[0] + mzk, p.pz.a[k], p.pz.b[k], p.pz.K[k]);
+      if (inx) dvx_dx = cpml(dvx_dx, Q->mx[0] + mxo + k * mxs, xah, xbh, xKh);
+      if (iny) dvy_dy = cpml(dvy_dy, Q->my[0] + myo + k * mys, ya, yb, yK);
+      if (inz) dvz_dz = cpml(dvz_dz, Q->mz[0] + mzk, Q->pz.a[k], Q->pz.b[k], Q->pz.K[k]);
       const T lp2m = cl + T(2) * cm;
       CPML_ST(lp2m * dvx_dx + cl * (dvy_dy + dvz_dz) + L.sxx, rxx);
       CPML_ST(cl * (dvx_dx + dvz_dz) + lp2m * dvy_dy + L.syy, ryy);
@@ -199,24 +213,24 @@ __global__ __launch_bounds__(BX *TY3) void k_stress3d(Step3D<T> p) {
     if (r_xy) {
       T dvy_dx = (vy_k - L.vy_xm) * p.odx;
       T dvx_dy = (L.vx_yp - vx_k) * p.ody;
-      if (inx) dvy_dx = cpml(dvy_dx, p.mx[1] + mxo + k * mxs, p.px.a[i], p.px.b[i], p.px.K[i]);
-      if (iny) dvx_dy = cpml(dvx_dy, p.my[1] + myo + k * mys, p.py.ah[j], p.py.bh[j], p.py.Kh[j]);
+      if (inx) dvy_dx = cpml(dvy_dx, Q->mx[1] + mxo + k * mxs, xa, xb, xK);
+      if (iny) dvx_dy = cpml(dvx_dy, Q->my[1] + myo + k * mys, yah, ybh, yKh);
       CPML_ST(cm * (dvy_dx + dvx_dy) + L.sxy, rxy);
     }
     // sigma_xz: 3D:908-925 (global k <= NZ-1)
     if (r_xz && k <= nz - 2) {
       T dvz_dx = (L.vz_k - L.vz_xm) * p.odx;
       T dvx_dz = (L.vx_p - vx_k) * p.odz;
-      if (inx) dvz_dx = cpml(dvz_dx, p.mx[2] + mxo + k * mxs, p.px.a[i], p.px.b[i], p.px.K[i]);
-      if (inz) dvx_dz = cpml(dvx_dz, p.mz[1] + mzk, p.pz.ah[k], p.pz.bh[k], p.pz.Kh[k]);
+      if (inx) dvz_dx = cpml(dvz_dx, Q->mx[2] + mxo + k * mxs, xa, xb, xK);
+      if (inz) dvx_dz = cpml(dvx_dz, Q->mz[1] + mzk, Q->pz.ah[k], Q->pz.bh[k], Q->pz.Kh[k]);
       CPML_ST(cm * (dvz_dx + dvx_dz) + L.sxz, rxz);
     }
     // sigma_yz: 3D:927-943
     if (r_yz && k <= nz - 2) {
       T dvz_dy = (L.vz_yp - L.vz_k) * p.ody;
       T dvy_dz = (L.vy_p - vy_k) * p.odz;
-      if (iny) dvz_dy = cpml(dvz_dy, p.my[2] + myo + k * mys, p.py.ah[j], p.py.bh[j], p.py.Kh[j]);
-      if (inz) dvy_dz = cpml(dvy_dz, p.mz[2] + mzk, p.pz.ah[k], p.pz.bh[k], p.pz.Kh[k]);
+      if (iny) dvz_dy = cpml(dvz_dy, Q->my[2] + myo + k * mys, yah, ybh, yKh);
+      if (inz) dvy_dz = cpml(dvy_dz, Q->mz[2] + mzk, Q->pz.ah[k], Q->pz.bh[k], Q->pz.Kh[k]);
       CPML_ST(cm * (dvz_dy + dvy_dz) + L.syz, ryz);
     }
     vz_m = L.vz_k;
@@ -273,12 +287,25 @@ __global__ __launch_bounds__(BX *TY3) void k_velocity3d(Step3D<T> p) {
     L.vy = CPML_LD(rvy, 0, 0, 0);
     L.vz = CPML_LD(rvz, 0, 0, 0);
   };
+  // x / y profile values of this column (loop invariant; only read in the
+  // absorbing slabs)
+  T xa = 0, xb = 0, xK = 1, xah = 0, xbh = 0, xKh = 1;
+  T ya = 0, yb = 0, yK = 1, yah = 0, ybh = 0, yKh = 1;
+  if (inx) {
+    xa = p.px.a[i]; xb = p.px.b[i]; xK = p.px.K[i];
+    xah = p.px.ah[i]; xbh = p.px.bh[i]; xKh = p.px.Kh[i];
+  }
+  if (iny) {
+    ya = p.py.a[j]; yb = p.py.b[j]; yK = p.py.K[j];
+    yah = p.py.ah[j]; ybh = p.py.bh[j]; yKh = p.py.Kh[j];
+  }
   int kk = 1;
   T sxz_m = CPML_LD(rxz, 0, 0, -1), syz_m = CPML_LD(ryz, 0, 0, -1);
   T szz_k = CPML_LD(rzz, 0, 0, 0);
   VPlane<T> L;
   load(L, kk);
   for (int k = t.kb; k < t.ke; ++k, ++kk) {
+    const auto Q = kernarg<Step3D<T>>();
     VPlane<T> N = L;
     if (k + 1 < t.ke) load(N, kk + 1);
     const bool inz = p.sz.in(k);
@@ -290,9 +317,9 @@ __global__ __launch_bounds__(BX *TY3) void k_velocity3d(Step3D<T> p) {
       T dsxx_dx = (L.sxx_k - L.sxx_xm) * p.odx;
       T dsxy_dy = (L.sxy_k - L.sxy_ym) * p.ody;
       T dsxz_dz = (L.sxz_k - sxz_m) * p.odz;
-      if (inx) dsxx_dx = cpml(dsxx_dx, p.mx[3] + mxo + k * mxs, p.px.a[i], p.px.b[i], p.px.K[i]);
-      if (iny) dsxy_dy = cpml(dsxy_dy, p.my[3] + myo + k * mys, p.py.a[j], p.py.b[j], p.py.K[j]);
-      if (inz) dsxz_dz = cpml(dsxz_dz, p.mz[3] + mzk, p.pz.a[k], p.pz.b[k], p.pz.K[k]);
+      if (inx) dsxx_dx = cpml(dsxx_dx, Q->mx[3] + mxo + k * mxs, xa, xb, xK);
+      if (iny) dsxy_dy = cpml(dsxy_dy, Q->my[3] + myo + k * mys, ya, yb, yK);
+      if (inz) dsxz_dz = cpml(dsxz_dz, Q->mz[3] + mzk, Q->pz.a[k], Q->pz.b[k], Q->pz.K[k]);
       vxn = cb * (dsxx_dx + dsxy_dy + dsxz_dz) + vxn;
     }
     // vy: 3D:995-1017
@@ -300,9 +327,9 @@ __global__ __launch_bounds__(BX *TY3) void k_velocity3d(Step3D<T> p) {
       T dsxy_dx = (L.sxy_xp - L.sxy_k) * p.odx;
       T dsyy_dy = (L.syy_yp - L.syy_k) * p.ody;
       T dsyz_dz = (L.syz_k - syz_m) * p.odz;
-      if (inx) dsxy_dx = cpml(dsxy_dx, p.mx[4] + mxo + k * mxs, p.px.ah[i], p.px.bh[i], p.px.Kh[i]);
-      if (iny) dsyy_dy = cpml(dsyy_dy, p.my[4] + myo + k * mys, p.py.ah[j], p.py.bh[j], p.py.Kh[j]);
-      if (inz) dsyz_dz = cpml(dsyz_dz, p.mz[4] + mzk, p.pz.a[k], p.pz.b[k], p.pz.K[k]);
+      if (inx) dsxy_dx = cpml(dsxy_dx, Q->mx[4] + mxo + k * mxs, xah, xbh, xKh);
+      if (iny) dsyy_dy = cpml(dsyy_dy, Q->my[4] + myo + k * mys, yah, ybh, yKh);
+      if (inz) dsyz_dz = cpml(dsyz_dz, Q->mz[4] + mzk, Q->pz.a[k], Q->pz.b[k], Q->pz.K[k]);
       vyn = cb * (dsxy_dx + dsyy_dy + dsyz_dz) + vyn;
     }
     // vz: 3D:1031-1052 (global k <= NZ-1)
@@ -310,17 +337,17 @@ __global__ __launch_bounds__(BX *TY3) void k_velocity3d(Step3D<T> p) {
       T dsxz_dx = (L.sxz_xp - L.sxz_k) * p.odx;
       T dsyz_dy = (L.syz_k - L.syz_ym) * p.ody;
       T dszz_dz = (L.szz_p - szz_k) * p.odz;
-      if (inx) dsxz_dx = cpml(dsxz_dx, p.mx[5] + mxo + k * mxs, p.px.ah[i], p.px.bh[i], p.px.Kh[i]);
-      if (iny) dsyz_dy = cpml(dsyz_dy, p.my[5] + myo + k * mys, p.py.a[j], p.py.b[j], p.py.K[j]);
-      if (inz) dszz_dz = cpml(dszz_dz, p.mz[5] + mzk, p.pz.ah[k], p.pz.bh[k], p.pz.Kh[k]);
+      if (inx) dsxz_dx = cpml(dsxz_dx, Q->mx[5] + mxo + k * mxs, xah, xbh, xKh);
+      if (iny) dsyz_dy = cpml(dsyz_dy, Q->my[5] + myo + k * mys, ya, yb, yK);
+      if (inz) dszz_dz = cpml(dszz_dz, Q->mz[5] + mzk, Q->pz.ah[k], Q->pz.bh[k], Q->pz.Kh[k]);
       vzn = cb * (dsxz_dx + dsyz_dy + dszz_dz) + vzn;
     }
     // point force on the cut plane, no density average (3D:1074-1082)
     if (src_col && k == p.ksrc) {
-      const int it = *p.step;
-      const T rs = p.rho[i + (long)j * p.ld + (long)k * p.ps];
-      vxn = vxn + p.fx[it - 1] * p.dt / rs;
-      vyn = vyn + p.fy[it - 1] * p.dt / rs;
+      const int it = *Q->step;
+      const T rs = Q->rho[i + (long)j * p.ld + (long)k * p.ps];
+      vxn = vxn + Q->fx[it - 1] * p.dt / rs;
+      vyn = vyn + Q->fy[it - 1] * p.dt / rs;
     }
     // six rigid faces (3D:1087-1121)
     if (edge_xy || k == 0 || k == nz - 1) {

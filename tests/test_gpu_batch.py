@@ -104,7 +104,7 @@ def test_batch_matches_oracle_bit_exact(order, prec):
 
 
 @pytest.mark.parametrize("tile_y,bpc,graph", [(8, 0, 16), (16, 0, 0), (4, 1, 7), (8, 4, 5),
-                                              (16, 8, 32)])
+                                              (16, 8, 32), (32, 0, 16), (64, 2, 3)])
 def test_batch_launch_shapes_identical(tile_y, bpc, graph):
     order, prec = 4, "f32"
     with make_batch(order, prec, shots_for(order, prec)) as b:
@@ -128,7 +128,7 @@ def test_batch_autotune_resets_state_and_caches(tmp_path):
     shots = shots_for(order, prec)
     with make_batch(order, prec, shots) as b:
         ty, bpc, ms = b.autotune(trial_steps=12, cache_path=cache)
-        assert ty in (4, 8, 16) and 0 <= bpc <= 64 and ms > 0
+        assert ty in (4, 8, 16, 32, 64) and 0 <= bpc <= 64 and ms > 0
         b.run()
         check_against_oracle(b, order, prec)
     assert cache.read_text().strip()
