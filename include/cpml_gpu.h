@@ -134,8 +134,9 @@ int cpml_gpu_velocity_norm(cpml_solver *h, double *vmax);
 /* Steps captured per HIP graph replay (default 16; 0 = launch eagerly). */
 int cpml_gpu_set_graph_steps(cpml_solver *h, int steps);
 /* Tiling: planes per block along z (3D, default 8) and rows of 64 points per
- * block (4, 8 or 16; default 4; the fused 3D step uses 12 rows for 16, its
- * block-size limit).  Results do not depend on either. */
+ * tile (4, 8 or 16; default 4; 2D also 32 or 64, walked by 16-row blocks; the
+ * fused 3D step uses 12 rows for 16, its block-size limit).  Results do not
+ * depend on either. */
 int cpml_gpu_set_kchunk(cpml_solver *h, int kchunk);
 int cpml_gpu_set_tile(cpml_solver *h, int tile_y);
 /* 3D only: run each time step as ONE fused kernel (stress and velocity in a
@@ -248,12 +249,13 @@ int cpml_abc_train_init(cpml_abc_net *net, double lr, double beta1, double beta2
  * at 0-based cell (i0, j0, k0).  Loads the teacher's current state into the
  * student, steps the student once (P), steps the teacher once (step `it`,
  * T) and adds the gradient of this sample to the mini-batch; the batch's
- * Adam step minimises, per face, sum (y - target)^2 / sum target^2 over the
- * owned strip cells of its samples (target = (T - P) / scales).  loss6 (may
- * be NULL): this sample's per-face relative error before the update. */
+ * Adam step minimises, per face, sum (y - target)^2 over the owned strip
+ * cells of its samples (target = (T - P) / scales) divided by the batch's
+ * target energy, floored at 1% of its input energy.  loss6 (may be NULL):
+ * this sample's per-face normalised error before the update. */
 int cpml_abc_train_step(cpml_abc_net *net, cpml_solver *student, cpml_solver *teacher, int i0,
                         int j0, int k0, int it, double *loss6);
-/* Per-face relative error and its gradient (float[CPML_ABC_WEIGHTS]) for the
+/* Per-face normalised error and its gradient (float[CPML_ABC_WEIGHTS]) for the
  * current student state (taken as P) against the teacher's current state
  * (taken as T); no step, no update (a partial mini-batch is dropped).  Needs
  * cpml_abc_train_init. */

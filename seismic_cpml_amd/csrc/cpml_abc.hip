@@ -369,7 +369,8 @@ static_assert(H2N + 1 <= XN + 1, "h2 reuses the input buffer");
 // then d) of the tile, centred like the input region IA = OA + 2; input cells
 // at depth -1 read the zero cell `izero`.  Lane l of a wave supplies row
 // m = l%16 (an output cell) and channels 4*(l/16)..+3 of A, and holds
-// D[m = 4*(l/16) + r][n = l%16].
+// D[m = 4*(l/16) + r][n = l%16].  Each wave keeps GPW groups of 16 output
+// cells in flight, so GPW independent MFMA chains hide the MFMA latency.
 template <int OA, int OD, bool LAST>
 __device__ __forceinline__ void mfma_layer(const unsigned short *in, int izero,
                                            unsigned short *out, const s4 *__restrict__ wf,
@@ -377,35 +378,52 @@ __device__ __forceinline__ void mfma_layer(const unsigned short *in, int izero,
   constexpr int IA = OA + 2;
   constexpr int ON = OA * OA * OD, NG = (ON + 15) / 16;
   constexpr int HALO = (OA - TT) / 2;
+  constexpr int NW = TB / 64, GPW = 4;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   s4 bw[TAPS];
 #pragma unroll
   for (int t = 0; t < TAPS; ++t) bw[t] = wf[t * 64 + lane];
   const int kq = (lane >> 4) * 4;
-  for (int gi = wave; gi < NG; gi += TB / 64) {
-    const int oc = min(gi * 16 + (lane & 15), ON - 1);
-    const int al = oc % OA, bl = (oc / OA) % OA, d = oc / (OA * OA);
-    f4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (int g0 = wave * GPW; g0 < NG; g0 += NW * GPW) {
+    int base[GPW];
+    bool top[GPW];
+    f4 acc[GPW];
+#pragma unroll
+    for (int u = 0; u < GPW; ++u) {
+      const int oc = min((g0 + u) * 16 + (lane & 15), ON - 1);
+      const int al = oc % OA, bl = (oc / OA) % OA, d = oc / (OA * OA);
+      base[u] = (d * IA + bl + 1) * IA + al + 1;
+      top[u] = d == 0;
+      acc[u] = f4{0.f, 0.f, 0.f, 0.f};
+    }
 #pragma unroll
     for (int t = 0; t < TAPS; ++t) {
-      const int d2 = d + t / 9 - 1;
-      const int ic = d2 < 0 ? izero : (d2 * IA + bl + 1 + t % 3 - 1) * IA + al + 1 + (t / 3) % 3 - 1;
-      const s4 av = *reinterpret_cast<const s4 *>(in + ic * C + kq);
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bw[t], acc, 0, 0, 0);
+      const int off = (t / 9 - 1) * IA * IA + (t % 3 - 1) * IA + ((t / 3) % 3 - 1);
+#pragma unroll
+      for (int u = 0; u < GPW; ++u) {
+        if (g0 + u >= NG) break;  // uniform over the wave
+        const int ic = (t < 9 && top[u]) ? izero : base[u] + off;
+        const s4 av = *reinterpret_cast<const s4 *>(in + ic * C + kq);
+        acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bw[t], acc[u], 0, 0, 0);
+      }
     }
     const int n = lane & 15;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int m = gi * 16 + 4 * (lane >> 4) + r;
-      if (m >= ON) continue;
-      const int ml = m % OA, mb = (m / OA) % OA, md = m / (OA * OA);
-      const int a = a0 - HALO + ml, b = b0 - HALO + mb;
-      const bool inside = a >= 0 && a < F.na && b >= 0 && b < F.nb;
-      if (!LAST) {
-        out[m * C + n] = inside ? to_bf16(acc[r]) : (unsigned short)0;
-      } else if (inside && n < NF) {
-        const long n4 = (long)S * F.na * F.nb;
-        corr[NF * F.off4 + n * n4 + ((long)md * F.nb + b) * F.na + a] = acc[r];
+    for (int u = 0; u < GPW; ++u) {
+      if (g0 + u >= NG) break;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = (g0 + u) * 16 + 4 * (lane >> 4) + r;
+        if (m >= ON) continue;
+        const int ml = m % OA, mb = (m / OA) % OA, md = m / (OA * OA);
+        const int a = a0 - HALO + ml, b = b0 - HALO + mb;
+        const bool inside = a >= 0 && a < F.na && b >= 0 && b < F.nb;
+        if (!LAST) {
+          out[m * C + n] = inside ? to_bf16(acc[u][r]) : (unsigned short)0;
+        } else if (inside && n < NF) {
+          const long n4 = (long)S * F.na * F.nb;
+          corr[NF * F.off4 + n * n4 + ((long)md * F.nb + b) * F.na + a] = acc[u][r];
+        }
       }
     }
   }
@@ -425,8 +443,13 @@ __global__ __launch_bounds__(TB) void k_infer(Geo g, Fields u, Scales inv,
   const int a0 = (tile % F.nta) * TT, b0 = (tile / F.nta) * TT;
 
   // input box: normalised fields in BF16, channels 9..15 zero
+  // x faces: depth runs along x, the contiguous direction, so the lanes walk
+  // the box depth first; y and z faces walk a (= x) first
+  const bool xface = F.ax == 0;
   for (int q = threadIdx.x; q < XN; q += TB) {
-    const int al = q % XA, bl = (q / XA) % XA, d = q / (XA * XA);
+    const int al = xface ? (q / D) % XA : q % XA;
+    const int bl = xface ? q / (D * XA) : (q / XA) % XA;
+    const int d = xface ? q % D : q / (XA * XA);
     const int a = a0 - 3 + al, b = b0 - 3 + bl;
     unsigned pk[C / 2] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (a >= 0 && a < F.na && b >= 0 && b < F.nb) {
@@ -439,7 +462,7 @@ __global__ __launch_bounds__(TB) void k_infer(Geo g, Fields u, Scales inv,
         pk[k >> 1] |= (k & 1) ? h << 16 : h;
       }
     }
-    uint4 *dst = reinterpret_cast<uint4 *>(bx + q * C);
+    uint4 *dst = reinterpret_cast<uint4 *>(bx + ((d * XA + bl) * XA + al) * C);
     dst[0] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
     dst[1] = make_uint4(pk[4], pk[5], pk[6], pk[7]);
   }
@@ -465,13 +488,18 @@ __global__ __launch_bounds__(TB) void k_apply(Geo g, Fields u, Scales sc, const 
   const int q = blockIdx.x * TB + threadIdx.x;
   const int n4 = S * F.na * F.nb;
   if (q >= n4) return;
-  const int a = q % F.na, b = (q / F.na) % F.nb, d = q / (F.na * F.nb);
+  // x faces walk the (contiguous) depth first
+  const bool xface = F.ax == 0;
+  const int a = xface ? (q / S) % F.na : q % F.na;
+  const int b = xface ? q / (S * F.na) : (q / F.na) % F.nb;
+  const int d = xface ? q % S : q / (F.na * F.nb);
+  const int cell = (d * F.nb + b) * F.na + a;
   int c[3];
   gcoord(g, F, d, a, b, c);
   if (owner(g, c) != f) return;
   const long o = c[0] + c[1] * g.ld + c[2] * g.ps;
 #pragma unroll
-  for (int k = 0; k < NF; ++k) u.f[k][o] += corr[NF * F.off4 + (long)k * n4 + q] * sc.s[k];
+  for (int k = 0; k < NF; ++k) u.f[k][o] += corr[NF * F.off4 + (long)k * n4 + cell] * sc.s[k];
 }
 
 // window copy: dst interior (i,j,k) <- src (i+i0, j+j0, k+k0), all 9 fields
@@ -513,7 +541,8 @@ struct cpml_abc_net {
         *G3 = nullptr, *T = nullptr, *M = nullptr, *part = nullptr, *corr = nullptr;
   double *red = nullptr;  // per face: box energy [0,6), squared error [6,12), target energy [12,18)
   int batch = 1, acc_count = 0;      // mini-batch: samples per Adam step, samples so far
-  double acc_tn[6] = {0, 0, 0, 0, 0, 0};
+  double acc_tn[6] = {0, 0, 0, 0, 0, 0};  // per face: target energy of the batch
+  double acc_xn[6] = {0, 0, 0, 0, 0, 0};  // per face: input (box) energy of the batch
   std::vector<void *> allocs;
   std::string err;
 };
@@ -604,10 +633,17 @@ hipError_t forward32(cpml_abc_net *n, const Geo &g, hipStream_t s) {
   return hipGetLastError();
 }
 
+// 1 / loss denominator: target energy, floored at 1% of the input energy;
+// 0 when both vanish (no wave in the box yet).
+double loss_scale(double target_energy, double input_energy) {
+  const double d = std::max(target_energy, 1e-2 * input_energy);
+  return d > 1e-30 ? 1.0 / d : 0.0;
+}
+
 // gather + forward + loss + backward of one sample into n->G (added to it when
 // accumulate); per-face squared error and owned target energy into err6, tn6
 int grad32(cpml_abc_net *n, const cpml::SolverView &st, const cpml::SolverView &te, int i0,
-           int j0, int k0, int accumulate, double *err6, double *tn6) {
+           int j0, int k0, int accumulate, double *err6, double *tn6, double *xn6 = nullptr) {
   const Geo g = make_geo(st);
   int rc = ensure_buffers(n, g);
   if (rc) return rc;
@@ -639,6 +675,7 @@ int grad32(cpml_abc_net *n, const cpml::SolverView &st, const cpml::SolverView &
   for (int f = 0; f < 6; ++f) {
     err6[f] = red[6 + f];
     tn6[f] = red[12 + f];
+    if (xn6) xn6[f] = red[f];
   }
   return CPML_OK;
 }
@@ -754,6 +791,7 @@ int cpml_abc_train_init(cpml_abc_net *n, double lr, double beta1, double beta2, 
   n->batch = batch;
   n->acc_count = 0;
   for (double &v : n->acc_tn) v = 0;
+  for (double &v : n->acc_xn) v = 0;
   n->lr = lr;
   n->b1 = beta1;
   n->b2 = beta2;
@@ -765,19 +803,20 @@ int cpml_abc_grad(cpml_abc_net *n, cpml_solver *student, cpml_solver *teacher, i
                   int k0, double *loss6, float *grad) {
   if (!n) return afail(n, CPML_ERR_ARG, "null net");
   cpml::SolverView st{}, te{};
-  double err[6], tn[6];
+  double err[6], tn[6], xn[6];
   int rc = views(n, student, &st, "student");
   if (!rc) rc = views(n, teacher, &te, "teacher");
-  if (!rc) rc = grad32(n, st, te, i0, j0, k0, 0, err, tn);
+  if (!rc) rc = grad32(n, st, te, i0, j0, k0, 0, err, tn, xn);
   if (rc) return rc;
   n->acc_count = 0;  // the gradient buffer no longer holds a partial mini-batch
   for (double &v : n->acc_tn) v = 0;
+  for (double &v : n->acc_xn) v = 0;
   if (loss6)
-    for (int f = 0; f < 6; ++f) loss6[f] = tn[f] > 0 ? err[f] / tn[f] : 0.0;
+    for (int f = 0; f < 6; ++f) loss6[f] = loss_scale(tn[f], xn[f]) * err[f];
   if (grad) {
     ACHK(n, hipMemcpy(grad, n->G, WN * sizeof(float), hipMemcpyDeviceToHost));
     for (int f = 0; f < 6; ++f) {
-      const float sc = tn[f] > 0 ? (float)(1.0 / tn[f]) : 0.f;
+      const float sc = (float)loss_scale(tn[f], xn[f]);
       for (int e = 0; e < WF; ++e) grad[f * WF + e] *= sc;
     }
   }
@@ -811,15 +850,20 @@ int cpml_abc_train_step(cpml_abc_net *n, cpml_solver *student, cpml_solver *teac
   // the runs may have switched state buffers: take fresh views
   if ((rc = views(n, student, &st, "student")) || (rc = views(n, teacher, &te, "teacher")))
     return rc;
-  double err[6], tn[6];
-  if ((rc = grad32(n, st, te, i0, j0, k0, n->acc_count > 0, err, tn))) return rc;
-  for (int f = 0; f < 6; ++f) n->acc_tn[f] += tn[f];
+  double err[6], tn[6], xn[6];
+  if ((rc = grad32(n, st, te, i0, j0, k0, n->acc_count > 0, err, tn, xn))) return rc;
+  for (int f = 0; f < 6; ++f) {
+    n->acc_tn[f] += tn[f];
+    n->acc_xn[f] += xn[f];
+  }
   if (loss6)
-    for (int f = 0; f < 6; ++f) loss6[f] = tn[f] > 0 ? err[f] / tn[f] : 0.0;
+    for (int f = 0; f < 6; ++f) loss6[f] = loss_scale(tn[f], xn[f]) * err[f];
   if (++n->acc_count < n->batch) return CPML_OK;
-  // Adam step on the mini-batch: gradient of sum(err) / sum(target energy), per face
+  // Adam step on the mini-batch: gradient of sum(err) / sum(target energy), per
+  // face, the denominator floored at 1% of the batch's input energy so that
+  // batches where the wave has not reached the strip yet do not dominate.
   Scales gs{};
-  for (int f = 0; f < 6; ++f) gs.s[f] = n->acc_tn[f] > 0 ? (float)(1.0 / n->acc_tn[f]) : 0.f;
+  for (int f = 0; f < 6; ++f) gs.s[f] = (float)loss_scale(n->acc_tn[f], n->acc_xn[f]);
   n->t += 1;
   const float bc1 = (float)(1.0 - std::pow(n->b1, n->t));
   const float bc2 = (float)(1.0 - std::pow(n->b2, n->t));
@@ -830,6 +874,7 @@ int cpml_abc_train_step(cpml_abc_net *n, cpml_solver *student, cpml_solver *teac
   n->wb_dirty = true;
   n->acc_count = 0;
   for (double &v : n->acc_tn) v = 0;
+  for (double &v : n->acc_xn) v = 0;
   ACHK(n, hipStreamSynchronize(st.stream));
   return CPML_OK;
 }

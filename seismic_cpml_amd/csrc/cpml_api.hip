@@ -1329,8 +1329,9 @@ int cpml_gpu_set_fused(cpml_solver *h, int on) {
 
 int cpml_gpu_set_tile(cpml_solver *h, int tile_y) {
   if (!h) return CPML_ERR_ARG;
-  if (tile_y != 4 && tile_y != 8 && tile_y != 16)
-    return fail(h, CPML_ERR_ARG, "tile rows must be 4, 8 or 16");
+  const bool ok3 = tile_y == 4 || tile_y == 8 || tile_y == 16;
+  if (!(ok3 || (h->ndim == 2 && (tile_y == 32 || tile_y == 64))))
+    return fail(h, CPML_ERR_ARG, "tile rows must be 4, 8 or 16 (2D also 32 or 64)");
   h->tile_y = tile_y;
   h->graph_dirty = true;
   return CPML_OK;

@@ -35,28 +35,34 @@ namespace {
 // gets one contiguous band of tiles (x fastest), so the rows and columns that
 // neighbouring tiles share are read through the same L2.  Sets this thread's
 // grid point (i, j).
-template <int TY>
+template <int TY, int RG>
 __device__ __forceinline__ void tile2d(int nx, int ny, int &i, int &j) {
-  const int ntx = (nx + BX - 1) / BX, nt = ntx * ((ny + TY - 1) / TY);
+  const int ntx = (nx + BX - 1) / BX, nt = ntx * ((ny + TY * RG - 1) / (TY * RG));
   const int h = blockIdx.x, xcd = h & 7, loc = h >> 3;
   const int q = nt >> 3, r = nt & 7;
   const int t = xcd * q + min(xcd, r) + loc;
   i = (t % ntx) * BX + threadIdx.x;
-  j = (t / ntx) * TY + threadIdx.y;
+  j = (t / ntx) * (TY * RG) + threadIdx.y;
 }
 
-template <typename T, int ORDER, int TY>
+// A tile is 64 points by TY*RG rows; the block (64 x TY threads) walks its RG
+// row groups, so each wavefront lives for several rows.
+template <typename T, int ORDER, int TY, int RG>
 __global__ __launch_bounds__(BX *TY) void k_stress2d(Step2D<T> p) {
   int i, j;
-  tile2d<TY>(p.nx, p.ny, i, j);
-  if (i < p.nx && j < p.ny) stress2d_point<T, ORDER>(p, i, j);
+  tile2d<TY, RG>(p.nx, p.ny, i, j);
+#pragma unroll
+  for (int r = 0; r < RG; ++r)
+    if (i < p.nx && j + r * TY < p.ny) stress2d_point<T, ORDER>(p, i, j + r * TY);
 }
 
-template <typename T, int ORDER, int TY>
+template <typename T, int ORDER, int TY, int RG>
 __global__ __launch_bounds__(BX *TY) void k_velocity2d(Step2D<T> p) {
   int i, j;
-  tile2d<TY>(p.nx, p.ny, i, j);
-  if (i < p.nx && j < p.ny) velocity2d_point<T, ORDER>(p, i, j);
+  tile2d<TY, RG>(p.nx, p.ny, i, j);
+#pragma unroll
+  for (int r = 0; r < RG; ++r)
+    if (i < p.nx && j + r * TY < p.ny) velocity2d_point<T, ORDER>(p, i, j + r * TY);
 }
 
 // ---------------------------------------------------------------- 3D ----
@@ -512,36 +518,45 @@ __global__ __launch_bounds__(EB) void k_vmax(const T *vx, const T *vy, const T *
 
 // ------------------------------------------------------------ launchers --
 
-template <typename T, int TY>
+template <typename T, int TY, int RG>
 static hipError_t stress2d_ty(int order, const Step2D<T> &p, hipStream_t s) {
-  dim3 grid(((p.nx + BX - 1) / BX) * ((p.ny + TY - 1) / TY)), block(BX, TY);
-  if (order == 2) hipLaunchKernelGGL((k_stress2d<T, 2, TY>), grid, block, 0, s, p);
-  else if (order == 4) hipLaunchKernelGGL((k_stress2d<T, 4, TY>), grid, block, 0, s, p);
-  else hipLaunchKernelGGL((k_stress2d<T, 8, TY>), grid, block, 0, s, p);
+  dim3 grid(((p.nx + BX - 1) / BX) * ((p.ny + TY * RG - 1) / (TY * RG))), block(BX, TY);
+  if (order == 2) hipLaunchKernelGGL((k_stress2d<T, 2, TY, RG>), grid, block, 0, s, p);
+  else if (order == 4) hipLaunchKernelGGL((k_stress2d<T, 4, TY, RG>), grid, block, 0, s, p);
+  else hipLaunchKernelGGL((k_stress2d<T, 8, TY, RG>), grid, block, 0, s, p);
   return hipGetLastError();
 }
 
-template <typename T, int TY>
+template <typename T, int TY, int RG>
 static hipError_t velocity2d_ty(int order, const Step2D<T> &p, hipStream_t s) {
-  dim3 grid(((p.nx + BX - 1) / BX) * ((p.ny + TY - 1) / TY)), block(BX, TY);
-  if (order == 2) hipLaunchKernelGGL((k_velocity2d<T, 2, TY>), grid, block, 0, s, p);
-  else if (order == 4) hipLaunchKernelGGL((k_velocity2d<T, 4, TY>), grid, block, 0, s, p);
-  else hipLaunchKernelGGL((k_velocity2d<T, 8, TY>), grid, block, 0, s, p);
+  dim3 grid(((p.nx + BX - 1) / BX) * ((p.ny + TY * RG - 1) / (TY * RG))), block(BX, TY);
+  if (order == 2) hipLaunchKernelGGL((k_velocity2d<T, 2, TY, RG>), grid, block, 0, s, p);
+  else if (order == 4) hipLaunchKernelGGL((k_velocity2d<T, 4, TY, RG>), grid, block, 0, s, p);
+  else hipLaunchKernelGGL((k_velocity2d<T, 8, TY, RG>), grid, block, 0, s, p);
   return hipGetLastError();
 }
 
+// p.tile_y: rows per tile, 4, 8, 16, or 32 / 64 walked by 16-row blocks
 template <typename T>
 hipError_t launch_stress2d(int order, const Step2D<T> &p, hipStream_t s) {
-  if (p.tile_y == 16) return stress2d_ty<T, 16>(order, p, s);
-  if (p.tile_y == 8) return stress2d_ty<T, 8>(order, p, s);
-  return stress2d_ty<T, 4>(order, p, s);
+  switch (p.tile_y) {
+    case 64: return stress2d_ty<T, 16, 4>(order, p, s);
+    case 32: return stress2d_ty<T, 16, 2>(order, p, s);
+    case 16: return stress2d_ty<T, 16, 1>(order, p, s);
+    case 8: return stress2d_ty<T, 8, 1>(order, p, s);
+    default: return stress2d_ty<T, 4, 1>(order, p, s);
+  }
 }
 
 template <typename T>
 hipError_t launch_velocity2d(int order, const Step2D<T> &p, hipStream_t s) {
-  if (p.tile_y == 16) return velocity2d_ty<T, 16>(order, p, s);
-  if (p.tile_y == 8) return velocity2d_ty<T, 8>(order, p, s);
-  return velocity2d_ty<T, 4>(order, p, s);
+  switch (p.tile_y) {
+    case 64: return velocity2d_ty<T, 16, 4>(order, p, s);
+    case 32: return velocity2d_ty<T, 16, 2>(order, p, s);
+    case 16: return velocity2d_ty<T, 16, 1>(order, p, s);
+    case 8: return velocity2d_ty<T, 8, 1>(order, p, s);
+    default: return velocity2d_ty<T, 4, 1>(order, p, s);
+  }
 }
 
 template <typename T>

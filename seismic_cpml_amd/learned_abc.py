@@ -148,9 +148,12 @@ def train(net: S.CpmlAbcNet, n: int = 128, runs: int = 16, nstep: int = 600,
         teacher.close()
         student.close()
         del keep_t, keep_s
-        history.append(float(np.mean(losses)))
+        # relative error of the samples that carry a wave (the precursor
+        # steps report 0), second half of the run
+        tail = [v for v in losses[len(losses) // 2:] if v > 0]
+        history.append(float(np.median(tail)) if tail else 0.0)
         if log:
-            log(f"run {r}: f0 {m.f0:.1f} Hz, src {m.src}, mean loss {history[-1]:.4e}")
+            log(f"run {r}: f0 {m.f0:.1f} Hz, src {m.src}, median loss {history[-1]:.4e}")
     return history
 
 

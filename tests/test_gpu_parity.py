@@ -207,3 +207,17 @@ def test_determinism_repeat_runs():
     b = run_gpu(case, prec, d=d)
     for k in ("vx", "vz", "syz", "ekin", "epot"):
         np.testing.assert_array_equal(a[k], b[k])
+
+
+@pytest.mark.parametrize("tile", [4, 8, 16, 32, 64])
+@pytest.mark.parametrize("case", ["2d4_layered", "2d2_layered"])
+def test_2d_tiling_does_not_change_results(case, tile):
+    """Rows per tile (one or several row groups per block) leave every field
+    bit-identical to the oracle."""
+    prec = "f32"
+    d = inputs(case, prec, 200)
+    g = run_gpu(case, prec, d=d, energy=False, tile=tile)
+    o = run_oracle(case, prec, d=d, nthreads=8, energy=False)
+    for k in _fields(case):
+        np.testing.assert_array_equal(g[k], o[k], err_msg=k)
+    np.testing.assert_array_equal(g["sisvx"], o["sisvx"])
