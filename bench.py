@@ -351,7 +351,8 @@ def bench_abc(args):
         trained = f"weights from {Path(args.abc_weights).name}"
     else:
         net = L.S.CpmlAbcNet(L.field_scales(), L.init_weights(args.abc_seed, std=0.02))
-        net.train_init(lr=2e-3, batch=8)
+        net.train_init(lr=2e-4, batch=8)
+        net.train_noise(1.0)
         t = time.perf_counter()
         L.train(net, n=128, runs=args.abc_runs, nstep=600, seed=args.abc_seed)
         trained = f"trained here: {args.abc_runs} teacher runs of 128^3, {time.perf_counter() - t:.0f} s"

@@ -214,11 +214,12 @@ int cpml_batch_last_error(cpml_batch *b, char *buf, int len);
  * replaces it by a 4-cell strip on every face: the grid is the interior plus
  * the strip, updated without C-PML (profiles with a = 0, K = 1), and after
  * each step (after the source and the rigid edges, before the receivers) a
- * per-face corrector adds a correction to the strip cells the face owns
+ * per-face corrector adds a correction to the outermost cells the face owns
  * (the nearest face; ties to the lowest of x-, x+, y-, y+, z-, z+).
  *
  * Corrector of one face: three linear 3x3x3 convolutions 9 -> 16 -> 16 -> 9
- * channels (no bias, no activation) over a 7-cell-deep box along the face,
+ * channels (no bias, no activation) over a 4-cell-deep box along the face,
+ * evaluated at depth 0,
  * inputs = the 9 fields divided by scales[9]; its output times the scales is
  * added to the fields.  Weights: float[6 faces][3 layers][27 taps][16 in]
  * [16 out], tap = (dd+1)*9 + (da+1)*3 + (db+1) for offsets along the inward
@@ -255,6 +256,10 @@ int cpml_abc_train_init(cpml_abc_net *net, double lr, double beta1, double beta2
  * this sample's per-face normalised error before the update. */
 int cpml_abc_train_step(cpml_abc_net *net, cpml_solver *student, cpml_solver *teacher, int i0,
                         int j0, int k0, int it, double *loss6);
+/* Input noise of train_step: each sample's box gets rel_noise times its RMS
+ * of white noise (target unchanged), penalising the corrector's response to
+ * patterns the data never shows (default 0). */
+int cpml_abc_train_noise(cpml_abc_net *net, double rel_noise);
 /* Per-face normalised error and its gradient (float[CPML_ABC_WEIGHTS]) for the
  * current student state (taken as P) against the teacher's current state
  * (taken as T); no step, no update (a partial mini-batch is dropped).  Needs
@@ -262,7 +267,8 @@ int cpml_abc_train_step(cpml_abc_net *net, cpml_solver *student, cpml_solver *te
 int cpml_abc_grad(cpml_abc_net *net, cpml_solver *student, cpml_solver *teacher, int i0, int j0,
                   int k0, double *loss6, float *grad);
 /* Corrector output for the current state of h, before ownership and scaling:
- * float[face][field 9][depth 4][b][a] (each face's block 9*4*na*nb),
+ * float[face][field 9][depth 4][b][a] (each face's block 9*4*na*nb; depths
+ * 1..3 are 0),
  * computed by the FP32 training path (use_mfma = 0) or the BF16 MFMA
  * inference kernel (use_mfma = 1). */
 int cpml_abc_correction(cpml_abc_net *net, cpml_solver *h, int use_mfma, float *out);

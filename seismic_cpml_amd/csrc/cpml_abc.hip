@@ -5,24 +5,30 @@
 // corrector adds a correction to the strip:
 //
 //   x  = the 9 fields (vx vy vz sxx syy szz sxy sxz syz), each divided by a
-//        fixed scale, in a 7-cell-deep box along the face (strip + 3 cells);
+//        fixed scale, in a 4-cell-deep box along the face;
 //   h1 = W1 * x,  h2 = W2 * h1,  y = W3 * h2     (3x3x3 convolutions, 16
 //        channels, zero padding outside the box and outside the grid);
-//   u += y * scale on the strip cells the face owns (each strip cell belongs
-//        to its nearest face, ties to the lowest face index x-,x+,y-,y+,z-,z+).
+//   u += y * scale on the outermost cells the face owns (each boundary cell
+//        belongs to its nearest face, ties to the lowest face index
+//        x-,x+,y-,y+,z-,z+).
+// The inner strip cells are updated by the plain step alone: their stencils
+// stay inside the grid, and what the wide C-PML does differently there (weak
+// damping) is four orders of magnitude below the correction the outermost
+// cell needs, whose stencil reaches past the grid edge.
 //
 // The corrector is linear (no activation, no bias): the elastic system is
 // linear, so an absorbing boundary that does not depend on the wave amplitude
 // or its sign is a linear map of the local state.  Only the outputs needed at
-// strip depth d < 4 are computed: h1 at d < 6, h2 at d < 5, y at d < 4.
+// depth 0 are computed: h1 at d < 3, h2 at d < 2, y at d = 0.
 //
 // Training (FP32 kernels below) regresses the correction on pairs produced by
 // the solver itself: a wide-C-PML "teacher" run gives the state S_n; the
 // student grid (teacher cells 16 inward, i.e. the inner 4 cells of a 20-cell
 // C-PML become the strip) is loaded with the window of S_n and stepped once
-// without C-PML, giving P(S_n); the target is T_{n+1} - P(S_n) on the strip,
-// with T_{n+1} the teacher's next state.  The loss of a face is
-// sum over owned strip cells (y - target)^2 / (sum over the box of x^2).
+// without C-PML, giving P(S_n); the target is T_{n+1} - P(S_n) on the owned
+// outermost cells, with T_{n+1} the teacher's next state.  The loss of a face
+// over a mini-batch is sum (y - target)^2 / max(sum target^2, 1% of the
+// input energy).
 // Inference runs the same network with BF16 weights and activations and FP32
 // accumulation on the MFMA units (v_mfma_f32_16x16x16_bf16).
 #include <hip/hip_runtime.h>
@@ -44,14 +50,15 @@ namespace cpml {
 namespace abc {
 
 constexpr int S = 4;            // strip depth (cells)
-constexpr int D = 7;            // box depth: strip + receptive field of 3 layers
+constexpr int SO = 1;           // depths the corrector writes (the outermost cell)
+constexpr int D = SO + 3;       // box depth: corrected cells + receptive field of 3 layers
 constexpr int C = 16;           // channel stride (9 fields, 16 hidden)
 constexpr int NF = 9;           // fields
 constexpr int TAPS = 27;
 constexpr int WL = TAPS * C * C;  // weights of one layer of one face: [27][16 in][16 out]
 constexpr int WF = 3 * WL;        // one face
 constexpr int WN = 6 * WF;        // all faces
-constexpr int DMAX1 = 6, DMAX2 = 5, DMAX3 = 4;
+constexpr int DMAX1 = SO + 2, DMAX2 = SO + 1, DMAX3 = SO;
 constexpr int NCHUNK = 64;        // cell chunks of the weight-gradient reduction
 constexpr int TB = 256;
 
@@ -177,7 +184,7 @@ __global__ __launch_bounds__(TB) void k_gather(Geo g, Fields s, Fields t, long t
     if (d < S) {
       const long to = c[0] + c[1] * tld + c[2] * tps + toff;
       float *tt = T + (F.off4 + q) * C;
-      const bool own = owner(g, c) == f;
+      const bool own = owner(g, c) == f && d < SO;
 #pragma unroll
       for (int k = 0; k < NF; ++k) {
         tt[k] = (t.f[k][to] - u[k]) * inv.s[k];
@@ -321,6 +328,38 @@ __global__ __launch_bounds__(TB) void k_loss(Geo g, const float *Y, const float 
   if (threadIdx.x == 0) atomicAdd(err + f, bs);
 }
 
+// Input noise for training: x += nu * rms(box of the face) * N(0, 1) on the 9
+// field channels (hash-based normal deviates, Irwin-Hall of four uniforms).
+// With the target unchanged this adds nu^2 * rms^2 * |W|^2 to the expected
+// loss, so the corrector learns not to amplify what the data never shows.
+__device__ __forceinline__ float hash_unit(unsigned x) {
+  x ^= x >> 16;
+  x *= 0x7feb352dU;
+  x ^= x >> 15;
+  x *= 0x846ca68bU;
+  x ^= x >> 16;
+  return (x >> 8) * (1.0f / 16777216.0f);
+}
+
+__global__ __launch_bounds__(TB) void k_noise(Geo g, float *X, const double *norm, float nu,
+                                              unsigned seed) {
+  const int f = blockIdx.y;
+  const Face &F = g.f[f];
+  const int q = blockIdx.x * TB + threadIdx.x;
+  const int n7 = D * F.na * F.nb;
+  if (q >= n7) return;
+  const float rms = (float)sqrt(norm[f] / (9.0 * n7));
+  float *x = X + (F.off7 + q) * C;
+  const unsigned base = (seed * 6u + f) * 0x9e3779b9U + (unsigned)q * 9u;
+#pragma unroll
+  for (int k = 0; k < NF; ++k) {
+    const unsigned h = (base + k) * 4u;
+    const float z = (hash_unit(h) + hash_unit(h + 1) + hash_unit(h + 2) + hash_unit(h + 3) - 2.f) *
+                    1.7320508f;  // unit variance
+    x[k] += nu * rms * z;
+  }
+}
+
 __global__ void k_adam(float *W, const float *G, Scales gscale, float *m, float *v, int n,
                        float lr, float b1, float b2, float eps, float bc1, float bc2) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -360,9 +399,9 @@ typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int TT = 8;                          // tile: 8 x 8 columns of the face
-constexpr int XA = TT + 6, XN = XA * XA * D;   // input box 14 x 14 x 7
-constexpr int H1A = TT + 4, H1N = H1A * H1A * 6;
-constexpr int H2A = TT + 2, H2N = H2A * H2A * 5;
+constexpr int XA = TT + 6, XN = XA * XA * D;   // input box 14 x 14 x D
+constexpr int H1A = TT + 4, H1N = H1A * H1A * (D - 1);
+constexpr int H2A = TT + 2, H2N = H2A * H2A * (D - 2);
 static_assert(H2N + 1 <= XN + 1, "h2 reuses the input buffer");
 
 // One layer on the MFMA units.  Output region OA x OA x OD (a fastest, then b,
@@ -472,13 +511,13 @@ __global__ __launch_bounds__(TB) void k_infer(Geo g, Fields u, Scales inv,
   }
   __syncthreads();
   const s4 *wf = reinterpret_cast<const s4 *>(Wb) + (long)f * 3 * TAPS * 64;
-  mfma_layer<H1A, 6, false>(bx, XN, bh1, wf, F, a0, b0, nullptr);
+  mfma_layer<H1A, D - 1, false>(bx, XN, bh1, wf, F, a0, b0, nullptr);
   __syncthreads();
   unsigned short *bh2 = bx;  // the input box is dead now
   if (threadIdx.x < C) bh2[H2N * C + threadIdx.x] = 0;
-  mfma_layer<H2A, 5, false>(bh1, H1N, bh2, wf + TAPS * 64, F, a0, b0, nullptr);
+  mfma_layer<H2A, D - 2, false>(bh1, H1N, bh2, wf + TAPS * 64, F, a0, b0, nullptr);
   __syncthreads();
-  mfma_layer<TT, 4, true>(bh2, H2N, nullptr, wf + 2 * TAPS * 64, F, a0, b0, corr);
+  mfma_layer<TT, SO, true>(bh2, H2N, nullptr, wf + 2 * TAPS * 64, F, a0, b0, corr);
 }
 
 // u += correction * scale on the strip cells each face owns
@@ -487,12 +526,12 @@ __global__ __launch_bounds__(TB) void k_apply(Geo g, Fields u, Scales sc, const 
   const Face &F = g.f[f];
   const int q = blockIdx.x * TB + threadIdx.x;
   const int n4 = S * F.na * F.nb;
-  if (q >= n4) return;
+  if (q >= SO * F.na * F.nb) return;
   // x faces walk the (contiguous) depth first
   const bool xface = F.ax == 0;
-  const int a = xface ? (q / S) % F.na : q % F.na;
-  const int b = xface ? q / (S * F.na) : (q / F.na) % F.nb;
-  const int d = xface ? q % S : q / (F.na * F.nb);
+  const int a = xface ? (q / SO) % F.na : q % F.na;
+  const int b = xface ? q / (SO * F.na) : (q / F.na) % F.nb;
+  const int d = xface ? q % SO : q / (F.na * F.nb);
   const int cell = (d * F.nb + b) * F.na + a;
   int c[3];
   gcoord(g, F, d, a, b, c);
@@ -535,6 +574,8 @@ struct cpml_abc_net {
   float *G = nullptr, *m = nullptr, *v = nullptr;
   int t = 0;
   double lr = 1e-3, b1 = 0.9, b2 = 0.999, eps = 1e-8;
+  float noise = 0.f;  // relative input noise during training
+  unsigned samples = 0;
   // work buffers, sized for the largest geometry seen
   long cap7 = 0, cap4 = 0;
   float *X = nullptr, *H1 = nullptr, *H2 = nullptr, *Y = nullptr, *G1 = nullptr, *G2 = nullptr,
@@ -643,7 +684,8 @@ double loss_scale(double target_energy, double input_energy) {
 // gather + forward + loss + backward of one sample into n->G (added to it when
 // accumulate); per-face squared error and owned target energy into err6, tn6
 int grad32(cpml_abc_net *n, const cpml::SolverView &st, const cpml::SolverView &te, int i0,
-           int j0, int k0, int accumulate, double *err6, double *tn6, double *xn6 = nullptr) {
+           int j0, int k0, int accumulate, double *err6, double *tn6, double *xn6 = nullptr,
+           float noise = 0.f) {
   const Geo g = make_geo(st);
   int rc = ensure_buffers(n, g);
   if (rc) return rc;
@@ -657,6 +699,8 @@ int grad32(cpml_abc_net *n, const cpml::SolverView &st, const cpml::SolverView &
   const long toff = i0 + j0 * te.ld + k0 * te.ps;
   hipLaunchKernelGGL(k_gather, grid, dim3(TB), 0, s, g, fields_of(st), fields_of(te), te.ld,
                      te.ps, toff, n->inv, n->X, n->T, n->M, n->red, n->red + 12);
+  if (noise > 0)
+    hipLaunchKernelGGL(k_noise, grid, dim3(TB), 0, s, g, n->X, n->red, noise, n->samples++);
   ACHK(n, forward32(n, g, s));
   hipLaunchKernelGGL(k_loss, grid, dim3(TB), 0, s, g, n->Y, n->T, n->M, n->G3, n->red + 6);
   const dim3 wg(NCHUNK, TAPS, 6);
@@ -799,6 +843,13 @@ int cpml_abc_train_init(cpml_abc_net *n, double lr, double beta1, double beta2, 
   return CPML_OK;
 }
 
+int cpml_abc_train_noise(cpml_abc_net *n, double rel_noise) {
+  if (!n) return afail(n, CPML_ERR_ARG, "null net");
+  if (!(rel_noise >= 0 && rel_noise <= 10)) return afail(n, CPML_ERR_ARG, "noise out of range");
+  n->noise = (float)rel_noise;
+  return CPML_OK;
+}
+
 int cpml_abc_grad(cpml_abc_net *n, cpml_solver *student, cpml_solver *teacher, int i0, int j0,
                   int k0, double *loss6, float *grad) {
   if (!n) return afail(n, CPML_ERR_ARG, "null net");
@@ -851,7 +902,7 @@ int cpml_abc_train_step(cpml_abc_net *n, cpml_solver *student, cpml_solver *teac
   if ((rc = views(n, student, &st, "student")) || (rc = views(n, teacher, &te, "teacher")))
     return rc;
   double err[6], tn[6], xn[6];
-  if ((rc = grad32(n, st, te, i0, j0, k0, n->acc_count > 0, err, tn, xn))) return rc;
+  if ((rc = grad32(n, st, te, i0, j0, k0, n->acc_count > 0, err, tn, xn, n->noise))) return rc;
   for (int f = 0; f < 6; ++f) {
     n->acc_tn[f] += tn[f];
     n->acc_xn[f] += xn[f];

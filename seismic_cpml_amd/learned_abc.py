@@ -236,9 +236,9 @@ def time_to_solution(model: Model, nstep: int, net: S.CpmlAbcNet, repeats: int =
 
 def corrector_flops(n_grid: int) -> float:
     """Algorithmic FLOPs of one corrector pass on an n_grid^3 grid: per face
-    column, layer 1 at 6 depths (27*9*16 MACs each), layer 2 at 5 depths
-    (27*16*16), layer 3 at 4 depths (27*16*9); 2 FLOPs per MAC."""
-    macs = 6 * 27 * 9 * 16 + 5 * 27 * 16 * 16 + 4 * 27 * 16 * 9
+    column, layer 1 at 3 depths (27*9*16 MACs each), layer 2 at 2 depths
+    (27*16*16), layer 3 at depth 0 (27*16*9); 2 FLOPs per MAC."""
+    macs = 3 * 27 * 9 * 16 + 2 * 27 * 16 * 16 + 1 * 27 * 16 * 9
     return 2.0 * macs * 6 * n_grid * n_grid
 
 
@@ -262,20 +262,23 @@ def main(argv=None):
 
     ap = argparse.ArgumentParser(description="train the learned thin absorbing boundary")
     ap.add_argument("--n", type=int, default=128, help="teacher interior cells per axis")
-    ap.add_argument("--runs", type=int, default=16)
+    ap.add_argument("--runs", type=int, default=96)
     ap.add_argument("--nstep", type=int, default=600)
-    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--lr", type=float, default=2e-4)
     ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--noise", type=float, default=1.0,
+                    help="relative input noise (stabilises long closed-loop runs)")
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--out", default=str(WEIGHTS_FILE))
     args = ap.parse_args(argv)
     net = S.CpmlAbcNet(field_scales(), init_weights(args.seed, std=0.02))
     net.train_init(lr=args.lr, batch=args.batch)
+    net.train_noise(args.noise)
     t = time.perf_counter()
     hist = train(net, n=args.n, runs=args.runs, nstep=args.nstep, seed=args.seed,
                  log=lambda m: print(m, flush=True))
     save_weights(net, args.out, n=args.n, runs=args.runs, nstep=args.nstep, lr=args.lr,
-                 seed=args.seed, loss=hist, seconds=time.perf_counter() - t)
+                 noise=args.noise, seed=args.seed, loss=hist, seconds=time.perf_counter() - t)
     print(f"saved {args.out} after {time.perf_counter() - t:.1f} s", flush=True)
 
 

@@ -85,6 +85,7 @@ SIGNATURES = {
     "cpml_gpu_set_boundary": [_VP, _I, _VP],
     "cpml_abc_train_init": [_VP, _D, _D, _D, _D, _I],
     "cpml_abc_train_step": [_VP, _VP, _VP, _I, _I, _I, _I, _VP],
+    "cpml_abc_train_noise": [_VP, _D],
     "cpml_abc_grad": [_VP, _VP, _VP, _I, _I, _I, _VP, _VP],
     "cpml_abc_correction": [_VP, _VP, _I, _VP],
     "cpml_abc_last_error": [_VP, C.c_char_p, _I],
@@ -424,6 +425,9 @@ class CpmlAbcNet:
     def train_init(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8, batch=8):
         self._call("cpml_abc_train_init", float(lr), float(beta1), float(beta2), float(eps),
                    int(batch))
+
+    def train_noise(self, rel_noise):
+        self._call("cpml_abc_train_noise", float(rel_noise))
 
     def train_step(self, student, teacher, offset, it):
         loss = np.zeros(6)
