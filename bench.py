@@ -393,7 +393,7 @@ def bench_abc(args):
             "frac": round(tflops / 2500.0, 4), "traffic": None}
     net.close()
     return {
-        "metric": "learned-ABC vs 20-cell C-PML time-to-solution (3D 384^3 FP32)",
+        "metric": f"learned-ABC vs 20-cell C-PML time-to-solution (3D {n}^3 FP32)",
         "value": round(tts["speedup"], 4), "unit": "x", "n_gpus": 1, "steps": nstep,
         "warmup": 16, "ms_per_step": round(tts["learned4"]["seconds"] / nstep * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
