@@ -135,14 +135,13 @@ int cpml_gpu_velocity_norm(cpml_solver *h, double *vmax);
 int cpml_gpu_set_graph_steps(cpml_solver *h, int steps);
 /* Tiling: planes per block along z (3D, default 8) and rows of 64 points per
  * tile (4, 8 or 16; default 4; 2D also 32 or 64, walked by 16-row blocks; the
- * fused 3D step uses 12 rows for 16, its block-size limit).  Results do not
- * depend on either. */
+ * fused 3D step owns 62-point rows, 6 of them per block for 4 or 8 and 14 for
+ * 16).  Results do not depend on either. */
 int cpml_gpu_set_kchunk(cpml_solver *h, int kchunk);
 int cpml_gpu_set_tile(cpml_solver *h, int tile_y);
 /* 3D only: run each time step as ONE fused kernel (stress and velocity in a
  * single pass over HBM, state kept in two ping-pong buffers: 9 more field
- * arrays).  Results are identical to the two-kernel path.  Tile rows 8 or 16
- * (set_tile; 4 selects 8). */
+ * arrays).  Results are identical to the two-kernel path. */
 int cpml_gpu_set_fused(cpml_solver *h, int on);
 /* When on, run() launches eagerly with a HIP event pair around every stress
  * and velocity kernel on the handle's stream; see cpml_gpu_kernel_time. */

@@ -123,6 +123,13 @@ struct Fused3D {
   Step3D<T> a;
   T *vx, *vy, *vz, *sxx, *syy, *szz, *sxy, *sxz, *syz;  // next state, interior origin
   T *mx[6], *my[6], *mz[6];                             // next memory variables
+  // Tiles whose points and ring touch no absorbing slab form a box in tile
+  // coordinates [box[0], box[1]) x [box[2], box[3]) x [box[4], box[5]); set by
+  // the launcher.
+  int box[6];
+  // z chunks: planes [0, z0), [z0, z1), [z1, nz) cut into zn[0], zn[1], zn[2]
+  // chunks of at most kchunk planes each; set by the launcher.
+  int z0, z1, zn[3];
 };
 
 // Receivers / step counter: one block.

@@ -15,7 +15,7 @@ HERE = Path(__file__).resolve().parent
 sys.path.insert(0, str(HERE / "golden"))
 
 import oracle_lib as orc  # noqa: E402
-from cases import CASES, values  # noqa: E402
+from cases import ALL_CASES, values  # noqa: E402
 
 GOLDEN = HERE / "golden"
 
@@ -27,7 +27,7 @@ def fixture(case: str, prec: str):
 def inputs(case: str, prec: str, nstep: int | None = None) -> dict:
     """All inputs of ``case`` in precision ``prec``, built the reference's way."""
     v = values(case, prec)
-    c = CASES[case]
+    c = ALL_CASES[case]
     dt = np.float64 if prec == "f64" else np.float32
     nx, ny, npml = c["NX"], c["NY"], c["NPML"]
     nz = c.get("NZ")
@@ -78,7 +78,7 @@ def run_oracle(case: str, prec: str, nstep: int | None = None, d: dict | None = 
     (lambda+2mu) coefficient, as the HIP kernels do, instead of the reference's
     DELTAT*(rho*cp*cp) (3D:145, 298)."""
     d = inputs(case, prec, nstep) if d is None else d
-    c = CASES[case]
+    c = ALL_CASES[case]
     if c.get("NZ") is None:
         return orc.run2d(c["NX"], c["NY"], c["order"], d["DELTAX"], d["DELTAY"], d["DELTAT"],
                          d["nstep"], d["lam"], d["mu"], d["rho"], d["prof_x"], d["prof_y"],

@@ -60,6 +60,15 @@ CASES = {
                      xfin="0", yfin="100.", model=False, nproc=2),
 }
 
+# Cases without reference fixtures, checked against the oracle only.
+# 3d_wide: 160 points along x (2.5 tiles of 64), so that some tiles touch no
+# absorbing slab; the source lies in one of them.
+OTHER_CASES = {
+    "3d_wide": dict(CASES["3d_small"], NX=160, NY=48, NZ=40, NSTEP=200, ISOURCE=91,
+                    JSOURCE=33),
+}
+ALL_CASES = {**CASES, **OTHER_CASES}
+
 VARIANT_OF = {
     "2d2_shipped": "ref2d2_shipped", "2d4_shipped": "ref2d4_shipped",
     "2d2_layered": "ref2d2_layered", "2d4_layered": "ref2d4_layered",
@@ -69,7 +78,7 @@ VARIANT_OF = {
 
 def values(name: str, prec: str) -> dict:
     """Evaluate the case's parameters in precision ``prec`` ('f64' or 'f32')."""
-    c = CASES[name]
+    c = ALL_CASES[name]
     L = lambda s: literal(s, prec)  # noqa: E731
     R = np.float64 if prec == "f64" else np.float32
     v = dict(c)
@@ -98,7 +107,7 @@ def values(name: str, prec: str) -> dict:
         v["ydeb"] = L("2300.")
         v["xfin"] = xsource
         v["yfin"] = L("300.")
-    elif name == "3d_small":  # xdeb = xsource - 100, xfin = xsource
+    elif name in ("3d_small", "3d_wide"):  # xdeb = xsource - 100, xfin = xsource
         v["xdeb"] = xsource - L("100.")
         v["ydeb"] = L(c["ydeb"])
         v["xfin"] = xsource

@@ -6,7 +6,7 @@ import numpy as np
 from seismic_cpml_amd import CpmlGpu
 
 from case_inputs import inputs
-from golden.cases import CASES
+from golden.cases import ALL_CASES
 
 F2D = ("vx", "vy", "sigmaxx", "sigmayy", "sigmaxy")
 F3D = ("vx", "vy", "vz", "sigmaxx", "sigmayy", "sigmazz", "sigmaxy", "sigmaxz", "sigmayz")
@@ -26,7 +26,7 @@ def oracle_mem_name(name: str) -> str:
 
 def make_solver(case: str, prec: str, d: dict, energy=True, graph_steps=16, kchunk=None,
                 tile=None, fused=False):
-    c = CASES[case]
+    c = ALL_CASES[case]
     p = 8 if prec == "f64" else 4
     ndim = 3 if c.get("NZ") else 2
     s = CpmlGpu(ndim, c["order"], p, c["NX"], c["NY"], c.get("NZ", 1), d["DELTAX"], d["DELTAY"],

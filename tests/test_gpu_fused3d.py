@@ -66,3 +66,14 @@ def test_fused_odd_grid_partial_tiles():
     b = run_gpu(case, prec, d=d, energy=False, kchunk=7, fused=True, tile=16)
     for k in FIELDS:
         np.testing.assert_array_equal(a[k], b[k])
+
+
+def test_fused_wide_grid_matches_two_phase():
+    """160 x 48 x 40: the two-phase path splits into interior-box and shell
+    launches here; the fused step covers the grid in one launch."""
+    case, prec = "3d_wide", "f32"
+    d = inputs(case, prec, 80)
+    a = run_gpu(case, prec, d=d, energy=False)
+    b = run_gpu(case, prec, d=d, energy=False, fused=True)
+    for k in FIELDS + ("sisvx", "sisvy"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)

@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 from case_inputs import fixture, inputs, run_oracle
-from golden.cases import CASES
+from golden.cases import ALL_CASES, CASES
 from gpu_runner import rel_l2, run_gpu
 
 pytestmark = pytest.mark.gpu
@@ -25,7 +25,7 @@ ALL = [(c, p) for c in CASES for p in ("f64", "f32")]
 
 
 def _fields(case):
-    if CASES[case].get("NZ"):
+    if ALL_CASES[case].get("NZ"):
         return ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
     return ("vx", "vy", "sxx", "syy", "sxy")
 
@@ -43,7 +43,7 @@ def test_gpu_matches_oracle_bit_exact(case, prec):
     np.testing.assert_array_equal(g["sisvx"], o["sisvx"])
     np.testing.assert_array_equal(g["sisvy"], o["sisvy"])
     tol = 1e-12 if prec == "f64" else 1e-4
-    if CASES[case].get("NZ"):
+    if ALL_CASES[case].get("NZ"):
         assert rel_l2(g["ekin"] + g["epot"], o["etotal"]) < tol
     else:
         assert rel_l2(g["ekin"], o["ekin"]) < tol
@@ -221,3 +221,35 @@ def test_2d_tiling_does_not_change_results(case, tile):
     for k in _fields(case):
         np.testing.assert_array_equal(g[k], o[k], err_msg=k)
     np.testing.assert_array_equal(g["sisvx"], o["sisvx"])
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_3d_interior_and_shell_tiles_match_oracle(prec):
+    """A grid where some tiles touch no absorbing slab: those run in the launch
+    without C-PML code, the rest in the shell launch.  Bit-exact against the
+    oracle for every field, memory variable and seismogram sample."""
+    case = "3d_wide"
+    d = inputs(case, prec, 200)
+    g = run_gpu(case, prec, d=d, memory=True)
+    o = run_oracle(case, prec, d=d, nthreads=8, derived_lp2m=True)
+    for k in _fields(case):
+        np.testing.assert_array_equal(g[k], o[k], err_msg=k)
+    mem = [k for k in g if k.startswith("m_")]
+    assert len(mem) == 18
+    for k in mem:
+        np.testing.assert_array_equal(g[k], o[k], err_msg=k)
+    for k in ("sisvx", "sisvy"):
+        np.testing.assert_array_equal(g[k], o[k], err_msg=k)
+    assert np.abs(g["sisvx"]).max() > 0
+    tol = 1e-12 if prec == "f64" else 1e-4
+    assert rel_l2(g["ekin"] + g["epot"], o["etotal"]) < tol
+
+
+@pytest.mark.parametrize("kchunk,tile", [(1, 4), (3, 8), (16, 16), (8, 16), (40, 4)])
+def test_3d_interior_box_tilings_agree(kchunk, tile):
+    case, prec = "3d_wide", "f32"
+    d = inputs(case, prec, 120)
+    ref = run_gpu(case, prec, d=d, energy=False)
+    g = run_gpu(case, prec, d=d, energy=False, kchunk=kchunk, tile=tile)
+    for k in _fields(case) + ("sisvx", "sisvy"):
+        np.testing.assert_array_equal(g[k], ref[k], err_msg=k)
