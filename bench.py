@@ -159,6 +159,9 @@ def bench_3d(args):
     nstep = args.warmup + steps
     w = workloads.layered_3d(n=args.n or 768, nstep=nstep)
     s = w.make_solver()
+    if args.fused:  # the fused step's own defaults: 14-row blocks, 48-plane chunks
+        args.tile = args.tile or 16
+        args.kchunk = args.kchunk or 48
     if args.kchunk:
         s.set_kchunk(args.kchunk)
     if args.fused:

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -188,6 +189,20 @@ int dev_alloc(cpml_solver *h, void **p, size_t bytes) {
   h->allocs.push_back(*p);
   h->bytes += (double)bytes;
   HIPCHK(h, hipMemsetAsync(*p, 0, bytes, h->stream));
+  return CPML_OK;
+}
+
+// n arrays of `bytes` each carved from one allocation (the arrays of one
+// state or the coefficients: one contiguous range of device memory, which
+// measured steadier and faster for the 3D step than separate allocations).
+int dev_alloc_group(cpml_solver *h, void **p, int n, size_t bytes, const bool *want) {
+  void *base = nullptr;
+  int used = 0;
+  for (int q = 0; q < n; ++q) used += want ? want[q] : 1;
+  int rc = dev_alloc(h, &base, bytes * (used > 0 ? used : 1));
+  if (rc) return rc;
+  for (int q = 0, k = 0; q < n; ++q)
+    if (!want || want[q]) p[q] = static_cast<char *>(base) + bytes * k++;
   return CPML_OK;
 }
 
@@ -828,10 +843,10 @@ int cpml_gpu_init(cpml_solver **hp, int ndim, int order, int prec, int NX, int N
     return bail(CPML_ERR_HIP);
   }
   const size_t fbytes = (size_t)h->total * h->esize;
-  for (int q = 0; q < 9 && rc == CPML_OK; ++q)
-    if (field_valid(h, q)) rc = dev_alloc(h, &h->field[q], fbytes);
-  for (int q = 0; q < (ndim == 3 ? 6 : 3) && rc == CPML_OK; ++q)
-    rc = dev_alloc(h, &h->media[q], fbytes);
+  bool want[9];
+  for (int q = 0; q < 9; ++q) want[q] = field_valid(h, q);
+  rc = dev_alloc_group(h, h->field, 9, fbytes, want);
+  if (rc == CPML_OK) rc = dev_alloc_group(h, h->media, ndim == 3 ? 6 : 3, fbytes, nullptr);
   const long nmax = std::max(NX, std::max(NY, h->nz));
   if (rc == CPML_OK) rc = dev_alloc(h, &h->prof, (size_t)18 * nmax * h->esize);
   if (rc == CPML_OK) rc = dev_alloc(h, &h->fx, (size_t)NSTEP * h->esize);
@@ -1301,9 +1316,9 @@ int cpml_gpu_set_fused(cpml_solver *h, int on) {
     if (hipMemGetInfo(&free_b, &tot_b) == hipSuccess &&
         (double)h->total * h->esize * 9 > 0.95 * (double)free_b)
       return fail(h, CPML_ERR_OOM, "not enough device memory for the second state buffer");
-    for (int q = 0; q < 9; ++q)
-      if (!h->field2[q] && (rc = dev_alloc(h, &h->field2[q], (size_t)h->total * h->esize)))
-        return rc;
+    if (!h->field2[0] &&
+        (rc = dev_alloc_group(h, h->field2, 9, (size_t)h->total * h->esize, nullptr)))
+      return rc;
     for (int w = 0; w < 18 && h->have_pml; ++w)
       if (mem_used(h->ndim, w) && !h->mem2[w] &&
           (rc = dev_alloc(h, &h->mem2[w], h->mem_elems[w] * h->esize)))
