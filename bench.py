@@ -309,7 +309,8 @@ def bench_batch(args):
     }
     roof = {"bound": "hbm", "kernel": "batch step", "achieved": round(bytes_per * value, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(bytes_per * value / HBM_PEAK_GBS, 4), "traffic": None}
+            "frac": round(bytes_per * value / HBM_PEAK_GBS, 4),
+            "traffic": _traffic(f"batch_{len(w.shots)}shots_tile{ty}_bpc{bpc}", "step")}
     cpu = None
     if not args.no_cpu_baseline:
         s0 = w.shots[0]

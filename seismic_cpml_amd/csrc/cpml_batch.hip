@@ -49,29 +49,18 @@ struct BandIter {
   }
 };
 
-__device__ __forceinline__ int shot_of(const int *__restrict__ tstart, int nshots, int t) {
-  int lo = 0, hi = nshots - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (tstart[mid] <= t) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
 // A tile is 64 points by TY*RG rows; the block (64 x TY threads) walks its RG
 // row groups one after another, so the shot's descriptor is read once per
 // TY*RG rows.
 template <typename T, int ORDER, int TY, int RG, bool STRESS>
 __global__ __launch_bounds__(BX *TY) void k_batch2d(const Step2D<T> *__restrict__ shots,
-                                                   const int *__restrict__ tstart, int nshots,
-                                                   int ntiles) {
+                                                   const int2 *__restrict__ ttab, int ntiles) {
   const BandIter b(ntiles);
   for (int u = b.loc; u < b.count; u += b.nb) {
     const int t = b.start + u;
-    const int s = shot_of(tstart, nshots, t);
-    const Step2D<T> &p = shots[s];
-    const int lt = t - tstart[s];
+    const int2 e = ttab[t];  // shot, tile within the shot
+    const Step2D<T> &p = shots[e.x];
+    const int lt = e.y;
     const int ntx = (p.nx + BX - 1) / BX;
     const int i = (lt % ntx) * BX + threadIdx.x;
     const int j0 = (lt / ntx) * (TY * RG) + threadIdx.y;
@@ -102,17 +91,17 @@ __global__ __launch_bounds__(kMaxRecBlock) void k_batch_record(const Record<T> *
 }
 
 template <typename T, int TY, int RG>
-hipError_t batch_phase_ty(int order, bool stress, const Step2D<T> *shots, const int *tstart,
-                          int nshots, int ntiles, int grid, hipStream_t s) {
+hipError_t batch_phase_ty(int order, bool stress, const Step2D<T> *shots, const int2 *ttab,
+                          int ntiles, int grid, hipStream_t s) {
   const dim3 g(grid), blk(BX, TY);
 #define CPML_B(O)                                                                            \
   do {                                                                                       \
     if (stress)                                                                              \
-      hipLaunchKernelGGL((k_batch2d<T, O, TY, RG, true>), g, blk, 0, s, shots, tstart,      \
-                         nshots, ntiles);                                                    \
+      hipLaunchKernelGGL((k_batch2d<T, O, TY, RG, true>), g, blk, 0, s, shots, ttab,        \
+                         ntiles);                                                            \
     else                                                                                     \
-      hipLaunchKernelGGL((k_batch2d<T, O, TY, RG, false>), g, blk, 0, s, shots, tstart,     \
-                         nshots, ntiles);                                                    \
+      hipLaunchKernelGGL((k_batch2d<T, O, TY, RG, false>), g, blk, 0, s, shots, ttab,       \
+                         ntiles);                                                            \
   } while (0)
   if (order == 2) CPML_B(2);
   else if (order == 4) CPML_B(4);
@@ -123,15 +112,15 @@ hipError_t batch_phase_ty(int order, bool stress, const Step2D<T> *shots, const 
 
 // rows per tile: 4, 8, 16 (one row group) or 32, 64 (2 or 4 groups of 16)
 template <typename T>
-hipError_t batch_phase(int order, int rows, bool stress, const void *shots, const int *tstart,
-                       int nshots, int ntiles, int grid, hipStream_t s) {
+hipError_t batch_phase(int order, int rows, bool stress, const void *shots, const int2 *ttab,
+                       int ntiles, int grid, hipStream_t s) {
   const auto *sh = static_cast<const Step2D<T> *>(shots);
   switch (rows) {
-    case 64: return batch_phase_ty<T, 16, 4>(order, stress, sh, tstart, nshots, ntiles, grid, s);
-    case 32: return batch_phase_ty<T, 16, 2>(order, stress, sh, tstart, nshots, ntiles, grid, s);
-    case 16: return batch_phase_ty<T, 16, 1>(order, stress, sh, tstart, nshots, ntiles, grid, s);
-    case 8: return batch_phase_ty<T, 8, 1>(order, stress, sh, tstart, nshots, ntiles, grid, s);
-    default: return batch_phase_ty<T, 4, 1>(order, stress, sh, tstart, nshots, ntiles, grid, s);
+    case 64: return batch_phase_ty<T, 16, 4>(order, stress, sh, ttab, ntiles, grid, s);
+    case 32: return batch_phase_ty<T, 16, 2>(order, stress, sh, ttab, ntiles, grid, s);
+    case 16: return batch_phase_ty<T, 16, 1>(order, stress, sh, ttab, ntiles, grid, s);
+    case 8: return batch_phase_ty<T, 8, 1>(order, stress, sh, ttab, ntiles, grid, s);
+    default: return batch_phase_ty<T, 4, 1>(order, stress, sh, ttab, ntiles, grid, s);
   }
 }
 
@@ -182,7 +171,8 @@ struct cpml_batch {
   char *pool = nullptr;
   void *d_desc = nullptr;   // Step2D<T>[nshots]
   void *d_rec = nullptr;    // Record<T>[nshots]
-  int *d_tstart = nullptr;  // [nshots]
+  int2 *d_ttab = nullptr;   // [tiles]: shot, tile within the shot
+  int ttab_cap = 0;         // entries allocated (the most tiles any tile height gives)
   int *d_steps = nullptr;   // [nshots]
   int ntiles = 0;
   int tile_y = 4, blocks_per_cu = 0, graph_steps = 16;
@@ -285,7 +275,7 @@ template <typename T>
 int upload_tables(cpml_batch *b) {
   std::vector<cpml::Step2D<T>> desc(b->nshots);
   std::vector<cpml::Record<T>> rec(b->nshots);
-  std::vector<int> tstart(b->nshots);
+  std::vector<int2> ttab;
   int nt = 0;
   for (int q = 0; q < b->nshots; ++q) {
     const Shot &s = b->shots[q];
@@ -293,8 +283,9 @@ int upload_tables(cpml_batch *b) {
       return bfail(b, CPML_ERR_STATE, "shot " + std::to_string(q) +
                                           ": set_medium and set_pml must be called before run");
     desc[q] = shot_step<T>(b, q);
-    tstart[q] = nt;
-    nt += (int)(((s.nx + cpml::BX - 1) / cpml::BX) * ((s.ny + b->tile_y - 1) / b->tile_y));
+    const int tq = (int)(((s.nx + cpml::BX - 1) / cpml::BX) * ((s.ny + b->tile_y - 1) / b->tile_y));
+    for (int t = 0; t < tq; ++t) ttab.push_back(make_int2(q, t));
+    nt += tq;
     cpml::Record<T> r{};
     r.vx = desc[q].vx;
     r.vy = desc[q].vy;
@@ -309,8 +300,8 @@ int upload_tables(cpml_batch *b) {
   b->ntiles = nt;
   BCHK(b, hipMemcpy(b->d_desc, desc.data(), desc.size() * sizeof(desc[0]), hipMemcpyHostToDevice));
   BCHK(b, hipMemcpy(b->d_rec, rec.data(), rec.size() * sizeof(rec[0]), hipMemcpyHostToDevice));
-  BCHK(b, hipMemcpy(b->d_tstart, tstart.data(), tstart.size() * sizeof(int),
-                    hipMemcpyHostToDevice));
+  if (nt > b->ttab_cap) return bfail(b, CPML_ERR_STATE, "tile table too small");
+  BCHK(b, hipMemcpy(b->d_ttab, ttab.data(), ttab.size() * sizeof(int2), hipMemcpyHostToDevice));
   return CPML_OK;
 }
 
@@ -323,18 +314,18 @@ hipError_t enqueue_bstep(cpml_batch *b) {
   const int grid = grid_of(b);
   hipError_t e;
   if (b->prec == 4) {
-    e = cpml::batch_phase<float>(b->order, b->tile_y, true, b->d_desc, b->d_tstart, b->nshots,
-                                 b->ntiles, grid, b->stream);
+    e = cpml::batch_phase<float>(b->order, b->tile_y, true, b->d_desc, b->d_ttab, b->ntiles,
+                                 grid, b->stream);
     if (e == hipSuccess)
-      e = cpml::batch_phase<float>(b->order, b->tile_y, false, b->d_desc, b->d_tstart,
-                                   b->nshots, b->ntiles, grid, b->stream);
+      e = cpml::batch_phase<float>(b->order, b->tile_y, false, b->d_desc, b->d_ttab,
+                                   b->ntiles, grid, b->stream);
     if (e == hipSuccess) e = cpml::batch_record<float>(b->d_rec, b->nshots, b->stream);
   } else {
-    e = cpml::batch_phase<double>(b->order, b->tile_y, true, b->d_desc, b->d_tstart, b->nshots,
-                                  b->ntiles, grid, b->stream);
+    e = cpml::batch_phase<double>(b->order, b->tile_y, true, b->d_desc, b->d_ttab, b->ntiles,
+                                  grid, b->stream);
     if (e == hipSuccess)
-      e = cpml::batch_phase<double>(b->order, b->tile_y, false, b->d_desc, b->d_tstart,
-                                    b->nshots, b->ntiles, grid, b->stream);
+      e = cpml::batch_phase<double>(b->order, b->tile_y, false, b->d_desc, b->d_ttab,
+                                    b->ntiles, grid, b->stream);
     if (e == hipSuccess) e = cpml::batch_record<double>(b->d_rec, b->nshots, b->stream);
   }
   return e;
@@ -527,7 +518,12 @@ int cpml_batch_init(cpml_batch **bp, int order, int prec, int nshots, const int 
   int rc = balloc(b, (void **)&b->pool, off);
   if (!rc) rc = balloc(b, &b->d_desc, (size_t)nshots * sizeof(cpml::Step2D<double>));
   if (!rc) rc = balloc(b, &b->d_rec, (size_t)nshots * sizeof(cpml::Record<double>));
-  if (!rc) rc = balloc(b, (void **)&b->d_tstart, (size_t)nshots * sizeof(int));
+  // tile table sized for the smallest tile height (4 rows: the most tiles)
+  long cap = 0;
+  for (int q = 0; q < nshots; ++q)
+    cap += (long)((b->shots[q].nx + cpml::BX - 1) / cpml::BX) * ((b->shots[q].ny + 3) / 4);
+  b->ttab_cap = (int)cap;
+  if (!rc) rc = balloc(b, (void **)&b->d_ttab, (size_t)std::max(cap, 1L) * sizeof(int2));
   if (!rc) rc = balloc(b, (void **)&b->d_steps, (size_t)nshots * sizeof(int));
   for (int q = 0; q < nshots && !rc; ++q) {
     Shot &s = b->shots[q];
