@@ -211,10 +211,12 @@ int cpml_batch_last_error(cpml_batch *b, char *buf, int len);
 /* The reference absorbs outgoing waves with a C-PML of NPOINTS_PML cells
  * (3D:399-667 profiles, 3D:836-1052 memory variables).  The learned form
  * replaces it by a 4-cell strip on every face: the grid is the interior plus
- * the strip, updated without C-PML (profiles with a = 0, K = 1), and after
- * each step (after the source and the rigid edges, before the receivers) a
- * per-face corrector adds a correction to the outermost cells the face owns
- * (the nearest face; ties to the lowest of x-, x+, y-, y+, z-, z+).
+ * the strip, updated either without C-PML (profiles with a = 0, K = 1) or
+ * with a 4-cell C-PML of its own (NPOINTS_PML = 4), and after each step
+ * (after the source and the rigid edges, before the receivers) a per-face
+ * corrector adds a correction to the outermost cells the face owns (the
+ * nearest face; ties to the lowest of x-, x+, y-, y+, z-, z+).  Over a thin
+ * C-PML the corrector learns what that C-PML misses against a wide one.
  *
  * Corrector of one face: three linear 3x3x3 convolutions 9 -> 16 -> 16 -> 9
  * channels (no bias, no activation) over a 4-cell-deep box along the face,
@@ -238,17 +240,19 @@ int cpml_abc_net_destroy(cpml_abc_net *net);
 int cpml_abc_net_get_weights(cpml_abc_net *net, float *weights);
 int cpml_abc_net_set_weights(cpml_abc_net *net, const float *weights);
 /* mode CPML_BOUNDARY_LEARNED attaches net (not owned; keep it alive while
- * attached) to a 3D FP32 handle whose profiles have no absorption;
- * CPML_BOUNDARY_CPML detaches it. */
+ * attached) to a 3D FP32 handle (the strip with or without its own thin
+ * C-PML); CPML_BOUNDARY_CPML detaches it. */
 int cpml_gpu_set_boundary(cpml_solver *h, int mode, cpml_abc_net *net);
 /* Adam optimiser state (zeroed); one Adam step every `batch` training steps. */
 int cpml_abc_train_init(cpml_abc_net *net, double lr, double beta1, double beta2, double eps,
                         int batch);
 /* One training step on one time step of a teacher run: the student grid
- * (no C-PML, no corrector, no source) is the window of the teacher starting
- * at 0-based cell (i0, j0, k0).  Loads the teacher's current state into the
- * student, steps the student once (P), steps the teacher once (step `it`,
- * T) and adds the gradient of this sample to the mini-batch; the batch's
+ * (no corrector, no source; no C-PML or the strip's own thin C-PML) is the
+ * window of the teacher starting at 0-based cell (i0, j0, k0).  Loads the
+ * teacher's current fields into the student (the student keeps its own
+ * memory variables), steps the student once (P), steps the teacher once
+ * (step `it`, T) and adds the gradient of this sample to the mini-batch; the
+ * batch's
  * Adam step minimises, per face, sum (y - target)^2 over the owned strip
  * cells of its samples (target = (T - P) / scales) divided by the batch's
  * target energy, floored at 1% of its input energy.  loss6 (may be NULL):

@@ -1,8 +1,8 @@
 // Learned thin absorbing boundary for the 3D solver (FP32 fields).
 //
 // The grid is the interior plus a 4-cell strip on every face, updated by the
-// plain staggered-grid step (no C-PML).  After each step a per-face
-// corrector adds a correction to the strip:
+// staggered-grid step either without C-PML or with a 4-cell C-PML of its
+// own.  After each step a per-face corrector adds a correction to the strip:
 //
 //   x  = the 9 fields (vx vy vz sxx syy szz sxy sxz syz), each divided by a
 //        fixed scale, in a 4-cell-deep box along the face;
@@ -24,8 +24,9 @@
 // Training (FP32 kernels below) regresses the correction on pairs produced by
 // the solver itself: a wide-C-PML "teacher" run gives the state S_n; the
 // student grid (teacher cells 16 inward, i.e. the inner 4 cells of a 20-cell
-// C-PML become the strip) is loaded with the window of S_n and stepped once
-// without C-PML, giving P(S_n); the target is T_{n+1} - P(S_n) on the owned
+// C-PML become the strip) is loaded with the window of S_n (fields; a thin
+// C-PML of the student keeps its own memory variables) and stepped once,
+// giving P(S_n); the target is T_{n+1} - P(S_n) on the owned
 // outermost cells, with T_{n+1} the teacher's next state.  The loss of a face
 // over a mini-batch is sum (y - target)^2 / max(sum target^2, 1% of the
 // input energy).
@@ -881,8 +882,6 @@ int cpml_abc_train_step(cpml_abc_net *n, cpml_solver *student, cpml_solver *teac
   int rc = views(n, student, &st, "student");
   if (!rc) rc = views(n, teacher, &te, "teacher");
   if (rc) return rc;
-  for (int q = 0; q < 6; ++q)
-    if (st.slab_w[q] != 0) return afail(n, CPML_ERR_ARG, "the student grid must have no C-PML");
   if (st.learned) return afail(n, CPML_ERR_ARG, "the student must run without a corrector");
   if (i0 < 0 || j0 < 0 || k0 < 0 || i0 + st.nx > te.nx || j0 + st.ny > te.ny ||
       k0 + st.nz > te.nz)

@@ -1290,10 +1290,7 @@ int cpml_gpu_set_boundary(cpml_solver *h, int mode, cpml_abc_net *net) {
   if (mode != CPML_BOUNDARY_LEARNED || !net)
     return fail(h, CPML_ERR_ARG, "mode must be CPML_BOUNDARY_CPML or CPML_BOUNDARY_LEARNED "
                                  "with a corrector");
-  if (!h->have_pml) return fail(h, CPML_ERR_STATE, "call set_pml (no absorption) first");
-  if (h->sx.w || h->sy.w || h->sz.w)
-    return fail(h, CPML_ERR_ARG, "the learned boundary replaces the C-PML: set profiles with "
-                                 "a = 0 and K = 1 everywhere");
+  if (!h->have_pml) return fail(h, CPML_ERR_STATE, "call set_pml first");
   std::string m;
   if ((rc = cpml::abc_prepare(net, view_of(h), &m))) return fail(h, rc, m);
   dev_free(h, h->abc_corr);

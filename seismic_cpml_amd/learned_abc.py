@@ -32,6 +32,7 @@ from . import solver as S
 
 NPML_WIDE = 20
 STRIP = 4
+STRIP_CPML = 4  # C-PML cells of the strip under the corrector (training default)
 OFFSET = NPML_WIDE - STRIP  # student/teacher window offset (cells)
 
 
@@ -129,15 +130,19 @@ def make_run(model: Model, pad: int, absorb: int, nstep: int, ext: int = 0,
 
 
 def train(net: S.CpmlAbcNet, n: int = 128, runs: int = 16, nstep: int = 600,
-          first_step: int = 30, stride: int = 1, seed: int = 1000, log=None):
+          first_step: int = 30, stride: int = 1, seed: int = 1000, log=None,
+          strip_cpml: int = 0):
     """Online training: for each seeded model a wide-C-PML teacher run
-    (interior n^3 + 20 cells) and a plain student grid (n + 8)^3.  Returns the
-    mean loss per run."""
+    (interior n^3 + 20 cells) and a student grid (n + 8)^3 whose strip has a
+    C-PML of strip_cpml cells (0: none); the corrector learns what the
+    student's step misses against the teacher.  Returns the mean loss per
+    run."""
+    net.strip_cpml = strip_cpml
     history = []
     for r in range(runs):
         m = layered_model(n, seed + r)
         teacher, _, keep_t = make_run(m, NPML_WIDE, NPML_WIDE, nstep)
-        student, _, keep_s = make_run(m, STRIP, 0, 1, source=False)
+        student, _, keep_s = make_run(m, STRIP, strip_cpml, 1, source=False)
         teacher.run(1, first_step - 1)
         losses = []
         for it in range(first_step, nstep + 1):
@@ -171,9 +176,9 @@ def _seis(s):
 def reflection(model: Model, nstep: int, net: S.CpmlAbcNet | None = None, per_axis: int = 8,
                reference=None):
     """Reflected energy (dB) and seismogram rel-L2 of the learned strip, the
-    plain 20-cell C-PML and the rigid (no absorption) strip, each against the
-    doubled-domain reference.  Returns a dict (and the reference seismograms
-    for reuse)."""
+    plain 20-cell C-PML, the rigid (no absorption) strip and the strip as a
+    4-cell C-PML, each against the doubled-domain reference.  Returns a dict
+    (and the reference seismograms for reuse)."""
     rec = lattice(model.n, per_axis)
     if reference is None:
         ref, _, keep = make_run(model, NPML_WIDE, NPML_WIDE, nstep, ext=model.n // 2,
@@ -190,9 +195,10 @@ def reflection(model: Model, nstep: int, net: S.CpmlAbcNet | None = None, per_ax
         return {"db": 10.0 * math.log10(max(err, 1e-300) / e_ref),
                 "rel_l2": math.sqrt(err / e_ref)}
 
-    runs = [("cpml20", NPML_WIDE, NPML_WIDE, None), ("rigid4", STRIP, 0, None)]
+    runs = [("cpml20", NPML_WIDE, NPML_WIDE, None), ("rigid4", STRIP, 0, None),
+            ("cpml4", STRIP, STRIP, None)]
     if net is not None:
-        runs.append(("learned4", STRIP, 0, net))
+        runs.append(("learned4", STRIP, net.strip_cpml, net))
     for name, pad, absorb, nn in runs:
         s, _, keep = make_run(model, pad, absorb, nstep, receivers=rec)
         if nn is not None:
@@ -207,14 +213,16 @@ def reflection(model: Model, nstep: int, net: S.CpmlAbcNet | None = None, per_ax
 
 
 def time_to_solution(model: Model, nstep: int, net: S.CpmlAbcNet, repeats: int = 1):
-    """Wall time of nstep steps: learned strip ((n+8)^3 + corrector) against
-    the 20-cell C-PML ((n+40)^3), both with the same step kernels and
+    """Wall time of nstep steps: learned strip ((n+8)^3 + corrector), the
+    strip as a 4-cell C-PML and the strip without absorption ((n+8)^3)
+    against the 20-cell C-PML ((n+40)^3), all with the same step kernels and
     graphs, no receivers."""
     import torch
 
     res = {}
     for name, pad, absorb, nn in (("cpml20", NPML_WIDE, NPML_WIDE, None),
-                                  ("learned4", STRIP, 0, net)):
+                                  ("learned4", STRIP, net.strip_cpml, net),
+                                  ("cpml4", STRIP, STRIP, None), ("rigid4", STRIP, 0, None)):
         s, _, keep = make_run(model, pad, absorb, nstep)
         if nn is not None:
             s.set_boundary(nn)
@@ -231,6 +239,7 @@ def time_to_solution(model: Model, nstep: int, net: S.CpmlAbcNet, repeats: int =
         s.close()
         del keep
     res["speedup"] = res["cpml20"]["seconds"] / res["learned4"]["seconds"]
+    res["speedup_cpml4"] = res["cpml20"]["seconds"] / res["cpml4"]["seconds"]
     return res
 
 
@@ -249,12 +258,15 @@ def save_weights(net: S.CpmlAbcNet, path=WEIGHTS_FILE, **meta):
     path = Path(path)
     path.parent.mkdir(parents=True, exist_ok=True)
     np.savez_compressed(path, weights=net.weights, scales=net.scales,
+                        strip_cpml=np.array(net.strip_cpml, np.int32),
                         meta=np.array(repr(meta)))
 
 
 def load_net(path=WEIGHTS_FILE):
     d = np.load(path, allow_pickle=False)
-    return S.CpmlAbcNet(d["scales"], d["weights"])
+    net = S.CpmlAbcNet(d["scales"], d["weights"])
+    net.strip_cpml = int(d["strip_cpml"]) if "strip_cpml" in d.files else 0
+    return net
 
 
 def main(argv=None):
@@ -266,9 +278,11 @@ def main(argv=None):
     ap.add_argument("--nstep", type=int, default=600)
     ap.add_argument("--lr", type=float, default=2e-4)
     ap.add_argument("--batch", type=int, default=8)
-    ap.add_argument("--noise", type=float, default=1.0,
+    ap.add_argument("--noise", type=float, default=0.3,
                     help="relative input noise (stabilises long closed-loop runs)")
     ap.add_argument("--seed", type=int, default=1000)
+    ap.add_argument("--strip-cpml", type=int, default=STRIP_CPML,
+                    help="C-PML cells of the student strip under the corrector (0: none)")
     ap.add_argument("--out", default=str(WEIGHTS_FILE))
     args = ap.parse_args(argv)
     net = S.CpmlAbcNet(field_scales(), init_weights(args.seed, std=0.02))
@@ -276,9 +290,10 @@ def main(argv=None):
     net.train_noise(args.noise)
     t = time.perf_counter()
     hist = train(net, n=args.n, runs=args.runs, nstep=args.nstep, seed=args.seed,
-                 log=lambda m: print(m, flush=True))
+                 log=lambda m: print(m, flush=True), strip_cpml=args.strip_cpml)
     save_weights(net, args.out, n=args.n, runs=args.runs, nstep=args.nstep, lr=args.lr,
-                 noise=args.noise, seed=args.seed, loss=hist, seconds=time.perf_counter() - t)
+                 noise=args.noise, seed=args.seed, strip_cpml=args.strip_cpml, loss=hist,
+                 seconds=time.perf_counter() - t)
     print(f"saved {args.out} after {time.perf_counter() - t:.1f} s", flush=True)
 
 

@@ -381,6 +381,9 @@ class CpmlAbcNet:
         w = None if weights is None else self._w(weights)
         self._h = C.c_void_p()
         self.scales = sc
+        # C-PML cells of the strip the corrector was trained over (0: none);
+        # recorded with the weights by the training driver
+        self.strip_cpml = 0
         _check(library().cpml_abc_net_create(C.byref(self._h), _ptr(w), _ptr(sc)), None,
                "cpml_abc_last_error")
 

@@ -194,3 +194,43 @@ def test_reflection_measurement_orders_boundaries():
     r, _ = learned_abc.reflection(m, 260, per_axis=5)
     assert r["cpml20"]["finite"] and r["rigid4"]["finite"]
     assert r["cpml20"]["db"] < r["rigid4"]["db"] - 10.0
+
+
+def test_zero_corrector_over_thin_cpml_leaves_step_unchanged():
+    """The corrector may sit on a strip with its own 4-cell C-PML; with zero
+    weights the step is that thin C-PML's, bit for bit."""
+    m = learned_abc.layered_model(24, seed=5, f0=12.0)
+    a, _, ka = learned_abc.make_run(m, 4, 4, 120)
+    b, _, kb = learned_abc.make_run(m, 4, 4, 120)
+    with CpmlAbcNet(SCALES) as net:
+        b.set_boundary(net)
+        a.run()
+        b.run()
+        for f in FIELDS:
+            np.testing.assert_array_equal(a.field(f), b.field(f), err_msg=f)
+        assert np.abs(a.field("vx")).max() > 0
+    a.close()
+    b.close()
+
+
+def test_training_over_thin_cpml_student_is_finite():
+    """Training against a student whose strip is a 4-cell C-PML (the student
+    keeps its own memory variables across samples): finite losses, and the
+    trained corrector over the thin C-PML runs finite in closed loop."""
+    with CpmlAbcNet(learned_abc.field_scales(), learned_abc.init_weights(4, std=0.02)) as net:
+        net.train_init(lr=2e-3, batch=4)
+        m = learned_abc.layered_model(24, seed=7, f0=12.0, src_margin=10)
+        teacher, _, kt = learned_abc.make_run(m, 20, 20, 300)
+        student, _, ks = learned_abc.make_run(m, 4, 4, 1, source=False)
+        teacher.run(1, 149)
+        losses = [net.train_step(student, teacher, (16, 16, 16), it).sum()
+                  for it in range(150, 301)]
+        assert np.isfinite(losses).all() and max(losses) > 0
+        teacher.close()
+        student.close()
+        s, _, k = learned_abc.make_run(m, 4, 4, 200)
+        s.set_boundary(net)
+        s.run()
+        assert np.isfinite(s.field("vx")).all()
+        s.set_boundary(None)
+        s.close()
