@@ -64,6 +64,32 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, unsig
 template <typename T>
 struct BufOp;
 
+// Cache policy of buffer accesses whose data no other thread touches (a
+// point's own old stress, coefficients, results): the non-temporal hint (bit
+// 1 of the policy operand on gfx950) streams them past the L2, which keeps
+// its capacity for the halo rows and planes that neighbouring tiles re-read.
+#ifndef CPML_NT_POLICY
+#define CPML_NT_POLICY 2
+#endif
+
+// The same hint on plain pointer accesses.
+template <typename T>
+__device__ __forceinline__ T ld_once(const T *p) {
+#if CPML_NT_POLICY
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+template <typename T>
+__device__ __forceinline__ void st_once(T *p, T x) {
+#if CPML_NT_POLICY
+  __builtin_nontemporal_store(x, p);
+#else
+  *p = x;
+#endif
+}
+
 template <>
 struct BufOp<float> {
   __device__ static float ld(__amdgpu_buffer_rsrc_t r, unsigned v, unsigned s) {
@@ -72,16 +98,31 @@ struct BufOp<float> {
   __device__ static void st(float x, __amdgpu_buffer_rsrc_t r, unsigned v, unsigned s) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, v, s, 0);
   }
+  __device__ static float ldn(__amdgpu_buffer_rsrc_t r, unsigned v, unsigned s) {
+    return __builtin_bit_cast(float,
+                              __builtin_amdgcn_raw_buffer_load_b32(r, v, s, CPML_NT_POLICY));
+  }
+  __device__ static void stn(float x, __amdgpu_buffer_rsrc_t r, unsigned v, unsigned s) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, v, s,
+                                          CPML_NT_POLICY);
+  }
 };
 
 template <>
 struct BufOp<double> {
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
   __device__ static double ld(__amdgpu_buffer_rsrc_t r, unsigned v, unsigned s) {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, v, s, 0));
   }
   __device__ static void st(double x, __amdgpu_buffer_rsrc_t r, unsigned v, unsigned s) {
-    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, v, s, 0);
+  }
+  __device__ static double ldn(__amdgpu_buffer_rsrc_t r, unsigned v, unsigned s) {
+    return __builtin_bit_cast(double,
+                              __builtin_amdgcn_raw_buffer_load_b64(r, v, s, CPML_NT_POLICY));
+  }
+  __device__ static void stn(double x, __amdgpu_buffer_rsrc_t r, unsigned v, unsigned s) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, v, s, CPML_NT_POLICY);
   }
 };
 

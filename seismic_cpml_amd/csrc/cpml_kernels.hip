@@ -129,7 +129,9 @@ struct Chunk3 {
 
 // load / store of (i+DX, j+dy, k+dz): the x shift goes to the immediate
 #define CPML_LD(r, DX, dy, dz) BufOp<T>::ld(r, g.v0 + (1 + (DX)) * (unsigned)sizeof(T), g.so(kk, dy, dz))
-#define CPML_ST(x, r) BufOp<T>::st(x, r, g.v0 + (unsigned)sizeof(T), g.so(kk, 0, 0))
+// loads of values only this point reads, and the stores (non-temporal)
+#define CPML_LDN(r) BufOp<T>::ldn(r, g.v0 + (unsigned)sizeof(T), g.so(kk, 0, 0))
+#define CPML_ST(x, r) BufOp<T>::stn(x, r, g.v0 + (unsigned)sizeof(T), g.so(kk, 0, 0))
 
 // Values the stress kernel reads at one plane.
 template <typename T>
@@ -169,14 +171,14 @@ __global__ __launch_bounds__(BX *TY3) void k_stress3d(Step3D<T> p) {
     L.vy_xm = CPML_LD(rvy, -1, 0, 0);
     L.vz_xm = CPML_LD(rvz, -1, 0, 0);
     L.vz_yp = CPML_LD(rvz, 0, 1, 0);
-    L.cl = CPML_LD(rcl, 0, 0, 0);
-    L.cm = CPML_LD(rcm, 0, 0, 0);
-    L.sxx = CPML_LD(rxx, 0, 0, 0);
-    L.syy = CPML_LD(ryy, 0, 0, 0);
-    L.szz = CPML_LD(rzz, 0, 0, 0);
-    L.sxy = CPML_LD(rxy, 0, 0, 0);
-    L.sxz = CPML_LD(rxz, 0, 0, 0);
-    L.syz = CPML_LD(ryz, 0, 0, 0);
+    L.cl = CPML_LDN(rcl);
+    L.cm = CPML_LDN(rcm);
+    L.sxx = CPML_LDN(rxx);
+    L.syy = CPML_LDN(ryy);
+    L.szz = CPML_LDN(rzz);
+    L.sxy = CPML_LDN(rxy);
+    L.sxz = CPML_LDN(rxz);
+    L.syz = CPML_LDN(ryz);
   };
   // x / y profile values of this column (loop invariant; only read in the
   // absorbing slabs)
@@ -288,10 +290,10 @@ __global__ __launch_bounds__(BX *TY3) void k_velocity3d(Step3D<T> p) {
     L.syy_yp = CPML_LD(ryy, 0, 1, 0);
     L.sxz_xp = CPML_LD(rxz, 1, 0, 0);
     L.syz_ym = CPML_LD(ryz, 0, -1, 0);
-    L.cb = CPML_LD(rcb, 0, 0, 0);
-    L.vx = CPML_LD(rvx, 0, 0, 0);
-    L.vy = CPML_LD(rvy, 0, 0, 0);
-    L.vz = CPML_LD(rvz, 0, 0, 0);
+    L.cb = CPML_LDN(rcb);
+    L.vx = CPML_LDN(rvx);
+    L.vy = CPML_LDN(rvy);
+    L.vz = CPML_LDN(rvz);
   };
   // x / y profile values of this column (loop invariant; only read in the
   // absorbing slabs)
@@ -372,6 +374,7 @@ __global__ __launch_bounds__(BX *TY3) void k_velocity3d(Step3D<T> p) {
 }
 
 #undef CPML_LD
+#undef CPML_LDN
 #undef CPML_ST
 
 // ------------------------------------------------------ receivers, step --

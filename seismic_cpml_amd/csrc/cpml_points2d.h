@@ -51,8 +51,9 @@ __device__ __forceinline__ void stress2d_point(const Step2D<T> &p, int i, int j)
     T dvy_dy = Op<T, ORDER>::d(p.vy, c - ld, ld) / p.den_y;
     if (inx) dvx_dx = cpml(dvx_dx, p.mx[0] + mxo, p.px.ah[i], p.px.bh[i], p.px.Kh[i]);
     if (iny) dvy_dy = cpml(dvy_dy, p.my[0] + myo, p.py.a[j], p.py.b[j], p.py.K[j]);
-    p.sxx[c] = p.sxx[c] + (lp2m * dvx_dx + lhx * dvy_dy) * p.dt;
-    p.syy[c] = p.syy[c] + (lhx * dvx_dx + lp2m * dvy_dy) * p.dt;
+    // own-point values: streamed past the L2 (ld_once / st_once)
+    st_once(p.sxx + c, ld_once(p.sxx + c) + (lp2m * dvx_dx + lhx * dvy_dy) * p.dt);
+    st_once(p.syy + c, ld_once(p.syy + c) + (lhx * dvx_dx + lp2m * dvy_dy) * p.dt);
   }
   // sigma_xy at (i, j+1/2), Fortran i=2..NX, j=1..NY-1
   if (i >= 1 && j <= p.ny - 2) {
@@ -64,7 +65,7 @@ __device__ __forceinline__ void stress2d_point(const Step2D<T> &p, int i, int j)
     if (iny)
       dvx_dy = cpml(dvx_dy, p.my[1] + myo, p.py.ah[j], p.py.bh[j],
                     ORDER == 4 ? p.py.K[j] : p.py.Kh[j]);
-    p.sxy[c] = p.sxy[c] + mhy * (dvy_dx + dvx_dy) * p.dt;
+    st_once(p.sxy + c, ld_once(p.sxy + c) + mhy * (dvy_dx + dvx_dy) * p.dt);
   }
 }
 
@@ -77,7 +78,7 @@ __device__ __forceinline__ void velocity2d_point(const Step2D<T> &p, int i, int 
   const bool inx = p.sx.in(i), iny = p.sy.in(j);
   const long mxo = p.sx.s(i) + (long)p.sx.w * j;
   const long myo = i + p.mld * p.sy.s(j);
-  T vxn = p.vx[c], vyn = p.vy[c];
+  T vxn = ld_once(p.vx + c), vyn = ld_once(p.vy + c);
   // vx at (i, j), Fortran i=2..NX, j=2..NY
   if (i >= 1 && j >= 1) {
     T dsxx_dx = Op<T, ORDER>::d(p.sxx, c - 1, 1) / p.den_x;
@@ -107,8 +108,8 @@ __device__ __forceinline__ void velocity2d_point(const Step2D<T> &p, int i, int 
     vxn = T(0);
     vyn = T(0);
   }
-  p.vx[c] = vxn;
-  p.vy[c] = vyn;
+  st_once(p.vx + c, vxn);
+  st_once(p.vy + c, vyn);
 }
 
 }  // namespace cpml
