@@ -346,12 +346,12 @@ __global__ __launch_bounds__(FW *(TY + 2)) void k_fused3d(Fused3D<T> P) {
       }
       if (st) {
         const unsigned so_s = soff(s);
-        BufOp<T>::stn(sn[0], rs(Q->sxx), vo, so_s);
-        BufOp<T>::stn(sn[1], rs(Q->syy), vo, so_s);
-        BufOp<T>::stn(sn[2], rs(Q->szz), vo, so_s);
-        BufOp<T>::stn(sn[3], rs(Q->sxy), vo, so_s);
-        BufOp<T>::stn(sn[4], rs(Q->sxz), vo, so_s);
-        BufOp<T>::stn(sn[5], rs(Q->syz), vo, so_s);
+        BufOp<T>::st(sn[0], rs(Q->sxx), vo, so_s);
+        BufOp<T>::st(sn[1], rs(Q->syy), vo, so_s);
+        BufOp<T>::st(sn[2], rs(Q->szz), vo, so_s);
+        BufOp<T>::st(sn[3], rs(Q->sxy), vo, so_s);
+        BufOp<T>::st(sn[4], rs(Q->sxz), vo, so_s);
+        BufOp<T>::st(sn[5], rs(Q->syz), vo, so_s);
       }
     }
 
@@ -426,9 +426,9 @@ __global__ __launch_bounds__(FW *(TY + 2)) void k_fused3d(Fused3D<T> P) {
         vzn = T(0);
       }
       const unsigned so_v = soff(kv);
-      BufOp<T>::stn(vxn, rs(Q->vx), vo, so_v);
-      BufOp<T>::stn(vyn, rs(Q->vy), vo, so_v);
-      BufOp<T>::stn(vzn, rs(Q->vz), vo, so_v);
+      BufOp<T>::st(vxn, rs(Q->vx), vo, so_v);
+      BufOp<T>::st(vyn, rs(Q->vy), vo, so_v);
+      BufOp<T>::st(vzn, rs(Q->vz), vo, so_v);
     }
 
     // ---- shift the planes
