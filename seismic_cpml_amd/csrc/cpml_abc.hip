@@ -400,6 +400,7 @@ typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int TT = 8;                          // tile: 8 x 8 columns of the face
+constexpr int TBI = 512;                       // threads of the inference kernel
 constexpr int XA = TT + 6, XN = XA * XA * D;   // input box 14 x 14 x D
 constexpr int H1A = TT + 4, H1N = H1A * H1A * (D - 1);
 constexpr int H2A = TT + 2, H2N = H2A * H2A * (D - 2);
@@ -418,33 +419,49 @@ __device__ __forceinline__ void mfma_layer(const unsigned short *in, int izero,
   constexpr int IA = OA + 2;
   constexpr int ON = OA * OA * OD, NG = (ON + 15) / 16;
   constexpr int HALO = (OA - TT) / 2;
-  constexpr int NW = TB / 64, GPW = 4;
+  constexpr int NW = TBI / 64, GPW = 4;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   s4 bw[TAPS];
 #pragma unroll
   for (int t = 0; t < TAPS; ++t) bw[t] = wf[t * 64 + lane];
   const int kq = (lane >> 4) * 4;
   for (int g0 = wave * GPW; g0 < NG; g0 += NW * GPW) {
+    // GPW independent 16-cell groups; a group past the layer's last one
+    // recomputes the last group (no branch inside the tap loop) and is not
+    // stored
     int base[GPW];
     bool top[GPW];
     f4 acc[GPW];
 #pragma unroll
     for (int u = 0; u < GPW; ++u) {
-      const int oc = min((g0 + u) * 16 + (lane & 15), ON - 1);
+      const int gu = min(g0 + u, NG - 1);
+      const int oc = min(gu * 16 + (lane & 15), ON - 1);
       const int al = oc % OA, bl = (oc / OA) % OA, d = oc / (OA * OA);
       base[u] = (d * IA + bl + 1) * IA + al + 1;
       top[u] = d == 0;
       acc[u] = f4{0.f, 0.f, 0.f, 0.f};
     }
+    auto a_of = [&](int u, int t) -> s4 {
+      const int off = (t / 9 - 1) * IA * IA + (t % 3 - 1) * IA + ((t / 3) % 3 - 1);
+      const int ic = (t < 9 && top[u]) ? izero : base[u] + off;
+      return *reinterpret_cast<const s4 *>(in + ic * C + kq);
+    };
+    // A operands of tap t + 1 are read from LDS while the MFMAs of tap t run
+    s4 cur[GPW], nxt[GPW];
+#pragma unroll
+    for (int u = 0; u < GPW; ++u) cur[u] = a_of(u, 0);
 #pragma unroll
     for (int t = 0; t < TAPS; ++t) {
-      const int off = (t / 9 - 1) * IA * IA + (t % 3 - 1) * IA + ((t / 3) % 3 - 1);
+      if (t + 1 < TAPS) {
 #pragma unroll
-      for (int u = 0; u < GPW; ++u) {
-        if (g0 + u >= NG) break;  // uniform over the wave
-        const int ic = (t < 9 && top[u]) ? izero : base[u] + off;
-        const s4 av = *reinterpret_cast<const s4 *>(in + ic * C + kq);
-        acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bw[t], acc[u], 0, 0, 0);
+        for (int u = 0; u < GPW; ++u) nxt[u] = a_of(u, t + 1);
+      }
+#pragma unroll
+      for (int u = 0; u < GPW; ++u)
+        acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(cur[u], bw[t], acc[u], 0, 0, 0);
+      if (t + 1 < TAPS) {
+#pragma unroll
+        for (int u = 0; u < GPW; ++u) cur[u] = nxt[u];
       }
     }
     const int n = lane & 15;
@@ -469,7 +486,7 @@ __device__ __forceinline__ void mfma_layer(const unsigned short *in, int izero,
   }
 }
 
-__global__ __launch_bounds__(TB) void k_infer(Geo g, Fields u, Scales inv,
+__global__ __launch_bounds__(TBI) void k_infer(Geo g, Fields u, Scales inv,
                                               const unsigned short *__restrict__ Wb,
                                               float *corr) {
   __shared__ __attribute__((aligned(16))) unsigned short bx[(XN + 1) * C];
@@ -486,7 +503,7 @@ __global__ __launch_bounds__(TB) void k_infer(Geo g, Fields u, Scales inv,
   // x faces: depth runs along x, the contiguous direction, so the lanes walk
   // the box depth first; y and z faces walk a (= x) first
   const bool xface = F.ax == 0;
-  for (int q = threadIdx.x; q < XN; q += TB) {
+  for (int q = threadIdx.x; q < XN; q += TBI) {
     const int al = xface ? (q / D) % XA : q % XA;
     const int bl = xface ? q / (D * XA) : (q / XA) % XA;
     const int d = xface ? q % D : q / (XA * XA);
@@ -749,7 +766,7 @@ int abc_prepare(cpml_abc_net *n, const SolverView &v, std::string *err) {
 hipError_t abc_enqueue(cpml_abc_net *n, float *corr, const SolverView &v) {
   const Geo g = make_geo(v);
   const Fields u = fields_of(v);
-  hipLaunchKernelGGL(k_infer, dim3(g.tiles), dim3(TB), 0, v.stream, g, u, n->inv, n->Wb, corr);
+  hipLaunchKernelGGL(k_infer, dim3(g.tiles), dim3(TBI), 0, v.stream, g, u, n->inv, n->Wb, corr);
   hipLaunchKernelGGL(k_apply, dim3((g.max4 + TB - 1) / TB, 6),
                      dim3(TB), 0, v.stream, g, u, n->scale, corr);
   return hipGetLastError();
@@ -940,7 +957,7 @@ int cpml_abc_correction(cpml_abc_net *n, cpml_solver *h, int use_mfma, float *ou
   if (use_mfma) {
     if ((rc = sync_bf16(n, s))) return rc;
     ACHK(n, hipMemsetAsync(n->corr, 0, (size_t)g.cells4 * NF * sizeof(float), s));
-    hipLaunchKernelGGL(k_infer, dim3(g.tiles), dim3(TB), 0, s, g, fields_of(v), n->inv, n->Wb,
+    hipLaunchKernelGGL(k_infer, dim3(g.tiles), dim3(TBI), 0, s, g, fields_of(v), n->inv, n->Wb,
                        n->corr);
   } else {
     ACHK(n, hipMemsetAsync(n->red, 0, 18 * sizeof(double), s));
