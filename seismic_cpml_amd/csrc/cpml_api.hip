@@ -308,6 +308,7 @@ cpml::Step2D<T> step2d(cpml_solver *h) {
   p.fy = static_cast<const T *>(h->fy);
   p.step = h->step;
   p.tile_y = h->tile_y;
+  p.streaming = h->bytes > cpml::kStreamingBytes;
   return p;
 }
 
@@ -353,6 +354,7 @@ cpml::Step3D<T> step3d(cpml_solver *h) {
   p.step = h->step;
   p.kchunk = h->kchunk;
   p.tile_y = h->tile_y;
+  p.streaming = h->bytes > cpml::kStreamingBytes;
   return p;
 }
 

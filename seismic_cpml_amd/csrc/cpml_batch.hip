@@ -52,7 +52,7 @@ struct BandIter {
 // A tile is 64 points by TY*RG rows; the block (64 x TY threads) walks its RG
 // row groups one after another, so the shot's descriptor is read once per
 // TY*RG rows.
-template <typename T, int ORDER, int TY, int RG, bool STRESS>
+template <typename T, int ORDER, int TY, int RG, bool STRESS, bool NT>
 __global__ __launch_bounds__(BX *TY) void k_batch2d(const Step2D<T> *__restrict__ shots,
                                                    const int2 *__restrict__ ttab, int ntiles) {
   const BandIter b(ntiles);
@@ -68,8 +68,8 @@ __global__ __launch_bounds__(BX *TY) void k_batch2d(const Step2D<T> *__restrict_
     for (int r = 0; r < RG; ++r) {
       const int j = j0 + r * TY;
       if (i < p.nx && j < p.ny) {
-        if (STRESS) stress2d_point<T, ORDER>(p, i, j);
-        else velocity2d_point<T, ORDER>(p, i, j);
+        if (STRESS) stress2d_point<T, ORDER, NT>(p, i, j);
+        else velocity2d_point<T, ORDER, NT>(p, i, j);
       }
     }
   }
@@ -91,36 +91,44 @@ __global__ __launch_bounds__(kMaxRecBlock) void k_batch_record(const Record<T> *
 }
 
 template <typename T, int TY, int RG>
-hipError_t batch_phase_ty(int order, bool stress, const Step2D<T> *shots, const int2 *ttab,
-                          int ntiles, int grid, hipStream_t s) {
+hipError_t batch_phase_ty(int order, bool stress, bool streaming, const Step2D<T> *shots,
+                          const int2 *ttab, int ntiles, int grid, hipStream_t s) {
   const dim3 g(grid), blk(BX, TY);
-#define CPML_B(O)                                                                            \
+#define CPML_B2(O, NT)                                                                       \
   do {                                                                                       \
     if (stress)                                                                              \
-      hipLaunchKernelGGL((k_batch2d<T, O, TY, RG, true>), g, blk, 0, s, shots, ttab,        \
+      hipLaunchKernelGGL((k_batch2d<T, O, TY, RG, true, NT>), g, blk, 0, s, shots, ttab,    \
                          ntiles);                                                            \
     else                                                                                     \
-      hipLaunchKernelGGL((k_batch2d<T, O, TY, RG, false>), g, blk, 0, s, shots, ttab,       \
+      hipLaunchKernelGGL((k_batch2d<T, O, TY, RG, false, NT>), g, blk, 0, s, shots, ttab,   \
                          ntiles);                                                            \
+  } while (0)
+#define CPML_B(O)              \
+  do {                         \
+    if (streaming)             \
+      CPML_B2(O, true);        \
+    else                       \
+      CPML_B2(O, false);       \
   } while (0)
   if (order == 2) CPML_B(2);
   else if (order == 4) CPML_B(4);
   else CPML_B(8);
 #undef CPML_B
+#undef CPML_B2
   return hipGetLastError();
 }
 
 // rows per tile: 4, 8, 16 (one row group) or 32, 64 (2 or 4 groups of 16)
 template <typename T>
-hipError_t batch_phase(int order, int rows, bool stress, const void *shots, const int2 *ttab,
-                       int ntiles, int grid, hipStream_t s) {
+hipError_t batch_phase(int order, int rows, bool stress, bool streaming, const void *shots,
+                       const int2 *ttab, int ntiles, int grid, hipStream_t s) {
   const auto *sh = static_cast<const Step2D<T> *>(shots);
   switch (rows) {
-    case 64: return batch_phase_ty<T, 16, 4>(order, stress, sh, ttab, ntiles, grid, s);
-    case 32: return batch_phase_ty<T, 16, 2>(order, stress, sh, ttab, ntiles, grid, s);
-    case 16: return batch_phase_ty<T, 16, 1>(order, stress, sh, ttab, ntiles, grid, s);
-    case 8: return batch_phase_ty<T, 8, 1>(order, stress, sh, ttab, ntiles, grid, s);
-    default: return batch_phase_ty<T, 4, 1>(order, stress, sh, ttab, ntiles, grid, s);
+    case 64: return batch_phase_ty<T, 16, 4>(order, stress, streaming, sh, ttab, ntiles, grid, s);
+    case 32: return batch_phase_ty<T, 16, 2>(order, stress, streaming, sh, ttab, ntiles, grid, s);
+    case 16: return batch_phase_ty<T, 16, 1>(order, stress, streaming, sh, ttab, ntiles, grid, s);
+    case 8: return batch_phase_ty<T, 8, 1>(order, stress, streaming, sh, ttab, ntiles, grid, s);
+    default: return batch_phase_ty<T, 4, 1>(order, stress, streaming, sh, ttab, ntiles, grid, s);
   }
 }
 
@@ -268,6 +276,7 @@ cpml::Step2D<T> shot_step(cpml_batch *b, int q) {
   p.fy = static_cast<const T *>(s.fy);
   p.step = b->d_steps + q;
   p.tile_y = b->tile_y;
+  p.streaming = b->bytes > cpml::kStreamingBytes;
   return p;
 }
 
@@ -312,19 +321,20 @@ int grid_of(const cpml_batch *b) {
 
 hipError_t enqueue_bstep(cpml_batch *b) {
   const int grid = grid_of(b);
+  const bool streaming = b->bytes > cpml::kStreamingBytes;
   hipError_t e;
   if (b->prec == 4) {
-    e = cpml::batch_phase<float>(b->order, b->tile_y, true, b->d_desc, b->d_ttab, b->ntiles,
+    e = cpml::batch_phase<float>(b->order, b->tile_y, true, streaming, b->d_desc, b->d_ttab, b->ntiles,
                                  grid, b->stream);
     if (e == hipSuccess)
-      e = cpml::batch_phase<float>(b->order, b->tile_y, false, b->d_desc, b->d_ttab,
+      e = cpml::batch_phase<float>(b->order, b->tile_y, false, streaming, b->d_desc, b->d_ttab,
                                    b->ntiles, grid, b->stream);
     if (e == hipSuccess) e = cpml::batch_record<float>(b->d_rec, b->nshots, b->stream);
   } else {
-    e = cpml::batch_phase<double>(b->order, b->tile_y, true, b->d_desc, b->d_ttab, b->ntiles,
+    e = cpml::batch_phase<double>(b->order, b->tile_y, true, streaming, b->d_desc, b->d_ttab, b->ntiles,
                                   grid, b->stream);
     if (e == hipSuccess)
-      e = cpml::batch_phase<double>(b->order, b->tile_y, false, b->d_desc, b->d_ttab,
+      e = cpml::batch_phase<double>(b->order, b->tile_y, false, streaming, b->d_desc, b->d_ttab,
                                     b->ntiles, grid, b->stream);
     if (e == hipSuccess) e = cpml::batch_record<double>(b->d_rec, b->nshots, b->stream);
   }

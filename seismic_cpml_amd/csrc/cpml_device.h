@@ -64,30 +64,23 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, unsig
 template <typename T>
 struct BufOp;
 
-// Cache policy of buffer accesses whose data no other thread touches (a
-// point's own old stress, coefficients, results): the non-temporal hint (bit
-// 1 of the policy operand on gfx950) streams them past the L2, which keeps
-// its capacity for the halo rows and planes that neighbouring tiles re-read.
-#ifndef CPML_NT_POLICY
-#define CPML_NT_POLICY 2
-#endif
+// Cache policy of accesses whose data no other thread touches (a point's own
+// old stress, coefficients, results): when the state is much larger than the
+// caches, the non-temporal hint (bit 1 of the policy operand on gfx950)
+// streams them past the L2, which keeps its capacity for the halo rows and
+// planes that neighbouring tiles re-read.  Grids whose state stays resident
+// in the caches from one step to the next keep plain accesses (NT = false).
+constexpr int kNonTemporal = 2;
 
-// The same hint on plain pointer accesses.
-template <typename T>
+template <bool NT, typename T>
 __device__ __forceinline__ T ld_once(const T *p) {
-#if CPML_NT_POLICY
-  return __builtin_nontemporal_load(p);
-#else
-  return *p;
-#endif
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
 }
-template <typename T>
+template <bool NT, typename T>
 __device__ __forceinline__ void st_once(T *p, T x) {
-#if CPML_NT_POLICY
-  __builtin_nontemporal_store(x, p);
-#else
-  *p = x;
-#endif
+  if constexpr (NT) __builtin_nontemporal_store(x, p);
+  else *p = x;
 }
 
 template <>
@@ -98,13 +91,13 @@ struct BufOp<float> {
   __device__ static void st(float x, __amdgpu_buffer_rsrc_t r, unsigned v, unsigned s) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, v, s, 0);
   }
-  __device__ static float ldn(__amdgpu_buffer_rsrc_t r, unsigned v, unsigned s) {
-    return __builtin_bit_cast(float,
-                              __builtin_amdgcn_raw_buffer_load_b32(r, v, s, CPML_NT_POLICY));
+  template <int POL>
+  __device__ static float ldp(__amdgpu_buffer_rsrc_t r, unsigned v, unsigned s) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, v, s, POL));
   }
-  __device__ static void stn(float x, __amdgpu_buffer_rsrc_t r, unsigned v, unsigned s) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, v, s,
-                                          CPML_NT_POLICY);
+  template <int POL>
+  __device__ static void stp(float x, __amdgpu_buffer_rsrc_t r, unsigned v, unsigned s) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, v, s, POL);
   }
 };
 
@@ -117,12 +110,13 @@ struct BufOp<double> {
   __device__ static void st(double x, __amdgpu_buffer_rsrc_t r, unsigned v, unsigned s) {
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, v, s, 0);
   }
-  __device__ static double ldn(__amdgpu_buffer_rsrc_t r, unsigned v, unsigned s) {
-    return __builtin_bit_cast(double,
-                              __builtin_amdgcn_raw_buffer_load_b64(r, v, s, CPML_NT_POLICY));
+  template <int POL>
+  __device__ static double ldp(__amdgpu_buffer_rsrc_t r, unsigned v, unsigned s) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, v, s, POL));
   }
-  __device__ static void stn(double x, __amdgpu_buffer_rsrc_t r, unsigned v, unsigned s) {
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, v, s, CPML_NT_POLICY);
+  template <int POL>
+  __device__ static void stp(double x, __amdgpu_buffer_rsrc_t r, unsigned v, unsigned s) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, v, s, POL);
   }
 };
 

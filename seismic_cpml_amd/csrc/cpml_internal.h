@@ -33,6 +33,11 @@ constexpr double kHolberg2 = -1.041182e-1;
 constexpr double kHolberg3 = 2.063707e-2;
 constexpr double kHolberg4 = -3.570998e-3;
 
+// State size (bytes) above which the launchers pick the kernels with
+// non-temporal own-point accesses: the 256 MiB last-level (infinity) cache of
+// the MI355X.  Smaller states stay cache-resident from one step to the next.
+constexpr double kStreamingBytes = 256.0 * 1024 * 1024;
+
 struct Slab {
   int wl = 0, wr = 0, w = 0;  // widths at the min and max side, total
   int n = 0;                  // grid extent along the axis
@@ -87,6 +92,7 @@ struct Step2D {
   const T *fx, *fy;
   const int *step;  // current step it (1-based), advanced by the record kernel
   int tile_y;       // rows per block (4, 8 or 16)
+  int streaming;    // state larger than the caches: non-temporal own-point accesses
 };
 
 template <typename T>
@@ -115,6 +121,7 @@ struct Step3D {
   const int *step;
   int kchunk;  // planes per block along z
   int tile_y;  // rows per block (4, 8 or 16)
+  int streaming;  // state larger than the caches: non-temporal own-point accesses
 };
 
 // Fused 3D step: reads the state in `a`, writes the next state here.

@@ -47,22 +47,22 @@ __device__ __forceinline__ void tile2d(int nx, int ny, int &i, int &j) {
 
 // A tile is 64 points by TY*RG rows; the block (64 x TY threads) walks its RG
 // row groups, so each wavefront lives for several rows.
-template <typename T, int ORDER, int TY, int RG>
+template <typename T, int ORDER, int TY, int RG, bool NT>
 __global__ __launch_bounds__(BX *TY) void k_stress2d(Step2D<T> p) {
   int i, j;
   tile2d<TY, RG>(p.nx, p.ny, i, j);
 #pragma unroll
   for (int r = 0; r < RG; ++r)
-    if (i < p.nx && j + r * TY < p.ny) stress2d_point<T, ORDER>(p, i, j + r * TY);
+    if (i < p.nx && j + r * TY < p.ny) stress2d_point<T, ORDER, NT>(p, i, j + r * TY);
 }
 
-template <typename T, int ORDER, int TY, int RG>
+template <typename T, int ORDER, int TY, int RG, bool NT>
 __global__ __launch_bounds__(BX *TY) void k_velocity2d(Step2D<T> p) {
   int i, j;
   tile2d<TY, RG>(p.nx, p.ny, i, j);
 #pragma unroll
   for (int r = 0; r < RG; ++r)
-    if (i < p.nx && j + r * TY < p.ny) velocity2d_point<T, ORDER>(p, i, j + r * TY);
+    if (i < p.nx && j + r * TY < p.ny) velocity2d_point<T, ORDER, NT>(p, i, j + r * TY);
 }
 
 // ---------------------------------------------------------------- 3D ----
@@ -129,9 +129,12 @@ struct Chunk3 {
 
 // load / store of (i+DX, j+dy, k+dz): the x shift goes to the immediate
 #define CPML_LD(r, DX, dy, dz) BufOp<T>::ld(r, g.v0 + (1 + (DX)) * (unsigned)sizeof(T), g.so(kk, dy, dz))
-// loads of values only this point reads, and the stores (non-temporal)
-#define CPML_LDN(r) BufOp<T>::ldn(r, g.v0 + (unsigned)sizeof(T), g.so(kk, 0, 0))
-#define CPML_ST(x, r) BufOp<T>::stn(x, r, g.v0 + (unsigned)sizeof(T), g.so(kk, 0, 0))
+// loads of values only this point reads, and the stores (non-temporal when
+// the state is larger than the caches)
+#define CPML_LDN(r) \
+  BufOp<T>::template ldp<NT ? kNonTemporal : 0>(r, g.v0 + (unsigned)sizeof(T), g.so(kk, 0, 0))
+#define CPML_ST(x, r) \
+  BufOp<T>::template stp<NT ? kNonTemporal : 0>(x, r, g.v0 + (unsigned)sizeof(T), g.so(kk, 0, 0))
 
 // Values the stress kernel reads at one plane.
 template <typename T>
@@ -141,7 +144,7 @@ struct SPlane {
   T cl, cm, sxx, syy, szz, sxy, sxz, syz;
 };
 
-template <typename T, int TY3>
+template <typename T, int TY3, bool NT>
 __global__ __launch_bounds__(BX *TY3) void k_stress3d(Step3D<T> p) {
   const Tile3 t = tile3<TY3>(p.nx, p.ny, p.nz, p.kchunk);
   if (!t.ok) return;
@@ -256,7 +259,7 @@ struct VPlane {
   T cb, vx, vy, vz;
 };
 
-template <typename T, int TY3>
+template <typename T, int TY3, bool NT>
 __global__ __launch_bounds__(BX *TY3) void k_velocity3d(Step3D<T> p) {
   const Tile3 t = tile3<TY3>(p.nx, p.ny, p.nz, p.kchunk);
   if (!t.ok) return;
@@ -521,22 +524,34 @@ __global__ __launch_bounds__(EB) void k_vmax(const T *vx, const T *vy, const T *
 
 // ------------------------------------------------------------ launchers --
 
+template <typename T, int TY, int RG, bool NT>
+static hipError_t stress2d_nt(int order, const Step2D<T> &p, hipStream_t s) {
+  dim3 grid(((p.nx + BX - 1) / BX) * ((p.ny + TY * RG - 1) / (TY * RG))), block(BX, TY);
+  if (order == 2) hipLaunchKernelGGL((k_stress2d<T, 2, TY, RG, NT>), grid, block, 0, s, p);
+  else if (order == 4) hipLaunchKernelGGL((k_stress2d<T, 4, TY, RG, NT>), grid, block, 0, s, p);
+  else hipLaunchKernelGGL((k_stress2d<T, 8, TY, RG, NT>), grid, block, 0, s, p);
+  return hipGetLastError();
+}
+
 template <typename T, int TY, int RG>
 static hipError_t stress2d_ty(int order, const Step2D<T> &p, hipStream_t s) {
+  return p.streaming ? stress2d_nt<T, TY, RG, true>(order, p, s)
+                     : stress2d_nt<T, TY, RG, false>(order, p, s);
+}
+
+template <typename T, int TY, int RG, bool NT>
+static hipError_t velocity2d_nt(int order, const Step2D<T> &p, hipStream_t s) {
   dim3 grid(((p.nx + BX - 1) / BX) * ((p.ny + TY * RG - 1) / (TY * RG))), block(BX, TY);
-  if (order == 2) hipLaunchKernelGGL((k_stress2d<T, 2, TY, RG>), grid, block, 0, s, p);
-  else if (order == 4) hipLaunchKernelGGL((k_stress2d<T, 4, TY, RG>), grid, block, 0, s, p);
-  else hipLaunchKernelGGL((k_stress2d<T, 8, TY, RG>), grid, block, 0, s, p);
+  if (order == 2) hipLaunchKernelGGL((k_velocity2d<T, 2, TY, RG, NT>), grid, block, 0, s, p);
+  else if (order == 4) hipLaunchKernelGGL((k_velocity2d<T, 4, TY, RG, NT>), grid, block, 0, s, p);
+  else hipLaunchKernelGGL((k_velocity2d<T, 8, TY, RG, NT>), grid, block, 0, s, p);
   return hipGetLastError();
 }
 
 template <typename T, int TY, int RG>
 static hipError_t velocity2d_ty(int order, const Step2D<T> &p, hipStream_t s) {
-  dim3 grid(((p.nx + BX - 1) / BX) * ((p.ny + TY * RG - 1) / (TY * RG))), block(BX, TY);
-  if (order == 2) hipLaunchKernelGGL((k_velocity2d<T, 2, TY, RG>), grid, block, 0, s, p);
-  else if (order == 4) hipLaunchKernelGGL((k_velocity2d<T, 4, TY, RG>), grid, block, 0, s, p);
-  else hipLaunchKernelGGL((k_velocity2d<T, 8, TY, RG>), grid, block, 0, s, p);
-  return hipGetLastError();
+  return p.streaming ? velocity2d_nt<T, TY, RG, true>(order, p, s)
+                     : velocity2d_nt<T, TY, RG, false>(order, p, s);
 }
 
 // p.tile_y: rows per tile, 4, 8, 16, or 32 / 64 walked by 16-row blocks
@@ -569,26 +584,36 @@ unsigned ntiles3(const Step3D<T> &p, int ty) {
 }
 
 // p.tile_y selects the tile height (4, 8 or 16 rows of 64 points)
+template <typename T, bool NT>
+static hipError_t stress3d_nt(const Step3D<T> &p, hipStream_t s) {
+  if (p.tile_y == 16)
+    hipLaunchKernelGGL((k_stress3d<T, 16, NT>), dim3(ntiles3(p, 16)), dim3(BX, 16), 0, s, p);
+  else if (p.tile_y == 8)
+    hipLaunchKernelGGL((k_stress3d<T, 8, NT>), dim3(ntiles3(p, 8)), dim3(BX, 8), 0, s, p);
+  else
+    hipLaunchKernelGGL((k_stress3d<T, 4, NT>), dim3(ntiles3(p, 4)), dim3(BX, 4), 0, s, p);
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_stress3d(const Step3D<T> &p, hipStream_t s) {
+  return p.streaming ? stress3d_nt<T, true>(p, s) : stress3d_nt<T, false>(p, s);
+}
+
+template <typename T, bool NT>
+static hipError_t velocity3d_nt(const Step3D<T> &p, hipStream_t s) {
   if (p.tile_y == 16)
-    hipLaunchKernelGGL((k_stress3d<T, 16>), dim3(ntiles3(p, 16)), dim3(BX, 16), 0, s, p);
+    hipLaunchKernelGGL((k_velocity3d<T, 16, NT>), dim3(ntiles3(p, 16)), dim3(BX, 16), 0, s, p);
   else if (p.tile_y == 8)
-    hipLaunchKernelGGL((k_stress3d<T, 8>), dim3(ntiles3(p, 8)), dim3(BX, 8), 0, s, p);
+    hipLaunchKernelGGL((k_velocity3d<T, 8, NT>), dim3(ntiles3(p, 8)), dim3(BX, 8), 0, s, p);
   else
-    hipLaunchKernelGGL((k_stress3d<T, 4>), dim3(ntiles3(p, 4)), dim3(BX, 4), 0, s, p);
+    hipLaunchKernelGGL((k_velocity3d<T, 4, NT>), dim3(ntiles3(p, 4)), dim3(BX, 4), 0, s, p);
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_velocity3d(const Step3D<T> &p, hipStream_t s) {
-  if (p.tile_y == 16)
-    hipLaunchKernelGGL((k_velocity3d<T, 16>), dim3(ntiles3(p, 16)), dim3(BX, 16), 0, s, p);
-  else if (p.tile_y == 8)
-    hipLaunchKernelGGL((k_velocity3d<T, 8>), dim3(ntiles3(p, 8)), dim3(BX, 8), 0, s, p);
-  else
-    hipLaunchKernelGGL((k_velocity3d<T, 4>), dim3(ntiles3(p, 4)), dim3(BX, 4), 0, s, p);
-  return hipGetLastError();
+  return p.streaming ? velocity3d_nt<T, true>(p, s) : velocity3d_nt<T, false>(p, s);
 }
 
 template <typename T>

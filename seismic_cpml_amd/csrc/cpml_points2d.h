@@ -35,7 +35,7 @@ struct Op<T, 8> {
 };
 
 // stress + C-PML at grid point (i, j): 2D2:556-600 (2D4:557-601)
-template <typename T, int ORDER>
+template <typename T, int ORDER, bool NT>
 __device__ __forceinline__ void stress2d_point(const Step2D<T> &p, int i, int j) {
   const long ld = p.ld;
   const long c = i + j * ld;
@@ -51,9 +51,9 @@ __device__ __forceinline__ void stress2d_point(const Step2D<T> &p, int i, int j)
     T dvy_dy = Op<T, ORDER>::d(p.vy, c - ld, ld) / p.den_y;
     if (inx) dvx_dx = cpml(dvx_dx, p.mx[0] + mxo, p.px.ah[i], p.px.bh[i], p.px.Kh[i]);
     if (iny) dvy_dy = cpml(dvy_dy, p.my[0] + myo, p.py.a[j], p.py.b[j], p.py.K[j]);
-    // own-point values: streamed past the L2 (ld_once / st_once)
-    st_once(p.sxx + c, ld_once(p.sxx + c) + (lp2m * dvx_dx + lhx * dvy_dy) * p.dt);
-    st_once(p.syy + c, ld_once(p.syy + c) + (lhx * dvx_dx + lp2m * dvy_dy) * p.dt);
+    // own-point values (ld_once / st_once: streamed past the L2 when NT)
+    st_once<NT>(p.sxx + c, ld_once<NT>(p.sxx + c) + (lp2m * dvx_dx + lhx * dvy_dy) * p.dt);
+    st_once<NT>(p.syy + c, ld_once<NT>(p.syy + c) + (lhx * dvx_dx + lp2m * dvy_dy) * p.dt);
   }
   // sigma_xy at (i, j+1/2), Fortran i=2..NX, j=1..NY-1
   if (i >= 1 && j <= p.ny - 2) {
@@ -65,20 +65,20 @@ __device__ __forceinline__ void stress2d_point(const Step2D<T> &p, int i, int j)
     if (iny)
       dvx_dy = cpml(dvx_dy, p.my[1] + myo, p.py.ah[j], p.py.bh[j],
                     ORDER == 4 ? p.py.K[j] : p.py.Kh[j]);
-    st_once(p.sxy + c, ld_once(p.sxy + c) + mhy * (dvy_dx + dvx_dy) * p.dt);
+    st_once<NT>(p.sxy + c, ld_once<NT>(p.sxy + c) + mhy * (dvy_dx + dvx_dy) * p.dt);
   }
 }
 
 // velocity + C-PML, source and rigid edges at grid point (i, j):
 // 2D2:606-680 (2D4:607-681)
-template <typename T, int ORDER>
+template <typename T, int ORDER, bool NT>
 __device__ __forceinline__ void velocity2d_point(const Step2D<T> &p, int i, int j) {
   const long ld = p.ld;
   const long c = i + j * ld;
   const bool inx = p.sx.in(i), iny = p.sy.in(j);
   const long mxo = p.sx.s(i) + (long)p.sx.w * j;
   const long myo = i + p.mld * p.sy.s(j);
-  T vxn = ld_once(p.vx + c), vyn = ld_once(p.vy + c);
+  T vxn = ld_once<NT>(p.vx + c), vyn = ld_once<NT>(p.vy + c);
   // vx at (i, j), Fortran i=2..NX, j=2..NY
   if (i >= 1 && j >= 1) {
     T dsxx_dx = Op<T, ORDER>::d(p.sxx, c - 1, 1) / p.den_x;
@@ -108,8 +108,8 @@ __device__ __forceinline__ void velocity2d_point(const Step2D<T> &p, int i, int 
     vxn = T(0);
     vyn = T(0);
   }
-  st_once(p.vx + c, vxn);
-  st_once(p.vy + c, vyn);
+  st_once<NT>(p.vx + c, vxn);
+  st_once<NT>(p.vy + c, vyn);
 }
 
 }  // namespace cpml
