@@ -277,7 +277,26 @@ int cpml_abc_grad(cpml_abc_net *net, cpml_solver *student, cpml_solver *teacher,
 int cpml_abc_correction(cpml_abc_net *net, cpml_solver *h, int use_mfma, float *out);
 int cpml_abc_last_error(cpml_abc_net *net, char *buf, int len);
 
+/* ---- PNM snapshots (2D; 2D2:716-731, 861-999) --------------------------- */
+
+/* Colour image of the current Vx (field_number 1) or Vy (2), computed on the
+ * device as the reference's create_color_image does (2D2:861-999): pixel
+ * values from v / max|v| (power 0.3, red positive, blue negative, white below
+ * 1% of the maximum), the source cross, a two-pixel black frame, the inner
+ * edges of the PML layers in use and the receiver squares, in the handle's
+ * precision of arithmetic.  rgb receives NX*NY*3 bytes, top row (j = NY)
+ * first; for order 4, (NX+2)*(NY+2)*3 bytes: the 4th-order program images its
+ * ghost-padded arrays (2D4:729-731).  Replaces the image half of
+ * 2D2:716-731. */
+int cpml_gpu_color_image(cpml_solver *h, int field_number, unsigned char *rgb);
+
 /* ---- host-side set-up helpers (no GPU needed) -------------------------- */
+
+/* Write rgb (NX*NY*3 bytes, top row first) as the reference's PNM P3 text
+ * file (2D2:900-903, 989: 'P3', ' NX NY', ' 255', one 'RRR GGG BBB' line per
+ * pixel). */
+int cpml_pnm_write(const char *path, int nx, int ny, const unsigned char *rgb);
+
 
 /* C-PML profile along one axis (2D2:283-394 for x; 2D2:396-465 for y;
  * 3D:399-667): d0 = -(NPOWER+1) cp ln(Rcoef) / (2 L), NPOWER = 2,

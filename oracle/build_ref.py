@@ -15,7 +15,7 @@ Patches applied to each variant (all textual, on a temporary copy):
   (the reference's own instruction, 2D2:131-133);
 * optional: medium read from ``model.bin`` instead of the homogeneous loop
   (2D2:468-474), Ricker source (the commented line 2D2:654), image output
-  disabled (2D2:728-731);
+  disabled (2D2:728-731) except in the variant that pins the images;
 * a binary dump written after the time loop (profiles, seismograms, energy,
   final wavefields and memory variables) so results can be compared at full
   precision -- the reference itself only writes single-precision ASCII.
@@ -72,6 +72,13 @@ for _p in ("f64", "f32"):
                                             ghost=1, model=True, ricker=True)
     VARIANTS[f"ref3d_small_{_p}"] = dict(src=SRC_3D, prec=_p, params=_SMALL_3D, ndim=3,
                                          ghost=0, nproc=2)
+# 2D2 and 2D4 as shipped with their PNM snapshots left on (2D2:716-731,
+# 861-999; 2D4:729-731), 200 steps: images of Vx and Vy at steps 5, 100, 200.
+for _p in ("f64", "f32"):
+    VARIANTS[f"ref2d2_images_{_p}"] = dict(src=SRC_2D2, prec=_p, params={"NSTEP": "200"},
+                                           ndim=2, ghost=0, images=True)
+    VARIANTS[f"ref2d4_images_{_p}"] = dict(src=SRC_2D4, prec=_p, params={"NSTEP": "200"},
+                                           ndim=2, ghost=1, images=True)
 
 
 def _set_params(text: str, params: dict) -> str:
@@ -166,7 +173,8 @@ def patch(text: str, v: dict) -> str:
             raise RuntimeError("source lines not found")
         text = text.replace(_FDGAUSS, "! " + _FDGAUSS.lstrip())
         text = text.replace(_RICKER_C, _RICKER_C[2:].rjust(len(_RICKER_C)))
-    text = text.replace("call create_color_image(", "if (.false.) call create_color_image(")
+    if not v.get("images"):
+        text = text.replace("call create_color_image(", "if (.false.) call create_color_image(")
     lines = text.split("\n")
     mark = next(n for n, l in enumerate(lines) if "end of time loop" in l)
     if "enddo" not in lines[mark]:

@@ -23,6 +23,7 @@ from .solver import (  # noqa: E402,F401
     nearest_receivers,
     pml_profile,
     source_time_function,
+    write_pnm,
 )
 
 __all__ = ["CpmlAbcNet", "CpmlBatch", "CpmlError", "CpmlGpu", "courant_number", "library", "nearest_receivers", "no_absorption",

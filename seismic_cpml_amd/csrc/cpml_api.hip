@@ -126,6 +126,7 @@ struct cpml_solver {
   bool fused = false;   // 3D: one fused kernel per step (ping-pong state)
   void *media[6] = {};  // lambda mu rho, and in 3D: DELTAT*lambda DELTAT*mu DELTAT/rho
   void *prof = nullptr; // [3 axes][6][n]
+  unsigned char *img = nullptr;  // PNM snapshot pixels (2D), allocated on first use
   void *mem[18] = {};
   void *mem2[18] = {};
   size_t mem_elems[18] = {};
@@ -1254,6 +1255,55 @@ int cpml_gpu_get_memory(cpml_solver *h, int which, void *out) {
                       tmp.data() + src * es, es);
       }
   return CPML_OK;
+}
+
+int cpml_gpu_color_image(cpml_solver *h, int field_number, unsigned char *rgb) {
+  if (!h || !rgb) return CPML_ERR_ARG;
+  if (h->ndim != 2) return fail(h, CPML_ERR_ARG, "PNM snapshots are 2D only");
+  if (field_number != 1 && field_number != 2)
+    return fail(h, CPML_ERR_ARG, "field_number must be 1 (Vx) or 2 (Vy)");
+  int rc = set_device(h);
+  if (rc) return rc;
+  const int gh = h->order == 4 ? 1 : 0;  // 2D4:729-731 images the ghost-padded arrays
+  const size_t bytes = (size_t)(h->nx + 2 * gh) * (h->ny + 2 * gh) * 3;
+  if (!h->img && (rc = dev_alloc(h, (void **)&h->img, bytes))) return rc;
+  cpml::Image g{};
+  g.ghost = gh;
+  g.isrc = h->isrc + 1;
+  g.jsrc = h->jsrc + 1;
+  g.npml = h->npml;
+  // a layer is drawn where the profiles absorb (USE_PML_XMIN ... 2D2:206-209)
+  g.use[0] = h->sx.wl > 0;
+  g.use[1] = h->sx.wr > 0;
+  g.use[2] = h->sy.wl > 0;
+  g.use[3] = h->sy.wr > 0;
+  unsigned long long *amax = h->scratch_max + 1;
+  HIPCHK(h, hipMemsetAsync(amax, 0, sizeof(unsigned long long), h->stream));
+  const int q = field_number == 1 ? CPML_VX : CPML_VY;
+  if (h->prec == 4)
+    HIPCHK(h, cpml::launch_color_image<float>(F<float>(h, q), h->nx, h->ny, h->ld, amax, g,
+                                              h->recidx, h->nrec, h->img, h->stream));
+  else
+    HIPCHK(h, cpml::launch_color_image<double>(F<double>(h, q), h->nx, h->ny, h->ld, amax, g,
+                                               h->recidx, h->nrec, h->img, h->stream));
+  HIPCHK(h, hipMemcpyAsync(rgb, h->img, bytes, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return CPML_OK;
+}
+
+int cpml_pnm_write(const char *path, int nx, int ny, const unsigned char *rgb) {
+  if (!path || !rgb || nx < 1 || ny < 1) return CPML_ERR_ARG;
+  FILE *fp = std::fopen(path, "w");
+  if (!fp) return CPML_ERR_ARG;
+  // the reference's text (2D2:900-903, 989): 'P3', list-directed NX NY and
+  // '255' (one leading blank each, as amdflang writes them), then one
+  // (i3,' ',i3,' ',i3) line per pixel
+  std::fprintf(fp, "P3\n %d %d\n 255\n", nx, ny);
+  const size_t n = (size_t)nx * ny;
+  for (size_t p = 0; p < n; ++p)
+    std::fprintf(fp, "%3d %3d %3d\n", rgb[3 * p], rgb[3 * p + 1], rgb[3 * p + 2]);
+  const bool ok = std::ferror(fp) == 0;
+  return std::fclose(fp) == 0 && ok ? CPML_OK : CPML_ERR_ARG;
 }
 
 int cpml_gpu_velocity_norm(cpml_solver *h, double *vmax) {

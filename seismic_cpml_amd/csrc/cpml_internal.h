@@ -199,6 +199,16 @@ template <typename T>
 hipError_t launch_energy(const Energy<T> &p, int nblocks, hipStream_t s);
 template <typename T>
 int energy_blocks(const Energy<T> &p);
+// PNM snapshot parameters (Fortran indices): source, PML thickness, the
+// layers in use (x min, x max, y min, y max), ghost border (1: the image is
+// the ghost-padded array of the 4th-order program).
+struct Image {
+  int isrc, jsrc, npml, use[4], ghost;
+};
+template <typename T>
+hipError_t launch_color_image(const T *f, int nx, int ny, long ld, unsigned long long *amax,
+                              const Image &g, const int *recidx, int nrec, unsigned char *rgb,
+                              hipStream_t s);
 template <typename T>
 hipError_t launch_vmax(const T *vx, const T *vy, const T *vz, int nx, int ny, int nz, long ld,
                        long ps, unsigned long long *out, hipStream_t s);

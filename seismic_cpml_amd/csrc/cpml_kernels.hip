@@ -648,6 +648,91 @@ hipError_t launch_vmax(const T *vx, const T *vy, const T *vz, int nx, int ny, in
   return hipGetLastError();
 }
 
+// ------------------------------------------ PNM snapshot (2D2:861-999) --
+
+// max |f| over the interior (as non-negative double bits, for atomicMax)
+template <typename T>
+__global__ __launch_bounds__(EB) void k_absmax(const T *f, int nx, int ny, long ld,
+                                               unsigned long long *out) {
+  const long n = (long)nx * ny;
+  double m = 0.0;
+  for (long t = blockIdx.x * (long)EB + threadIdx.x; t < n; t += (long)gridDim.x * EB) {
+    const double d = (double)fabs(f[(t % nx) + (t / nx) * ld]);
+    m = d > m ? d : m;
+  }
+  __shared__ double sm[EB];
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = EB / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s && sm[threadIdx.x + s] > sm[threadIdx.x])
+      sm[threadIdx.x] = sm[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(out, (unsigned long long)__double_as_longlong(sm[0]));
+}
+
+// One pixel per thread, top row first; the first matching rule wins, then a
+// receiver square paints over it.  Arithmetic in T (the single-precision
+// build of the reference computes the image in real(4)).  With g.ghost = 1
+// the image is the ghost-padded (NX+2) x (NY+2) array of 2D4:729-731: pixel
+// ix shows grid point ix - 1 (the zero halo at the border), while the source,
+// receivers and PML lines keep their interior indices, as there.
+template <typename T>
+__global__ __launch_bounds__(EB) void k_color_image(const T *f, int gnx, int gny, long ld,
+                                                    const unsigned long long *amax_bits,
+                                                    Image g, const int *recidx, int nrec,
+                                                    unsigned char *rgb) {
+  const int nx = gnx + 2 * g.ghost, ny = gny + 2 * g.ghost;  // image size
+  const long q = blockIdx.x * (long)EB + threadIdx.x;
+  if (q >= (long)nx * ny) return;
+  const int ix = (int)(q % nx) + 1, iy = ny - (int)(q / nx);  // Fortran indices
+  const T v = f[(ix - 1 - g.ghost) + (long)(iy - 1 - g.ghost) * ld];
+  const T amax = (T)__longlong_as_double((long long)*amax_bits);
+  T nv = v / amax;
+  if (nv < T(-1)) nv = T(-1);
+  if (nv > T(1)) nv = T(1);
+  int R, G, B;
+  const int WC = 5, TC = 1, SQ = 3;  // cross half-width, half-thickness; square half-size
+  if ((ix >= g.isrc - WC && ix <= g.isrc + WC && iy >= g.jsrc - TC && iy <= g.jsrc + TC) ||
+      (ix >= g.isrc - TC && ix <= g.isrc + TC && iy >= g.jsrc - WC && iy <= g.jsrc + WC)) {
+    R = 255, G = 157, B = 0;
+  } else if (ix <= 2 || ix >= nx - 1 || iy <= 2 || iy >= ny - 1) {
+    R = 0, G = 0, B = 0;
+  } else if ((g.use[0] && ix == g.npml) || (g.use[1] && ix == nx - g.npml) ||
+             (g.use[2] && iy == g.npml) || (g.use[3] && iy == ny - g.npml)) {
+    R = 255, G = 150, B = 0;
+  } else if (fabs(v) <= amax * T(0.01)) {
+    R = 255, G = 255, B = 255;
+  } else {
+    const T x = T(255) * pow(fabs(nv), T(0.30));
+    const T fl = floor(x);
+    const int c = (int)fl + (x - fl >= T(0.5) ? 1 : 0);  // nint, x >= 0
+    if (nv >= T(0)) R = c, G = 0, B = 0;
+    else R = 0, G = 0, B = c;
+  }
+  for (int r = 0; r < nrec; ++r) {
+    const int xr = (int)(recidx[r] % ld) + 1, yr = (int)(recidx[r] / ld) + 1;
+    if (ix >= xr - SQ && ix <= xr + SQ && iy >= yr - SQ && iy <= yr + SQ) R = 30, G = 180, B = 60;
+  }
+  rgb[3 * q] = (unsigned char)R;
+  rgb[3 * q + 1] = (unsigned char)G;
+  rgb[3 * q + 2] = (unsigned char)B;
+}
+
+template <typename T>
+hipError_t launch_color_image(const T *f, int nx, int ny, long ld, unsigned long long *amax,
+                              const Image &g, const int *recidx, int nrec, unsigned char *rgb,
+                              hipStream_t s) {
+  const long n = (long)nx * ny;
+  const long b = (n + EB - 1) / EB;
+  const long bi = ((long)(nx + 2 * g.ghost) * (ny + 2 * g.ghost) + EB - 1) / EB;
+  hipLaunchKernelGGL(k_absmax<T>, dim3((unsigned)(b > 2048 ? 2048 : b)), dim3(EB), 0, s, f, nx,
+                     ny, ld, amax);
+  hipLaunchKernelGGL(k_color_image<T>, dim3((unsigned)bi), dim3(EB), 0, s, f, nx, ny, ld, amax,
+                     g, recidx, nrec, rgb);
+  return hipGetLastError();
+}
+
 #define CPML_INSTANTIATE(T)                                                                     \
   template hipError_t launch_stress2d<T>(int, const Step2D<T> &, hipStream_t);                \
   template hipError_t launch_velocity2d<T>(int, const Step2D<T> &, hipStream_t);              \
@@ -657,7 +742,10 @@ hipError_t launch_vmax(const T *vx, const T *vy, const T *vz, int nx, int ny, in
   template hipError_t launch_energy<T>(const Energy<T> &, int, hipStream_t);                  \
   template int energy_blocks<T>(const Energy<T> &);                                           \
   template hipError_t launch_vmax<T>(const T *, const T *, const T *, int, int, int, long, long, \
-                                     unsigned long long *, hipStream_t);
+                                     unsigned long long *, hipStream_t);                      \
+  template hipError_t launch_color_image<T>(const T *, int, int, long, unsigned long long *,    \
+                                            const Image &, const int *, int, unsigned char *,  \
+                                            hipStream_t);
 
 CPML_INSTANTIATE(float)
 CPML_INSTANTIATE(double)

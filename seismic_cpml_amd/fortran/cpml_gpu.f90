@@ -29,7 +29,7 @@ module cpml_gpu
        cpml_gpu_set_pml_3d, cpml_gpu_set_source, cpml_gpu_set_receivers, cpml_gpu_set_energy, &
        cpml_gpu_set_stability, cpml_gpu_run, cpml_gpu_get_seismograms, cpml_gpu_get_energy, &
        cpml_gpu_get_field, cpml_gpu_set_field, cpml_gpu_velocity_norm, cpml_gpu_elapsed_ms, &
-       cpml_gpu_error_message
+       cpml_gpu_error_message, cpml_gpu_create_color_image
   ! many 2D shots in one launch per phase (no reference counterpart: a survey
   ! is a loop of reference runs)
   public :: cpml_gpu_batch_init, cpml_gpu_batch_finalize, cpml_gpu_batch_set_medium, &
@@ -123,6 +123,18 @@ module cpml_gpu
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: h
       real(c_double), intent(out) :: v
+    end function
+    integer(c_int) function c_color_image(h, fnum, rgb) bind(C, name="cpml_gpu_color_image")
+      import :: c_ptr, c_int, c_int8_t
+      type(c_ptr), value :: h
+      integer(c_int), value :: fnum
+      integer(c_int8_t), dimension(*), intent(out) :: rgb
+    end function
+    integer(c_int) function c_pnm_write(path, nx, ny, rgb) bind(C, name="cpml_pnm_write")
+      import :: c_int, c_int8_t, c_char
+      character(kind=c_char), dimension(*), intent(in) :: path
+      integer(c_int), value :: nx, ny
+      integer(c_int8_t), dimension(*), intent(in) :: rgb
     end function
     integer(c_int) function c_elapsed_ms(h, v) bind(C, name="cpml_gpu_elapsed_ms")
       import :: c_ptr, c_int, c_double
@@ -359,6 +371,30 @@ contains
     double precision, intent(out) :: velocnorm
     integer, intent(out) :: ierr
     ierr = c_velocity_norm(h, velocnorm)
+  end subroutine
+
+  ! create_color_image of 2D2:861-999 (2D4:729-731 for order 4, ghost-padded):
+  ! the image of Vx (field_number 1) or Vy (2) is computed on the device and
+  ! written as the reference's file image<it>_V[xy].pnm (2D2:890-896)
+  subroutine cpml_gpu_create_color_image(h, NX, NY, order, it, field_number, ierr)
+    type(c_ptr), intent(in) :: h
+    integer, intent(in) :: NX, NY, order, it, field_number
+    integer, intent(out) :: ierr
+    integer(c_int8_t), allocatable :: rgb(:)
+    character(len=100) :: file_name
+    integer :: g
+    g = 0
+    if (order == 4) g = 1
+    allocate(rgb(3 * (NX + 2*g) * (NY + 2*g)))
+    ierr = c_color_image(h, field_number, rgb)
+    if (ierr /= 0) return
+    if (field_number == 1) then
+      write(file_name, "('image',i6.6,'_Vx.pnm')") it
+    else
+      write(file_name, "('image',i6.6,'_Vy.pnm')") it
+    endif
+    ierr = c_pnm_write(trim(file_name) // c_null_char, NX + 2*g, NY + 2*g, rgb)
+    deallocate(rgb)
   end subroutine
 
   subroutine cpml_gpu_elapsed_ms(h, ms, ierr)

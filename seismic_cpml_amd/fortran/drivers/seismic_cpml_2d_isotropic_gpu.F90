@@ -7,8 +7,9 @@
 ! WP=8 (double precision, as shipped) and WP=4 (the single-precision form
 ! the reference recommends at 2D2:131-133).
 !
-! Outputs: Vx_file_NNN.dat, Vy_file_NNN.dat, energy.dat (reference formats)
-! and gpu_dump.bin (seismograms, energy and final vx, vy at full precision).
+! Outputs: Vx_file_NNN.dat, Vy_file_NNN.dat, energy.dat, imageNNNNNN_V[xy].pnm
+! (reference formats) and gpu_dump.bin (seismograms, energy and final vx, vy
+! at full precision).
 #ifndef WP
 #define WP 8
 #endif
@@ -126,6 +127,11 @@ program seismic_cpml_2d_isotropic_gpu
       print *, 'Time: ', real((it_end-1)*DELTAT, 4), ' seconds'
       print *, 'Max norm velocity vector V (m/s) = ', velocnorm
       print *, 'total energy = ', total_energy_kinetic(it_end) + total_energy_potential(it_end)
+      ! PNM snapshots of Vx and Vy (2D2:728-731), computed on the device
+      call cpml_gpu_create_color_image(h, NX, NY, ORDER, it_end, 1, ierr)
+      call check(h, ierr, 'color_image')
+      call cpml_gpu_create_color_image(h, NX, NY, ORDER, it_end, 2, ierr)
+      call check(h, ierr, 'color_image')
     endif
     it = it_end + 1
   enddo

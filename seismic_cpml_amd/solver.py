@@ -55,6 +55,8 @@ SIGNATURES = {
     "cpml_gpu_set_field": [_VP, _I, _VP],
     "cpml_gpu_get_memory": [_VP, _I, _VP],
     "cpml_gpu_velocity_norm": [_VP, _DP],
+    "cpml_gpu_color_image": [_VP, C.c_int, C.c_void_p],
+    "cpml_pnm_write": [C.c_char_p, C.c_int, C.c_int, C.c_void_p],
     "cpml_gpu_set_graph_steps": [_VP, _I],
     "cpml_gpu_set_kchunk": [_VP, _I],
     "cpml_gpu_set_tile": [_VP, _I],
@@ -119,6 +121,16 @@ def library():
             f.restype = C.c_int
         _lib = lib
     return _lib
+
+
+def write_pnm(path, rgb):
+    """Write rgb (rows, columns, 3) uint8, top row first, as the reference's
+    PNM P3 text file (2D2:900-903, 989)."""
+    rgb = np.ascontiguousarray(rgb, np.uint8)
+    ny, nx = rgb.shape[:2]
+    rc = library().cpml_pnm_write(str(path).encode(), nx, ny, _ptr(rgb))
+    if rc:
+        raise CpmlError(rc, f"cannot write {path}")
 
 
 def _ptr(a):
@@ -332,6 +344,15 @@ class CpmlGpu:
         v = C.c_double()
         self._call("cpml_gpu_velocity_norm", C.byref(v))
         return v.value
+
+    def color_image(self, field_number):
+        """PNM snapshot of Vx (1) or Vy (2), computed on the device
+        (2D2:861-999): uint8 (rows, columns, 3), top row first; order 4 gives
+        the ghost-padded (NY+2, NX+2) image of 2D4:729-731."""
+        g = 1 if self.order == 4 else 0
+        out = np.zeros((self.ny + 2 * g, self.nx + 2 * g, 3), np.uint8)
+        self._call("cpml_gpu_color_image", int(field_number), _ptr(out))
+        return out
 
     def kernel_time(self, kernel):
         n, ms = C.c_int(), C.c_double()

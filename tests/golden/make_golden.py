@@ -2,6 +2,7 @@
 reference programs (built by oracle/build_ref.py from /root/reference).
 
     python tests/golden/make_golden.py            # all cases, both precisions
+    python tests/golden/make_golden.py images     # only the 2D2 PNM snapshots
 
 Each fixture ``<case>_<prec>.npz`` holds the reference's own outputs at full
 precision: damping profiles, receiver indices, seismograms, energy curve(s),
@@ -9,6 +10,8 @@ the SHA-256 digest of each final wavefield with a regular subsample of it
 (every 4th point per axis in 2D, every 3rd in 3D), and for each C-PML memory array the masks of x/y/z
 lines on which it is non-zero (the arrays themselves are not stored).  For the
 layered cases the medium (rho, mu, lambda) fed to the program is stored too.
+``images_<2d2|2d4>_<prec>.npz`` hold the PNM snapshots (header and pixels)
+that the shipped 2D2 and 2D4 programs write at steps 5, 100 and 200.
 This script needs /root/reference and amdflang; the fixtures it writes do not.
 """
 from __future__ import annotations
@@ -75,13 +78,58 @@ def make(case: str, prec: str, workroot: Path) -> Path:
     return path
 
 
+IMAGE_STEPS = (5, 100, 200)
+
+
+def read_pnm(path: Path):
+    """Header lines (verbatim) and pixels (rows top to bottom, RGB uint8) of a
+    P3 image."""
+    lines = path.read_text().split("\n")
+    head = "\n".join(lines[:3]) + "\n"
+    nx, ny = (int(t) for t in lines[1].split())
+    px = np.array([[int(t) for t in l.split()] for l in lines[3:3 + nx * ny]], np.uint8)
+    return head, px.reshape(ny, nx, 3)
+
+
+def make_images(workroot: Path, prec: str = "f64", prog: str = "2d2") -> Path:
+    """images_<prog>_<prec>.npz: the PNM snapshots of the shipped 2D2 or 2D4
+    program (200 steps) -- header text and pixels of Vx and Vy at steps 5,
+    100, 200."""
+    wd = workroot / f"ref{prog}_images_{prec}"
+    build_ref.run(f"ref{prog}_images_{prec}", wd)
+    save = {}
+    for it in IMAGE_STEPS:
+        if not (wd / f"image{it:06d}_Vx.pnm").is_file():
+            continue  # 2D4 displays every 200 steps (2D4:187)
+        for comp in ("Vx", "Vy"):
+            head, px = read_pnm(wd / f"image{it:06d}_{comp}.pnm")
+            save[f"{comp}_{it}"] = px
+            save["header"] = np.array(head)
+    path = HERE / f"images_{prog}_{prec}.npz"
+    np.savez_compressed(path, **save)
+    return path
+
+
 def main(argv):
+    imgs = [(g, p) for g in ("2d2", "2d4") for p in ("f64", "f32")]
+    if argv == ["images"]:
+        build_ref.build([f"ref{g}_images_{p}" for g, p in imgs])
+        with tempfile.TemporaryDirectory(prefix="cpml_golden_") as tmp:
+            for g, p in imgs:
+                path = make_images(Path(tmp), p, g)
+                print(f"{path.name}: {path.stat().st_size / 1024:.0f} KiB")
+        return
     names = argv or list(CASES)
     build_ref.build([f"{VARIANT_OF[n]}_{p}" for n in names for p in ("f64", "f32")])
     with tempfile.TemporaryDirectory(prefix="cpml_golden_") as tmp:
         for n in names:
             for p in ("f64", "f32"):
                 path = make(n, p, Path(tmp))
+                print(f"{path.name}: {path.stat().st_size / 1024:.0f} KiB")
+        if not argv:
+            build_ref.build([f"ref{g}_images_{p}" for g, p in imgs])
+            for g, p in imgs:
+                path = make_images(Path(tmp), p, g)
                 print(f"{path.name}: {path.stat().st_size / 1024:.0f} KiB")
 
 

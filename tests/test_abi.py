@@ -158,3 +158,15 @@ def test_batch_without_gpu_reports_nodevice():
         sc.CpmlBatch(2, 8, [dict(NX=32, NY=32, DELTAX=10.0, DELTAY=10.0, DELTAT=1e-3,
                                  NPOINTS_PML=4)], 10)
     assert e.value.code == solver.CPML_ERR_NODEVICE
+
+
+def test_pnm_writer_matches_reference_file_format(tmp_path):
+    """cpml_pnm_write (host only) writes the reference's P3 text: the header
+    and pixel lines of tests/golden/images_2d2_f64.npz byte for byte."""
+    from pathlib import Path
+    from seismic_cpml_amd import write_pnm
+    g = np.load(Path(__file__).resolve().parent / "golden" / "images_2d2_f64.npz")
+    px = g["Vx_100"]
+    write_pnm(tmp_path / "a.pnm", px)
+    want = str(g["header"]) + "".join(f"{r:3d} {gg:3d} {b:3d}\n" for r, gg, b in px.reshape(-1, 3))
+    assert (tmp_path / "a.pnm").read_text() == want

@@ -4,7 +4,7 @@ compiled reference's outputs.
 The 2D drivers run the shipped configurations of 2D2 and 2D4; their
 seismograms must equal the reference's bit-for-bit in both precisions and
 their ASCII Vx_file_001.dat must match the reference's single-precision
-values.  The reduced 3D build matches the 3D golden case within the 3D
+values, and their PNM snapshots the reference's image files byte for byte.  The reduced 3D build matches the 3D golden case within the 3D
 tolerances (1e-12 float64, 1e-5 float32 rel-L2).
 """
 import subprocess
@@ -49,12 +49,26 @@ def read_dump(path, dt, nstep, nrec, nplane):
     return out
 
 
+def check_images(tmp_path, order, prec):
+    """The driver's PNM snapshots at the steps the reference's images cover
+    (tests/golden/images_2d[24]_<prec>.npz): the same file bytes."""
+    g = np.load(Path(__file__).resolve().parent / "golden" / f"images_2d{order}_{prec}.npz")
+    steps = sorted({int(k.split("_")[1]) for k in g.files if k.startswith("Vx_")})
+    for it in steps:
+        for comp in ("Vx", "Vy"):
+            text = (tmp_path / f"image{it:06d}_{comp}.pnm").read_text()
+            px = g[f"{comp}_{it}"].reshape(-1, 3)
+            want = str(g["header"]) + "".join(f"{r:3d} {gg:3d} {b:3d}\n" for r, gg, b in px)
+            assert text == want, f"image {it} {comp}"
+
+
 @pytest.mark.parametrize("order,prec", [(2, "f64"), (2, "f32"), (4, "f64"), (4, "f32")])
 def test_2d_driver_matches_reference(order, prec, tmp_path):
     name = f"seismic_cpml_2d_{'second' if order == 2 else 'fourth'}_{prec}"
     case = f"2d{order}_shipped"
     out = run_driver(name, tmp_path)
     assert "Max norm velocity vector" in out
+    check_images(tmp_path, order, prec)
     f = fixture(case, prec)
     dt = np.float64 if prec == "f64" else np.float32
     nstep = 2000 if order == 2 else 4000

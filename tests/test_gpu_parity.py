@@ -253,3 +253,24 @@ def test_3d_interior_box_tilings_agree(kchunk, tile):
     g = run_gpu(case, prec, d=d, energy=False, kchunk=kchunk, tile=tile)
     for k in _fields(case) + ("sisvx", "sisvy"):
         np.testing.assert_array_equal(g[k], ref[k], err_msg=k)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("prog,steps", [("2d2", (5, 100, 200)), ("2d4", (5, 200))])
+def test_gpu_images_match_reference(prog, steps, prec):
+    """PNM snapshots computed on the device (cpml_gpu_color_image) against the
+    reference's image files (tests/golden/images_<prog>_<prec>.npz): every
+    pixel equal; the 4th-order program images its ghost-padded arrays."""
+    from pathlib import Path
+    from gpu_runner import make_solver
+    g = np.load(Path(__file__).resolve().parent / "golden" / f"images_{prog}_{prec}.npz")
+    case = f"{prog}_shipped"
+    d = inputs(case, prec, max(steps))
+    with make_solver(case, prec, d, energy=False) as s:
+        done = 0
+        for it in steps:
+            s.run(done + 1, it)
+            done = it
+            for comp, fn in (("Vx", 1), ("Vy", 2)):
+                np.testing.assert_array_equal(s.color_image(fn), g[f"{comp}_{it}"],
+                                              err_msg=f"{comp} step {it}")
