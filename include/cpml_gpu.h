@@ -277,7 +277,7 @@ int cpml_abc_grad(cpml_abc_net *net, cpml_solver *student, cpml_solver *teacher,
 int cpml_abc_correction(cpml_abc_net *net, cpml_solver *h, int use_mfma, float *out);
 int cpml_abc_last_error(cpml_abc_net *net, char *buf, int len);
 
-/* ---- PNM snapshots (2D; 2D2:716-731, 861-999) --------------------------- */
+/* ---- PNM snapshots (2D2:716-731, 861-999; 3D:1231-1234) ----------------- */
 
 /* Colour image of the current Vx (field_number 1) or Vy (2), computed on the
  * device as the reference's create_color_image does (2D2:861-999): pixel
@@ -286,8 +286,10 @@ int cpml_abc_last_error(cpml_abc_net *net, char *buf, int len);
  * edges of the PML layers in use and the receiver squares, in the handle's
  * precision of arithmetic.  rgb receives NX*NY*3 bytes, top row (j = NY)
  * first; for order 4, (NX+2)*(NY+2)*3 bytes: the 4th-order program images its
- * ghost-padded arrays (2D4:729-731).  Replaces the image half of
- * 2D2:716-731. */
+ * ghost-padded arrays (2D4:729-731).  3D handles image the plane k = NZ/2
+ * (the top plane of the reference's cut-plane rank, 3D:346, 1231-1234), with
+ * every receiver drawn at its (i, j).  Replaces the image half of
+ * 2D2:716-731 and 3D:1231-1234. */
 int cpml_gpu_color_image(cpml_solver *h, int field_number, unsigned char *rgb);
 
 /* ---- host-side set-up helpers (no GPU needed) -------------------------- */
@@ -296,6 +298,19 @@ int cpml_gpu_color_image(cpml_solver *h, int field_number, unsigned char *rgb);
  * file (2D2:900-903, 989: 'P3', ' NX NY', ' 255', one 'RRR GGG BBB' line per
  * pixel). */
 int cpml_pnm_write(const char *path, int nx, int ny, const unsigned char *rgb);
+
+/* Write ntr traces of ns samples as the SU file of the 8th-order program
+ * (ADE8:987-1003 for seismograms, ADE8:935-953 for snapshots): per trace a
+ * 240-byte header of 120 native int16 words, all zero but word 58 = ns and
+ * word 59 = dt_value truncated toward zero, then the ns samples as native
+ * float32.  traces: ntr blocks of ns values (the Fortran array
+ * traces(ns, ntr)), float (prec 4) or double (prec 8, rounded to float).
+ * dt_value is what the header carries: DELTAT*1e6 (microseconds) for
+ * seismograms, DELTAY*1e3 for snapshots.  The reference stores both words
+ * through integer(2) assignments; values outside 0..32767 (ns >= 1) are
+ * refused here (CPML_ERR_ARG) rather than written wrapped. */
+int cpml_su_write(const char *path, int ntr, int ns, double dt_value, const void *traces,
+                  int prec);
 
 
 /* C-PML profile along one axis (2D2:283-394 for x; 2D2:396-465 for y;

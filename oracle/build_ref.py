@@ -18,7 +18,8 @@ Patches applied to each variant (all textual, on a temporary copy):
   disabled (2D2:728-731) except in the variant that pins the images;
 * a binary dump written after the time loop (profiles, seismograms, energy,
   final wavefields and memory variables) so results can be compared at full
-  precision -- the reference itself only writes single-precision ASCII.
+  precision -- the reference itself only writes single-precision ASCII.  The
+  ADE8 variant has no dump: it runs only for the SU files it writes.
 
 The 3D program is run with 2 MPI ranks (MPICH from /opt/conda), the smallest
 decomposition it accepts (3D:381-394).
@@ -44,6 +45,7 @@ MPI_INC = Path("/opt/conda/include")
 MPI_LIB = Path("/opt/conda/lib")
 
 SRC_2D2 = "seismic_CPML_2D_isotropic_second_order.f90"
+SRC_ADE8 = "seismic_ADEPML_2D_elastic_RK4_eighth_order.f90"
 SRC_2D4 = "seismic_CPML_2D_isotropic_fourth_order.f90"
 SRC_3D = "seismic_CPML_3D_isotropic_MPI_OpenMP.f90"
 
@@ -79,6 +81,13 @@ for _p in ("f64", "f32"):
                                            ndim=2, ghost=0, images=True)
     VARIANTS[f"ref2d4_images_{_p}"] = dict(src=SRC_2D4, prec=_p, params={"NSTEP": "200"},
                                            ndim=2, ghost=1, images=True)
+    VARIANTS[f"ref3d_images_{_p}"] = dict(src=SRC_3D, prec=_p, params=_SMALL_3D, ndim=3,
+                                          ghost=0, images=True)
+# ADE8 on a 61 x 61 grid for 60 steps, snapshots every 30: run only for the SU
+# files it writes (seismograms ADE8:987-1003, snapshots ADE8:935-953); no dump.
+VARIANTS["refade8_su_f64"] = dict(src=SRC_ADE8, prec="f64", ndim=2, ghost=0, dump=False,
+                                  params={"NX": "61", "NY": "61", "NSTEP": "60",
+                                          "IT_DISPLAY": "30"})
 
 
 def _set_params(text: str, params: dict) -> str:
@@ -175,6 +184,8 @@ def patch(text: str, v: dict) -> str:
         text = text.replace(_RICKER_C, _RICKER_C[2:].rjust(len(_RICKER_C)))
     if not v.get("images"):
         text = text.replace("call create_color_image(", "if (.false.) call create_color_image(")
+    if not v.get("dump", True):
+        return _to_single(text) if v["prec"] == "f32" else text
     lines = text.split("\n")
     mark = next(n for n, l in enumerate(lines) if "end of time loop" in l)
     if "enddo" not in lines[mark]:
@@ -245,6 +256,15 @@ def run(name: str, workdir: Path, model: tuple | None = None, timeout: int = 600
         raise RuntimeError(f"{exe} not built")
     workdir = Path(workdir)
     workdir.mkdir(parents=True, exist_ok=True)
+    if not v.get("dump", True):  # ADE8 writes into these folders (ADE8:942, 994)
+        for sub in ("seismograms", "snapshots"):
+            (workdir / sub).mkdir(exist_ok=True)
+        with open(workdir / "stdout.txt", "w") as log:
+            r = subprocess.run([str(exe)], cwd=workdir, stdout=log, stderr=subprocess.STDOUT,
+                               timeout=timeout)
+        if r.returncode != 0:
+            raise RuntimeError(f"{name} exited with {r.returncode}; see {workdir}/stdout.txt")
+        return {}
     dt = np.float64 if v["prec"] == "f64" else np.float32
     ints = _shipped_ints(v)
     nx, ny, nstep, nrec = ints["NX"], ints["NY"], ints["NSTEP"], ints["NREC"]

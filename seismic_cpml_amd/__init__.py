@@ -24,7 +24,9 @@ from .solver import (  # noqa: E402,F401
     pml_profile,
     source_time_function,
     write_pnm,
+    write_su,
 )
 
-__all__ = ["CpmlAbcNet", "CpmlBatch", "CpmlError", "CpmlGpu", "courant_number", "library", "nearest_receivers", "no_absorption",
-           "pml_profile", "source_time_function"]
+__all__ = ["CpmlAbcNet", "CpmlBatch", "CpmlError", "CpmlGpu", "courant_number", "library",
+           "nearest_receivers", "no_absorption", "pml_profile", "source_time_function",
+           "write_pnm", "write_su"]

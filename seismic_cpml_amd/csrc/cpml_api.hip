@@ -1259,7 +1259,6 @@ int cpml_gpu_get_memory(cpml_solver *h, int which, void *out) {
 
 int cpml_gpu_color_image(cpml_solver *h, int field_number, unsigned char *rgb) {
   if (!h || !rgb) return CPML_ERR_ARG;
-  if (h->ndim != 2) return fail(h, CPML_ERR_ARG, "PNM snapshots are 2D only");
   if (field_number != 1 && field_number != 2)
     return fail(h, CPML_ERR_ARG, "field_number must be 1 (Vx) or 2 (Vy)");
   int rc = set_device(h);
@@ -1277,15 +1276,19 @@ int cpml_gpu_color_image(cpml_solver *h, int field_number, unsigned char *rgb) {
   g.use[1] = h->sx.wr > 0;
   g.use[2] = h->sy.wl > 0;
   g.use[3] = h->sy.wr > 0;
+  g.ps = h->ps;
+  // 3D: plane k = NZ/2, the top plane of the reference's cut-plane rank
+  // (3D:346 rank_cut_plane = nb_procs/2 - 1, images of vx(:,:,NZ_LOCAL) 3D:1231-1234)
+  const long plane = h->ndim == 3 ? (long)(h->nz / 2 - 1) * h->ps : 0;
   unsigned long long *amax = h->scratch_max + 1;
   HIPCHK(h, hipMemsetAsync(amax, 0, sizeof(unsigned long long), h->stream));
   const int q = field_number == 1 ? CPML_VX : CPML_VY;
   if (h->prec == 4)
-    HIPCHK(h, cpml::launch_color_image<float>(F<float>(h, q), h->nx, h->ny, h->ld, amax, g,
-                                              h->recidx, h->nrec, h->img, h->stream));
+    HIPCHK(h, cpml::launch_color_image<float>(F<float>(h, q) + plane, h->nx, h->ny, h->ld, amax,
+                                              g, h->recidx, h->nrec, h->img, h->stream));
   else
-    HIPCHK(h, cpml::launch_color_image<double>(F<double>(h, q), h->nx, h->ny, h->ld, amax, g,
-                                               h->recidx, h->nrec, h->img, h->stream));
+    HIPCHK(h, cpml::launch_color_image<double>(F<double>(h, q) + plane, h->nx, h->ny, h->ld,
+                                               amax, g, h->recidx, h->nrec, h->img, h->stream));
   HIPCHK(h, hipMemcpyAsync(rgb, h->img, bytes, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return CPML_OK;
@@ -1303,6 +1306,33 @@ int cpml_pnm_write(const char *path, int nx, int ny, const unsigned char *rgb) {
   for (size_t p = 0; p < n; ++p)
     std::fprintf(fp, "%3d %3d %3d\n", rgb[3 * p], rgb[3 * p + 1], rgb[3 * p + 2]);
   const bool ok = std::ferror(fp) == 0;
+  return std::fclose(fp) == 0 && ok ? CPML_OK : CPML_ERR_ARG;
+}
+
+int cpml_su_write(const char *path, int ntr, int ns, double dt_value, const void *traces,
+                  int prec) {
+  if (!path || !traces || ntr < 1 || ns < 1 || ns > 32767 || (prec != 4 && prec != 8))
+    return CPML_ERR_ARG;
+  const double dtw = std::trunc(dt_value);
+  if (!(dtw >= 0 && dtw <= 32767)) return CPML_ERR_ARG;
+  short head[120] = {};
+  head[57] = (short)ns;   // head(58) = nstep (ADE8:992) or NY (ADE8:938)
+  head[58] = (short)dtw;  // head(59) = deltat*1e6 (ADE8:993) or DELTAY*1E3 (ADE8:939)
+  FILE *fp = std::fopen(path, "wb");
+  if (!fp) return CPML_ERR_ARG;
+  std::vector<float> tr(ns);
+  bool ok = true;
+  for (int j = 0; j < ntr && ok; ++j) {
+    if (prec == 4) {
+      std::memcpy(tr.data(), static_cast<const float *>(traces) + (size_t)j * ns,
+                  (size_t)ns * sizeof(float));
+    } else {
+      const double *src = static_cast<const double *>(traces) + (size_t)j * ns;
+      for (int k = 0; k < ns; ++k) tr[k] = (float)src[k];  // real(): nearest
+    }
+    ok = std::fwrite(head, sizeof(head), 1, fp) == 1 &&
+         std::fwrite(tr.data(), sizeof(float), ns, fp) == (size_t)ns;
+  }
   return std::fclose(fp) == 0 && ok ? CPML_OK : CPML_ERR_ARG;
 }
 

@@ -204,6 +204,7 @@ int energy_blocks(const Energy<T> &p);
 // the ghost-padded array of the 4th-order program).
 struct Image {
   int isrc, jsrc, npml, use[4], ghost;
+  long ps;  // plane stride of the receiver offsets
 };
 template <typename T>
 hipError_t launch_color_image(const T *f, int nx, int ny, long ld, unsigned long long *amax,

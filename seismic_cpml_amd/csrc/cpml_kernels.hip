@@ -676,7 +676,8 @@ __global__ __launch_bounds__(EB) void k_absmax(const T *f, int nx, int ny, long 
 // build of the reference computes the image in real(4)).  With g.ghost = 1
 // the image is the ghost-padded (NX+2) x (NY+2) array of 2D4:729-731: pixel
 // ix shows grid point ix - 1 (the zero halo at the border), while the source,
-// receivers and PML lines keep their interior indices, as there.
+// receivers and PML lines keep their interior indices, as there.  Receivers
+// are drawn at their (x, y) whatever their plane (g.ps = plane stride).
 template <typename T>
 __global__ __launch_bounds__(EB) void k_color_image(const T *f, int gnx, int gny, long ld,
                                                     const unsigned long long *amax_bits,
@@ -711,7 +712,8 @@ __global__ __launch_bounds__(EB) void k_color_image(const T *f, int gnx, int gny
     else R = 0, G = 0, B = c;
   }
   for (int r = 0; r < nrec; ++r) {
-    const int xr = (int)(recidx[r] % ld) + 1, yr = (int)(recidx[r] / ld) + 1;
+    const long rq = recidx[r] % g.ps;  // offset within the receiver's plane
+    const int xr = (int)(rq % ld) + 1, yr = (int)(rq / ld) + 1;
     if (ix >= xr - SQ && ix <= xr + SQ && iy >= yr - SQ && iy <= yr + SQ) R = 30, G = 180, B = 60;
   }
   rgb[3 * q] = (unsigned char)R;

@@ -29,7 +29,7 @@ module cpml_gpu
        cpml_gpu_set_pml_3d, cpml_gpu_set_source, cpml_gpu_set_receivers, cpml_gpu_set_energy, &
        cpml_gpu_set_stability, cpml_gpu_run, cpml_gpu_get_seismograms, cpml_gpu_get_energy, &
        cpml_gpu_get_field, cpml_gpu_set_field, cpml_gpu_velocity_norm, cpml_gpu_elapsed_ms, &
-       cpml_gpu_error_message, cpml_gpu_create_color_image
+       cpml_gpu_error_message, cpml_gpu_create_color_image, cpml_gpu_write_su
   ! many 2D shots in one launch per phase (no reference counterpart: a survey
   ! is a loop of reference runs)
   public :: cpml_gpu_batch_init, cpml_gpu_batch_finalize, cpml_gpu_batch_set_medium, &
@@ -135,6 +135,14 @@ module cpml_gpu
       character(kind=c_char), dimension(*), intent(in) :: path
       integer(c_int), value :: nx, ny
       integer(c_int8_t), dimension(*), intent(in) :: rgb
+    end function
+    integer(c_int) function c_su_write(path, ntr, ns, dt_value, traces, prec) &
+         bind(C, name="cpml_su_write")
+      import :: c_int, c_char, c_double
+      character(kind=c_char), dimension(*), intent(in) :: path
+      integer(c_int), value :: ntr, ns, prec
+      real(c_double), value :: dt_value
+      type(*), dimension(*), intent(in) :: traces
     end function
     integer(c_int) function c_elapsed_ms(h, v) bind(C, name="cpml_gpu_elapsed_ms")
       import :: c_ptr, c_int, c_double
@@ -373,9 +381,10 @@ contains
     ierr = c_velocity_norm(h, velocnorm)
   end subroutine
 
-  ! create_color_image of 2D2:861-999 (2D4:729-731 for order 4, ghost-padded):
-  ! the image of Vx (field_number 1) or Vy (2) is computed on the device and
-  ! written as the reference's file image<it>_V[xy].pnm (2D2:890-896)
+  ! create_color_image of 2D2:861-999 (2D4:729-731 for order 4, ghost-padded;
+  ! 3D:1231-1234 for a 3D handle, the plane k = NZ/2): the image of Vx
+  ! (field_number 1) or Vy (2) is computed on the device and written as the
+  ! reference's file image<it>_V[xy].pnm (2D2:890-896)
   subroutine cpml_gpu_create_color_image(h, NX, NY, order, it, field_number, ierr)
     type(c_ptr), intent(in) :: h
     integer, intent(in) :: NX, NY, order, it, field_number
@@ -395,6 +404,18 @@ contains
     endif
     ierr = c_pnm_write(trim(file_name) // c_null_char, NX + 2*g, NY + 2*g, rgb)
     deallocate(rgb)
+  end subroutine
+
+  ! SU file of ADE8:987-1003 (seismograms: dt_value = DELTAT*1e6) and
+  ! ADE8:935-953 (snapshots: dt_value = DELTAY*1e3): traces(ns, ntr) of kind
+  ! prec (4 or 8), each written as float32 after a 120-word int16 header
+  subroutine cpml_gpu_write_su(file_name, traces, ns, ntr, prec, dt_value, ierr)
+    character(len=*), intent(in) :: file_name
+    type(*), dimension(*), intent(in) :: traces
+    integer, intent(in) :: ns, ntr, prec
+    double precision, intent(in) :: dt_value
+    integer, intent(out) :: ierr
+    ierr = c_su_write(trim(file_name) // c_null_char, ntr, ns, dt_value, traces, prec)
   end subroutine
 
   subroutine cpml_gpu_elapsed_ms(h, ms, ierr)

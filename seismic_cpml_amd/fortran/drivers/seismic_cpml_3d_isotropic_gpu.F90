@@ -7,8 +7,10 @@
 ! by cpml_gpu calls.  Built for WP=8 and WP=4; the grid and step count can be
 ! overridden at compile time (NX_, NY_, NZ_, NSTEP_, NPML_, YDEB_, YFIN_).
 !
-! Outputs: Vx_file_NNN.dat, Vy_file_NNN.dat, energy.dat (one column of total
-! energy as 3D:1253-1257) and gpu_dump.bin (full precision).
+! Outputs: at each display step the timestamp file timestampNNNNNN
+! (3D:1219-1229) and the PNM snapshots of Vx and Vy on the cut plane
+! (3D:1231-1234); at the end Vx_file_NNN.dat, Vy_file_NNN.dat, energy.dat (one
+! column of total energy as 3D:1253-1257) and gpu_dump.bin (full precision).
 #ifndef WP
 #define WP 8
 #endif
@@ -80,6 +82,11 @@ program seismic_cpml_3d_isotropic_gpu
   type(c_ptr) :: h
   integer :: it, it_end, ierr
   double precision :: velocnorm, courant_number, ms
+  integer(8) :: clock_start, clock_now, clock_rate
+  double precision :: tCPU
+  integer :: int_tCPU, ihours, iminutes, iseconds
+  character(len=100) :: outputname
+  integer, parameter :: IOUT = 41
 
   print *, '3D elastic finite-difference code in velocity and stress formulation with C-PML'
   print *, 'time loop on one GPU (cpml_gpu), real kind ', wp
@@ -124,6 +131,7 @@ program seismic_cpml_3d_isotropic_gpu
   call cpml_gpu_set_stability(h, IT_DISPLAY, STABILITY_THRESHOLD, ierr)
   call check(h, ierr, 'set_stability')
 
+  call system_clock(clock_start, clock_rate)
   it = 1
   do while (it <= NSTEP)
     it_end = (it + IT_DISPLAY - 1) / IT_DISPLAY * IT_DISPLAY
@@ -138,6 +146,35 @@ program seismic_cpml_3d_isotropic_gpu
       print *, 'Time: ', real((it_end-1)*DELTAT, 4), ' seconds'
       print *, 'Max norm velocity vector V (m/s) = ', velocnorm
       print *, 'Total energy = ', ekin(it_end) + epot(it_end)
+      ! elapsed wall-clock time and the progress file (3D:1196-1229)
+      call system_clock(clock_now)
+      tCPU = dble(clock_now - clock_start) / dble(clock_rate)
+      int_tCPU = int(tCPU)
+      ihours = int_tCPU / 3600
+      iminutes = (int_tCPU - 3600*ihours) / 60
+      iseconds = int_tCPU - 3600*ihours - 60*iminutes
+      write(*,*) 'Elapsed time in seconds = ', real(tCPU, wp)
+      write(*,"(' Elapsed time in hh:mm:ss = ',i4,' h ',i2.2,' m ',i2.2,' s')") &
+           ihours, iminutes, iseconds
+      write(*,*) 'Mean elapsed time per time step in seconds = ', real(tCPU / dble(it_end), wp)
+      write(*,*)
+      write(outputname, "('timestamp',i6.6)") it_end
+      open(unit=IOUT, file=outputname, status='unknown')
+      write(IOUT,*) 'Time step # ', it_end
+      write(IOUT,*) 'Time: ', real((it_end-1)*DELTAT, 4), ' seconds'
+      write(IOUT,*) 'Max norm velocity vector V (m/s) = ', real(velocnorm, wp)
+      write(IOUT,*) 'Total energy = ', real(ekin(it_end) + epot(it_end), wp)
+      write(IOUT,*) 'Elapsed time in seconds = ', real(tCPU, wp)
+      write(IOUT,"(' Elapsed time in hh:mm:ss = ',i4,' h ',i2.2,' m ',i2.2,' s')") &
+           ihours, iminutes, iseconds
+      write(IOUT,*) 'Mean elapsed time per time step in seconds = ', &
+           real(tCPU / dble(it_end), wp)
+      close(IOUT)
+      ! snapshots of the cut plane (3D:1231-1234)
+      call cpml_gpu_create_color_image(h, NX, NY, 2, it_end, 1, ierr)
+      call check(h, ierr, 'create_color_image')
+      call cpml_gpu_create_color_image(h, NX, NY, 2, it_end, 2, ierr)
+      call check(h, ierr, 'create_color_image')
     endif
     it = it_end + 1
   enddo

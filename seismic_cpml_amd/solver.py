@@ -57,6 +57,7 @@ SIGNATURES = {
     "cpml_gpu_velocity_norm": [_VP, _DP],
     "cpml_gpu_color_image": [_VP, C.c_int, C.c_void_p],
     "cpml_pnm_write": [C.c_char_p, C.c_int, C.c_int, C.c_void_p],
+    "cpml_su_write": [C.c_char_p, _I, _I, C.c_double, C.c_void_p, _I],
     "cpml_gpu_set_graph_steps": [_VP, _I],
     "cpml_gpu_set_kchunk": [_VP, _I],
     "cpml_gpu_set_tile": [_VP, _I],
@@ -131,6 +132,20 @@ def write_pnm(path, rgb):
     rc = library().cpml_pnm_write(str(path).encode(), nx, ny, _ptr(rgb))
     if rc:
         raise CpmlError(rc, f"cannot write {path}")
+
+
+def write_su(path, traces, dt_value):
+    """Write traces (ntr, ns) float32/float64 as the 8th-order program's SU
+    file (ADE8:987-1003, 935-953): header word 58 = ns, word 59 =
+    int(dt_value) -- DELTAT*1e6 for seismograms, DELTAY*1e3 for snapshots."""
+    a = np.asarray(traces)
+    a = np.ascontiguousarray(a, np.float64 if a.dtype == np.float64 else np.float32)
+    if a.ndim != 2:
+        raise ValueError("traces must be (ntr, ns)")
+    rc = library().cpml_su_write(str(path).encode(), a.shape[0], a.shape[1], float(dt_value),
+                                 _ptr(a), a.itemsize)
+    if rc:
+        raise CpmlError(rc, f"cannot write {path} (ns and the dt word must be 0..32767)")
 
 
 def _ptr(a):
@@ -348,7 +363,8 @@ class CpmlGpu:
     def color_image(self, field_number):
         """PNM snapshot of Vx (1) or Vy (2), computed on the device
         (2D2:861-999): uint8 (rows, columns, 3), top row first; order 4 gives
-        the ghost-padded (NY+2, NX+2) image of 2D4:729-731."""
+        the ghost-padded (NY+2, NX+2) image of 2D4:729-731, a 3D grid the
+        image of its plane k = NZ/2 (3D:1231-1234)."""
         g = 1 if self.order == 4 else 0
         out = np.zeros((self.ny + 2 * g, self.nx + 2 * g, 3), np.uint8)
         self._call("cpml_gpu_color_image", int(field_number), _ptr(out))

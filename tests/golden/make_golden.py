@@ -2,7 +2,9 @@
 reference programs (built by oracle/build_ref.py from /root/reference).
 
     python tests/golden/make_golden.py            # all cases, both precisions
-    python tests/golden/make_golden.py images     # only the 2D2 PNM snapshots
+    python tests/golden/make_golden.py images     # only the 2D PNM snapshots
+    python tests/golden/make_golden.py display3d  # only the 3D display-step files
+    python tests/golden/make_golden.py su         # only the ADE8 SU files
 
 Each fixture ``<case>_<prec>.npz`` holds the reference's own outputs at full
 precision: damping profiles, receiver indices, seismograms, energy curve(s),
@@ -11,7 +13,10 @@ the SHA-256 digest of each final wavefield with a regular subsample of it
 lines on which it is non-zero (the arrays themselves are not stored).  For the
 layered cases the medium (rho, mu, lambda) fed to the program is stored too.
 ``images_<2d2|2d4>_<prec>.npz`` hold the PNM snapshots (header and pixels)
-that the shipped 2D2 and 2D4 programs write at steps 5, 100 and 200.
+that the shipped 2D2 and 2D4 programs write at steps 5, 100 and 200;
+``display_3d_<prec>.npz`` the timestamp files and cut-plane PNM snapshots of
+the reduced 3D program at its display steps; ``su_ade8.npz`` the SU
+seismogram and snapshot files of a reduced run of the 8th-order program.
 This script needs /root/reference and amdflang; the fixtures it writes do not.
 """
 from __future__ import annotations
@@ -110,13 +115,70 @@ def make_images(workroot: Path, prec: str = "f64", prog: str = "2d2") -> Path:
     return path
 
 
+DISPLAY_STEPS_3D = (5, 100, 200, 300)
+
+
+def make_display3d(workroot: Path, prec: str = "f64") -> Path:
+    """display_3d_<prec>.npz: what the reduced 3D program (the 3d_small
+    configuration, 2 ranks) writes at its display steps 5, 100, 200, 300
+    (3D:1183-1234) -- the text of each timestamp file, the PNM snapshots of
+    Vx and Vy on the cut plane, and the final Vx and Vy of that plane
+    (global k = NZ/2), which is what the step-300 images show."""
+    wd = workroot / f"ref3d_images_{prec}"
+    out = build_ref.run(f"ref3d_images_{prec}", wd)
+    save = {}
+    for it in DISPLAY_STEPS_3D:
+        save[f"timestamp_{it}"] = np.array((wd / f"timestamp{it:06d}").read_text())
+        for comp in ("Vx", "Vy"):
+            head, px = read_pnm(wd / f"image{it:06d}_{comp}.pnm")
+            save[f"{comp}_{it}"] = px
+            save["header"] = np.array(head)
+    k = out["vx"].shape[0] // 2 - 1
+    save["vx_plane"] = out["vx"][k]
+    save["vy_plane"] = out["vy"][k]
+    save["ix_rec"], save["iy_rec"] = out["ix_rec"], out["iy_rec"]
+    path = HERE / f"display_3d_{prec}.npz"
+    np.savez_compressed(path, **save)
+    return path
+
+
+SU_FILES = ("seismograms/seisVx.su", "seismograms/seisVy.su", "snapshots/00030snapVx.su",
+            "snapshots/00060snapVy.su")
+
+
+def make_su(workroot: Path) -> Path:
+    """su_ade8.npz: the SU files (raw bytes) that the 8th-order ADE-PML program
+    writes on a 61 x 61 grid in 60 steps -- its two seismogram files
+    (ADE8:987-1003) and a snapshot of each component (ADE8:935-953)."""
+    wd = workroot / "refade8_su_f64"
+    build_ref.run("refade8_su_f64", wd)
+    save = {Path(f).name.replace(".su", ""): np.frombuffer((wd / f).read_bytes(), np.uint8)
+            for f in SU_FILES}
+    path = HERE / "su_ade8.npz"
+    np.savez_compressed(path, **save)
+    return path
+
+
 def main(argv):
+    if argv == ["su"]:
+        build_ref.build(["refade8_su_f64"])
+        with tempfile.TemporaryDirectory(prefix="cpml_golden_") as tmp:
+            path = make_su(Path(tmp))
+            print(f"{path.name}: {path.stat().st_size / 1024:.0f} KiB")
+        return
     imgs = [(g, p) for g in ("2d2", "2d4") for p in ("f64", "f32")]
     if argv == ["images"]:
         build_ref.build([f"ref{g}_images_{p}" for g, p in imgs])
         with tempfile.TemporaryDirectory(prefix="cpml_golden_") as tmp:
             for g, p in imgs:
                 path = make_images(Path(tmp), p, g)
+                print(f"{path.name}: {path.stat().st_size / 1024:.0f} KiB")
+        return
+    if argv == ["display3d"]:
+        build_ref.build([f"ref3d_images_{p}" for p in ("f64", "f32")])
+        with tempfile.TemporaryDirectory(prefix="cpml_golden_") as tmp:
+            for p in ("f64", "f32"):
+                path = make_display3d(Path(tmp), p)
                 print(f"{path.name}: {path.stat().st_size / 1024:.0f} KiB")
         return
     names = argv or list(CASES)
@@ -131,6 +193,13 @@ def main(argv):
             for g, p in imgs:
                 path = make_images(Path(tmp), p, g)
                 print(f"{path.name}: {path.stat().st_size / 1024:.0f} KiB")
+            build_ref.build([f"ref3d_images_{p}" for p in ("f64", "f32")])
+            for p in ("f64", "f32"):
+                path = make_display3d(Path(tmp), p)
+                print(f"{path.name}: {path.stat().st_size / 1024:.0f} KiB")
+            build_ref.build(["refade8_su_f64"])
+            path = make_su(Path(tmp))
+            print(f"{path.name}: {path.stat().st_size / 1024:.0f} KiB")
 
 
 if __name__ == "__main__":

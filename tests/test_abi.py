@@ -170,3 +170,45 @@ def test_pnm_writer_matches_reference_file_format(tmp_path):
     write_pnm(tmp_path / "a.pnm", px)
     want = str(g["header"]) + "".join(f"{r:3d} {gg:3d} {b:3d}\n" for r, gg, b in px.reshape(-1, 3))
     assert (tmp_path / "a.pnm").read_text() == want
+
+
+def _su_traces(raw, ns):
+    """(header words, traces) of an SU file with ns samples per trace."""
+    rec = np.frombuffer(raw, np.uint8).reshape(-1, 240 + 4 * ns)
+    return rec[:, :240].copy().view(np.int16), rec[:, 240:].copy().view(np.float32)
+
+
+@pytest.mark.parametrize("name,ns,dt_value", [
+    ("seisVx", 60, 3.e-3 * 1e6), ("seisVy", 60, 3.e-3 * 1e6),
+    ("00030snapVx", 61, 10. * 1e3), ("00060snapVy", 61, 10. * 1e3)])
+def test_su_writer_matches_reference_files(name, ns, dt_value, tmp_path):
+    """cpml_su_write (host only) against the SU files the 8th-order program
+    wrote (tests/golden/su_ade8.npz, a 61 x 61 run of 60 steps): its traces,
+    written again with the reference's header values (ns; DELTAT*1e6 for
+    seismograms, DELTAY*1e3 for snapshots), give the same bytes, from float32
+    traces and from float64 ones (which round to the same floats)."""
+    from seismic_cpml_amd import write_su
+    raw = np.load(Path(__file__).resolve().parent / "golden" / "su_ade8.npz")[name].tobytes()
+    head, tr = _su_traces(raw, ns)
+    assert tr.shape[0] == 61 and (head[:, 57] == ns).all()
+    assert np.count_nonzero(head) == 2 * 61  # only words 58 and 59 are set
+    assert np.abs(tr).max() > 0
+    for a in (tr, tr.astype(np.float64)):
+        write_su(tmp_path / "a.su", a, dt_value)
+        assert (tmp_path / "a.su").read_bytes() == raw
+
+
+def test_su_writer_rounds_and_refuses(tmp_path):
+    """float64 samples are rounded to nearest float32 (Fortran real()); the dt
+    word is truncated toward zero (integer(2) assignment); an ns or dt word
+    outside 0..32767 is refused rather than wrapped."""
+    from seismic_cpml_amd import write_su
+    x = np.array([[1 + 2.0 ** -24 + 2.0 ** -40, -1e-30, 3.0]])  # just above a float tie
+    write_su(tmp_path / "b.su", x, 2999.9)
+    head, tr = _su_traces((tmp_path / "b.su").read_bytes(), 3)
+    assert head[0, 58] == 2999 and head[0, 57] == 3
+    np.testing.assert_array_equal(tr[0], x[0].astype(np.float32))
+    assert tr[0, 0] == np.float32(1 + 2.0 ** -23)
+    for bad in ((np.zeros((1, 32768)), 1.0), (x, 32768.0), (x, -1.0)):
+        with pytest.raises(solver.CpmlError):
+            write_su(tmp_path / "c.su", *bad)

@@ -4,8 +4,10 @@ compiled reference's outputs.
 The 2D drivers run the shipped configurations of 2D2 and 2D4; their
 seismograms must equal the reference's bit-for-bit in both precisions and
 their ASCII Vx_file_001.dat must match the reference's single-precision
-values, and their PNM snapshots the reference's image files byte for byte.  The reduced 3D build matches the 3D golden case within the 3D
-tolerances (1e-12 float64, 1e-5 float32 rel-L2).
+values, and their PNM snapshots the reference's image files byte for byte.
+The reduced 3D build matches the 3D golden case within the 3D tolerances
+(1e-12 float64, 1e-5 float32 rel-L2), and its timestamp files and cut-plane
+snapshots the reference's.
 """
 import subprocess
 from pathlib import Path
@@ -97,3 +99,37 @@ def test_3d_driver_matches_reference(prec, tmp_path):
     assert rel_l2(d["sisvx"], f["sisvx"]) < tol
     assert rel_l2(d["sisvy"], f["sisvy"]) < tol
     assert rel_l2(d["ekin"] + d["epot"], f["etotal"]) < (1e-12 if prec == "f64" else 1e-4)
+    check_display_3d(tmp_path, prec)
+
+
+def check_display_3d(tmp_path, prec):
+    """The timestamp files and cut-plane PNM snapshots at the display steps
+    against the reference's (tests/golden/display_3d_<prec>.npz).  Timestamp
+    lines: the labels and the step and time text equal, the velocity norm and
+    energy within the 3D tolerances (the fields match to rounding), the
+    elapsed-time lines in the reference's form.  Images: every pixel equal in
+    f64, at most 0.5% of the pixels in f32 (a value within rounding of a
+    colour-level boundary)."""
+    import re
+    g = np.load(Path(__file__).resolve().parent / "golden" / f"display_3d_{prec}.npz")
+    rtol = 1e-10 if prec == "f64" else 1e-5
+    for it in (5, 100, 200, 300):
+        ours = (tmp_path / f"timestamp{it:06d}").read_text().rstrip("\n").split("\n")
+        ref = str(g[f"timestamp_{it}"]).rstrip("\n").split("\n")
+        assert len(ours) == len(ref) == 7
+        assert ours[:2] == ref[:2], (ours[:2], ref[:2])
+        for n in (2, 3):
+            la, _, va = ours[n].rpartition("=")
+            lb, _, vb = ref[n].rpartition("=")
+            assert la == lb
+            assert abs(float(va) / float(vb) - 1) < (rtol if n == 2 else 10 * rtol), (ours[n], ref[n])
+        for n in (4, 6):
+            assert ours[n].rpartition("=")[0] == ref[n].rpartition("=")[0]
+            float(ours[n].rpartition("=")[2])
+        assert re.fullmatch(r" Elapsed time in hh:mm:ss = [ \d]{4} h \d\d m \d\d s", ours[5])
+        for comp in ("Vx", "Vy"):
+            text = (tmp_path / f"image{it:06d}_{comp}.pnm").read_text()
+            assert text.startswith(str(g["header"]))
+            px = np.array(text.split()[4:], np.int64).reshape(g[f"{comp}_{it}"].shape)
+            bad = (px != g[f"{comp}_{it}"]).any(-1).sum()
+            assert bad <= (0 if prec == "f64" else 0.005 * px.shape[0] * px.shape[1]), (it, comp, bad)

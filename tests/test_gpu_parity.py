@@ -12,12 +12,18 @@ Bars (written here):
     2*DELTAT*mu instead of DELTAT*rho*cp^2) within 1e-12 rel-L2 in float64
     and 1e-5 in float32.
 """
+import sys
+from pathlib import Path
+
 import numpy as np
 import pytest
 
 from case_inputs import fixture, inputs, run_oracle
 from golden.cases import ALL_CASES, CASES
 from gpu_runner import rel_l2, run_gpu
+
+sys.path.insert(0, str(Path(__file__).resolve().parents[1] / "oracle"))
+from color_image import color_image  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -256,15 +262,21 @@ def test_3d_interior_box_tilings_agree(kchunk, tile):
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-@pytest.mark.parametrize("prog,steps", [("2d2", (5, 100, 200)), ("2d4", (5, 200))])
+@pytest.mark.parametrize("prog,steps", [("2d2", (5, 100, 200)), ("2d4", (5, 200)),
+                                        ("3d", (5, 100, 200, 300))])
 def test_gpu_images_match_reference(prog, steps, prec):
     """PNM snapshots computed on the device (cpml_gpu_color_image) against the
-    reference's image files (tests/golden/images_<prog>_<prec>.npz): every
-    pixel equal; the 4th-order program images its ghost-padded arrays."""
-    from pathlib import Path
+    reference's image files (tests/golden/images_<prog>_<prec>.npz, and for 3D
+    the cut-plane images of the reduced program, display_3d_<prec>.npz): every
+    pixel equal; the 4th-order program images its ghost-padded arrays.  The
+    3D fields match the reference to rounding, not bit for bit, so there the
+    device image must also equal the oracle's colouring of the device's own
+    cut plane, and in f32 at most 0.5% of the pixels may differ from the
+    reference's (a value within rounding of a colour-level boundary)."""
     from gpu_runner import make_solver
-    g = np.load(Path(__file__).resolve().parent / "golden" / f"images_{prog}_{prec}.npz")
-    case = f"{prog}_shipped"
+    name = f"display_3d_{prec}.npz" if prog == "3d" else f"images_{prog}_{prec}.npz"
+    g = np.load(Path(__file__).resolve().parent / "golden" / name)
+    case = "3d_small" if prog == "3d" else f"{prog}_shipped"
     d = inputs(case, prec, max(steps))
     with make_solver(case, prec, d, energy=False) as s:
         done = 0
@@ -272,5 +284,15 @@ def test_gpu_images_match_reference(prog, steps, prec):
             s.run(done + 1, it)
             done = it
             for comp, fn in (("Vx", 1), ("Vy", 2)):
-                np.testing.assert_array_equal(s.color_image(fn), g[f"{comp}_{it}"],
+                img = s.color_image(fn)
+                if prog == "3d":
+                    plane = s.field(comp.lower())[s.nz // 2 - 1]
+                    own = color_image(plane, d["ISOURCE"], d["JSOURCE"], d["ix_rec"],
+                                      d["iy_rec"], ALL_CASES[case]["NPML"])
+                    np.testing.assert_array_equal(img, own, err_msg=f"{comp} step {it}")
+                if prog == "3d" and prec == "f32":
+                    bad = (img != g[f"{comp}_{it}"]).any(-1).sum()
+                    assert bad <= 0.005 * img.shape[0] * img.shape[1], (comp, it, bad)
+                    continue
+                np.testing.assert_array_equal(img, g[f"{comp}_{it}"],
                                               err_msg=f"{comp} step {it}")
