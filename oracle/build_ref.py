@@ -48,6 +48,7 @@ SRC_2D2 = "seismic_CPML_2D_isotropic_second_order.f90"
 SRC_ADE8 = "seismic_ADEPML_2D_elastic_RK4_eighth_order.f90"
 SRC_2D4 = "seismic_CPML_2D_isotropic_fourth_order.f90"
 SRC_3D = "seismic_CPML_3D_isotropic_MPI_OpenMP.f90"
+SRC_3DV = "seismic_CPML_3D_viscoelastic_MPI.f90"
 
 # Reduced heterogeneous 2D case: 128 x 96, 5 m, seeded layered medium read
 # from model.bin, Ricker source at the centre, 8 receivers along the top
@@ -83,6 +84,22 @@ for _p in ("f64", "f32"):
                                            ndim=2, ghost=1, images=True)
     VARIANTS[f"ref3d_images_{_p}"] = dict(src=SRC_3D, prec=_p, params=_SMALL_3D, ndim=3,
                                           ghost=0, images=True)
+# The 4th-order 3D program (3D-visco) in its elastic limit: after the
+# attenuation fit, tau_epsilon := tau_sigma, so phi = 0 and Mu_nu = 1 exactly,
+# the memory variables e* stay zero and the update is the elastic one
+# (3D-visco:466-477, 977-1300).  Reduced grid, 2 ranks, receivers moved near
+# the source (the shipped ones lie 500 m and 2260 m away, 3D-visco:824-829).
+_SMALL_3DV = {
+    "NX": "40", "NY": "60", "NZ": "40", "NPROC": "2", "NPOINTS_PML": "8", "NSTEP": "400",
+    "IT_DISPLAY": "100",
+}
+_RECEIVERS_3DV = (("xrec(1)=xsource+500.d0", "xrec(1)=xsource+20.d0"),
+                  ("yrec(1)=ysource+500.d0", "yrec(1)=ysource+40.d0"),
+                  ("yrec(2)=ysource+2260.d0", "yrec(2)=ysource+120.d0"),
+                  ("xrec(3)=xsource+500.d0", "xrec(3)=xsource+20.d0"),
+                  ("yrec(3)=ysource+2260.d0", "yrec(3)=ysource+120.d0"))
+VARIANTS["ref3dv4_elastic_f64"] = dict(src=SRC_3DV, prec="f64", params=_SMALL_3DV, ndim=3,
+                                       ghost=0, nproc=2, visco=True)
 # ADE8 on a 61 x 61 grid for 60 steps, snapshots every 30: run only for the SU
 # files it writes (seismograms ADE8:987-1003, snapshots ADE8:935-953); no dump.
 VARIANTS["refade8_su_f64"] = dict(src=SRC_ADE8, prec="f64", ndim=2, ghost=0, dump=False,
@@ -170,9 +187,60 @@ _DUMP_3D = (
     "  close(77)\n")
 
 
+_DUMP_3DV = (
+    "  write(ref_dumpname,\"('ref_dump_',i3.3,'.bin')\") rank\n"
+    "  open(unit=77,file=ref_dumpname,access='stream',form='unformatted',status='replace')\n"
+    "  write(77) a_x,b_x,K_x,a_x_half,b_x_half,K_x_half\n"
+    "  write(77) a_y,b_y,K_y,a_y_half,b_y_half,K_y_half\n"
+    "  write(77) a_z,b_z,K_z,a_z_half,b_z_half,K_z_half\n"
+    "  write(77) vx(1:NX,1:NY,1:NZ_LOCAL),vy(1:NX,1:NY,1:NZ_LOCAL),vz(1:NX,1:NY,1:NZ_LOCAL)\n"
+    "  write(77) sigmaxx(1:NX,1:NY,1:NZ_LOCAL),sigmayy(1:NX,1:NY,1:NZ_LOCAL)\n"
+    "  write(77) sigmazz(1:NX,1:NY,1:NZ_LOCAL),sigmaxy(1:NX,1:NY,1:NZ_LOCAL)\n"
+    "  write(77) sigmaxz(1:NX,1:NY,1:NZ_LOCAL),sigmayz(1:NX,1:NY,1:NZ_LOCAL)\n"
+    + "".join(f"  write(77) {m}(1:NX,1:NY,1:NZ_LOCAL)\n" for m in (
+        "memory_dvx_dx", "memory_dvx_dy", "memory_dvx_dz", "memory_dvy_dx", "memory_dvy_dy",
+        "memory_dvy_dz", "memory_dvz_dx", "memory_dvz_dy", "memory_dvz_dz",
+        "memory_dsigmaxx_dx", "memory_dsigmayy_dy", "memory_dsigmazz_dz",
+        "memory_dsigmaxy_dx", "memory_dsigmaxy_dy", "memory_dsigmaxz_dx",
+        "memory_dsigmaxz_dz", "memory_dsigmayz_dy", "memory_dsigmayz_dz")) +
+    "  if (rank == rank_cut_plane) then\n"
+    "    write(77) sisvx,sisvy,total_energy\n"
+    "    write(77) ix_rec,iy_rec\n"
+    "  endif\n"
+    "  close(77)\n")
+_ELASTIC_LIMIT = ("  tau_epsilon_nu1 = tau_sigma_nu1\n"
+                  "  tau_epsilon_nu2 = tau_sigma_nu2\n")
+
+
+def _patch_visco(text: str) -> str:
+    """Elastic limit, receivers, mpif.h instead of the mpi module (whose .mod
+    is not built for amdflang), images off."""
+    lines = text.split("\n")
+    n = next(i for i, l in enumerate(lines) if "tau_epsilon_nu2,tau_sigma_nu2)" in l)
+    lines.insert(n + 1, _ELASTIC_LIMIT.rstrip("\n"))
+    text = "\n".join(lines)
+    for old, new in _RECEIVERS_3DV:
+        if text.count(old) != 1:
+            raise RuntimeError(f"receiver line {old!r} not found")
+        text = text.replace(old, new)
+    if text.count("  use mpi\n") != 1:
+        raise RuntimeError("use mpi not found")
+    text = text.replace("  use mpi\n", "")
+    text = text.replace("  implicit none\n", "  implicit none\n  include 'mpif.h'\n"
+                        "  character(len=64) :: ref_dumpname\n", 1)
+    text = text.replace("call create_color_image(", "if (.false.) call create_color_image(")
+    lines = text.split("\n")
+    mark = next(n for n, l in enumerate(lines) if "end of time loop" in l)
+    mark = next(n for n in range(mark + 1, len(lines)) if lines[n].strip() == "enddo")
+    lines.insert(mark + 1, _DUMP_3DV.rstrip("\n"))
+    return "\n".join(lines)
+
+
 def patch(text: str, v: dict) -> str:
     """Return the patched program text for variant ``v``."""
     text = _set_params(text, v["params"])
+    if v.get("visco"):
+        return _patch_visco(text)
     if v.get("model"):
         text, k = _MODEL_LOOP.subn(_MODEL_READ, text)
         if k != 1:
@@ -220,8 +288,9 @@ def build(names=None, verbose=False) -> list[str]:
             exe = OUT_DIR / name
             cmd = ["amdflang", "-O3", "-o", str(exe), str(f90)]
             if v["ndim"] == 3:
-                cmd = ["amdflang", "-O3", f"-I{MPI_INC}", "-o", str(exe), str(f90),
-                       f"-L{MPI_LIB}", "-lmpifort", "-lmpi", f"-Wl,-rpath,{MPI_LIB}"]
+                # -I REF_DIR: 3D-visco:1578 includes the reference's SolvOpt file
+                cmd = ["amdflang", "-O3", f"-I{MPI_INC}", f"-I{REF_DIR}", "-o", str(exe),
+                       str(f90), f"-L{MPI_LIB}", "-lmpifort", "-lmpi", f"-Wl,-rpath,{MPI_LIB}"]
             r = subprocess.run(cmd, cwd=tmp, capture_output=True, text=True)
             if r.returncode != 0:
                 raise RuntimeError(f"compile {name} failed:\n{r.stdout}\n{r.stderr}")

@@ -57,7 +57,15 @@ extern "C" {
                          int ksrc, const REAL *fx, const REAL *fy, int nrec,             \
                          const int *ixr, const int *iyr, const int *izr, int ldseis,     \
                          REAL *sisvx, REAL *sisvy, REAL *sisvz, const int *ewin,         \
-                         int energy_slabs, REAL *etotal, REAL *state, int nthreads);
+                         int energy_slabs, REAL *etotal, REAL *state, int nthreads);    \
+  int oracle_run3d4_##SFX(int nx, int ny, int nz, REAL dx, REAL dy, REAL dz, REAL dt,    \
+                          int it_begin, int it_end, const REAL *lambda, const REAL *mu,  \
+                          const REAL *rho, const REAL *prof_x, const REAL *prof_y,       \
+                          const REAL *prof_z, int isrc, int jsrc, int ksrc,              \
+                          const REAL *fx, const REAL *fy, int nrec, const int *ixr,      \
+                          const int *iyr, const int *izr, int ldseis, REAL *sisvx,       \
+                          REAL *sisvy, const int *ewin, int energy_slabs, REAL *etotal,  \
+                          int zcut, REAL *state, int nthreads);
 
 ORACLE_DECLS(double, d)
 ORACLE_DECLS(float, f)

@@ -600,6 +600,266 @@ int ORC(run3d)(int nx, int ny, int nz, REAL dx, REAL dy, REAL dz, REAL dt, int i
   return 0;
 }
 
+/* --------------------------------------------------------------------------
+ * 3D, 4th order in space: the elastic part of
+ * seismic_CPML_3D_viscoelastic_MPI.f90 (3D-visco), i.e. that program with
+ * tau_epsilon = tau_sigma (phi = 0, Mu_nu = 1: 3D-visco:466-477), in which
+ * the memory variables e* stay zero and only the unrelaxed update remains.
+ *   stress   3D-visco:977-1229, velocity 3D-visco:1244-1327,
+ *   source   3D-visco:1329-1353, Dirichlet 3D-visco:1355-1398 (two planes
+ *   per face), seismograms 3D-visco:1371-1378, energy 3D-visco:1380-1423.
+ * Staggered 4th-order difference: (27 f(+) - 27 f(-) - f(++) + f(--)) *
+ * ONE_OVER_DELTA / 24, summed left to right as written there.  Loop ranges
+ * as 3D-visco in global indices (NPROC slabs concatenated):
+ *   sxx syy szz: i 1..NX-1, j 2..NY, k 2..NZ     sxy: i 2..NX, j 1..NY-1, all k
+ *   sxz: i 2..NX, j 1..NY, k 1..NZ-1              syz: i 1..NX, j 1..NY-1, k 1..NZ-1
+ *   vx: i 2..NX, j 2..NY, k 2..NZ    vy: i 1..NX-1, j 1..NY-1, k 2..NZ
+ *   vz: i 1..NX-1, j 2..NY, k 1..NZ-1
+ * Lame coefficients as 3D-visco:981-985 with Mu_nu = 1:
+ *   lambda_u = (lambda + (2/3) mu) - (2/3) mu, lambda_u + 2 mu, mu.
+ * zcut > 0 reproduces the reference's MPI run with a slab interface after
+ * global plane zcut: its halo exchange sends two planes in one direction
+ * per field (3D-visco:963-975, 1230-1244), so across the interface it reads
+ * a never-filled (zero) ghost plane for vz(k+1) in sxx/syy/szz and for
+ * sxz(k+1), syz(k+1) in vx, vy on the lower side (k = zcut), and for
+ * vx(k-1), vy(k-1) in sxz, syz and szz(k-1) in vz on the upper side
+ * (k = zcut+1).  zcut = 0 is the single-domain stencil the GPU computes.
+ * Reads outside 1..N use the zero ghost planes (arrays 0:N+1 in x and y,
+ * -1:NZ_LOCAL+2 in z).
+ * ------------------------------------------------------------------------ */
+int ORC(run3d4)(int nx, int ny, int nz, REAL dx, REAL dy, REAL dz, REAL dt, int it_begin,
+                int it_end, const REAL *lambda, const REAL *mu, const REAL *rho,
+                const REAL *prof_x, const REAL *prof_y, const REAL *prof_z, int isrc, int jsrc,
+                int ksrc, const REAL *fx, const REAL *fy, int nrec, const int *ixr,
+                const int *iyr, const int *izr, int ldseis, REAL *sisvx, REAL *sisvy,
+                const int *ewin, int energy_slabs, REAL *etotal, int zcut, REAL *state,
+                int nthreads)
+{
+  if (nx < 4 || ny < 4 || nz < 4) return 1;
+  const size_t nn = (size_t)nx * ny * nz;
+  /* ghost-padded copies of the nine fields: 2 planes each side, every axis */
+  const int G = 2;
+  const size_t px = (size_t)nx + 2 * G, py = (size_t)ny + 2 * G, pz = (size_t)nz + 2 * G;
+  const size_t pn = px * py * pz;
+  REAL *buf = (REAL *)calloc(pn * 9, sizeof(REAL));
+  if (!buf) return 2;
+  REAL *f[9];
+  for (int q = 0; q < 9; ++q) f[q] = buf + (size_t)q * pn;
+  REAL *vx = f[0], *vy = f[1], *vz = f[2], *sxx = f[3], *syy = f[4], *szz = f[5];
+  REAL *sxy = f[6], *sxz = f[7], *syz = f[8];
+  const size_t sy = px, sz = px * py;
+#define P(i, j, k) ((size_t)((i) - 1 + G) + (size_t)((j) - 1 + G) * sy + (size_t)((k) - 1 + G) * sz)
+#define Q(i, j, k) ((size_t)((i) - 1) + (size_t)((j) - 1) * nx + (size_t)((k) - 1) * (size_t)nx * ny)
+  for (int q = 0; q < 9; ++q)
+    for (int k = 1; k <= nz; ++k)
+      for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) f[q][P(i, j, k)] = state[(size_t)q * nn + Q(i, j, k)];
+  REAL *m_dvx_dx = state + 9 * nn, *m_dvx_dy = state + 10 * nn, *m_dvx_dz = state + 11 * nn;
+  REAL *m_dvy_dx = state + 12 * nn, *m_dvy_dy = state + 13 * nn, *m_dvy_dz = state + 14 * nn;
+  REAL *m_dvz_dx = state + 15 * nn, *m_dvz_dy = state + 16 * nn, *m_dvz_dz = state + 17 * nn;
+  REAL *m_dsxx_dx = state + 18 * nn, *m_dsyy_dy = state + 19 * nn, *m_dszz_dz = state + 20 * nn;
+  REAL *m_dsxy_dx = state + 21 * nn, *m_dsxy_dy = state + 22 * nn, *m_dsxz_dx = state + 23 * nn;
+  REAL *m_dsxz_dz = state + 24 * nn, *m_dsyz_dy = state + 25 * nn, *m_dsyz_dz = state + 26 * nn;
+  const REAL *a_x = prof_x, *b_x = prof_x + nx, *K_x = prof_x + 2 * nx;
+  const REAL *a_x_half = prof_x + 3 * nx, *b_x_half = prof_x + 4 * nx, *K_x_half = prof_x + 5 * nx;
+  const REAL *a_y = prof_y, *b_y = prof_y + ny, *K_y = prof_y + 2 * ny;
+  const REAL *a_y_half = prof_y + 3 * ny, *b_y_half = prof_y + 4 * ny, *K_y_half = prof_y + 5 * ny;
+  const REAL *a_z = prof_z, *b_z = prof_z + nz, *K_z = prof_z + 2 * nz;
+  const REAL *a_z_half = prof_z + 3 * nz, *b_z_half = prof_z + 4 * nz, *K_z_half = prof_z + 5 * nz;
+  const REAL odx = LIT(1.0) / dx, ody = LIT(1.0) / dy, odz = LIT(1.0) / dz; /* 3D-visco:162-164 */
+  const REAL c23 = LIT(2.0) / LIT(3.0);                                      /* 2.d0/DIM */
+  const REAL Z = LIT(0.0);
+  /* 4th-order staggered difference, 3D-visco:987 and the like */
+#define D4(fp, fm, fpp, fmm, od) ((((LIT(27.0) * (fp) - LIT(27.0) * (fm)) - (fpp)) + (fmm)) * (od) / LIT(24.0))
+
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+
+  for (int it = it_begin; it <= it_end; ++it) {
+    /* normal stresses (3D-visco:977-1078) */
+#pragma omp parallel for schedule(static)
+    for (int k = 2; k <= nz; ++k)
+      for (int j = 2; j <= ny; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+          const size_t p = P(i, j, k), q = Q(i, j, k);
+          const REAL m = mu[q], l = lambda[q];
+          const REAL lu = (l + c23 * m) - c23 * m;
+          const REAL l2u = lu + LIT(2.0) * m;
+          const REAL vzp = (zcut > 0 && k == zcut) ? Z : vz[p + sz];
+          REAL dvx_dx = D4(vx[p + 1], vx[p], vx[p + 2], vx[p - 1], odx);
+          REAL dvy_dy = D4(vy[p], vy[p - sy], vy[p + sy], vy[p - 2 * sy], ody);
+          REAL dvz_dz = D4(vz[p], vz[p - sz], vzp, vz[p - 2 * sz], odz);
+          m_dvx_dx[q] = b_x_half[i - 1] * m_dvx_dx[q] + a_x_half[i - 1] * dvx_dx;
+          m_dvy_dy[q] = b_y[j - 1] * m_dvy_dy[q] + a_y[j - 1] * dvy_dy;
+          m_dvz_dz[q] = b_z[k - 1] * m_dvz_dz[q] + a_z[k - 1] * dvz_dz;
+          dvx_dx = dvx_dx / K_x_half[i - 1] + m_dvx_dx[q];
+          dvy_dy = dvy_dy / K_y[j - 1] + m_dvy_dy[q];
+          dvz_dz = dvz_dz / K_z[k - 1] + m_dvz_dz[q];
+          sxx[p] = sxx[p] + (l2u * dvx_dx + lu * dvy_dy + lu * dvz_dz) * dt;
+          syy[p] = syy[p] + (lu * dvx_dx + l2u * dvy_dy + lu * dvz_dz) * dt;
+          szz[p] = szz[p] + (lu * dvx_dx + lu * dvy_dy + l2u * dvz_dz) * dt;
+        }
+    /* sigma_xy (3D-visco:1080-1139) */
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= nz; ++k)
+      for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 2; i <= nx; ++i) {
+          const size_t p = P(i, j, k), q = Q(i, j, k);
+          REAL dvy_dx = D4(vy[p], vy[p - 1], vy[p + 1], vy[p - 2], odx);
+          REAL dvx_dy = D4(vx[p + sy], vx[p], vx[p + 2 * sy], vx[p - sy], ody);
+          m_dvy_dx[q] = b_x[i - 1] * m_dvy_dx[q] + a_x[i - 1] * dvy_dx;
+          m_dvx_dy[q] = b_y_half[j - 1] * m_dvx_dy[q] + a_y_half[j - 1] * dvx_dy;
+          dvy_dx = dvy_dx / K_x[i - 1] + m_dvy_dx[q];
+          dvx_dy = dvx_dy / K_y_half[j - 1] + m_dvx_dy[q];
+          sxy[p] = sxy[p] + mu[q] * (dvx_dy + dvy_dx) * dt;
+        }
+    /* sigma_xz, sigma_yz (3D-visco:1141-1226) */
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= nz - 1; ++k) {
+      const int cut = zcut > 0 && k == zcut + 1;
+      for (int j = 1; j <= ny; ++j)
+        for (int i = 2; i <= nx; ++i) {
+          const size_t p = P(i, j, k), q = Q(i, j, k);
+          REAL dvz_dx = D4(vz[p], vz[p - 1], vz[p + 1], vz[p - 2], odx);
+          REAL dvx_dz = D4(vx[p + sz], vx[p], vx[p + 2 * sz], cut ? Z : vx[p - sz], odz);
+          m_dvz_dx[q] = b_x[i - 1] * m_dvz_dx[q] + a_x[i - 1] * dvz_dx;
+          m_dvx_dz[q] = b_z_half[k - 1] * m_dvx_dz[q] + a_z_half[k - 1] * dvx_dz;
+          dvz_dx = dvz_dx / K_x[i - 1] + m_dvz_dx[q];
+          dvx_dz = dvx_dz / K_z_half[k - 1] + m_dvx_dz[q];
+          sxz[p] = sxz[p] + mu[q] * (dvx_dz + dvz_dx) * dt;
+        }
+      for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx; ++i) {
+          const size_t p = P(i, j, k), q = Q(i, j, k);
+          REAL dvz_dy = D4(vz[p + sy], vz[p], vz[p + 2 * sy], vz[p - sy], ody);
+          REAL dvy_dz = D4(vy[p + sz], vy[p], vy[p + 2 * sz], cut ? Z : vy[p - sz], odz);
+          m_dvz_dy[q] = b_y_half[j - 1] * m_dvz_dy[q] + a_y_half[j - 1] * dvz_dy;
+          m_dvy_dz[q] = b_z_half[k - 1] * m_dvy_dz[q] + a_z_half[k - 1] * dvy_dz;
+          dvz_dy = dvz_dy / K_y_half[j - 1] + m_dvz_dy[q];
+          dvy_dz = dvy_dz / K_z_half[k - 1] + m_dvy_dz[q];
+          syz[p] = syz[p] + mu[q] * (dvy_dz + dvz_dy) * dt;
+        }
+    }
+    /* vx, vy (3D-visco:1244-1285) */
+#pragma omp parallel for schedule(static)
+    for (int k = 2; k <= nz; ++k) {
+      const int cut = zcut > 0 && k == zcut;
+      for (int j = 2; j <= ny; ++j)
+        for (int i = 2; i <= nx; ++i) {
+          const size_t p = P(i, j, k), q = Q(i, j, k);
+          REAL dsxx_dx = D4(sxx[p], sxx[p - 1], sxx[p + 1], sxx[p - 2], odx);
+          REAL dsxy_dy = D4(sxy[p], sxy[p - sy], sxy[p + sy], sxy[p - 2 * sy], ody);
+          REAL dsxz_dz = D4(sxz[p], sxz[p - sz], cut ? Z : sxz[p + sz], sxz[p - 2 * sz], odz);
+          m_dsxx_dx[q] = b_x[i - 1] * m_dsxx_dx[q] + a_x[i - 1] * dsxx_dx;
+          m_dsxy_dy[q] = b_y[j - 1] * m_dsxy_dy[q] + a_y[j - 1] * dsxy_dy;
+          m_dsxz_dz[q] = b_z[k - 1] * m_dsxz_dz[q] + a_z[k - 1] * dsxz_dz;
+          dsxx_dx = dsxx_dx / K_x[i - 1] + m_dsxx_dx[q];
+          dsxy_dy = dsxy_dy / K_y[j - 1] + m_dsxy_dy[q];
+          dsxz_dz = dsxz_dz / K_z[k - 1] + m_dsxz_dz[q];
+          vx[p] = dt / rho[q] * (dsxx_dx + dsxy_dy + dsxz_dz) + vx[p];
+        }
+      for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+          const size_t p = P(i, j, k), q = Q(i, j, k);
+          REAL dsxy_dx = D4(sxy[p + 1], sxy[p], sxy[p + 2], sxy[p - 1], odx);
+          REAL dsyy_dy = D4(syy[p + sy], syy[p], syy[p + 2 * sy], syy[p - sy], ody);
+          REAL dsyz_dz = D4(syz[p], syz[p - sz], cut ? Z : syz[p + sz], syz[p - 2 * sz], odz);
+          m_dsxy_dx[q] = b_x_half[i - 1] * m_dsxy_dx[q] + a_x_half[i - 1] * dsxy_dx;
+          m_dsyy_dy[q] = b_y_half[j - 1] * m_dsyy_dy[q] + a_y_half[j - 1] * dsyy_dy;
+          m_dsyz_dz[q] = b_z[k - 1] * m_dsyz_dz[q] + a_z[k - 1] * dsyz_dz;
+          dsxy_dx = dsxy_dx / K_x_half[i - 1] + m_dsxy_dx[q];
+          dsyy_dy = dsyy_dy / K_y_half[j - 1] + m_dsyy_dy[q];
+          dsyz_dz = dsyz_dz / K_z[k - 1] + m_dsyz_dz[q];
+          vy[p] = dt / rho[q] * (dsxy_dx + dsyy_dy + dsyz_dz) + vy[p];
+        }
+    }
+    /* vz (3D-visco:1287-1305) */
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= nz - 1; ++k) {
+      const int cut = zcut > 0 && k == zcut + 1;
+      for (int j = 2; j <= ny; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+          const size_t p = P(i, j, k), q = Q(i, j, k);
+          REAL dsxz_dx = D4(sxz[p + 1], sxz[p], sxz[p + 2], sxz[p - 1], odx);
+          REAL dsyz_dy = D4(syz[p], syz[p - sy], syz[p + sy], syz[p - 2 * sy], ody);
+          REAL dszz_dz = D4(szz[p + sz], szz[p], szz[p + 2 * sz], cut ? Z : szz[p - sz], odz);
+          m_dsxz_dx[q] = b_x_half[i - 1] * m_dsxz_dx[q] + a_x_half[i - 1] * dsxz_dx;
+          m_dsyz_dy[q] = b_y[j - 1] * m_dsyz_dy[q] + a_y[j - 1] * dsyz_dy;
+          m_dszz_dz[q] = b_z_half[k - 1] * m_dszz_dz[q] + a_z_half[k - 1] * dszz_dz;
+          dsxz_dx = dsxz_dx / K_x_half[i - 1] + m_dsxz_dx[q];
+          dsyz_dy = dsyz_dy / K_y[j - 1] + m_dsyz_dy[q];
+          dszz_dz = dszz_dz / K_z_half[k - 1] + m_dszz_dz[q];
+          vz[p] = dt / rho[q] * (dsxz_dx + dsyz_dy + dszz_dz) + vz[p];
+        }
+    }
+    /* source (3D-visco:1329-1353) */
+    if (isrc >= 1 && fx && fy) {
+      const size_t p = P(isrc, jsrc, ksrc), q = Q(isrc, jsrc, ksrc);
+      vx[p] = vx[p] + fx[it - 1] * dt / rho[q];
+      vy[p] = vy[p] + fy[it - 1] * dt / rho[q];
+    }
+    /* Dirichlet: planes 1 and N of every axis (the ghost planes stay zero),
+     * 3D-visco:1355-1398 */
+    for (int k = 1; k <= nz; ++k)
+      for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i)
+          if (i == 1 || i == nx || j == 1 || j == ny || k == 1 || k == nz) {
+            const size_t p = P(i, j, k);
+            vx[p] = vy[p] = vz[p] = Z;
+          }
+    /* seismograms (3D-visco:1371-1378) */
+    for (int r = 0; r < nrec; ++r) {
+      const size_t p = P(ixr[r], iyr[r], izr[r]);
+      sisvx[(size_t)(it - 1) + (size_t)r * ldseis] = vx[p];
+      sisvy[(size_t)(it - 1) + (size_t)r * ldseis] = vy[p];
+    }
+    /* energy (3D-visco:1380-1429): per-slab partial sums added in slab
+     * order; sigma_R = sigma in the elastic limit up to the rounding of the
+     * relaxed Lame coefficients */
+    if (ewin && etotal) {
+      const int i0 = ewin[0], i1 = ewin[1], j0 = ewin[2], j1 = ewin[3], k0 = ewin[4], k1 = ewin[5];
+      const int nsl = energy_slabs > 0 ? energy_slabs : 1;
+      const int nzl = nz / nsl;
+      REAL tot = 0;
+      for (int s = 0; s < nsl; ++s) {
+        const int ka = s * nzl + 1 > k0 ? s * nzl + 1 : k0;
+        const int kb = (s + 1) * nzl < k1 ? (s + 1) * nzl : k1;
+        REAL ek = 0, ep = 0;
+        for (int k = ka; k <= kb; ++k)
+          for (int j = j0; j <= j1; ++j)
+            for (int i = i0; i <= i1; ++i) {
+              const size_t p = P(i, j, k), q = Q(i, j, k);
+              const REAL r = rho[q], l = lambda[q], m = mu[q];
+              ek = ek + LIT(0.5) * r * (vx[p] * vx[p] + vy[p] * vy[p] + vz[p] * vz[p]);
+              const REAL den = LIT(2.0) * m * (LIT(3.0) * l + LIT(2.0) * m);
+              const REAL exx = (LIT(2.0) * (l + m) * sxx[p] - l * syy[p] - l * szz[p]) / den;
+              const REAL eyy = (LIT(2.0) * (l + m) * syy[p] - l * sxx[p] - l * szz[p]) / den;
+              const REAL exy = sxy[p] / (LIT(2.0) * m);
+              const REAL exz = sxz[p] / (LIT(2.0) * m);
+              const REAL eyz = syz[p] / (LIT(2.0) * m);
+              ep = ep + LIT(0.5) * (exx * sxx[p] + eyy * syy[p] + eyy * syy[p] +
+                                    LIT(2.0) * exy * sxy[p] + LIT(2.0) * exz * sxz[p] +
+                                    LIT(2.0) * eyz * syz[p]);
+            }
+        tot = s == 0 ? ek + ep : tot + (ek + ep);
+      }
+      etotal[it - 1] = tot;
+    }
+  }
+  for (int q = 0; q < 9; ++q)
+    for (int k = 1; k <= nz; ++k)
+      for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) state[(size_t)q * nn + Q(i, j, k)] = f[q][P(i, j, k)];
+#undef D4
+#undef P
+#undef Q
+  free(buf);
+  return 0;
+}
+
 #undef ORC
 #undef CAT
 #undef CAT_

@@ -43,6 +43,8 @@ def inputs(case: str, prec: str, nstep: int | None = None) -> dict:
     if nz is not None:
         d["prof_z"] = orc.profile(nz, v["DELTAZ"], v["DELTAT"], npml, v["cp"], v["f0"], prec,
                                   clamp_alpha=False)
+    if nz is not None and c["order"] == 4:
+        return _inputs_3d4(case, prec, d, c)
     kind = 2 if c["src"] == "ricker" else 1
     d["fx"], d["fy"] = orc.stf(nstep, v["DELTAT"], v["f0"], v["t0"], v["factor"], v["ANGLE"],
                                kind, prec)
@@ -72,13 +74,46 @@ def inputs(case: str, prec: str, nstep: int | None = None) -> dict:
     return d
 
 
+def _inputs_3d4(case: str, prec: str, d: dict, c: dict) -> dict:
+    """3D 4th order: the reference's own profiles and receiver indices (float64
+    fixture, rounded for float32), source at (ISOURCE, JSOURCE, NZ/2) and
+    receivers on that plane (3D-visco:1350-1352, 1374-1377), energy window
+    NPOINTS_PML..N-NPOINTS_PML+1 on every axis (3D-visco:1383-1388)."""
+    f = fixture(case, "f64")
+    dt = d["dtype"]
+    nx, ny, nz, npml = c["NX"], c["NY"], c["NZ"], c["NPML"]
+    for k in ("prof_x", "prof_y", "prof_z"):
+        d[k] = f[k].astype(dt)
+    d["fx"], d["fy"] = orc.stf(d["nstep"], d["DELTAT"], d["f0"], d["t0"], d["factor"],
+                               d["ANGLE"], 1, prec)
+    d["ix_rec"], d["iy_rec"] = f["ix_rec"].astype(np.int32), f["iy_rec"].astype(np.int32)
+    shp = (nz, ny, nx)
+    d["rho"] = np.full(shp, d["density"], dt)
+    d["mu"] = np.full(shp, d["mu"], dt)
+    d["lam"] = np.full(shp, d["lambda"], dt)
+    d["ksrc"] = nz // 2
+    d["iz_rec"] = np.full(c["NREC"], nz // 2, np.int32)
+    d["ewin"] = (npml, nx - npml + 1, npml, ny - npml + 1, npml, nz - npml + 1)
+    return d
+
+
 def run_oracle(case: str, prec: str, nstep: int | None = None, d: dict | None = None,
-               nthreads: int = 0, energy=True, derived_lp2m: bool = False) -> dict:
+               nthreads: int = 0, energy=True, derived_lp2m: bool = False,
+               zcut: int = 0) -> dict:
     """derived_lp2m: 3D only -- use DELTAT*lambda + 2*(DELTAT*mu) for the
     (lambda+2mu) coefficient, as the HIP kernels do, instead of the reference's
-    DELTAT*(rho*cp*cp) (3D:145, 298)."""
+    DELTAT*(rho*cp*cp) (3D:145, 298).  zcut: 3D 4th order only -- reproduce
+    the reference's slab-interface reads after plane zcut (0: single domain,
+    what the HIP kernels compute)."""
     d = inputs(case, prec, nstep) if d is None else d
     c = ALL_CASES[case]
+    if c.get("NZ") is not None and c["order"] == 4:
+        return orc.run3d4(c["NX"], c["NY"], c["NZ"], d["DELTAX"], d["DELTAY"], d["DELTAZ"],
+                          d["DELTAT"], d["nstep"], d["lam"], d["mu"], d["rho"], d["prof_x"],
+                          d["prof_y"], d["prof_z"], d["ISOURCE"], d["JSOURCE"], d["ksrc"],
+                          d["fx"], d["fy"], d["ix_rec"], d["iy_rec"], d["iz_rec"],
+                          ewin=d["ewin"] if energy else None, energy_slabs=c["nproc"],
+                          prec=prec, zcut=zcut, nthreads=nthreads)
     if c.get("NZ") is None:
         return orc.run2d(c["NX"], c["NY"], c["order"], d["DELTAX"], d["DELTAY"], d["DELTAT"],
                          d["nstep"], d["lam"], d["mu"], d["rho"], d["prof_x"], d["prof_y"],

@@ -67,12 +67,22 @@ OTHER_CASES = {
     "3d_wide": dict(CASES["3d_small"], NX=160, NY=48, NZ=40, NSTEP=200, ISOURCE=91,
                     JSOURCE=33),
 }
-ALL_CASES = {**CASES, **OTHER_CASES}
+# 3D 4th order: the elastic limit of the 3D viscoelastic program
+# (oracle/build_ref.py _SMALL_3DV; 3D-visco:152-222), 2 MPI ranks, float64
+# reference only.  Profiles and receiver indices are the reference's own
+# (K_MAX_PML = 7, its own d0 and receiver rule), read from the fixture.
+CASES_3D4 = {
+    "3d4_small": dict(order=4, NX=40, NY=60, NZ=40, DELTAX="4.", NPML=8, cp="3000.",
+                      cs="2000.", density="2000.", NSTEP=400, DELTAT="4.e-4", f0="18.",
+                      factor="1.e7", ANGLE="0.", NREC=3, src="gauss1", xdeb=None, model=False,
+                      ISOURCE=8 + 20, JSOURCE=60 // 5 + 1, nproc=2, precs=("f64",)),
+}
+ALL_CASES = {**CASES, **OTHER_CASES, **CASES_3D4}
 
 VARIANT_OF = {
     "2d2_shipped": "ref2d2_shipped", "2d4_shipped": "ref2d4_shipped",
     "2d2_layered": "ref2d2_layered", "2d4_layered": "ref2d4_layered",
-    "3d_small": "ref3d_small",
+    "3d_small": "ref3d_small", "3d4_small": "ref3dv4_elastic",
 }
 
 
@@ -91,7 +101,7 @@ def values(name: str, prec: str) -> dict:
     else:
         v["DELTAT"] = L(c["DELTAT"])
     v["cp"] = L(c["cp"])
-    v["cs"] = v["cp"] / L("1.732")
+    v["cs"] = L(c["cs"]) if "cs" in c else v["cp"] / L("1.732")
     v["density"] = L(c["density"])
     v["f0"] = L(c["f0"])
     v["t0"] = L("1.20") / v["f0"]

@@ -12,6 +12,8 @@ the SHA-256 digest of each final wavefield with a regular subsample of it
 (every 4th point per axis in 2D, every 3rd in 3D), and for each C-PML memory array the masks of x/y/z
 lines on which it is non-zero (the arrays themselves are not stored).  For the
 layered cases the medium (rho, mu, lambda) fed to the program is stored too.
+``3d4_small_f64.npz`` is the 4th-order 3D case: the viscoelastic 3D program
+in its elastic limit (float64 only).
 ``images_<2d2|2d4>_<prec>.npz`` hold the PNM snapshots (header and pixels)
 that the shipped 2D2 and 2D4 programs write at steps 5, 100 and 200;
 ``display_3d_<prec>.npz`` the timestamp files and cut-plane PNM snapshots of
@@ -34,7 +36,7 @@ sys.path.insert(0, str(ROOT / "oracle"))
 sys.path.insert(0, str(HERE))
 
 import build_ref  # noqa: E402
-from cases import CASES, VARIANT_OF, layered_model  # noqa: E402
+from cases import ALL_CASES, CASES, CASES_3D4, VARIANT_OF, layered_model  # noqa: E402
 
 FIELDS_2D = ("vx", "vy", "sxx", "syy", "sxy")
 MEM_2D = ("m_dvx_dx", "m_dvx_dy", "m_dvy_dx", "m_dvy_dy",
@@ -49,7 +51,7 @@ def field_digest(a) -> str:
 
 
 def make(case: str, prec: str, workroot: Path) -> Path:
-    c = CASES[case]
+    c = ALL_CASES[case]
     variant = f"{VARIANT_OF[case]}_{prec}"
     model = None
     if c.get("model"):
@@ -181,11 +183,12 @@ def main(argv):
                 path = make_display3d(Path(tmp), p)
                 print(f"{path.name}: {path.stat().st_size / 1024:.0f} KiB")
         return
-    names = argv or list(CASES)
-    build_ref.build([f"{VARIANT_OF[n]}_{p}" for n in names for p in ("f64", "f32")])
+    names = argv or list(CASES) + list(CASES_3D4)
+    precs = {n: ALL_CASES[n].get("precs", ("f64", "f32")) for n in names}
+    build_ref.build([f"{VARIANT_OF[n]}_{p}" for n in names for p in precs[n]])
     with tempfile.TemporaryDirectory(prefix="cpml_golden_") as tmp:
         for n in names:
-            for p in ("f64", "f32"):
+            for p in precs[n]:
                 path = make(n, p, Path(tmp))
                 print(f"{path.name}: {path.stat().st_size / 1024:.0f} KiB")
         if not argv:

@@ -135,3 +135,35 @@ def run3d(nx, ny, nz, dx, dy, dz, deltat, nstep, lam, mu, rho, lp2m, prof_x, pro
     out.update(state=state, sisvx=sis[0][:nrec], sisvy=sis[1][:nrec], sisvz=sis[2][:nrec],
                etotal=etot)
     return out
+
+
+def run3d4(nx, ny, nz, dx, dy, dz, deltat, nstep, lam, mu, rho, prof_x, prof_y, prof_z,
+           isrc, jsrc, ksrc, fx, fy, ixr, iyr, izr, ewin=None, energy_slabs=1, prec="f64",
+           zcut=0, state=None, it_begin=1, it_end=None, nthreads=0):
+    """Run the 4th-order 3D loop (elastic part of the 3D viscoelastic program);
+    arrays (nz, ny, nx).  zcut > 0 reproduces the reference's 2-rank halo
+    reads at the slab interface after plane zcut."""
+    dt, cr, s = _real(prec)
+    it_end = nstep if it_end is None else it_end
+    nrec = len(ixr)
+    if state is None:
+        state = np.zeros((27, nz, ny, nx), dt)
+    state = np.ascontiguousarray(state, dt)
+    sis = [np.zeros((max(nrec, 1), nstep), dt) for _ in range(2)]
+    etot = np.zeros(nstep, dt)
+    ew = None if ewin is None else np.asarray(ewin, np.int32)
+    lam, mu, rho = (np.ascontiguousarray(a, dt) for a in (lam, mu, rho))
+    px, py, pz, fx, fy = (np.ascontiguousarray(a, dt) for a in (prof_x, prof_y, prof_z, fx, fy))
+    ixr, iyr, izr = (np.ascontiguousarray(a, np.int32) for a in (ixr, iyr, izr))
+    rc = getattr(lib(), f"oracle_run3d4_{s}")(
+        C.c_int(nx), C.c_int(ny), C.c_int(nz), cr(dx), cr(dy), cr(dz), cr(deltat),
+        C.c_int(it_begin), C.c_int(it_end), _ptr(lam), _ptr(mu), _ptr(rho),
+        _ptr(px), _ptr(py), _ptr(pz), C.c_int(isrc), C.c_int(jsrc), C.c_int(ksrc), _ptr(fx),
+        _ptr(fy), C.c_int(nrec), _ptr(ixr), _ptr(iyr), _ptr(izr), C.c_int(nstep),
+        _ptr(sis[0]), _ptr(sis[1]), _ptr(ew), C.c_int(energy_slabs), _ptr(etot),
+        C.c_int(zcut), _ptr(state), C.c_int(nthreads))
+    if rc != 0:
+        raise RuntimeError(f"oracle_run3d4 returned {rc}")
+    out = {k: state[q] for q, k in enumerate(STATE3D)}
+    out.update(state=state, sisvx=sis[0][:nrec], sisvy=sis[1][:nrec], etotal=etot)
+    return out
