@@ -56,11 +56,15 @@ enum { CPML_KERNEL_STRESS = 0, CPML_KERNEL_VELOCITY = 1 };
 
 /* ---- lifecycle ------------------------------------------------------- */
 
-/* Replaces the L1 parameter block of 2D2:137-218 / 2D4:131-219 / 3D:123-218.
+/* Replaces the L1 parameter block of 2D2:137-218 / 2D4:131-219 / 3D:123-218
+ * (/ 3D-visco:152-245 = seismic_CPML_3D_viscoelastic_MPI.f90 for 3D order 4).
  * ndim 2 (NZ ignored) or 3; order 2, 4 or 8 in 2D (2D2 / 2D4 / Holberg
- * operator of ADE8:263-266), 2 in 3D; prec 4 or 8.  NSTEP sizes the source
- * and seismogram tables.  Validates everything on the host before any HIP
- * call; returns CPML_ERR_NODEVICE when no GPU is visible. */
+ * operator of ADE8:263-266), 2 or 4 in 3D (order 4: the elastic update of
+ * 3D-visco:977-1398 on one domain, with that program's loop ranges, Lame
+ * coefficients and two-plane rigid faces; the source goes at KSOURCE as
+ * there at the cut plane); prec 4 or 8.  NSTEP sizes the source and
+ * seismogram tables.  Validates everything on the host before any HIP call;
+ * returns CPML_ERR_NODEVICE when no GPU is visible. */
 int cpml_gpu_init(cpml_solver **h, int ndim, int order, int prec, int NX, int NY, int NZ,
                   double DELTAX, double DELTAY, double DELTAZ, double DELTAT, int NPOINTS_PML,
                   int NSTEP);

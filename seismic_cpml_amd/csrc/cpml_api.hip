@@ -321,6 +321,8 @@ cpml::Step3D<T> step3d(cpml_solver *h) {
   p.cmu = M<T>(h, 4);
   p.cbuoy = M<T>(h, 5);
   p.rho = M<T>(h, 2);
+  p.lam = M<T>(h, 0);
+  p.mu = M<T>(h, 1);
   for (int q = 0; q < 6; ++q) {
     p.mx[q] = static_cast<T *>(mbuf(h, kSlot3Dx[q]));
     p.my[q] = static_cast<T *>(mbuf(h, kSlot3Dy[q]));
@@ -428,13 +430,15 @@ hipError_t enqueue_fused(cpml_solver *h) {
 
 template <typename T>
 hipError_t enqueue_stress(cpml_solver *h) {
-  return h->ndim == 2 ? cpml::launch_stress2d<T>(h->order, step2d<T>(h), h->stream)
-                      : cpml::launch_stress3d<T>(step3d<T>(h), h->stream);
+  if (h->ndim == 2) return cpml::launch_stress2d<T>(h->order, step2d<T>(h), h->stream);
+  return h->order == 4 ? cpml::launch_stress3d4<T>(step3d<T>(h), h->stream)
+                       : cpml::launch_stress3d<T>(step3d<T>(h), h->stream);
 }
 template <typename T>
 hipError_t enqueue_velocity(cpml_solver *h) {
-  return h->ndim == 2 ? cpml::launch_velocity2d<T>(h->order, step2d<T>(h), h->stream)
-                      : cpml::launch_velocity3d<T>(step3d<T>(h), h->stream);
+  if (h->ndim == 2) return cpml::launch_velocity2d<T>(h->order, step2d<T>(h), h->stream);
+  return h->order == 4 ? cpml::launch_velocity3d4<T>(step3d<T>(h), h->stream)
+                       : cpml::launch_velocity3d<T>(step3d<T>(h), h->stream);
 }
 template <typename T>
 hipError_t enqueue_tail(cpml_solver *h) {
@@ -763,7 +767,8 @@ int cpml_gpu_init(cpml_solver **hp, int ndim, int order, int prec, int NX, int N
   if (prec != 4 && prec != 8) return fail(nullptr, CPML_ERR_ARG, "prec must be 4 or 8");
   if (ndim == 2 && order != 2 && order != 4 && order != 8)
     return fail(nullptr, CPML_ERR_ARG, "2D order must be 2, 4 or 8");
-  if (ndim == 3 && order != 2) return fail(nullptr, CPML_ERR_ARG, "3D order must be 2");
+  if (ndim == 3 && order != 2 && order != 4)
+    return fail(nullptr, CPML_ERR_ARG, "3D order must be 2 or 4");
   if (NX < 2 || NY < 2 || (ndim == 3 && NZ < 2))
     return fail(nullptr, CPML_ERR_STENCIL, "grid must have at least 2 points per axis");
   if (!(DELTAX > 0) || !(DELTAY > 0) || !(DELTAT > 0) || (ndim == 3 && !(DELTAZ > 0)))
@@ -1263,7 +1268,8 @@ int cpml_gpu_color_image(cpml_solver *h, int field_number, unsigned char *rgb) {
     return fail(h, CPML_ERR_ARG, "field_number must be 1 (Vx) or 2 (Vy)");
   int rc = set_device(h);
   if (rc) return rc;
-  const int gh = h->order == 4 ? 1 : 0;  // 2D4:729-731 images the ghost-padded arrays
+  // 2D4:729-731 images the ghost-padded arrays (3D-visco:1494 the interior)
+  const int gh = h->ndim == 2 && h->order == 4 ? 1 : 0;
   const size_t bytes = (size_t)(h->nx + 2 * gh) * (h->ny + 2 * gh) * 3;
   if (!h->img && (rc = dev_alloc(h, (void **)&h->img, bytes))) return rc;
   cpml::Image g{};
@@ -1373,6 +1379,8 @@ int cpml_gpu_set_boundary(cpml_solver *h, int mode, cpml_abc_net *net) {
     return fail(h, CPML_ERR_ARG, "mode must be CPML_BOUNDARY_CPML or CPML_BOUNDARY_LEARNED "
                                  "with a corrector");
   if (!h->have_pml) return fail(h, CPML_ERR_STATE, "call set_pml first");
+  if (h->ndim == 3 && h->order != 2)
+    return fail(h, CPML_ERR_ARG, "the learned boundary is for the 3D second-order step");
   std::string m;
   if ((rc = cpml::abc_prepare(net, view_of(h), &m))) return fail(h, rc, m);
   dev_free(h, h->abc_corr);
@@ -1387,7 +1395,8 @@ int cpml_gpu_set_boundary(cpml_solver *h, int mode, cpml_abc_net *net) {
 
 int cpml_gpu_set_fused(cpml_solver *h, int on) {
   if (!h) return CPML_ERR_ARG;
-  if (on && h->ndim != 3) return fail(h, CPML_ERR_ARG, "the fused step is 3D only");
+  if (on && (h->ndim != 3 || h->order != 2))
+    return fail(h, CPML_ERR_ARG, "the fused step is 3D second order only");
   int rc = set_device(h);
   if (rc) return rc;
   if (on && !h->fused) {

@@ -100,6 +100,7 @@ struct Step3D {
   T *vx, *vy, *vz, *sxx, *syy, *szz, *sxy, *sxz, *syz;
   const T *clam, *cmu, *cbuoy;  // DELTAT*lambda, DELTAT*mu, DELTAT/rho
   const T *rho;                 // raw density (source term, 3D:1081-1082)
+  const T *lam, *mu;            // raw Lame parameters (4th-order step)
   // x-slab: [0] dvx_dx [1] dvy_dx [2] dvz_dx [3] dsxx_dx [4] dsxy_dx [5] dsxz_dx
   T *mx[6];
   // y-slab: [0] dvy_dy [1] dvx_dy [2] dvz_dy [3] dsxy_dy [4] dsyy_dy [5] dsyz_dy
@@ -191,6 +192,11 @@ template <typename T>
 hipError_t launch_stress3d(const Step3D<T> &p, hipStream_t s);
 template <typename T>
 hipError_t launch_velocity3d(const Step3D<T> &p, hipStream_t s);
+// 4th-order 3D step (elastic part of the 3D viscoelastic program)
+template <typename T>
+hipError_t launch_stress3d4(const Step3D<T> &p, hipStream_t s);
+template <typename T>
+hipError_t launch_velocity3d4(const Step3D<T> &p, hipStream_t s);
 template <typename T>
 hipError_t launch_fused3d(const Fused3D<T> &p, hipStream_t s);
 template <typename T>

@@ -365,7 +365,7 @@ class CpmlGpu:
         (2D2:861-999): uint8 (rows, columns, 3), top row first; order 4 gives
         the ghost-padded (NY+2, NX+2) image of 2D4:729-731, a 3D grid the
         image of its plane k = NZ/2 (3D:1231-1234)."""
-        g = 1 if self.order == 4 else 0
+        g = 1 if self.order == 4 and self.ndim == 2 else 0
         out = np.zeros((self.ny + 2 * g, self.nx + 2 * g, 3), np.uint8)
         self._call("cpml_gpu_color_image", int(field_number), _ptr(out))
         return out

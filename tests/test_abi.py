@@ -44,7 +44,7 @@ def _init(*args):
 @pytest.mark.parametrize("args,code", [
     ((4, 2, 8, 10, 10, 1, 1.0, 1.0, 1.0, 1e-3, 2, 10), solver.CPML_ERR_ARG),   # ndim
     ((2, 6, 8, 10, 10, 1, 1.0, 1.0, 1.0, 1e-3, 2, 10), solver.CPML_ERR_ARG),   # order
-    ((3, 4, 8, 10, 10, 10, 1.0, 1.0, 1.0, 1e-3, 2, 10), solver.CPML_ERR_ARG),  # 3D order 4
+    ((3, 8, 8, 10, 10, 10, 1.0, 1.0, 1.0, 1e-3, 2, 10), solver.CPML_ERR_ARG),  # 3D order 8
     ((2, 2, 2, 10, 10, 1, 1.0, 1.0, 1.0, 1e-3, 2, 10), solver.CPML_ERR_ARG),   # prec
     ((2, 2, 8, 1, 10, 1, 1.0, 1.0, 1.0, 1e-3, 2, 10), solver.CPML_ERR_STENCIL),
     ((2, 2, 8, 10, 10, 1, -1.0, 1.0, 1.0, 1e-3, 2, 10), solver.CPML_ERR_ARG),
