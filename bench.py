@@ -18,7 +18,8 @@ FP32, 12 seeded layers, Ricker 10 Hz at the centre, 256 receivers on the top
 interior row, 4000 steps; ``batch`` -- 128 2D 4th-order shots with seeded
 uneven grids 512..2048 per side, 3000 steps, batched solver against 128
 sequential plain-solver runs; ``shipped`` -- 2D2 as shipped (101 x 641, FP64,
-2000 steps).
+2000 steps); ``3d4`` -- the default 3D workload with the 4th-order step (the
+elastic update of seismic_CPML_3D_viscoelastic_MPI.f90).
 """
 from __future__ import annotations
 
@@ -72,7 +73,8 @@ def cpu_baseline_2d(nx, ny, order, dx, dt, lam, mu, rho, px, py, fx, fy, seconds
 
 def cpu_baseline(w, lam, mu, rho, nzs=128, seconds=15.0, threads=None):
     """Time the oracle's 3D loop (reference structure, full-grid memory
-    variables) on the top nzs planes of the same medium."""
+    variables; the 4th-order loop for an order-4 workload) on the top nzs
+    planes of the same medium."""
     import numpy as np
 
     orc = _orc()
@@ -91,6 +93,10 @@ def cpu_baseline(w, lam, mu, rho, nzs=128, seconds=15.0, threads=None):
     args = dict(prec=prec, nthreads=threads, energy_slabs=1)
 
     def run(nst, st):
+        if w.order == 4:
+            return orc.run3d4(n, n, nzs, w.dx, w.dx, w.dx, w.dt, nst, lam_h, mu_h, rho_h, px,
+                              py, pzs, n // 2, n // 2, nzs // 2, fx[:nst], fy[:nst], ixr, ixr,
+                              np.array([nzs // 2], np.int32), state=st, **args)
         return orc.run3d(n, n, nzs, w.dx, w.dx, w.dx, w.dt, nst, lam_h, mu_h, rho_h, None, px,
                          py, pzs, n // 2, n // 2, nzs // 2, fx[:nst], fy[:nst], ixr, ixr,
                          np.array([nzs // 2], np.int32), state=st, **args)
@@ -150,14 +156,16 @@ def _roofline(nb, avg, key=None):
             "traffic": _traffic(key, dom) if key else None, "algorithmic_bytes": int(nb[dom])}
 
 
-def bench_3d(args):
+def bench_3d(args, order=2):
     import torch
 
     from seismic_cpml_amd import workloads
 
     steps = args.steps or 2000
     nstep = args.warmup + steps
-    w = workloads.layered_3d(n=args.n or 768, nstep=nstep)
+    if order == 4 and args.fused:
+        raise SystemExit("bench.py: the fused step is second order only")
+    w = workloads.layered_3d(n=args.n or 768, nstep=nstep, order=order)
     s = w.make_solver()
     if args.fused:  # the fused step's own defaults: 14-row blocks, 48-plane chunks
         args.tile = args.tile or 16
@@ -189,7 +197,8 @@ def bench_3d(args):
         lam, mu, rho = w.medium_dev
         cpu = cpu_baseline(w, lam, mu, rho, seconds=args.cpu_seconds)
     config = {
-        "workload": f"3D isotropic elastic C-PML {w.nx}^3, NPML 20, 2nd order, layered "
+        "workload": f"3D isotropic elastic C-PML {w.nx}^3, NPML 20, "
+                    f"{'2nd' if order == 2 else '4th'} order, layered "
                     "+-5% smoothed medium, Ricker 8 Hz, 1024 receivers",
         "grid": [w.nx, w.ny, w.nz], "npml": w.npml, "deltat": w.dt,
         "kernel_path": "fused" if fused else "two-phase",
@@ -203,6 +212,10 @@ def bench_3d(args):
         "kernel_ms": {k: round(v, 4) for k, v in avg.items()},
         "max_velocity_norm": vnorm,
     }
+    if order == 4:
+        key = f"3d4_{w.nx}_kchunk{args.kchunk or 8}"
+        return _line("Gcell-updates/s (3D 768^3 FP32 4th-order C-PML time loop)", value, args,
+                     steps, wall / steps * 1e3, "f32", config, _roofline(nb, avg, key), cpu)
     key = (f"3d_{w.nx}_{'fused' if fused else 'two-phase'}_tile{args.tile or 4}_"
            f"kchunk{args.kchunk or 8}")
     return _line("Gcell-updates/s (3D 768^3 FP32 C-PML time loop)", value, args, steps,
@@ -401,7 +414,8 @@ def bench_abc(args):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("3d", "2d", "batch", "shipped", "abc"), default="3d")
+    ap.add_argument("--workload", choices=("3d", "3d4", "2d", "batch", "shipped", "abc"),
+                    default="3d")
     ap.add_argument("--steps", type=int, default=0, help="timed steps (0: the workload's own)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=0, help="grid points per axis (3d, 2d)")
@@ -439,8 +453,8 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    fn = {"3d": bench_3d, "2d": bench_2d, "batch": bench_batch, "shipped": bench_shipped,
-          "abc": bench_abc}
+    fn = {"3d": bench_3d, "3d4": lambda a: bench_3d(a, order=4), "2d": bench_2d,
+          "batch": bench_batch, "shipped": bench_shipped, "abc": bench_abc}
     print(json.dumps(fn[args.workload](args)), flush=True)
 
 

@@ -127,8 +127,9 @@ def layered_2d(n=4096, nstep=4000, seed=2, order=4, prec=4, courant=0.5, npml=20
 
 
 def layered_3d(n=768, nstep=2000, seed=3, prec=4, courant=0.5, npml=20, dx=10.0, f0=8.0,
-               nrec_side=32, sigma=8.0, amp=0.05, device="cuda") -> Workload:
-    """3D layered + smoothed perturbation, generated on the GPU with torch."""
+               nrec_side=32, sigma=8.0, amp=0.05, device="cuda", order=2) -> Workload:
+    """3D layered + smoothed perturbation, generated on the GPU with torch.
+    order 4 selects the 4th-order step (Courant 0.5 is inside its 6/7 limit)."""
     import torch
 
     tdt = torch.float32 if prec == 4 else torch.float64
@@ -171,7 +172,7 @@ def layered_3d(n=768, nstep=2000, seed=3, prec=4, courant=0.5, npml=20, dx=10.0,
     ix = ix.ravel().astype(np.int32)
     iy = iy.ravel().astype(np.int32)
     iz = np.full(ix.size, n - npml, np.int32)
-    return Workload(f"3d_layered_{n}", 3, 2, prec, n, n, n, dx, deltat, npml, nstep, f0,
+    return Workload(f"3d_layered_{n}", 3, order, prec, n, n, n, dx, deltat, npml, nstep, f0,
                     (n // 2, n // 2, n // 2), (ix, iy, iz), (px, py, pz), (fx, fy),
                     medium_dev=(lam, mu, rho), cp_max=cp_max)
 
