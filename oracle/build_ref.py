@@ -49,6 +49,7 @@ SRC_ADE8 = "seismic_ADEPML_2D_elastic_RK4_eighth_order.f90"
 SRC_2D4 = "seismic_CPML_2D_isotropic_fourth_order.f90"
 SRC_3D = "seismic_CPML_3D_isotropic_MPI_OpenMP.f90"
 SRC_3DV = "seismic_CPML_3D_viscoelastic_MPI.f90"
+SRC_2DVE = "seismic_CPML_2D_velocity_and_stress_second_order_viscoelastic.f90"
 
 # Reduced heterogeneous 2D case: 128 x 96, 5 m, seeded layered medium read
 # from model.bin, Ricker source at the centre, 8 receivers along the top
@@ -100,6 +101,14 @@ _RECEIVERS_3DV = (("xrec(1)=xsource+500.d0", "xrec(1)=xsource+20.d0"),
                   ("yrec(3)=ysource+2260.d0", "yrec(3)=ysource+120.d0"))
 VARIANTS["ref3dv4_elastic_f64"] = dict(src=SRC_3DV, prec="f64", params=_SMALL_3DV, ndim=3,
                                        ghost=0, nproc=2, visco=True)
+# 2D viscoelastic C-PML, 2nd order (2D-visco2), as shipped (Qp 65, Qs 55,
+# N_SLS 3, attenuation on) on a reduced 161 x 161 grid, 700 steps, source at
+# the centre, 3 receivers; float64 only (its SolvOpt include stays double).
+VARIANTS["ref2dve_f64"] = dict(src=SRC_2DVE, prec="f64", ndim=2, ghost=0, visco2d=True,
+                               params={"NX": "161", "NY": "161", "NSTEP": "700",
+                                       "xsource": "120.d0", "ysource": "120.d0", "NREC": "3",
+                                       "xdeb": "60.d0", "ydeb": "190.d0", "xfin": "190.d0",
+                                       "yfin": "60.d0"})
 # ADE8 on a 61 x 61 grid for 60 steps, snapshots every 30: run only for the SU
 # files it writes (seismograms ADE8:987-1003, snapshots ADE8:935-953); no dump.
 VARIANTS["refade8_su_f64"] = dict(src=SRC_ADE8, prec="f64", ndim=2, ghost=0, dump=False,
@@ -236,11 +245,41 @@ def _patch_visco(text: str) -> str:
     return "\n".join(lines)
 
 
+_DUMP_2DVE = (
+    "  open(unit=77,file='ref_dump.bin',access='stream',form='unformatted',status='replace')\n"
+    "  write(77) a_x,b_x,K_x,a_x_half,b_x_half,K_x_half\n"
+    "  write(77) a_y,b_y,K_y,a_y_half,b_y_half,K_y_half\n"
+    "  write(77) sisvx,sisvy,sispressure\n"
+    "  write(77) vx,vy,sigma_xx,sigma_yy,sigma_xy\n"
+    "  write(77) memory_dvx_dx,memory_dvx_dy,memory_dvy_dx,memory_dvy_dy\n"
+    "  write(77) memory_dsigma_xx_dx,memory_dsigma_yy_dy,memory_dsigma_xy_dx,memory_dsigma_xy_dy\n"
+    "  write(77) memory_variable_R_e1_dot,memory_variable_R_e11_dot,memory_variable_R_e13_dot\n"
+    "  write(77) tau_epsilon_nu1,tau_sigma_nu1,tau_epsilon_nu2,tau_sigma_nu2\n"
+    "  write(77) lambda_unrelaxed,mu_unrelaxed,rho\n"
+    "  write(77) ix_rec,iy_rec\n"
+    "  close(77)\n")
+
+
+def _patch_visco2d(text: str) -> str:
+    """Images and the file-removal shell call off, binary dump after the loop."""
+    text = text.replace("call create_color_image(", "if (.false.) call create_color_image(")
+    old = "  call system('rm -f Vx_file*.dat Vy_file*.dat image*.pnm image*.gif')"
+    if text.count(old) != 1:
+        raise RuntimeError("system call line not found")
+    text = text.replace(old, "")
+    lines = text.split("\n")
+    mark = next(n for n, l in enumerate(lines) if "end of time loop" in l)
+    lines.insert(mark + 1, _DUMP_2DVE.rstrip("\n"))
+    return "\n".join(lines)
+
+
 def patch(text: str, v: dict) -> str:
     """Return the patched program text for variant ``v``."""
     text = _set_params(text, v["params"])
     if v.get("visco"):
         return _patch_visco(text)
+    if v.get("visco2d"):
+        return _patch_visco2d(text)
     if v.get("model"):
         text, k = _MODEL_LOOP.subn(_MODEL_READ, text)
         if k != 1:
@@ -287,6 +326,8 @@ def build(names=None, verbose=False) -> list[str]:
             f90.write_text(patch(src, v))
             exe = OUT_DIR / name
             cmd = ["amdflang", "-O3", "-o", str(exe), str(f90)]
+            if v.get("visco2d"):  # 2D-visco2:1337 includes the SolvOpt file
+                cmd = ["amdflang", "-O3", f"-I{REF_DIR}", "-o", str(exe), str(f90)]
             if v["ndim"] == 3:
                 # -I REF_DIR: 3D-visco:1578 includes the reference's SolvOpt file
                 cmd = ["amdflang", "-O3", f"-I{MPI_INC}", f"-I{REF_DIR}", "-o", str(exe),
@@ -325,6 +366,8 @@ def run(name: str, workdir: Path, model: tuple | None = None, timeout: int = 600
         raise RuntimeError(f"{exe} not built")
     workdir = Path(workdir)
     workdir.mkdir(parents=True, exist_ok=True)
+    if v.get("visco2d"):
+        return _run_visco2d(v, exe, workdir, timeout)
     if not v.get("dump", True):  # ADE8 writes into these folders (ADE8:942, 994)
         for sub in ("seismograms", "snapshots"):
             (workdir / sub).mkdir(exist_ok=True)
@@ -417,6 +460,43 @@ def run(name: str, workdir: Path, model: tuple | None = None, timeout: int = 600
             assert off == len(raw), (rank, off, len(raw))
         for key, parts in fields.items():
             out[key] = np.concatenate(parts, axis=0)
+    return out
+
+
+def _run_visco2d(v: dict, exe: Path, workdir: Path, timeout: int) -> dict:
+    """Run the 2D viscoelastic variant; its dump as (ny, nx) arrays, the
+    attenuation memory variables as (N_SLS, ny, nx)."""
+    nx, ny = int(v["params"]["NX"]), int(v["params"]["NY"])
+    nstep, nrec, nsls = int(v["params"]["NSTEP"]), int(v["params"]["NREC"]), 3
+    with open(workdir / "stdout.txt", "w") as log:
+        r = subprocess.run([str(exe)], cwd=workdir, stdout=log, stderr=subprocess.STDOUT,
+                           timeout=timeout)
+    if r.returncode != 0:
+        raise RuntimeError(f"exited with {r.returncode}; see {workdir}/stdout.txt")
+    raw = (workdir / "ref_dump.bin").read_bytes()
+    off = 0
+
+    def take(count, typ=np.float64):
+        nonlocal off
+        a = np.frombuffer(raw, dtype=typ, count=count, offset=off)
+        off += a.nbytes
+        return a.copy()
+
+    out = {"prof_x": take(6 * nx).reshape(6, nx), "prof_y": take(6 * ny).reshape(6, ny)}
+    for k in ("sisvx", "sisvy", "sispressure"):
+        out[k] = take(nstep * nrec).reshape(nrec, nstep)
+    for k in ("vx", "vy", "sxx", "syy", "sxy", "m_dvx_dx", "m_dvx_dy", "m_dvy_dx", "m_dvy_dy",
+              "m_dsxx_dx", "m_dsyy_dy", "m_dsxy_dx", "m_dsxy_dy"):
+        out[k] = take(nx * ny).reshape(ny, nx)
+    for k in ("r_e1", "r_e11", "r_e13"):
+        out[k] = take(nx * ny * nsls).reshape(nsls, ny, nx)
+    for k in ("tau_eps_nu1", "tau_sig_nu1", "tau_eps_nu2", "tau_sig_nu2"):
+        out[k] = take(nsls)
+    for k in ("lam", "mu", "rho"):
+        out[k] = take(nx * ny).reshape(ny, nx)
+    out["ix_rec"] = take(nrec, np.int32)
+    out["iy_rec"] = take(nrec, np.int32)
+    assert off == len(raw), (off, len(raw))
     return out
 
 

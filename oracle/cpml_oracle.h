@@ -65,7 +65,15 @@ extern "C" {
                           const REAL *fx, const REAL *fy, int nrec, const int *ixr,      \
                           const int *iyr, const int *izr, int ldseis, REAL *sisvx,       \
                           REAL *sisvy, const int *ewin, int energy_slabs, REAL *etotal,  \
-                          int zcut, REAL *state, int nthreads);
+                          int zcut, REAL *state, int nthreads);                          \
+  int oracle_run2dve_##SFX(int nx, int ny, REAL dx, REAL dy, REAL dt, int it_begin,       \
+                           int it_end, const REAL *lambda, const REAL *mu, const REAL *rho, \
+                           const REAL *prof_x, const REAL *prof_y, int nsls,              \
+                           const REAL *tau_eps1, const REAL *tau_sig1,                    \
+                           const REAL *tau_eps2, const REAL *tau_sig2, int isrc, int jsrc, \
+                           const REAL *fx, const REAL *fy, int nrec, const int *ixr,      \
+                           const int *iyr, int ldseis, REAL *sisvx, REAL *sisvy,          \
+                           REAL *sisp, REAL *state, int nthreads);
 
 ORACLE_DECLS(double, d)
 ORACLE_DECLS(float, f)

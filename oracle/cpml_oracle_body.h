@@ -860,6 +860,178 @@ int ORC(run3d4)(int nx, int ny, int nz, REAL dx, REAL dy, REAL dz, REAL dt, int 
   return 0;
 }
 
+/* --------------------------------------------------------------------------
+ * 2D viscoelastic C-PML, 2nd order: seismic_CPML_2D_velocity_and_stress_
+ * second_order_viscoelastic.f90 (2D-visco2) with VISCOELASTIC_ATTENUATION.
+ *   coefficients 2D-visco2:374-381 (from tau_epsilon, tau_sigma per SLS),
+ *   stress + ADE memory variables 2D-visco2:736-833 (Robertsson et al.
+ *   1994 eq. 13-14: R_new = (R_old + d*DELTAT_phi - R_old*DELTAT/(2 tau_s))
+ *   * 1/(1 + DELTAT/(2 tau_s)), stress increment uses (R_new + R_old)/2),
+ *   velocity 2D-visco2:839-870, source 2D-visco2:873-918 (force table given),
+ *   Dirichlet 2D-visco2:921-929, seismograms and pressure 2D-visco2:932-990.
+ * Derivatives multiply by ONE_OVER_DELTA (2D-visco2:209-210).  The memory
+ * variables R are updated in place: each point reads only its own old value.
+ * state: vx vy sxx syy sxy, 8 C-PML memories (as run2d), then R_e1, R_e11,
+ * R_e13, each nsls planes; all (ny, nx).
+ * ------------------------------------------------------------------------ */
+int ORC(run2dve)(int nx, int ny, REAL dx, REAL dy, REAL dt, int it_begin, int it_end,
+                 const REAL *lambda, const REAL *mu, const REAL *rho, const REAL *prof_x,
+                 const REAL *prof_y, int nsls, const REAL *tau_eps1, const REAL *tau_sig1,
+                 const REAL *tau_eps2, const REAL *tau_sig2, int isrc, int jsrc,
+                 const REAL *fx, const REAL *fy, int nrec, const int *ixr, const int *iyr,
+                 int ldseis, REAL *sisvx, REAL *sisvy, REAL *sisp, REAL *state, int nthreads)
+{
+  if (nx < 2 || ny < 2 || nsls < 1 || nsls > 8) return 1;
+  const size_t nn = (size_t)nx * ny;
+  REAL *vx = state, *vy = state + nn, *sxx = state + 2 * nn, *syy = state + 3 * nn;
+  REAL *sxy = state + 4 * nn;
+  REAL *m_dvx_dx = state + 5 * nn, *m_dvx_dy = state + 6 * nn, *m_dvy_dx = state + 7 * nn;
+  REAL *m_dvy_dy = state + 8 * nn, *m_dsxx_dx = state + 9 * nn, *m_dsyy_dy = state + 10 * nn;
+  REAL *m_dsxy_dx = state + 11 * nn, *m_dsxy_dy = state + 12 * nn;
+  REAL *re1 = state + 13 * nn, *re11 = re1 + (size_t)nsls * nn, *re13 = re11 + (size_t)nsls * nn;
+  const REAL *a_x = prof_x, *b_x = prof_x + nx, *K_x = prof_x + 2 * nx;
+  const REAL *a_x_half = prof_x + 3 * nx, *b_x_half = prof_x + 4 * nx, *K_x_half = prof_x + 5 * nx;
+  const REAL *a_y = prof_y, *b_y = prof_y + ny, *K_y = prof_y + 2 * ny;
+  const REAL *a_y_half = prof_y + 3 * ny, *b_y_half = prof_y + 4 * ny, *K_y_half = prof_y + 5 * ny;
+  const REAL odx = LIT(1.0) / dx, ody = LIT(1.0) / dy; /* 2D-visco2:209-210 */
+  const REAL two_thirds = LIT(2.0) / LIT(3.0);          /* 2D-visco2:221 */
+  /* 2D-visco2:374-381 */
+  REAL H1[8], H2[8], M1[8], M2[8], P1[8], P2[8], S1 = 0, S2 = 0;
+  for (int l = 0; l < nsls; ++l) {
+    S1 = S1 + tau_eps1[l] / tau_sig1[l];
+    S2 = S2 + tau_eps2[l] / tau_sig2[l];
+  }
+  for (int l = 0; l < nsls; ++l) {
+    const REAL o1 = LIT(1.0) / tau_sig1[l], o2 = LIT(1.0) / tau_sig2[l];
+    H1[l] = LIT(0.5) * dt / tau_sig1[l];
+    H2[l] = LIT(0.5) * dt / tau_sig2[l];
+    M1[l] = LIT(1.0) / (LIT(1.0) + LIT(0.5) * dt * o1);
+    M2[l] = LIT(1.0) / (LIT(1.0) + LIT(0.5) * dt * o2);
+    P1[l] = dt * (LIT(1.0) - tau_eps1[l] / tau_sig1[l]) / tau_sig1[l] / S1;
+    P2[l] = dt * (LIT(1.0) - tau_eps2[l] / tau_sig2[l]) / tau_sig2[l] / S2;
+  }
+#define I2(i, j) ((size_t)((i) - 1) + (size_t)((j) - 1) * nx)
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  for (int it = it_begin; it <= it_end; ++it) {
+    /* sigma_xx, sigma_yy (2D-visco2:742-800) */
+#pragma omp parallel for schedule(static)
+    for (int j = 2; j <= ny; ++j)
+      for (int i = 1; i <= nx - 1; ++i) {
+        const size_t q = I2(i, j);
+        const REAL lh = LIT(0.5) * (lambda[q + 1] + lambda[q]);
+        const REAL mh = LIT(0.5) * (mu[q + 1] + mu[q]);
+        const REAL lpm = lh + mh;
+        const REAL l2m = lh + LIT(2.0) * mh;
+        REAL dvx_dx = (vx[q + 1] - vx[q]) * odx;
+        REAL dvy_dy = (vy[q] - vy[q - nx]) * ody;
+        m_dvx_dx[q] = b_x_half[i - 1] * m_dvx_dx[q] + a_x_half[i - 1] * dvx_dx;
+        m_dvy_dy[q] = b_y[j - 1] * m_dvy_dy[q] + a_y[j - 1] * dvy_dy;
+        dvx_dx = dvx_dx / K_x_half[i - 1] + m_dvx_dx[q];
+        dvy_dy = dvy_dy / K_y[j - 1] + m_dvy_dy[q];
+        REAL se1 = 0, se11 = 0;
+        for (int l = 0; l < nsls; ++l) {
+          REAL *r1 = re1 + (size_t)l * nn + q, *r11 = re11 + (size_t)l * nn + q;
+          const REAL o1 = *r1, o11 = *r11;
+          const REAL n1 = (o1 + (dvx_dx + dvy_dy) * P1[l] - o1 * H1[l]) * M1[l];
+          const REAL n11 = (o11 + LIT(0.5) * (dvx_dx - dvy_dy) * P2[l] - o11 * H2[l]) * M2[l];
+          *r1 = n1;
+          *r11 = n11;
+          se1 = se1 + n1 + o1;
+          se11 = se11 + n11 + o11;
+        }
+        sxx[q] = sxx[q] + (l2m * dvx_dx + lh * dvy_dy +
+                           (LIT(0.5) * lpm * se1 + mh * se11)) * dt;
+        syy[q] = syy[q] + (lh * dvx_dx + l2m * dvy_dy +
+                           (LIT(0.5) * lpm * se1 - mh * se11)) * dt;
+      }
+    /* sigma_xy (2D-visco2:802-833) */
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j <= ny - 1; ++j)
+      for (int i = 2; i <= nx; ++i) {
+        const size_t q = I2(i, j);
+        const REAL mhy = LIT(0.5) * (mu[q + nx] + mu[q]);
+        REAL dvy_dx = (vy[q] - vy[q - 1]) * odx;
+        REAL dvx_dy = (vx[q + nx] - vx[q]) * ody;
+        m_dvy_dx[q] = b_x[i - 1] * m_dvy_dx[q] + a_x[i - 1] * dvy_dx;
+        m_dvx_dy[q] = b_y_half[j - 1] * m_dvx_dy[q] + a_y_half[j - 1] * dvx_dy;
+        dvy_dx = dvy_dx / K_x[i - 1] + m_dvy_dx[q];
+        dvx_dy = dvx_dy / K_y_half[j - 1] + m_dvx_dy[q];
+        REAL se13 = 0;
+        for (int l = 0; l < nsls; ++l) {
+          REAL *r13 = re13 + (size_t)l * nn + q;
+          const REAL o13 = *r13;
+          const REAL n13 = (o13 + (dvy_dx + dvx_dy) * P2[l] - o13 * H2[l]) * M2[l];
+          *r13 = n13;
+          se13 = se13 + n13 + o13;
+        }
+        sxy[q] = sxy[q] + mhy * (dvy_dx + dvx_dy + LIT(0.5) * se13) * dt;
+      }
+    /* vx (2D-visco2:839-851) */
+#pragma omp parallel for schedule(static)
+    for (int j = 2; j <= ny; ++j)
+      for (int i = 2; i <= nx; ++i) {
+        const size_t q = I2(i, j);
+        REAL dsxx_dx = (sxx[q] - sxx[q - 1]) * odx;
+        REAL dsxy_dy = (sxy[q] - sxy[q - nx]) * ody;
+        m_dsxx_dx[q] = b_x[i - 1] * m_dsxx_dx[q] + a_x[i - 1] * dsxx_dx;
+        m_dsxy_dy[q] = b_y[j - 1] * m_dsxy_dy[q] + a_y[j - 1] * dsxy_dy;
+        dsxx_dx = dsxx_dx / K_x[i - 1] + m_dsxx_dx[q];
+        dsxy_dy = dsxy_dy / K_y[j - 1] + m_dsxy_dy[q];
+        vx[q] = vx[q] + (dsxx_dx + dsxy_dy) * dt / rho[q];
+      }
+    /* vy (2D-visco2:853-870) */
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j <= ny - 1; ++j)
+      for (int i = 1; i <= nx - 1; ++i) {
+        const size_t q = I2(i, j);
+        const REAL rh = LIT(0.25) * (rho[q] + rho[q + 1] + rho[q + 1 + nx] + rho[q + nx]);
+        REAL dsxy_dx = (sxy[q + 1] - sxy[q]) * odx;
+        REAL dsyy_dy = (syy[q + nx] - syy[q]) * ody;
+        m_dsxy_dx[q] = b_x_half[i - 1] * m_dsxy_dx[q] + a_x_half[i - 1] * dsxy_dx;
+        m_dsyy_dy[q] = b_y_half[j - 1] * m_dsyy_dy[q] + a_y_half[j - 1] * dsyy_dy;
+        dsxy_dx = dsxy_dx / K_x_half[i - 1] + m_dsxy_dx[q];
+        dsyy_dy = dsyy_dy / K_y_half[j - 1] + m_dsyy_dy[q];
+        vy[q] = vy[q] + (dsxy_dx + dsyy_dy) * dt / rh;
+      }
+    /* source (2D-visco2:913-918): fx, fy are force_x, force_y of the step */
+    if (isrc >= 1 && fx && fy) {
+      const size_t q = I2(isrc, jsrc);
+      const REAL rh = LIT(0.25) * (rho[q] + rho[q + 1] + rho[q + 1 + nx] + rho[q + nx]);
+      vx[q] = vx[q] + fx[it - 1] * dt / rho[q];
+      vy[q] = vy[q] + fy[it - 1] * dt / rh;
+    }
+    /* Dirichlet (2D-visco2:921-929) */
+    for (int j = 1; j <= ny; ++j) {
+      vx[I2(1, j)] = vy[I2(1, j)] = 0;
+      vx[I2(nx, j)] = vy[I2(nx, j)] = 0;
+    }
+    for (int i = 1; i <= nx; ++i) {
+      vx[I2(i, 1)] = vy[I2(i, 1)] = 0;
+      vx[I2(i, ny)] = vy[I2(i, ny)] = 0;
+    }
+    /* seismograms and pressure (2D-visco2:932-990) */
+    for (int r = 0; r < nrec; ++r) {
+      const size_t q = I2(ixr[r], iyr[r]);
+      sisvx[(size_t)(it - 1) + (size_t)r * ldseis] = vx[q];
+      sisvy[(size_t)(it - 1) + (size_t)r * ldseis] = vy[q];
+      if (sisp) {
+        const REAL lh = LIT(0.5) * (lambda[q + 1] + lambda[q]);
+        const REAL mh = LIT(0.5) * (mu[q + 1] + mu[q]);
+        const REAL den = LIT(4.0) * mh * (lh + mh);
+        const REAL exx = ((lh + LIT(2.0) * mh) * sxx[q] - lh * syy[q]) / den;
+        const REAL eyy = ((lh + LIT(2.0) * mh) * syy[q] - lh * sxx[q]) / den;
+        sisp[(size_t)(it - 1) + (size_t)r * ldseis] = -((lh + two_thirds * mh) * (exx + eyy));
+      }
+    }
+  }
+#undef I2
+  return 0;
+}
+
 #undef ORC
 #undef CAT
 #undef CAT_

@@ -126,3 +126,29 @@ def run_oracle(case: str, prec: str, nstep: int | None = None, d: dict | None = 
                      d["ksrc"], d["fx"], d["fy"], d["ix_rec"], d["iy_rec"], d["iz_rec"],
                      ewin=d["ewin"] if energy else None, energy_slabs=c.get("nproc", 1),
                      prec=prec, nthreads=nthreads)
+
+
+def visco2d_inputs(prec: str):
+    """Inputs of the reduced 2D viscoelastic case (2D-visco2:149-315 with the
+    oracle/build_ref.py ref2dve_f64 overrides) and its fixture: profiles, tau
+    values, medium and receivers are the reference's own (float64 fixture,
+    rounded for float32); force table = Ricker (2D-visco2:895) divided by
+    DELTAX*DELTAY (2D-visco2:899), angle 0 (force along y only)."""
+    f = np.load(GOLDEN / "2dve_small_f64.npz")
+    dt = np.float64 if prec == "f64" else np.float32
+    R = dt
+    d = dict(NX=161, NY=161, nstep=700, NPML=10, dtype=dt)
+    d["DELTAX"] = d["DELTAY"] = R(1.5)
+    d["DELTAT"] = R(2.2e-4) if prec == "f64" else np.float32(2.2e-4)
+    f0 = R(35.0)
+    t0 = R(1.20) / f0
+    fx, fy = orc.stf(d["nstep"], d["DELTAT"], f0, t0, R(1.0), R(0.0), 2, prec)
+    area = d["DELTAX"] * d["DELTAY"]
+    d["fx"], d["fy"] = (fx / area).astype(dt), (fy / area).astype(dt)
+    d["ISOURCE"] = d["JSOURCE"] = int(120.0 / 1.5 + 1)
+    for k in ("prof_x", "prof_y", "lam", "mu", "rho"):
+        d[k] = f[k].astype(dt)
+    d["taus"] = tuple(f[k].astype(dt) for k in ("tau_eps_nu1", "tau_sig_nu1", "tau_eps_nu2",
+                                                 "tau_sig_nu2"))
+    d["ix_rec"], d["iy_rec"] = f["ix_rec"].astype(np.int32), f["iy_rec"].astype(np.int32)
+    return d, f

@@ -167,3 +167,41 @@ def run3d4(nx, ny, nz, dx, dy, dz, deltat, nstep, lam, mu, rho, prof_x, prof_y, 
     out = {k: state[q] for q, k in enumerate(STATE3D)}
     out.update(state=state, sisvx=sis[0][:nrec], sisvy=sis[1][:nrec], etotal=etot)
     return out
+
+
+STATE2DVE = STATE2D_BASE = ("vx", "vy", "sxx", "syy", "sxy", "m_dvx_dx", "m_dvx_dy", "m_dvy_dx",
+                            "m_dvy_dy", "m_dsxx_dx", "m_dsyy_dy", "m_dsxy_dx", "m_dsxy_dy")
+
+
+def run2dve(nx, ny, dx, dy, deltat, nstep, lam, mu, rho, prof_x, prof_y, taus, isrc, jsrc,
+            fx, fy, ixr, iyr, prec="f64", state=None, it_begin=1, it_end=None, nthreads=0):
+    """2D viscoelastic C-PML loop (2nd order); arrays (ny, nx).  taus: tau_epsilon
+    and tau_sigma of nu1 and nu2, each (nsls,).  Returns fields, C-PML memories,
+    r_e1 / r_e11 / r_e13 (nsls, ny, nx) and sisvx, sisvy, sispressure."""
+    dt, cr, s = _real(prec)
+    it_end = nstep if it_end is None else it_end
+    nrec = len(ixr)
+    te1, ts1, te2, ts2 = (np.ascontiguousarray(a, dt) for a in taus)
+    nsls = len(te1)
+    if state is None:
+        state = np.zeros((13 + 3 * nsls, ny, nx), dt)
+    state = np.ascontiguousarray(state, dt)
+    sis = [np.zeros((max(nrec, 1), nstep), dt) for _ in range(3)]
+    lam, mu, rho = (np.ascontiguousarray(a, dt) for a in (lam, mu, rho))
+    px, py, fx, fy = (np.ascontiguousarray(a, dt) for a in (prof_x, prof_y, fx, fy))
+    ixr, iyr = (np.ascontiguousarray(a, np.int32) for a in (ixr, iyr))
+    rc = getattr(lib(), f"oracle_run2dve_{s}")(
+        C.c_int(nx), C.c_int(ny), cr(dx), cr(dy), cr(deltat), C.c_int(it_begin),
+        C.c_int(it_end), _ptr(lam), _ptr(mu), _ptr(rho), _ptr(px), _ptr(py), C.c_int(nsls),
+        _ptr(te1), _ptr(ts1), _ptr(te2), _ptr(ts2), C.c_int(isrc), C.c_int(jsrc), _ptr(fx),
+        _ptr(fy), C.c_int(nrec), _ptr(ixr), _ptr(iyr), C.c_int(nstep), _ptr(sis[0]),
+        _ptr(sis[1]), _ptr(sis[2]), _ptr(state), C.c_int(nthreads))
+    if rc != 0:
+        raise RuntimeError(f"oracle_run2dve returned {rc}")
+    out = {k: state[q] for q, k in enumerate(STATE2DVE)}
+    out["r_e1"] = state[13:13 + nsls]
+    out["r_e11"] = state[13 + nsls:13 + 2 * nsls]
+    out["r_e13"] = state[13 + 2 * nsls:13 + 3 * nsls]
+    out.update(state=state, sisvx=sis[0][:nrec], sisvy=sis[1][:nrec],
+               sispressure=sis[2][:nrec])
+    return out
