@@ -281,6 +281,28 @@ int cpml_abc_grad(cpml_abc_net *net, cpml_solver *student, cpml_solver *teacher,
 int cpml_abc_correction(cpml_abc_net *net, cpml_solver *h, int use_mfma, float *out);
 int cpml_abc_last_error(cpml_abc_net *net, char *buf, int len);
 
+/* ---- 2D viscoelastic attenuation (2D-visco2 =
+ * seismic_CPML_2D_velocity_and_stress_second_order_viscoelastic.f90) ----- */
+
+/* Switches a 2D second-order handle to the viscoelastic step of 2D-visco2
+ * (VISCOELASTIC_ATTENUATION, 2D-visco2:736-929): N_SLS standard linear solids
+ * per modulus with the relaxation times the reference's SolvOpt fit gives
+ * (tau_epsilon/tau_sigma for Qp: nu1, for Qs: nu2; 2D-visco2:360-372), memory
+ * variables R_e1, R_e11, R_e13 (zeroed here), derivatives multiplied by
+ * ONE_OVER_DELTA as that program does, and the pressure seismogram of
+ * 2D-visco2:958-990 recorded beside vx, vy.  The medium passed to
+ * cpml_gpu_set_medium is the unrelaxed one (lambda_unrelaxed, mu_unrelaxed,
+ * rho); the force table is the program's force / (DELTAX*DELTAY)
+ * (2D-visco2:899).  N_SLS = 0 returns to the elastic step.  N_SLS <= 8. */
+int cpml_gpu_set_attenuation(cpml_solver *h, int N_SLS, const double *tau_epsilon_nu1,
+                             const double *tau_sigma_nu1, const double *tau_epsilon_nu2,
+                             const double *tau_sigma_nu2);
+/* R_e1 (which 0), R_e11 (1) or R_e13 (2): N_SLS planes of NX*NY (x fastest),
+ * the memory_variable_R_*_dot arrays of 2D-visco2:324-326. */
+int cpml_gpu_get_attenuation_memory(cpml_solver *h, int which, void *out);
+/* sispressure(NSTEP, NREC) of 2D-visco2:297, 990 (viscoelastic step only). */
+int cpml_gpu_get_pressure(cpml_solver *h, void *sispressure);
+
 /* ---- PNM snapshots (2D2:716-731, 861-999; 3D:1231-1234) ----------------- */
 
 /* Colour image of the current Vx (field_number 1) or Vy (2), computed on the

@@ -159,6 +159,10 @@ struct cpml_solver {
   int host_step = 0;
   cpml_abc_net *abc = nullptr;  // learned boundary corrector (not owned)
   void *abc_corr = nullptr;     // its per-step correction buffer
+  // 2D viscoelastic attenuation (cpml_gpu_set_attenuation)
+  int nsls = 0;                 // 0: elastic
+  double tau[4][cpml::kMaxSls] = {};  // tau_epsilon_nu1, tau_sigma_nu1, _nu2, _nu2
+  void *re[3] = {};             // R_e1, R_e11, R_e13: nsls dense nx*ny planes each
   std::string err;
 };
 
@@ -313,6 +317,41 @@ cpml::Step2D<T> step2d(cpml_solver *h) {
   return p;
 }
 
+// 2D-visco2:374-381: per-SLS coefficients in the handle's precision
+template <typename T>
+cpml::Visco2D<T> visco2d(cpml_solver *h) {
+  cpml::Visco2D<T> v{};
+  v.s = step2d<T>(h);
+  v.re1 = static_cast<T *>(h->re[0]);
+  v.re11 = static_cast<T *>(h->re[1]);
+  v.re13 = static_cast<T *>(h->re[2]);
+  v.nn = (long)h->nx * h->ny;
+  v.nsls = h->nsls;
+  const T dt = T(h->dt);
+  T s1 = 0, s2 = 0;
+  for (int l = 0; l < h->nsls; ++l) {
+    s1 = s1 + T(h->tau[0][l]) / T(h->tau[1][l]);
+    s2 = s2 + T(h->tau[2][l]) / T(h->tau[3][l]);
+  }
+  for (int l = 0; l < h->nsls; ++l) {
+    const T te1 = T(h->tau[0][l]), ts1 = T(h->tau[1][l]);
+    const T te2 = T(h->tau[2][l]), ts2 = T(h->tau[3][l]);
+    const T o1 = T(1) / ts1, o2 = T(1) / ts2;
+    v.H1[l] = T(0.5) * dt / ts1;
+    v.H2[l] = T(0.5) * dt / ts2;
+    v.M1[l] = T(1) / (T(1) + T(0.5) * dt * o1);
+    v.M2[l] = T(1) / (T(1) + T(0.5) * dt * o2);
+    v.P1[l] = dt * (T(1) - te1 / ts1) / ts1 / s1;
+    v.P2[l] = dt * (T(1) - te2 / ts2) / ts2 / s2;
+  }
+  v.odx = T(1) / T(h->dx);  // 2D-visco2:209-210
+  v.ody = T(1) / T(h->dy);
+  v.sisp = static_cast<T *>(h->seis[2]);
+  v.recidx = h->recidx;
+  v.nrec = h->nrec;
+  return v;
+}
+
 template <typename T>
 cpml::Step3D<T> step3d(cpml_solver *h) {
   cpml::Step3D<T> p{};
@@ -430,12 +469,15 @@ hipError_t enqueue_fused(cpml_solver *h) {
 
 template <typename T>
 hipError_t enqueue_stress(cpml_solver *h) {
+  if (h->ndim == 2 && h->nsls > 0) return cpml::launch_stress2dve<T>(visco2d<T>(h), h->stream);
   if (h->ndim == 2) return cpml::launch_stress2d<T>(h->order, step2d<T>(h), h->stream);
   return h->order == 4 ? cpml::launch_stress3d4<T>(step3d<T>(h), h->stream)
                        : cpml::launch_stress3d<T>(step3d<T>(h), h->stream);
 }
 template <typename T>
 hipError_t enqueue_velocity(cpml_solver *h) {
+  if (h->ndim == 2 && h->nsls > 0)
+    return cpml::launch_velocity2dve<T>(visco2d<T>(h), h->stream);
   if (h->ndim == 2) return cpml::launch_velocity2d<T>(h->order, step2d<T>(h), h->stream);
   return h->order == 4 ? cpml::launch_velocity3d4<T>(step3d<T>(h), h->stream)
                        : cpml::launch_velocity3d<T>(step3d<T>(h), h->stream);
@@ -1067,7 +1109,7 @@ int cpml_gpu_set_receivers(cpml_solver *h, int NREC, const int *ix_rec, const in
   if ((rc = dev_alloc(h, (void **)&h->recidx, idx.size() * sizeof(int)))) return rc;
   const size_t sbytes = (size_t)h->nstep * std::max(NREC, 1) * h->esize;
   for (int q = 0; q < 3; ++q)
-    if ((rc = dev_alloc(h, &h->seis[q], (q == 2 && h->ndim == 2) ? h->esize : sbytes)))
+    if ((rc = dev_alloc(h, &h->seis[q], sbytes)))  // 2D: [2] holds pressure (viscoelastic)
       return rc;
   HIPCHK(h, hipMemcpy(h->recidx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
   h->nrec = NREC;
@@ -1340,6 +1382,69 @@ int cpml_su_write(const char *path, int ntr, int ns, double dt_value, const void
          std::fwrite(tr.data(), sizeof(float), ns, fp) == (size_t)ns;
   }
   return std::fclose(fp) == 0 && ok ? CPML_OK : CPML_ERR_ARG;
+}
+
+int cpml_gpu_set_attenuation(cpml_solver *h, int N_SLS, const double *tau_epsilon_nu1,
+                             const double *tau_sigma_nu1, const double *tau_epsilon_nu2,
+                             const double *tau_sigma_nu2) {
+  if (!h) return CPML_ERR_ARG;
+  if (h->ndim != 2 || h->order != 2)
+    return fail(h, CPML_ERR_ARG, "attenuation is for the 2D second-order step");
+  if (N_SLS < 0 || N_SLS > cpml::kMaxSls)
+    return fail(h, CPML_ERR_ARG, "N_SLS must be 0.." + std::to_string(cpml::kMaxSls));
+  const double *t[4] = {tau_epsilon_nu1, tau_sigma_nu1, tau_epsilon_nu2, tau_sigma_nu2};
+  for (int q = 0; q < 4 && N_SLS > 0; ++q) {
+    if (!t[q]) return fail(h, CPML_ERR_ARG, "null tau array");
+    for (int l = 0; l < N_SLS; ++l)
+      if (!(t[q][l] > 0) || !std::isfinite(t[q][l]))
+        return fail(h, CPML_ERR_ARG, "tau values must be finite and > 0");
+  }
+  int rc = set_device(h);
+  if (rc) return rc;
+  for (int k = 0; k < 3; ++k) {
+    dev_free(h, h->re[k]);
+    h->re[k] = nullptr;
+  }
+  h->nsls = N_SLS;
+  for (int q = 0; q < 4; ++q)
+    for (int l = 0; l < N_SLS; ++l) h->tau[q][l] = t[q][l];
+  const size_t bytes = (size_t)N_SLS * h->nx * h->ny * h->esize;
+  for (int k = 0; k < 3 && N_SLS > 0; ++k) {
+    if ((rc = dev_alloc(h, &h->re[k], bytes))) return rc;
+    HIPCHK(h, hipMemsetAsync(h->re[k], 0, bytes, h->stream));
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->graph_dirty = true;
+  return CPML_OK;
+}
+
+int cpml_gpu_get_attenuation_memory(cpml_solver *h, int which, void *out) {
+  if (!h || !out) return CPML_ERR_ARG;
+  if (h->nsls == 0) return fail(h, CPML_ERR_STATE, "attenuation is off");
+  if (which < 0 || which > 2) return fail(h, CPML_ERR_ARG, "which must be 0, 1 or 2");
+  int rc = set_device(h);
+  if (rc) return rc;
+  HIPCHK(h, hipMemcpy(out, h->re[which], (size_t)h->nsls * h->nx * h->ny * h->esize,
+                      hipMemcpyDeviceToHost));
+  return CPML_OK;
+}
+
+int cpml_gpu_get_pressure(cpml_solver *h, void *sispressure) {
+  if (!h || !sispressure) return CPML_ERR_ARG;
+  if (h->ndim != 2 || h->nsls == 0)
+    return fail(h, CPML_ERR_STATE, "pressure is recorded by the 2D viscoelastic step only");
+  int rc = set_device(h);
+  if (rc) return rc;
+  if (h->nrec == 0) return CPML_OK;
+  const size_t n = (size_t)h->nstep * h->nrec;
+  std::vector<char> tmp(n * h->esize);
+  HIPCHK(h, hipMemcpy(tmp.data(), h->seis[2], tmp.size(), hipMemcpyDeviceToHost));
+  char *o = static_cast<char *>(sispressure);
+  for (int it = 0; it < h->nstep; ++it)
+    for (int r = 0; r < h->nrec; ++r)
+      std::memcpy(o + ((size_t)it + (size_t)r * h->nstep) * h->esize,
+                  tmp.data() + ((size_t)it * h->nrec + r) * h->esize, h->esize);
+  return CPML_OK;
 }
 
 int cpml_gpu_velocity_norm(cpml_solver *h, double *vmax) {

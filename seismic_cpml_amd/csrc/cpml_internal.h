@@ -192,6 +192,27 @@ template <typename T>
 hipError_t launch_stress3d(const Step3D<T> &p, hipStream_t s);
 template <typename T>
 hipError_t launch_velocity3d(const Step3D<T> &p, hipStream_t s);
+// 2D viscoelastic step: the 2D step plus the attenuation memory variables
+// (dense nx*ny per SLS plane) and their per-SLS coefficients.
+constexpr int kMaxSls = 8;
+template <typename T>
+struct Visco2D {
+  Step2D<T> s;
+  T *re1, *re11, *re13;
+  long nn;   // nx * ny
+  int nsls;
+  T P1[kMaxSls], P2[kMaxSls];  // DELTAT_phi_nu1, _nu2
+  T H1[kMaxSls], H2[kMaxSls];  // HALF_DELTAT_over_tau_sigma_nu1, _nu2
+  T M1[kMaxSls], M2[kMaxSls];  // multiplication_factor_tau_sigma_nu1, _nu2
+  T odx, ody;                  // ONE_OVER_DELTAX, ONE_OVER_DELTAY
+  T *sisp;                     // pressure seismograms [NSTEP][NREC]
+  const int *recidx;
+  int nrec;
+};
+template <typename T>
+hipError_t launch_stress2dve(const Visco2D<T> &v, hipStream_t s);
+template <typename T>
+hipError_t launch_velocity2dve(const Visco2D<T> &v, hipStream_t s);
 // 4th-order 3D step (elastic part of the 3D viscoelastic program)
 template <typename T>
 hipError_t launch_stress3d4(const Step3D<T> &p, hipStream_t s);

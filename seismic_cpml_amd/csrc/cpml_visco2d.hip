@@ -1,0 +1,176 @@
+// 2D viscoelastic C-PML step (2nd order) for gfx950:
+// seismic_CPML_2D_velocity_and_stress_second_order_viscoelastic.f90 with
+// VISCOELASTIC_ATTENUATION (2D-visco2 below):
+//   stress + attenuation  2D-visco2:736-833
+//   velocity              2D-visco2:839-870, source 913-918, edges 921-929
+//   pressure seismogram   2D-visco2:958-990
+// Arithmetic statement by statement as there (derivatives multiply by
+// ONE_OVER_DELTA, 2D-visco2:209-210); built with -ffp-contract=off, so the
+// float64 results equal the reference bit for bit.
+//
+// Attenuation: N_SLS standard linear solids per modulus, memory variables
+// R_e1 (dilatation), R_e11 (deviatoric normal) and R_e13 (shear), updated by
+// the ADE scheme of Robertsson, Blanch and Symes (1994) eq. 14:
+//   R_new = (R_old + d * DELTAT_phi - R_old * DELTAT / (2 tau_sigma))
+//           / (1 + DELTAT / (2 tau_sigma))
+// and entering the stress increment as (R_new + R_old) / 2.  Each point reads
+// only its own old value, so R is updated in place (the reference keeps an
+// _old copy of every array and copies it each step, 2D-visco2:736-738).
+//
+// One point per thread, 64 x 4 blocks; R is stored dense (nx * ny per SLS
+// plane) next to the padded wavefields.
+#include <hip/hip_runtime.h>
+
+#include "cpml_device.h"
+#include "cpml_internal.h"
+
+namespace cpml {
+namespace {
+
+constexpr int VTY = 4;
+
+template <typename T>
+__device__ __forceinline__ bool point2(const Visco2D<T> &v, int &i, int &j) {
+  i = blockIdx.x * BX + threadIdx.x;
+  j = blockIdx.y * VTY + threadIdx.y;
+  return i < v.s.nx && j < v.s.ny;
+}
+
+template <typename T>
+__global__ __launch_bounds__(BX *VTY) void k_stress2dve(Visco2D<T> v) {
+  int i, j;
+  if (!point2(v, i, j)) return;
+  const Step2D<T> &p = v.s;
+  const long ld = p.ld, c = i + j * ld;
+  const long r = i + (long)j * p.nx;  // dense index of the memory variables
+  const bool inx = p.sx.in(i), iny = p.sy.in(j);
+  const long mxo = p.sx.s(i) + (long)p.sx.w * j;
+  const long myo = i + p.mld * p.sy.s(j);
+  // sigma_xx, sigma_yy: Fortran i = 1..NX-1, j = 2..NY (2D-visco2:742-800)
+  if (i <= p.nx - 2 && j >= 1) {
+    const T lh = T(0.5) * (p.lam[c + 1] + p.lam[c]);
+    const T mh = T(0.5) * (p.mu[c + 1] + p.mu[c]);
+    const T lpm = lh + mh;
+    const T l2m = lh + T(2) * mh;
+    T dvx_dx = (p.vx[c + 1] - p.vx[c]) * v.odx;
+    T dvy_dy = (p.vy[c] - p.vy[c - ld]) * v.ody;
+    if (inx) dvx_dx = cpml(dvx_dx, p.mx[0] + mxo, p.px.ah[i], p.px.bh[i], p.px.Kh[i]);
+    if (iny) dvy_dy = cpml(dvy_dy, p.my[0] + myo, p.py.a[j], p.py.b[j], p.py.K[j]);
+    T se1 = 0, se11 = 0;
+    for (int l = 0; l < v.nsls; ++l) {
+      T *r1 = v.re1 + r + l * v.nn, *r11 = v.re11 + r + l * v.nn;
+      const T o1 = *r1, o11 = *r11;
+      const T n1 = (o1 + (dvx_dx + dvy_dy) * v.P1[l] - o1 * v.H1[l]) * v.M1[l];
+      const T n11 = (o11 + T(0.5) * (dvx_dx - dvy_dy) * v.P2[l] - o11 * v.H2[l]) * v.M2[l];
+      *r1 = n1;
+      *r11 = n11;
+      se1 = se1 + n1 + o1;
+      se11 = se11 + n11 + o11;
+    }
+    p.sxx[c] = p.sxx[c] + (l2m * dvx_dx + lh * dvy_dy + (T(0.5) * lpm * se1 + mh * se11)) * p.dt;
+    p.syy[c] = p.syy[c] + (lh * dvx_dx + l2m * dvy_dy + (T(0.5) * lpm * se1 - mh * se11)) * p.dt;
+  }
+  // sigma_xy: Fortran i = 2..NX, j = 1..NY-1 (2D-visco2:802-833)
+  if (i >= 1 && j <= p.ny - 2) {
+    const T mhy = T(0.5) * (p.mu[c + ld] + p.mu[c]);
+    T dvy_dx = (p.vy[c] - p.vy[c - 1]) * v.odx;
+    T dvx_dy = (p.vx[c + ld] - p.vx[c]) * v.ody;
+    if (inx) dvy_dx = cpml(dvy_dx, p.mx[1] + mxo, p.px.a[i], p.px.b[i], p.px.K[i]);
+    if (iny) dvx_dy = cpml(dvx_dy, p.my[1] + myo, p.py.ah[j], p.py.bh[j], p.py.Kh[j]);
+    T se13 = 0;
+    for (int l = 0; l < v.nsls; ++l) {
+      T *r13 = v.re13 + r + l * v.nn;
+      const T o13 = *r13;
+      const T n13 = (o13 + (dvy_dx + dvx_dy) * v.P2[l] - o13 * v.H2[l]) * v.M2[l];
+      *r13 = n13;
+      se13 = se13 + n13 + o13;
+    }
+    p.sxy[c] = p.sxy[c] + mhy * (dvy_dx + dvx_dy + T(0.5) * se13) * p.dt;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(BX *VTY) void k_velocity2dve(Visco2D<T> v) {
+  int i, j;
+  if (!point2(v, i, j)) return;
+  const Step2D<T> &p = v.s;
+  const long ld = p.ld, c = i + j * ld;
+  const bool inx = p.sx.in(i), iny = p.sy.in(j);
+  const long mxo = p.sx.s(i) + (long)p.sx.w * j;
+  const long myo = i + p.mld * p.sy.s(j);
+  T vxn = p.vx[c], vyn = p.vy[c];
+  // vx: Fortran i = 2..NX, j = 2..NY (2D-visco2:839-851)
+  if (i >= 1 && j >= 1) {
+    T dsxx_dx = (p.sxx[c] - p.sxx[c - 1]) * v.odx;
+    T dsxy_dy = (p.sxy[c] - p.sxy[c - ld]) * v.ody;
+    if (inx) dsxx_dx = cpml(dsxx_dx, p.mx[2] + mxo, p.px.a[i], p.px.b[i], p.px.K[i]);
+    if (iny) dsxy_dy = cpml(dsxy_dy, p.my[2] + myo, p.py.a[j], p.py.b[j], p.py.K[j]);
+    vxn = vxn + (dsxx_dx + dsxy_dy) * p.dt / p.rho[c];
+  }
+  // vy: Fortran i = 1..NX-1, j = 1..NY-1 (2D-visco2:853-870)
+  if (i <= p.nx - 2 && j <= p.ny - 2) {
+    const T rhh = T(0.25) * (p.rho[c] + p.rho[c + 1] + p.rho[c + 1 + ld] + p.rho[c + ld]);
+    T dsxy_dx = (p.sxy[c + 1] - p.sxy[c]) * v.odx;
+    T dsyy_dy = (p.syy[c + ld] - p.syy[c]) * v.ody;
+    if (inx) dsxy_dx = cpml(dsxy_dx, p.mx[3] + mxo, p.px.ah[i], p.px.bh[i], p.px.Kh[i]);
+    if (iny) dsyy_dy = cpml(dsyy_dy, p.my[3] + myo, p.py.ah[j], p.py.bh[j], p.py.Kh[j]);
+    vyn = vyn + (dsxy_dx + dsyy_dy) * p.dt / rhh;
+  }
+  // point force (2D-visco2:913-918; the table holds force / (DELTAX*DELTAY))
+  if (i == p.isrc && j == p.jsrc) {
+    const int it = *p.step;
+    const T rhh = T(0.25) * (p.rho[c] + p.rho[c + 1] + p.rho[c + 1 + ld] + p.rho[c + ld]);
+    vxn = vxn + p.fx[it - 1] * p.dt / p.rho[c];
+    vyn = vyn + p.fy[it - 1] * p.dt / rhh;
+  }
+  // rigid edges (2D-visco2:921-929)
+  if (i == 0 || i == p.nx - 1 || j == 0 || j == p.ny - 1) {
+    vxn = T(0);
+    vyn = T(0);
+  }
+  p.vx[c] = vxn;
+  p.vy[c] = vyn;
+}
+
+// Pressure at the receivers from sigma_xx, sigma_yy in plane strain
+// (2D-visco2:979-990), before the record kernel advances the step.
+template <typename T>
+__global__ void k_pressure2d(Visco2D<T> v) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= v.nrec) return;
+  const Step2D<T> &p = v.s;
+  const long c = v.recidx[r];
+  const int it = *p.step;
+  const T lh = T(0.5) * (p.lam[c + 1] + p.lam[c]);
+  const T mh = T(0.5) * (p.mu[c + 1] + p.mu[c]);
+  const T den = T(4) * mh * (lh + mh);
+  const T exx = ((lh + T(2) * mh) * p.sxx[c] - lh * p.syy[c]) / den;
+  const T eyy = ((lh + T(2) * mh) * p.syy[c] - lh * p.sxx[c]) / den;
+  const T two_thirds = T(2) / T(3);
+  v.sisp[(long)(it - 1) * v.nrec + r] = -((lh + two_thirds * mh) * (exx + eyy));
+}
+
+}  // namespace
+
+template <typename T>
+hipError_t launch_stress2dve(const Visco2D<T> &v, hipStream_t s) {
+  const dim3 g((v.s.nx + BX - 1) / BX, (v.s.ny + VTY - 1) / VTY);
+  hipLaunchKernelGGL(k_stress2dve<T>, g, dim3(BX, VTY), 0, s, v);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_velocity2dve(const Visco2D<T> &v, hipStream_t s) {
+  const dim3 g((v.s.nx + BX - 1) / BX, (v.s.ny + VTY - 1) / VTY);
+  hipLaunchKernelGGL(k_velocity2dve<T>, g, dim3(BX, VTY), 0, s, v);
+  if (v.nrec > 0)
+    hipLaunchKernelGGL(k_pressure2d<T>, dim3((v.nrec + 63) / 64), dim3(64), 0, s, v);
+  return hipGetLastError();
+}
+
+template hipError_t launch_stress2dve<float>(const Visco2D<float> &, hipStream_t);
+template hipError_t launch_stress2dve<double>(const Visco2D<double> &, hipStream_t);
+template hipError_t launch_velocity2dve<float>(const Visco2D<float> &, hipStream_t);
+template hipError_t launch_velocity2dve<double>(const Visco2D<double> &, hipStream_t);
+
+}  // namespace cpml

@@ -58,6 +58,9 @@ SIGNATURES = {
     "cpml_gpu_color_image": [_VP, C.c_int, C.c_void_p],
     "cpml_pnm_write": [C.c_char_p, C.c_int, C.c_int, C.c_void_p],
     "cpml_su_write": [C.c_char_p, _I, _I, C.c_double, C.c_void_p, _I],
+    "cpml_gpu_set_attenuation": [_VP, _I, _VP, _VP, _VP, _VP],
+    "cpml_gpu_get_attenuation_memory": [_VP, _I, _VP],
+    "cpml_gpu_get_pressure": [_VP, _VP],
     "cpml_gpu_set_graph_steps": [_VP, _I],
     "cpml_gpu_set_kchunk": [_VP, _I],
     "cpml_gpu_set_tile": [_VP, _I],
@@ -221,6 +224,7 @@ class CpmlGpu:
         self.nstep = NSTEP
         self.dtype = _dtype(prec)
         self.nrec = 0
+        self.nsls = 0
         self._h = C.c_void_p()
         _check(library().cpml_gpu_init(C.byref(self._h), ndim, order, prec, NX, NY,
                                        NZ if ndim == 3 else 1, float(DELTAX), float(DELTAY),
@@ -354,6 +358,27 @@ class CpmlGpu:
         out = np.zeros(self.shape, self.dtype)
         self._call("cpml_gpu_get_memory", MEMORY.index(name), _ptr(out))
         return out
+
+    def set_attenuation(self, tau_epsilon_nu1, tau_sigma_nu1, tau_epsilon_nu2, tau_sigma_nu2):
+        """2D second order: the viscoelastic step of
+        seismic_CPML_2D_velocity_and_stress_second_order_viscoelastic.f90
+        with these relaxation times (one per SLS); empty arrays: elastic."""
+        t = [np.ascontiguousarray(a, np.float64) for a in (tau_epsilon_nu1, tau_sigma_nu1,
+                                                             tau_epsilon_nu2, tau_sigma_nu2)]
+        self.nsls = len(t[0])
+        self._call("cpml_gpu_set_attenuation", self.nsls, *[_ptr(a) for a in t])
+
+    def attenuation_memory(self, which):
+        """R_e1 (0), R_e11 (1) or R_e13 (2) as (N_SLS, ny, nx)."""
+        out = np.zeros((self.nsls, self.ny, self.nx), self.dtype)
+        self._call("cpml_gpu_get_attenuation_memory", int(which), _ptr(out))
+        return out
+
+    def pressure(self):
+        """Pressure seismograms (nrec, nstep) of the viscoelastic step."""
+        out = np.zeros((max(self.nrec, 1), self.nstep), self.dtype)
+        self._call("cpml_gpu_get_pressure", _ptr(out))
+        return out[:self.nrec]
 
     def velocity_norm(self):
         v = C.c_double()
