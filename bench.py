@@ -19,7 +19,8 @@ interior row, 4000 steps; ``batch`` -- 128 2D 4th-order shots with seeded
 uneven grids 512..2048 per side, 3000 steps, batched solver against 128
 sequential plain-solver runs; ``shipped`` -- 2D2 as shipped (101 x 641, FP64,
 2000 steps); ``3d4`` -- the default 3D workload with the 4th-order step (the
-elastic update of seismic_CPML_3D_viscoelastic_MPI.f90).
+elastic update of seismic_CPML_3D_viscoelastic_MPI.f90); ``2dve`` -- the 2D
+workload in 2nd order with viscoelastic attenuation (3 SLS).
 """
 from __future__ import annotations
 
@@ -265,6 +266,69 @@ def bench_2d(args):
                   _roofline(nb, avg, f"2d_{n}_tile{args.tile or 4}"), cpu)
 
 
+def bench_2dve(args):
+    """2D viscoelastic C-PML (2nd order, 3 SLS): the layered 2D workload with
+    attenuation on."""
+    import numpy as np
+    import torch
+
+    from seismic_cpml_amd import workloads
+
+    steps = args.steps or 4000
+    nstep = args.warmup + steps
+    n = args.n or 4096
+    w = workloads.layered_2d(n=n, nstep=nstep, order=2)
+    taus = workloads.taus_for(w.f0)
+    s = w.make_solver()
+    s.set_attenuation(*taus)
+    slabs = s.slab_widths()
+    wall = _timed_run(s, args.warmup, nstep)
+    torch.cuda.synchronize()
+    value = w.cells * steps / wall / 1e9
+    nb = workloads.bytes_per_step(w, slabs, nsls=3)
+    kt = {name: s.kernel_time(k) for k, name in ((0, "stress"), (1, "velocity"))}
+    avg = {name: ms / max(c, 1) for name, (c, ms) in kt.items()}
+    s.close()
+    cpu = None
+    if not args.no_cpu_baseline:
+        orc = _orc()
+        lam, mu, rho = w.medium_host
+        rows = 2048
+        px, py = w.prof
+        fx, fy = w.force
+        threads = min(16, os.cpu_count() or 1)
+        ixr = np.array([n // 2], np.int32)
+        iyr = np.array([rows // 2], np.int32)
+
+        def run(nst, st):
+            return orc.run2dve(n, rows, w.dx, w.dx, w.dt, nst, lam[:rows], mu[:rows],
+                               rho[:rows], px, py[:, :rows], taus, n // 2, rows // 2,
+                               fx[:nst], fy[:nst], ixr, iyr, prec="f32", state=st,
+                               nthreads=threads)
+
+        out = run(1, None)
+        t = time.perf_counter()
+        out = run(2, out["state"])
+        t1 = (time.perf_counter() - t) / 2
+        k = max(1, min(2000, int(args.cpu_seconds / max(t1, 1e-3))))
+        t = time.perf_counter()
+        run(k, out["state"])
+        el = time.perf_counter() - t
+        cpu = {"value": round(n * rows * k / el / 1e9, 4), "unit": "Gcell/s", "cores": threads,
+               "kind": "port",
+               "sample": f"oracle C loop (reference structure, OpenMP) on {n}x{rows} of the "
+                         f"same medium, {k} steps, {el:.1f} s"}
+    config = {
+        "workload": f"2D viscoelastic C-PML {n}^2, 2nd order, 3 SLS (Qp 65, Qs 55), NPML 20, "
+                    "12 seeded layers, Ricker 10 Hz at the centre, 256 receivers",
+        "grid": [w.nx, w.ny], "npml": w.npml, "deltat": w.dt, "slab_widths": slabs,
+        "bytes_per_cell_update": round(nb["step"] / w.cells, 3),
+        "kernel_ms": {k: round(v, 4) for k, v in avg.items()},
+    }
+    return _line("Gcell-updates/s (2D 4096^2 viscoelastic FP32 C-PML time loop)", value, args,
+                 steps, wall / steps * 1e3, "f32", config, _roofline(nb, avg, None), cpu)
+
+
 def bench_batch(args):
     import numpy as np
     import torch
@@ -414,8 +478,8 @@ def bench_abc(args):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("3d", "3d4", "2d", "batch", "shipped", "abc"),
-                    default="3d")
+    ap.add_argument("--workload",
+                    choices=("3d", "3d4", "2d", "2dve", "batch", "shipped", "abc"), default="3d")
     ap.add_argument("--steps", type=int, default=0, help="timed steps (0: the workload's own)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=0, help="grid points per axis (3d, 2d)")
@@ -454,6 +518,7 @@ def main():
         torch.cuda.synchronize()
         del a, b
     fn = {"3d": bench_3d, "3d4": lambda a: bench_3d(a, order=4), "2d": bench_2d,
+          "2dve": bench_2dve,
           "batch": bench_batch, "shipped": bench_shipped, "abc": bench_abc}
     print(json.dumps(fn[args.workload](args)), flush=True)
 
