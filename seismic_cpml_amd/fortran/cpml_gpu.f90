@@ -29,7 +29,8 @@ module cpml_gpu
        cpml_gpu_set_pml_3d, cpml_gpu_set_source, cpml_gpu_set_receivers, cpml_gpu_set_energy, &
        cpml_gpu_set_stability, cpml_gpu_run, cpml_gpu_get_seismograms, cpml_gpu_get_energy, &
        cpml_gpu_get_field, cpml_gpu_set_field, cpml_gpu_velocity_norm, cpml_gpu_elapsed_ms, &
-       cpml_gpu_error_message, cpml_gpu_create_color_image, cpml_gpu_write_su
+       cpml_gpu_error_message, cpml_gpu_create_color_image, cpml_gpu_write_su, &
+       cpml_gpu_set_attenuation, cpml_gpu_get_pressure
   ! many 2D shots in one launch per phase (no reference counterpart: a survey
   ! is a loop of reference runs)
   public :: cpml_gpu_batch_init, cpml_gpu_batch_finalize, cpml_gpu_batch_set_medium, &
@@ -118,6 +119,18 @@ module cpml_gpu
       type(c_ptr), value :: h
       integer(c_int), value :: which
       type(*), dimension(*), intent(in) :: in
+    end function
+    integer(c_int) function c_set_attenuation(h, nsls, te1, ts1, te2, ts2) &
+         bind(C, name="cpml_gpu_set_attenuation")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: nsls
+      real(c_double), dimension(*), intent(in) :: te1, ts1, te2, ts2
+    end function
+    integer(c_int) function c_get_pressure(h, sp) bind(C, name="cpml_gpu_get_pressure")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      type(*), dimension(*) :: sp
     end function
     integer(c_int) function c_velocity_norm(h, v) bind(C, name="cpml_gpu_velocity_norm")
       import :: c_ptr, c_int, c_double
@@ -379,6 +392,28 @@ contains
     double precision, intent(out) :: velocnorm
     integer, intent(out) :: ierr
     ierr = c_velocity_norm(h, velocnorm)
+  end subroutine
+
+  ! viscoelastic step of seismic_CPML_2D_velocity_and_stress_second_order_
+  ! viscoelastic.f90 (2D, order 2): N_SLS relaxation times per modulus, as
+  ! its SolvOpt fit gives them (nu1: Qp, nu2: Qs); N_SLS = 0: elastic
+  subroutine cpml_gpu_set_attenuation(h, N_SLS, tau_epsilon_nu1, tau_sigma_nu1, &
+                                      tau_epsilon_nu2, tau_sigma_nu2, ierr)
+    type(c_ptr), intent(in) :: h
+    integer, intent(in) :: N_SLS
+    double precision, dimension(*), intent(in) :: tau_epsilon_nu1, tau_sigma_nu1, &
+         tau_epsilon_nu2, tau_sigma_nu2
+    integer, intent(out) :: ierr
+    ierr = c_set_attenuation(h, N_SLS, tau_epsilon_nu1, tau_sigma_nu1, tau_epsilon_nu2, &
+                             tau_sigma_nu2)
+  end subroutine
+
+  ! sispressure(NSTEP, NREC) of the viscoelastic step (its kind is the handle's)
+  subroutine cpml_gpu_get_pressure(h, sispressure, ierr)
+    type(c_ptr), intent(in) :: h
+    type(*), dimension(*) :: sispressure
+    integer, intent(out) :: ierr
+    ierr = c_get_pressure(h, sispressure)
   end subroutine
 
   ! create_color_image of 2D2:861-999 (2D4:729-731 for order 4, ghost-padded;
