@@ -82,6 +82,10 @@ DRIVERS = [
     ("seismic_cpml_3d_isotropic_gpu.F90", "seismic_cpml_3d_small_f32",
      {"WP": 4, "NX_": 40, "NY_": 44, "NZ_": 36, "NSTEP_": 300, "NPML_": 8, "YDEB_": 300,
       "YFIN_": 100}),
+    ("seismic_cpml_2d_visco_gpu.F90", "seismic_cpml_2d_visco", {}),
+    ("seismic_cpml_2d_visco_gpu.F90", "seismic_cpml_2d_visco_small",
+     {"NX_": 161, "NY_": 161, "NSTEP_": 700, "XSRC_": 120, "YSRC_": 120, "NREC_": 3,
+      "XDEB_": 60, "YDEB_": 190, "XFIN_": 190, "YFIN_": 60}),
 ]
 
 

@@ -7,7 +7,8 @@ their ASCII Vx_file_001.dat must match the reference's single-precision
 values, and their PNM snapshots the reference's image files byte for byte.
 The reduced 3D build matches the 3D golden case within the 3D tolerances
 (1e-12 float64, 1e-5 float32 rel-L2), and its timestamp files and cut-plane
-snapshots the reference's.
+snapshots the reference's.  The 2D viscoelastic driver's seismograms (vx, vy,
+pressure) and final fields equal the reference's bit for bit.
 """
 import subprocess
 from pathlib import Path
@@ -133,3 +134,25 @@ def check_display_3d(tmp_path, prec):
             px = np.array(text.split()[4:], np.int64).reshape(g[f"{comp}_{it}"].shape)
             bad = (px != g[f"{comp}_{it}"]).any(-1).sum()
             assert bad <= (0 if prec == "f64" else 0.005 * px.shape[0] * px.shape[1]), (it, comp, bad)
+
+
+def test_visco2d_driver_matches_reference(tmp_path):
+    """The viscoelastic driver (cpml_gpu_set_attenuation / cpml_gpu_get_pressure
+    from Fortran) on the reduced 161 x 161, 700-step case: vx, vy and pressure
+    seismograms and the final vx, vy equal the reference's bit for bit
+    (tests/golden/2dve_small_f64.npz)."""
+    from golden.make_golden import field_digest
+    run_driver("seismic_cpml_2d_visco_small", tmp_path)
+    g = np.load(Path(__file__).resolve().parent / "golden" / "2dve_small_f64.npz")
+    raw = np.fromfile(tmp_path / "gpu_dump.bin", np.float64)
+    nstep, nrec, n = 700, 3, 161
+    off = 0
+    for k in ("sisvx", "sisvy", "sispressure"):
+        a = raw[off:off + nstep * nrec].reshape(nrec, nstep)
+        off += nstep * nrec
+        np.testing.assert_array_equal(a, g[k], err_msg=k)
+    for k in ("vx", "vy"):
+        a = raw[off:off + n * n].reshape(n, n)
+        off += n * n
+        assert field_digest(a) == str(g[f"sha_{k}"]), k
+    assert off == raw.size
