@@ -284,8 +284,10 @@ int cpml_abc_last_error(cpml_abc_net *net, char *buf, int len);
 /* ---- 2D viscoelastic attenuation (2D-visco2 =
  * seismic_CPML_2D_velocity_and_stress_second_order_viscoelastic.f90) ----- */
 
-/* Switches a 2D second-order handle to the viscoelastic step of 2D-visco2
- * (VISCOELASTIC_ATTENUATION, 2D-visco2:736-929): N_SLS standard linear solids
+/* Switches a 2D handle to the viscoelastic step of 2D-visco2 (order 2) or of
+ * seismic_CPML_2D_velocity_and_stress_fourth_order_viscoelastic.f90 (order 4:
+ * the same step with the (9/8, -1/24) difference operator on its ghost-padded
+ * arrays) (VISCOELASTIC_ATTENUATION, 2D-visco2:736-929): N_SLS standard linear solids
  * per modulus with the relaxation times the reference's SolvOpt fit gives
  * (tau_epsilon/tau_sigma for Qp: nu1, for Qs: nu2; 2D-visco2:360-372), memory
  * variables R_e1, R_e11, R_e13 (zeroed here), derivatives multiplied by

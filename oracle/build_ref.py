@@ -50,6 +50,7 @@ SRC_2D4 = "seismic_CPML_2D_isotropic_fourth_order.f90"
 SRC_3D = "seismic_CPML_3D_isotropic_MPI_OpenMP.f90"
 SRC_3DV = "seismic_CPML_3D_viscoelastic_MPI.f90"
 SRC_2DVE = "seismic_CPML_2D_velocity_and_stress_second_order_viscoelastic.f90"
+SRC_2DVE4 = "seismic_CPML_2D_velocity_and_stress_fourth_order_viscoelastic.f90"
 
 # Reduced heterogeneous 2D case: 128 x 96, 5 m, seeded layered medium read
 # from model.bin, Ricker source at the centre, 8 receivers along the top
@@ -104,11 +105,13 @@ VARIANTS["ref3dv4_elastic_f64"] = dict(src=SRC_3DV, prec="f64", params=_SMALL_3D
 # 2D viscoelastic C-PML, 2nd order (2D-visco2), as shipped (Qp 65, Qs 55,
 # N_SLS 3, attenuation on) on a reduced 161 x 161 grid, 700 steps, source at
 # the centre, 3 receivers; float64 only (its SolvOpt include stays double).
+# ref2dve4_f64: the same case of the 4th-order program (2D-visco4).
 VARIANTS["ref2dve_f64"] = dict(src=SRC_2DVE, prec="f64", ndim=2, ghost=0, visco2d=True,
                                params={"NX": "161", "NY": "161", "NSTEP": "700",
                                        "xsource": "120.d0", "ysource": "120.d0", "NREC": "3",
                                        "xdeb": "60.d0", "ydeb": "190.d0", "xfin": "190.d0",
                                        "yfin": "60.d0"})
+VARIANTS["ref2dve4_f64"] = dict(VARIANTS["ref2dve_f64"], src=SRC_2DVE4, ghost=1)
 # ADE8 on a 61 x 61 grid for 60 steps, snapshots every 30: run only for the SU
 # files it writes (seismograms ADE8:987-1003, snapshots ADE8:935-953); no dump.
 VARIANTS["refade8_su_f64"] = dict(src=SRC_ADE8, prec="f64", ndim=2, ghost=0, dump=False,
@@ -482,10 +485,18 @@ def _run_visco2d(v: dict, exe: Path, workdir: Path, timeout: int) -> dict:
         off += a.nbytes
         return a.copy()
 
+    g = v["ghost"]  # the 4th-order program's fields and media are 0:NX+1, 0:NY+1
+
+    def grid():
+        a = take((nx + 2 * g) * (ny + 2 * g)).reshape(ny + 2 * g, nx + 2 * g)
+        return a[g:g + ny, g:g + nx].copy()
+
     out = {"prof_x": take(6 * nx).reshape(6, nx), "prof_y": take(6 * ny).reshape(6, ny)}
     for k in ("sisvx", "sisvy", "sispressure"):
         out[k] = take(nstep * nrec).reshape(nrec, nstep)
-    for k in ("vx", "vy", "sxx", "syy", "sxy", "m_dvx_dx", "m_dvx_dy", "m_dvy_dx", "m_dvy_dy",
+    for k in ("vx", "vy", "sxx", "syy", "sxy"):
+        out[k] = grid()
+    for k in ("m_dvx_dx", "m_dvx_dy", "m_dvy_dx", "m_dvy_dy",
               "m_dsxx_dx", "m_dsyy_dy", "m_dsxy_dx", "m_dsxy_dy"):
         out[k] = take(nx * ny).reshape(ny, nx)
     for k in ("r_e1", "r_e11", "r_e13"):
@@ -493,7 +504,7 @@ def _run_visco2d(v: dict, exe: Path, workdir: Path, timeout: int) -> dict:
     for k in ("tau_eps_nu1", "tau_sig_nu1", "tau_eps_nu2", "tau_sig_nu2"):
         out[k] = take(nsls)
     for k in ("lam", "mu", "rho"):
-        out[k] = take(nx * ny).reshape(ny, nx)
+        out[k] = grid()
     out["ix_rec"] = take(nrec, np.int32)
     out["iy_rec"] = take(nrec, np.int32)
     assert off == len(raw), (off, len(raw))

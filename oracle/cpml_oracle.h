@@ -66,7 +66,8 @@ extern "C" {
                           const int *iyr, const int *izr, int ldseis, REAL *sisvx,       \
                           REAL *sisvy, const int *ewin, int energy_slabs, REAL *etotal,  \
                           int zcut, REAL *state, int nthreads);                          \
-  int oracle_run2dve_##SFX(int nx, int ny, REAL dx, REAL dy, REAL dt, int it_begin,       \
+  int oracle_run2dve_##SFX(int nx, int ny, int order, REAL dx, REAL dy, REAL dt,          \
+                           int it_begin,                                                  \
                            int it_end, const REAL *lambda, const REAL *mu, const REAL *rho, \
                            const REAL *prof_x, const REAL *prof_y, int nsls,              \
                            const REAL *tau_eps1, const REAL *tau_sig1,                    \

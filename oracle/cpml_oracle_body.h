@@ -861,8 +861,11 @@ int ORC(run3d4)(int nx, int ny, int nz, REAL dx, REAL dy, REAL dz, REAL dt, int 
 }
 
 /* --------------------------------------------------------------------------
- * 2D viscoelastic C-PML, 2nd order: seismic_CPML_2D_velocity_and_stress_
- * second_order_viscoelastic.f90 (2D-visco2) with VISCOELASTIC_ATTENUATION.
+ * 2D viscoelastic C-PML: seismic_CPML_2D_velocity_and_stress_
+ * second_order_viscoelastic.f90 (2D-visco2; order 2) and
+ * ..._fourth_order_viscoelastic.f90 (2D-visco4; order 4, which differs only
+ * in its difference operator and its 0:NX+1 ghost-padded fields) with
+ * VISCOELASTIC_ATTENUATION.
  *   coefficients 2D-visco2:374-381 (from tau_epsilon, tau_sigma per SLS),
  *   stress + ADE memory variables 2D-visco2:736-833 (Robertsson et al.
  *   1994 eq. 13-14: R_new = (R_old + d*DELTAT_phi - R_old*DELTAT/(2 tau_s))
@@ -874,17 +877,27 @@ int ORC(run3d4)(int nx, int ny, int nz, REAL dx, REAL dy, REAL dz, REAL dt, int 
  * state: vx vy sxx syy sxy, 8 C-PML memories (as run2d), then R_e1, R_e11,
  * R_e13, each nsls planes; all (ny, nx).
  * ------------------------------------------------------------------------ */
-int ORC(run2dve)(int nx, int ny, REAL dx, REAL dy, REAL dt, int it_begin, int it_end,
+int ORC(run2dve)(int nx, int ny, int order, REAL dx, REAL dy, REAL dt, int it_begin, int it_end,
                  const REAL *lambda, const REAL *mu, const REAL *rho, const REAL *prof_x,
                  const REAL *prof_y, int nsls, const REAL *tau_eps1, const REAL *tau_sig1,
                  const REAL *tau_eps2, const REAL *tau_sig2, int isrc, int jsrc,
                  const REAL *fx, const REAL *fy, int nrec, const int *ixr, const int *iyr,
                  int ldseis, REAL *sisvx, REAL *sisvy, REAL *sisp, REAL *state, int nthreads)
 {
-  if (nx < 2 || ny < 2 || nsls < 1 || nsls > 8) return 1;
+  if (nx < 2 || ny < 2 || nsls < 1 || nsls > 8 || (order != 2 && order != 4)) return 1;
   const size_t nn = (size_t)nx * ny;
-  REAL *vx = state, *vy = state + nn, *sxx = state + 2 * nn, *syy = state + 3 * nn;
-  REAL *sxy = state + 4 * nn;
+  /* ghost-padded copies of the five fields (two cells each side; the
+   * 4th-order program's arrays are 0:NX+1, 0:NY+1 and its ghosts stay 0) */
+  const int G = 2;
+  const size_t px = (size_t)nx + 2 * G, pn = px * ((size_t)ny + 2 * G);
+  REAL *buf = (REAL *)calloc(pn * 5, sizeof(REAL));
+  if (!buf) return 2;
+  REAL *vx = buf, *vy = buf + pn, *sxx = buf + 2 * pn, *syy = buf + 3 * pn, *sxy = buf + 4 * pn;
+  for (int f = 0; f < 5; ++f)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i)
+        buf[(size_t)f * pn + (size_t)(i + G) + (size_t)(j + G) * px] =
+            state[(size_t)f * nn + (size_t)i + (size_t)j * nx];
   REAL *m_dvx_dx = state + 5 * nn, *m_dvx_dy = state + 6 * nn, *m_dvy_dx = state + 7 * nn;
   REAL *m_dvy_dy = state + 8 * nn, *m_dsxx_dx = state + 9 * nn, *m_dsyy_dy = state + 10 * nn;
   REAL *m_dsxy_dx = state + 11 * nn, *m_dsxy_dy = state + 12 * nn;
@@ -894,6 +907,9 @@ int ORC(run2dve)(int nx, int ny, REAL dx, REAL dy, REAL dt, int it_begin, int it
   const REAL *a_y = prof_y, *b_y = prof_y + ny, *K_y = prof_y + 2 * ny;
   const REAL *a_y_half = prof_y + 3 * ny, *b_y_half = prof_y + 4 * ny, *K_y_half = prof_y + 5 * ny;
   const REAL odx = LIT(1.0) / dx, ody = LIT(1.0) / dy; /* 2D-visco2:209-210 */
+  /* 2D-visco4: NINE_OVER_8_DELTA, ONE_OVER_24_DELTA */
+  const REAL n8x = LIT(9.0) / (LIT(8.0) * dx), n8y = LIT(9.0) / (LIT(8.0) * dy);
+  const REAL o24x = LIT(1.0) / (LIT(24.0) * dx), o24y = LIT(1.0) / (LIT(24.0) * dy);
   const REAL two_thirds = LIT(2.0) / LIT(3.0);          /* 2D-visco2:221 */
   /* 2D-visco2:374-381 */
   REAL H1[8], H2[8], M1[8], M2[8], P1[8], P2[8], S1 = 0, S2 = 0;
@@ -911,6 +927,11 @@ int ORC(run2dve)(int nx, int ny, REAL dx, REAL dy, REAL dt, int it_begin, int it
     P2[l] = dt * (LIT(1.0) - tau_eps2[l] / tau_sig2[l]) / tau_sig2[l] / S2;
   }
 #define I2(i, j) ((size_t)((i) - 1) + (size_t)((j) - 1) * nx)
+#define F2(i, j) ((size_t)((i) - 1 + G) + (size_t)((j) - 1 + G) * px)
+/* staggered difference: (f+ - f0) * ONE_OVER_DELTA (order 2), or
+ * (f+ - f0) * NINE_OVER_8_DELTA + (f- - f++) * ONE_OVER_24_DELTA (order 4) */
+#define DV(fp, f0, fm, fpp, od, n8, o24) \
+  (order == 2 ? ((fp) - (f0)) * (od) : ((fp) - (f0)) * (n8) + ((fm) - (fpp)) * (o24))
 #ifdef _OPENMP
   if (nthreads > 0) omp_set_num_threads(nthreads);
 #else
@@ -921,13 +942,13 @@ int ORC(run2dve)(int nx, int ny, REAL dx, REAL dy, REAL dt, int it_begin, int it
 #pragma omp parallel for schedule(static)
     for (int j = 2; j <= ny; ++j)
       for (int i = 1; i <= nx - 1; ++i) {
-        const size_t q = I2(i, j);
+        const size_t q = I2(i, j), f = F2(i, j);
         const REAL lh = LIT(0.5) * (lambda[q + 1] + lambda[q]);
         const REAL mh = LIT(0.5) * (mu[q + 1] + mu[q]);
         const REAL lpm = lh + mh;
         const REAL l2m = lh + LIT(2.0) * mh;
-        REAL dvx_dx = (vx[q + 1] - vx[q]) * odx;
-        REAL dvy_dy = (vy[q] - vy[q - nx]) * ody;
+        REAL dvx_dx = DV(vx[f + 1], vx[f], vx[f - 1], vx[f + 2], odx, n8x, o24x);
+        REAL dvy_dy = DV(vy[f], vy[f - px], vy[f - 2 * px], vy[f + px], ody, n8y, o24y);
         m_dvx_dx[q] = b_x_half[i - 1] * m_dvx_dx[q] + a_x_half[i - 1] * dvx_dx;
         m_dvy_dy[q] = b_y[j - 1] * m_dvy_dy[q] + a_y[j - 1] * dvy_dy;
         dvx_dx = dvx_dx / K_x_half[i - 1] + m_dvx_dx[q];
@@ -943,19 +964,19 @@ int ORC(run2dve)(int nx, int ny, REAL dx, REAL dy, REAL dt, int it_begin, int it
           se1 = se1 + n1 + o1;
           se11 = se11 + n11 + o11;
         }
-        sxx[q] = sxx[q] + (l2m * dvx_dx + lh * dvy_dy +
+        sxx[f] = sxx[f] + (l2m * dvx_dx + lh * dvy_dy +
                            (LIT(0.5) * lpm * se1 + mh * se11)) * dt;
-        syy[q] = syy[q] + (lh * dvx_dx + l2m * dvy_dy +
+        syy[f] = syy[f] + (lh * dvx_dx + l2m * dvy_dy +
                            (LIT(0.5) * lpm * se1 - mh * se11)) * dt;
       }
     /* sigma_xy (2D-visco2:802-833) */
 #pragma omp parallel for schedule(static)
     for (int j = 1; j <= ny - 1; ++j)
       for (int i = 2; i <= nx; ++i) {
-        const size_t q = I2(i, j);
+        const size_t q = I2(i, j), f = F2(i, j);
         const REAL mhy = LIT(0.5) * (mu[q + nx] + mu[q]);
-        REAL dvy_dx = (vy[q] - vy[q - 1]) * odx;
-        REAL dvx_dy = (vx[q + nx] - vx[q]) * ody;
+        REAL dvy_dx = DV(vy[f], vy[f - 1], vy[f - 2], vy[f + 1], odx, n8x, o24x);
+        REAL dvx_dy = DV(vx[f + px], vx[f], vx[f - px], vx[f + 2 * px], ody, n8y, o24y);
         m_dvy_dx[q] = b_x[i - 1] * m_dvy_dx[q] + a_x[i - 1] * dvy_dx;
         m_dvx_dy[q] = b_y_half[j - 1] * m_dvx_dy[q] + a_y_half[j - 1] * dvx_dy;
         dvy_dx = dvy_dx / K_x[i - 1] + m_dvy_dx[q];
@@ -968,66 +989,74 @@ int ORC(run2dve)(int nx, int ny, REAL dx, REAL dy, REAL dt, int it_begin, int it
           *r13 = n13;
           se13 = se13 + n13 + o13;
         }
-        sxy[q] = sxy[q] + mhy * (dvy_dx + dvx_dy + LIT(0.5) * se13) * dt;
+        sxy[f] = sxy[f] + mhy * (dvy_dx + dvx_dy + LIT(0.5) * se13) * dt;
       }
     /* vx (2D-visco2:839-851) */
 #pragma omp parallel for schedule(static)
     for (int j = 2; j <= ny; ++j)
       for (int i = 2; i <= nx; ++i) {
-        const size_t q = I2(i, j);
-        REAL dsxx_dx = (sxx[q] - sxx[q - 1]) * odx;
-        REAL dsxy_dy = (sxy[q] - sxy[q - nx]) * ody;
+        const size_t q = I2(i, j), f = F2(i, j);
+        REAL dsxx_dx = DV(sxx[f], sxx[f - 1], sxx[f - 2], sxx[f + 1], odx, n8x, o24x);
+        REAL dsxy_dy = DV(sxy[f], sxy[f - px], sxy[f - 2 * px], sxy[f + px], ody, n8y, o24y);
         m_dsxx_dx[q] = b_x[i - 1] * m_dsxx_dx[q] + a_x[i - 1] * dsxx_dx;
         m_dsxy_dy[q] = b_y[j - 1] * m_dsxy_dy[q] + a_y[j - 1] * dsxy_dy;
         dsxx_dx = dsxx_dx / K_x[i - 1] + m_dsxx_dx[q];
         dsxy_dy = dsxy_dy / K_y[j - 1] + m_dsxy_dy[q];
-        vx[q] = vx[q] + (dsxx_dx + dsxy_dy) * dt / rho[q];
+        vx[f] = vx[f] + (dsxx_dx + dsxy_dy) * dt / rho[q];
       }
     /* vy (2D-visco2:853-870) */
 #pragma omp parallel for schedule(static)
     for (int j = 1; j <= ny - 1; ++j)
       for (int i = 1; i <= nx - 1; ++i) {
-        const size_t q = I2(i, j);
+        const size_t q = I2(i, j), f = F2(i, j);
         const REAL rh = LIT(0.25) * (rho[q] + rho[q + 1] + rho[q + 1 + nx] + rho[q + nx]);
-        REAL dsxy_dx = (sxy[q + 1] - sxy[q]) * odx;
-        REAL dsyy_dy = (syy[q + nx] - syy[q]) * ody;
+        REAL dsxy_dx = DV(sxy[f + 1], sxy[f], sxy[f - 1], sxy[f + 2], odx, n8x, o24x);
+        REAL dsyy_dy = DV(syy[f + px], syy[f], syy[f - px], syy[f + 2 * px], ody, n8y, o24y);
         m_dsxy_dx[q] = b_x_half[i - 1] * m_dsxy_dx[q] + a_x_half[i - 1] * dsxy_dx;
         m_dsyy_dy[q] = b_y_half[j - 1] * m_dsyy_dy[q] + a_y_half[j - 1] * dsyy_dy;
         dsxy_dx = dsxy_dx / K_x_half[i - 1] + m_dsxy_dx[q];
         dsyy_dy = dsyy_dy / K_y_half[j - 1] + m_dsyy_dy[q];
-        vy[q] = vy[q] + (dsxy_dx + dsyy_dy) * dt / rh;
+        vy[f] = vy[f] + (dsxy_dx + dsyy_dy) * dt / rh;
       }
     /* source (2D-visco2:913-918): fx, fy are force_x, force_y of the step */
     if (isrc >= 1 && fx && fy) {
-      const size_t q = I2(isrc, jsrc);
+      const size_t q = I2(isrc, jsrc), f = F2(isrc, jsrc);
       const REAL rh = LIT(0.25) * (rho[q] + rho[q + 1] + rho[q + 1 + nx] + rho[q + nx]);
-      vx[q] = vx[q] + fx[it - 1] * dt / rho[q];
-      vy[q] = vy[q] + fy[it - 1] * dt / rh;
+      vx[f] = vx[f] + fx[it - 1] * dt / rho[q];
+      vy[f] = vy[f] + fy[it - 1] * dt / rh;
     }
     /* Dirichlet (2D-visco2:921-929) */
     for (int j = 1; j <= ny; ++j) {
-      vx[I2(1, j)] = vy[I2(1, j)] = 0;
-      vx[I2(nx, j)] = vy[I2(nx, j)] = 0;
+      vx[F2(1, j)] = vy[F2(1, j)] = 0;
+      vx[F2(nx, j)] = vy[F2(nx, j)] = 0;
     }
     for (int i = 1; i <= nx; ++i) {
-      vx[I2(i, 1)] = vy[I2(i, 1)] = 0;
-      vx[I2(i, ny)] = vy[I2(i, ny)] = 0;
+      vx[F2(i, 1)] = vy[F2(i, 1)] = 0;
+      vx[F2(i, ny)] = vy[F2(i, ny)] = 0;
     }
     /* seismograms and pressure (2D-visco2:932-990) */
     for (int r = 0; r < nrec; ++r) {
-      const size_t q = I2(ixr[r], iyr[r]);
-      sisvx[(size_t)(it - 1) + (size_t)r * ldseis] = vx[q];
-      sisvy[(size_t)(it - 1) + (size_t)r * ldseis] = vy[q];
+      const size_t q = I2(ixr[r], iyr[r]), f = F2(ixr[r], iyr[r]);
+      sisvx[(size_t)(it - 1) + (size_t)r * ldseis] = vx[f];
+      sisvy[(size_t)(it - 1) + (size_t)r * ldseis] = vy[f];
       if (sisp) {
         const REAL lh = LIT(0.5) * (lambda[q + 1] + lambda[q]);
         const REAL mh = LIT(0.5) * (mu[q + 1] + mu[q]);
         const REAL den = LIT(4.0) * mh * (lh + mh);
-        const REAL exx = ((lh + LIT(2.0) * mh) * sxx[q] - lh * syy[q]) / den;
-        const REAL eyy = ((lh + LIT(2.0) * mh) * syy[q] - lh * sxx[q]) / den;
+        const REAL exx = ((lh + LIT(2.0) * mh) * sxx[f] - lh * syy[f]) / den;
+        const REAL eyy = ((lh + LIT(2.0) * mh) * syy[f] - lh * sxx[f]) / den;
         sisp[(size_t)(it - 1) + (size_t)r * ldseis] = -((lh + two_thirds * mh) * (exx + eyy));
       }
     }
   }
+  for (int f = 0; f < 5; ++f)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i)
+        state[(size_t)f * nn + (size_t)i + (size_t)j * nx] =
+            buf[(size_t)f * pn + (size_t)(i + G) + (size_t)(j + G) * px];
+  free(buf);
+#undef DV
+#undef F2
 #undef I2
   return 0;
 }

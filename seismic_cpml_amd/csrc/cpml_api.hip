@@ -346,6 +346,10 @@ cpml::Visco2D<T> visco2d(cpml_solver *h) {
   }
   v.odx = T(1) / T(h->dx);  // 2D-visco2:209-210
   v.ody = T(1) / T(h->dy);
+  v.n8x = T(9) / (T(8) * T(h->dx));  // 2D-visco4: NINE_OVER_8_DELTAX
+  v.n8y = T(9) / (T(8) * T(h->dy));
+  v.o24x = T(1) / (T(24) * T(h->dx));  // ONE_OVER_24_DELTAX
+  v.o24y = T(1) / (T(24) * T(h->dy));
   v.sisp = static_cast<T *>(h->seis[2]);
   v.recidx = h->recidx;
   v.nrec = h->nrec;
@@ -469,7 +473,8 @@ hipError_t enqueue_fused(cpml_solver *h) {
 
 template <typename T>
 hipError_t enqueue_stress(cpml_solver *h) {
-  if (h->ndim == 2 && h->nsls > 0) return cpml::launch_stress2dve<T>(visco2d<T>(h), h->stream);
+  if (h->ndim == 2 && h->nsls > 0)
+    return cpml::launch_stress2dve<T>(h->order, visco2d<T>(h), h->stream);
   if (h->ndim == 2) return cpml::launch_stress2d<T>(h->order, step2d<T>(h), h->stream);
   return h->order == 4 ? cpml::launch_stress3d4<T>(step3d<T>(h), h->stream)
                        : cpml::launch_stress3d<T>(step3d<T>(h), h->stream);
@@ -477,7 +482,7 @@ hipError_t enqueue_stress(cpml_solver *h) {
 template <typename T>
 hipError_t enqueue_velocity(cpml_solver *h) {
   if (h->ndim == 2 && h->nsls > 0)
-    return cpml::launch_velocity2dve<T>(visco2d<T>(h), h->stream);
+    return cpml::launch_velocity2dve<T>(h->order, visco2d<T>(h), h->stream);
   if (h->ndim == 2) return cpml::launch_velocity2d<T>(h->order, step2d<T>(h), h->stream);
   return h->order == 4 ? cpml::launch_velocity3d4<T>(step3d<T>(h), h->stream)
                        : cpml::launch_velocity3d<T>(step3d<T>(h), h->stream);
@@ -1388,8 +1393,8 @@ int cpml_gpu_set_attenuation(cpml_solver *h, int N_SLS, const double *tau_epsilo
                              const double *tau_sigma_nu1, const double *tau_epsilon_nu2,
                              const double *tau_sigma_nu2) {
   if (!h) return CPML_ERR_ARG;
-  if (h->ndim != 2 || h->order != 2)
-    return fail(h, CPML_ERR_ARG, "attenuation is for the 2D second-order step");
+  if (h->ndim != 2 || (h->order != 2 && h->order != 4))
+    return fail(h, CPML_ERR_ARG, "attenuation is for the 2D second- and fourth-order steps");
   if (N_SLS < 0 || N_SLS > cpml::kMaxSls)
     return fail(h, CPML_ERR_ARG, "N_SLS must be 0.." + std::to_string(cpml::kMaxSls));
   const double *t[4] = {tau_epsilon_nu1, tau_sigma_nu1, tau_epsilon_nu2, tau_sigma_nu2};

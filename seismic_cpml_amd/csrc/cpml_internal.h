@@ -205,14 +205,15 @@ struct Visco2D {
   T H1[kMaxSls], H2[kMaxSls];  // HALF_DELTAT_over_tau_sigma_nu1, _nu2
   T M1[kMaxSls], M2[kMaxSls];  // multiplication_factor_tau_sigma_nu1, _nu2
   T odx, ody;                  // ONE_OVER_DELTAX, ONE_OVER_DELTAY
+  T n8x, n8y, o24x, o24y;      // NINE_OVER_8_DELTA*, ONE_OVER_24_DELTA* (order 4)
   T *sisp;                     // pressure seismograms [NSTEP][NREC]
   const int *recidx;
   int nrec;
 };
 template <typename T>
-hipError_t launch_stress2dve(const Visco2D<T> &v, hipStream_t s);
+hipError_t launch_stress2dve(int order, const Visco2D<T> &v, hipStream_t s);
 template <typename T>
-hipError_t launch_velocity2dve(const Visco2D<T> &v, hipStream_t s);
+hipError_t launch_velocity2dve(int order, const Visco2D<T> &v, hipStream_t s);
 // 4th-order 3D step (elastic part of the 3D viscoelastic program)
 template <typename T>
 hipError_t launch_stress3d4(const Step3D<T> &p, hipStream_t s);

@@ -1,6 +1,8 @@
-// 2D viscoelastic C-PML step (2nd order) for gfx950:
+// 2D viscoelastic C-PML step for gfx950:
 // seismic_CPML_2D_velocity_and_stress_second_order_viscoelastic.f90 with
-// VISCOELASTIC_ATTENUATION (2D-visco2 below):
+// VISCOELASTIC_ATTENUATION (2D-visco2 below), and as ORDER 4 the
+// ..._fourth_order_viscoelastic.f90 program (2D-visco4), which differs only
+// in its difference operator:
 //   stress + attenuation  2D-visco2:736-833
 //   velocity              2D-visco2:839-870, source 913-918, edges 921-929
 //   pressure seismogram   2D-visco2:958-990
@@ -36,7 +38,15 @@ __device__ __forceinline__ bool point2(const Visco2D<T> &v, int &i, int &j) {
   return i < v.s.nx && j < v.s.ny;
 }
 
-template <typename T>
+// staggered difference: (f+ - f0) * ONE_OVER_DELTA (2D-visco2), or
+// (f+ - f0) * NINE_OVER_8_DELTA + (f- - f++) * ONE_OVER_24_DELTA (2D-visco4)
+template <int ORDER, typename T>
+__device__ __forceinline__ T dv(T fp, T f0, T fm, T fpp, T od, T n8, T o24) {
+  if (ORDER == 2) return (fp - f0) * od;
+  return (fp - f0) * n8 + (fm - fpp) * o24;
+}
+
+template <typename T, int ORDER>
 __global__ __launch_bounds__(BX *VTY) void k_stress2dve(Visco2D<T> v) {
   int i, j;
   if (!point2(v, i, j)) return;
@@ -52,8 +62,9 @@ __global__ __launch_bounds__(BX *VTY) void k_stress2dve(Visco2D<T> v) {
     const T mh = T(0.5) * (p.mu[c + 1] + p.mu[c]);
     const T lpm = lh + mh;
     const T l2m = lh + T(2) * mh;
-    T dvx_dx = (p.vx[c + 1] - p.vx[c]) * v.odx;
-    T dvy_dy = (p.vy[c] - p.vy[c - ld]) * v.ody;
+    T dvx_dx = dv<ORDER>(p.vx[c + 1], p.vx[c], p.vx[c - 1], p.vx[c + 2], v.odx, v.n8x, v.o24x);
+    T dvy_dy = dv<ORDER>(p.vy[c], p.vy[c - ld], p.vy[c - 2 * ld], p.vy[c + ld], v.ody, v.n8y,
+                         v.o24y);
     if (inx) dvx_dx = cpml(dvx_dx, p.mx[0] + mxo, p.px.ah[i], p.px.bh[i], p.px.Kh[i]);
     if (iny) dvy_dy = cpml(dvy_dy, p.my[0] + myo, p.py.a[j], p.py.b[j], p.py.K[j]);
     T se1 = 0, se11 = 0;
@@ -73,8 +84,9 @@ __global__ __launch_bounds__(BX *VTY) void k_stress2dve(Visco2D<T> v) {
   // sigma_xy: Fortran i = 2..NX, j = 1..NY-1 (2D-visco2:802-833)
   if (i >= 1 && j <= p.ny - 2) {
     const T mhy = T(0.5) * (p.mu[c + ld] + p.mu[c]);
-    T dvy_dx = (p.vy[c] - p.vy[c - 1]) * v.odx;
-    T dvx_dy = (p.vx[c + ld] - p.vx[c]) * v.ody;
+    T dvy_dx = dv<ORDER>(p.vy[c], p.vy[c - 1], p.vy[c - 2], p.vy[c + 1], v.odx, v.n8x, v.o24x);
+    T dvx_dy = dv<ORDER>(p.vx[c + ld], p.vx[c], p.vx[c - ld], p.vx[c + 2 * ld], v.ody, v.n8y,
+                         v.o24y);
     if (inx) dvy_dx = cpml(dvy_dx, p.mx[1] + mxo, p.px.a[i], p.px.b[i], p.px.K[i]);
     if (iny) dvx_dy = cpml(dvx_dy, p.my[1] + myo, p.py.ah[j], p.py.bh[j], p.py.Kh[j]);
     T se13 = 0;
@@ -89,7 +101,7 @@ __global__ __launch_bounds__(BX *VTY) void k_stress2dve(Visco2D<T> v) {
   }
 }
 
-template <typename T>
+template <typename T, int ORDER>
 __global__ __launch_bounds__(BX *VTY) void k_velocity2dve(Visco2D<T> v) {
   int i, j;
   if (!point2(v, i, j)) return;
@@ -101,8 +113,10 @@ __global__ __launch_bounds__(BX *VTY) void k_velocity2dve(Visco2D<T> v) {
   T vxn = p.vx[c], vyn = p.vy[c];
   // vx: Fortran i = 2..NX, j = 2..NY (2D-visco2:839-851)
   if (i >= 1 && j >= 1) {
-    T dsxx_dx = (p.sxx[c] - p.sxx[c - 1]) * v.odx;
-    T dsxy_dy = (p.sxy[c] - p.sxy[c - ld]) * v.ody;
+    T dsxx_dx = dv<ORDER>(p.sxx[c], p.sxx[c - 1], p.sxx[c - 2], p.sxx[c + 1], v.odx, v.n8x,
+                          v.o24x);
+    T dsxy_dy = dv<ORDER>(p.sxy[c], p.sxy[c - ld], p.sxy[c - 2 * ld], p.sxy[c + ld], v.ody,
+                          v.n8y, v.o24y);
     if (inx) dsxx_dx = cpml(dsxx_dx, p.mx[2] + mxo, p.px.a[i], p.px.b[i], p.px.K[i]);
     if (iny) dsxy_dy = cpml(dsxy_dy, p.my[2] + myo, p.py.a[j], p.py.b[j], p.py.K[j]);
     vxn = vxn + (dsxx_dx + dsxy_dy) * p.dt / p.rho[c];
@@ -110,8 +124,10 @@ __global__ __launch_bounds__(BX *VTY) void k_velocity2dve(Visco2D<T> v) {
   // vy: Fortran i = 1..NX-1, j = 1..NY-1 (2D-visco2:853-870)
   if (i <= p.nx - 2 && j <= p.ny - 2) {
     const T rhh = T(0.25) * (p.rho[c] + p.rho[c + 1] + p.rho[c + 1 + ld] + p.rho[c + ld]);
-    T dsxy_dx = (p.sxy[c + 1] - p.sxy[c]) * v.odx;
-    T dsyy_dy = (p.syy[c + ld] - p.syy[c]) * v.ody;
+    T dsxy_dx = dv<ORDER>(p.sxy[c + 1], p.sxy[c], p.sxy[c - 1], p.sxy[c + 2], v.odx, v.n8x,
+                          v.o24x);
+    T dsyy_dy = dv<ORDER>(p.syy[c + ld], p.syy[c], p.syy[c - ld], p.syy[c + 2 * ld], v.ody,
+                          v.n8y, v.o24y);
     if (inx) dsxy_dx = cpml(dsxy_dx, p.mx[3] + mxo, p.px.ah[i], p.px.bh[i], p.px.Kh[i]);
     if (iny) dsyy_dy = cpml(dsyy_dy, p.my[3] + myo, p.py.ah[j], p.py.bh[j], p.py.Kh[j]);
     vyn = vyn + (dsxy_dx + dsyy_dy) * p.dt / rhh;
@@ -153,24 +169,30 @@ __global__ void k_pressure2d(Visco2D<T> v) {
 }  // namespace
 
 template <typename T>
-hipError_t launch_stress2dve(const Visco2D<T> &v, hipStream_t s) {
+hipError_t launch_stress2dve(int order, const Visco2D<T> &v, hipStream_t s) {
   const dim3 g((v.s.nx + BX - 1) / BX, (v.s.ny + VTY - 1) / VTY);
-  hipLaunchKernelGGL(k_stress2dve<T>, g, dim3(BX, VTY), 0, s, v);
+  if (order == 4)
+    hipLaunchKernelGGL((k_stress2dve<T, 4>), g, dim3(BX, VTY), 0, s, v);
+  else
+    hipLaunchKernelGGL((k_stress2dve<T, 2>), g, dim3(BX, VTY), 0, s, v);
   return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_velocity2dve(const Visco2D<T> &v, hipStream_t s) {
+hipError_t launch_velocity2dve(int order, const Visco2D<T> &v, hipStream_t s) {
   const dim3 g((v.s.nx + BX - 1) / BX, (v.s.ny + VTY - 1) / VTY);
-  hipLaunchKernelGGL(k_velocity2dve<T>, g, dim3(BX, VTY), 0, s, v);
+  if (order == 4)
+    hipLaunchKernelGGL((k_velocity2dve<T, 4>), g, dim3(BX, VTY), 0, s, v);
+  else
+    hipLaunchKernelGGL((k_velocity2dve<T, 2>), g, dim3(BX, VTY), 0, s, v);
   if (v.nrec > 0)
     hipLaunchKernelGGL(k_pressure2d<T>, dim3((v.nrec + 63) / 64), dim3(64), 0, s, v);
   return hipGetLastError();
 }
 
-template hipError_t launch_stress2dve<float>(const Visco2D<float> &, hipStream_t);
-template hipError_t launch_stress2dve<double>(const Visco2D<double> &, hipStream_t);
-template hipError_t launch_velocity2dve<float>(const Visco2D<float> &, hipStream_t);
-template hipError_t launch_velocity2dve<double>(const Visco2D<double> &, hipStream_t);
+template hipError_t launch_stress2dve<float>(int, const Visco2D<float> &, hipStream_t);
+template hipError_t launch_stress2dve<double>(int, const Visco2D<double> &, hipStream_t);
+template hipError_t launch_velocity2dve<float>(int, const Visco2D<float> &, hipStream_t);
+template hipError_t launch_velocity2dve<double>(int, const Visco2D<double> &, hipStream_t);
 
 }  // namespace cpml

@@ -128,16 +128,17 @@ def run_oracle(case: str, prec: str, nstep: int | None = None, d: dict | None = 
                      prec=prec, nthreads=nthreads)
 
 
-def visco2d_inputs(prec: str):
+def visco2d_inputs(prec: str, order: int = 2):
     """Inputs of the reduced 2D viscoelastic case (2D-visco2:149-315 with the
-    oracle/build_ref.py ref2dve_f64 overrides) and its fixture: profiles, tau
+    oracle/build_ref.py ref2dve_f64 overrides; order 4: the same case of the
+    4th-order program, ref2dve4_f64) and its fixture: profiles, tau
     values, medium and receivers are the reference's own (float64 fixture,
     rounded for float32); force table = Ricker (2D-visco2:895) divided by
     DELTAX*DELTAY (2D-visco2:899), angle 0 (force along y only)."""
-    f = np.load(GOLDEN / "2dve_small_f64.npz")
+    f = np.load(GOLDEN / ("2dve_small_f64.npz" if order == 2 else "2dve4_small_f64.npz"))
     dt = np.float64 if prec == "f64" else np.float32
     R = dt
-    d = dict(NX=161, NY=161, nstep=700, NPML=10, dtype=dt)
+    d = dict(NX=161, NY=161, nstep=700, NPML=10, dtype=dt, order=order)
     d["DELTAX"] = d["DELTAY"] = R(1.5)
     d["DELTAT"] = R(2.2e-4) if prec == "f64" else np.float32(2.2e-4)
     f0 = R(35.0)

@@ -20,7 +20,8 @@ that the shipped 2D2 and 2D4 programs write at steps 5, 100 and 200;
 ``display_3d_<prec>.npz`` the timestamp files and cut-plane PNM snapshots of
 the reduced 3D program at its display steps; ``su_ade8.npz`` the SU
 seismogram and snapshot files of a reduced run of the 8th-order program;
-``2dve_small_f64.npz`` the reduced 2D viscoelastic (attenuation) case.
+``2dve_small_f64.npz`` / ``2dve4_small_f64.npz`` the reduced 2D
+viscoelastic (attenuation) case of the 2nd- / 4th-order program.
 This script needs /root/reference and amdflang; the fixtures it writes do not.
 """
 from __future__ import annotations
@@ -146,13 +147,14 @@ def make_display3d(workroot: Path, prec: str = "f64") -> Path:
     return path
 
 
-def make_visco2d(workroot: Path) -> Path:
-    """2dve_small_f64.npz: the 2D viscoelastic program (2nd order, attenuation
-    on) on 161 x 161 for 700 steps: its profiles, tau values of the fitted
-    SLS, medium, receivers, seismograms (vx, vy, pressure), and the digest
-    and every-4th-point subsample of each final field, C-PML memory and
-    attenuation memory variable."""
-    out = build_ref.run("ref2dve_f64", workroot / "ref2dve_f64")
+def make_visco2d(workroot: Path, order: int = 2) -> Path:
+    """2dve_small_f64.npz (2dve4_small_f64.npz): the 2D viscoelastic program
+    of 2nd (4th) order, attenuation on, on 161 x 161 for 700 steps: its
+    profiles, tau values of the fitted SLS, medium, receivers, seismograms
+    (vx, vy, pressure), and the digest and every-4th-point subsample of each
+    final field, C-PML memory and attenuation memory variable."""
+    tag = "2dve" if order == 2 else "2dve4"
+    out = build_ref.run(f"ref{tag}_f64", workroot / f"ref{tag}_f64")
     save = {k: out[k] for k in ("prof_x", "prof_y", "sisvx", "sisvy", "sispressure",
                                 "tau_eps_nu1", "tau_sig_nu1", "tau_eps_nu2", "tau_sig_nu2",
                                 "lam", "mu", "rho", "ix_rec", "iy_rec")}
@@ -160,7 +162,7 @@ def make_visco2d(workroot: Path) -> Path:
               "m_dsxx_dx", "m_dsyy_dy", "m_dsxy_dx", "m_dsxy_dy", "r_e1", "r_e11", "r_e13"):
         save[f"sha_{k}"] = np.array(field_digest(out[k]))
         save[f"sub_{k}"] = out[k][..., ::4, ::4]
-    path = HERE / "2dve_small_f64.npz"
+    path = HERE / f"{tag}_small_f64.npz"
     np.savez_compressed(path, **save)
     return path
 
@@ -184,10 +186,11 @@ def make_su(workroot: Path) -> Path:
 
 def main(argv):
     if argv == ["visco2d"]:
-        build_ref.build(["ref2dve_f64"])
+        build_ref.build(["ref2dve_f64", "ref2dve4_f64"])
         with tempfile.TemporaryDirectory(prefix="cpml_golden_") as tmp:
-            path = make_visco2d(Path(tmp))
-            print(f"{path.name}: {path.stat().st_size / 1024:.0f} KiB")
+            for order in (2, 4):
+                path = make_visco2d(Path(tmp), order)
+                print(f"{path.name}: {path.stat().st_size / 1024:.0f} KiB")
         return
     if argv == ["su"]:
         build_ref.build(["refade8_su_f64"])
@@ -230,9 +233,10 @@ def main(argv):
             build_ref.build(["refade8_su_f64"])
             path = make_su(Path(tmp))
             print(f"{path.name}: {path.stat().st_size / 1024:.0f} KiB")
-            build_ref.build(["ref2dve_f64"])
-            path = make_visco2d(Path(tmp))
-            print(f"{path.name}: {path.stat().st_size / 1024:.0f} KiB")
+            build_ref.build(["ref2dve_f64", "ref2dve4_f64"])
+            for order in (2, 4):
+                path = make_visco2d(Path(tmp), order)
+                print(f"{path.name}: {path.stat().st_size / 1024:.0f} KiB")
 
 
 if __name__ == "__main__":

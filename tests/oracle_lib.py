@@ -174,7 +174,8 @@ STATE2DVE = STATE2D_BASE = ("vx", "vy", "sxx", "syy", "sxy", "m_dvx_dx", "m_dvx_
 
 
 def run2dve(nx, ny, dx, dy, deltat, nstep, lam, mu, rho, prof_x, prof_y, taus, isrc, jsrc,
-            fx, fy, ixr, iyr, prec="f64", state=None, it_begin=1, it_end=None, nthreads=0):
+            fx, fy, ixr, iyr, prec="f64", state=None, it_begin=1, it_end=None, nthreads=0,
+            order=2):
     """2D viscoelastic C-PML loop (2nd order); arrays (ny, nx).  taus: tau_epsilon
     and tau_sigma of nu1 and nu2, each (nsls,).  Returns fields, C-PML memories,
     r_e1 / r_e11 / r_e13 (nsls, ny, nx) and sisvx, sisvy, sispressure."""
@@ -191,8 +192,9 @@ def run2dve(nx, ny, dx, dy, deltat, nstep, lam, mu, rho, prof_x, prof_y, taus, i
     px, py, fx, fy = (np.ascontiguousarray(a, dt) for a in (prof_x, prof_y, fx, fy))
     ixr, iyr = (np.ascontiguousarray(a, np.int32) for a in (ixr, iyr))
     rc = getattr(lib(), f"oracle_run2dve_{s}")(
-        C.c_int(nx), C.c_int(ny), cr(dx), cr(dy), cr(deltat), C.c_int(it_begin),
-        C.c_int(it_end), _ptr(lam), _ptr(mu), _ptr(rho), _ptr(px), _ptr(py), C.c_int(nsls),
+        C.c_int(nx), C.c_int(ny), C.c_int(order), cr(dx), cr(dy), cr(deltat),
+        C.c_int(it_begin), C.c_int(it_end), _ptr(lam), _ptr(mu), _ptr(rho), _ptr(px), _ptr(py),
+        C.c_int(nsls),
         _ptr(te1), _ptr(ts1), _ptr(te2), _ptr(ts2), C.c_int(isrc), C.c_int(jsrc), _ptr(fx),
         _ptr(fy), C.c_int(nrec), _ptr(ixr), _ptr(iyr), C.c_int(nstep), _ptr(sis[0]),
         _ptr(sis[1]), _ptr(sis[2]), _ptr(state), C.c_int(nthreads))

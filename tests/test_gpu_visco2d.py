@@ -1,6 +1,7 @@
 """2D viscoelastic C-PML on the GPU (cpml_gpu_set_attenuation; kernels
-k_stress2dve / k_velocity2dve / k_pressure2d) against the oracle (run2dve),
-which equals the compiled reference bit for bit (tests/test_oracle_visco2d.py).
+k_stress2dve / k_velocity2dve / k_pressure2d, orders 2 and 4: the second- and
+fourth-order viscoelastic programs) against the oracle (run2dve), which
+equals the compiled reference bit for bit (tests/test_oracle_visco2d.py).
 
 Bars: float64 -- every field, C-PML memory variable, attenuation memory
 variable and seismogram sample (vx, vy, pressure) bit-exact against the oracle
@@ -21,7 +22,7 @@ pytestmark = pytest.mark.gpu
 def run_gpu(d, prec, nstep=None, graph_steps=16):
     from seismic_cpml_amd import CpmlGpu
     nstep = nstep or d["nstep"]
-    with CpmlGpu(2, 2, 8 if prec == "f64" else 4, d["NX"], d["NY"], 1, d["DELTAX"],
+    with CpmlGpu(2, d["order"], 8 if prec == "f64" else 4, d["NX"], d["NY"], 1, d["DELTAX"],
                  d["DELTAY"], 1.0, d["DELTAT"], d["NPML"], nstep) as s:
         s.set_medium(d["lam"], d["mu"], d["rho"])
         s.set_pml(d["prof_x"], d["prof_y"])
@@ -44,7 +45,7 @@ def run_oracle(d, prec, nstep=None):
     return orc.run2dve(d["NX"], d["NY"], d["DELTAX"], d["DELTAY"], d["DELTAT"], nstep,
                        d["lam"], d["mu"], d["rho"], d["prof_x"], d["prof_y"], d["taus"],
                        d["ISOURCE"], d["JSOURCE"], d["fx"][:nstep], d["fy"][:nstep],
-                       d["ix_rec"], d["iy_rec"], prec=prec, nthreads=8)
+                       d["ix_rec"], d["iy_rec"], prec=prec, nthreads=8, order=d["order"])
 
 
 KEYS = ("vx", "vy", "sxx", "syy", "sxy", "m_dvx_dx", "m_dvx_dy", "m_dvy_dx", "m_dvy_dy",
@@ -52,9 +53,10 @@ KEYS = ("vx", "vy", "sxx", "syy", "sxy", "m_dvx_dx", "m_dvx_dy", "m_dvy_dx", "m_
         "sisvx", "sisvy", "sispressure")
 
 
+@pytest.mark.parametrize("order", [2, 4])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-def test_visco2d_matches_oracle_bit_exact(prec):
-    d, f = visco2d_inputs(prec)
+def test_visco2d_matches_oracle_bit_exact(prec, order):
+    d, f = visco2d_inputs(prec, order)
     g = run_gpu(d, prec)
     o = run_oracle(d, prec)
     for k in KEYS:

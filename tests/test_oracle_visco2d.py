@@ -1,10 +1,12 @@
 """2D viscoelastic C-PML (seismic_CPML_2D_velocity_and_stress_second_order_
-viscoelastic.f90, attenuation on): the oracle (run2dve) against the compiled
-reference's own outputs (tests/golden/2dve_small_f64.npz, 161 x 161, 700
+viscoelastic.f90 and the fourth-order program, attenuation on): the oracle
+(run2dve) against the compiled reference's own outputs
+(tests/golden/2dve_small_f64.npz, 2dve4_small_f64.npz: 161 x 161, 700
 steps, Qp 65 / Qs 55 with 3 standard linear solids fitted by the reference's
 SolvOpt).  Bar: bit-exact on every seismogram sample (vx, vy, pressure),
 final field, C-PML memory and attenuation memory variable."""
 import numpy as np
+import pytest
 
 from case_inputs import visco2d_inputs
 from golden.make_golden import field_digest
@@ -18,11 +20,12 @@ def run(d, nstep=None, prec="f64"):
     return orc.run2dve(d["NX"], d["NY"], d["DELTAX"], d["DELTAY"], d["DELTAT"],
                        nstep or d["nstep"], d["lam"], d["mu"], d["rho"], d["prof_x"],
                        d["prof_y"], d["taus"], d["ISOURCE"], d["JSOURCE"], d["fx"], d["fy"],
-                       d["ix_rec"], d["iy_rec"], prec=prec, nthreads=4)
+                       d["ix_rec"], d["iy_rec"], prec=prec, nthreads=4, order=d["order"])
 
 
-def test_visco2d_oracle_bit_exact():
-    d, f = visco2d_inputs("f64")
+@pytest.mark.parametrize("order", [2, 4])
+def test_visco2d_oracle_bit_exact(order):
+    d, f = visco2d_inputs("f64", order)
     o = run(d)
     for k in ("sisvx", "sisvy", "sispressure"):
         np.testing.assert_array_equal(o[k], f[k], err_msg=k)
