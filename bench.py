@@ -326,7 +326,7 @@ def bench_2dve(args):
         "kernel_ms": {k: round(v, 4) for k, v in avg.items()},
     }
     return _line("Gcell-updates/s (2D 4096^2 viscoelastic FP32 C-PML time loop)", value, args,
-                 steps, wall / steps * 1e3, "f32", config, _roofline(nb, avg, None), cpu)
+                 steps, wall / steps * 1e3, "f32", config, _roofline(nb, avg, f"2dve_{n}"), cpu)
 
 
 def bench_batch(args):

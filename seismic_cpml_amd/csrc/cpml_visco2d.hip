@@ -53,6 +53,16 @@ __global__ __launch_bounds__(BX *VTY) void k_stress2dve(Visco2D<T> v) {
   const Step2D<T> &p = v.s;
   const long ld = p.ld, c = i + j * ld;
   const long r = i + (long)j * p.nx;  // dense index of the memory variables
+  // the kernel reads v and writes sigma and R: restrict lets the reads of a
+  // point issue together instead of waiting behind its stores
+  const T *__restrict__ vx = p.vx;
+  const T *__restrict__ vy = p.vy;
+  T *__restrict__ sxx = p.sxx;
+  T *__restrict__ syy = p.syy;
+  T *__restrict__ sxy = p.sxy;
+  T *__restrict__ re1 = v.re1;
+  T *__restrict__ re11 = v.re11;
+  T *__restrict__ re13 = v.re13;
   const bool inx = p.sx.in(i), iny = p.sy.in(j);
   const long mxo = p.sx.s(i) + (long)p.sx.w * j;
   const long myo = i + p.mld * p.sy.s(j);
@@ -62,14 +72,13 @@ __global__ __launch_bounds__(BX *VTY) void k_stress2dve(Visco2D<T> v) {
     const T mh = T(0.5) * (p.mu[c + 1] + p.mu[c]);
     const T lpm = lh + mh;
     const T l2m = lh + T(2) * mh;
-    T dvx_dx = dv<ORDER>(p.vx[c + 1], p.vx[c], p.vx[c - 1], p.vx[c + 2], v.odx, v.n8x, v.o24x);
-    T dvy_dy = dv<ORDER>(p.vy[c], p.vy[c - ld], p.vy[c - 2 * ld], p.vy[c + ld], v.ody, v.n8y,
-                         v.o24y);
+    T dvx_dx = dv<ORDER>(vx[c + 1], vx[c], vx[c - 1], vx[c + 2], v.odx, v.n8x, v.o24x);
+    T dvy_dy = dv<ORDER>(vy[c], vy[c - ld], vy[c - 2 * ld], vy[c + ld], v.ody, v.n8y, v.o24y);
     if (inx) dvx_dx = cpml(dvx_dx, p.mx[0] + mxo, p.px.ah[i], p.px.bh[i], p.px.Kh[i]);
     if (iny) dvy_dy = cpml(dvy_dy, p.my[0] + myo, p.py.a[j], p.py.b[j], p.py.K[j]);
     T se1 = 0, se11 = 0;
     for (int l = 0; l < v.nsls; ++l) {
-      T *r1 = v.re1 + r + l * v.nn, *r11 = v.re11 + r + l * v.nn;
+      T *r1 = re1 + r + l * v.nn, *r11 = re11 + r + l * v.nn;
       const T o1 = *r1, o11 = *r11;
       const T n1 = (o1 + (dvx_dx + dvy_dy) * v.P1[l] - o1 * v.H1[l]) * v.M1[l];
       const T n11 = (o11 + T(0.5) * (dvx_dx - dvy_dy) * v.P2[l] - o11 * v.H2[l]) * v.M2[l];
@@ -78,26 +87,25 @@ __global__ __launch_bounds__(BX *VTY) void k_stress2dve(Visco2D<T> v) {
       se1 = se1 + n1 + o1;
       se11 = se11 + n11 + o11;
     }
-    p.sxx[c] = p.sxx[c] + (l2m * dvx_dx + lh * dvy_dy + (T(0.5) * lpm * se1 + mh * se11)) * p.dt;
-    p.syy[c] = p.syy[c] + (lh * dvx_dx + l2m * dvy_dy + (T(0.5) * lpm * se1 - mh * se11)) * p.dt;
+    sxx[c] = sxx[c] + (l2m * dvx_dx + lh * dvy_dy + (T(0.5) * lpm * se1 + mh * se11)) * p.dt;
+    syy[c] = syy[c] + (lh * dvx_dx + l2m * dvy_dy + (T(0.5) * lpm * se1 - mh * se11)) * p.dt;
   }
   // sigma_xy: Fortran i = 2..NX, j = 1..NY-1 (2D-visco2:802-833)
   if (i >= 1 && j <= p.ny - 2) {
     const T mhy = T(0.5) * (p.mu[c + ld] + p.mu[c]);
-    T dvy_dx = dv<ORDER>(p.vy[c], p.vy[c - 1], p.vy[c - 2], p.vy[c + 1], v.odx, v.n8x, v.o24x);
-    T dvx_dy = dv<ORDER>(p.vx[c + ld], p.vx[c], p.vx[c - ld], p.vx[c + 2 * ld], v.ody, v.n8y,
-                         v.o24y);
+    T dvy_dx = dv<ORDER>(vy[c], vy[c - 1], vy[c - 2], vy[c + 1], v.odx, v.n8x, v.o24x);
+    T dvx_dy = dv<ORDER>(vx[c + ld], vx[c], vx[c - ld], vx[c + 2 * ld], v.ody, v.n8y, v.o24y);
     if (inx) dvy_dx = cpml(dvy_dx, p.mx[1] + mxo, p.px.a[i], p.px.b[i], p.px.K[i]);
     if (iny) dvx_dy = cpml(dvx_dy, p.my[1] + myo, p.py.ah[j], p.py.bh[j], p.py.Kh[j]);
     T se13 = 0;
     for (int l = 0; l < v.nsls; ++l) {
-      T *r13 = v.re13 + r + l * v.nn;
+      T *r13 = re13 + r + l * v.nn;
       const T o13 = *r13;
       const T n13 = (o13 + (dvy_dx + dvx_dy) * v.P2[l] - o13 * v.H2[l]) * v.M2[l];
       *r13 = n13;
       se13 = se13 + n13 + o13;
     }
-    p.sxy[c] = p.sxy[c] + mhy * (dvy_dx + dvx_dy + T(0.5) * se13) * p.dt;
+    sxy[c] = sxy[c] + mhy * (dvy_dx + dvx_dy + T(0.5) * se13) * p.dt;
   }
 }
 
@@ -110,13 +118,17 @@ __global__ __launch_bounds__(BX *VTY) void k_velocity2dve(Visco2D<T> v) {
   const bool inx = p.sx.in(i), iny = p.sy.in(j);
   const long mxo = p.sx.s(i) + (long)p.sx.w * j;
   const long myo = i + p.mld * p.sy.s(j);
-  T vxn = p.vx[c], vyn = p.vy[c];
+  const T *__restrict__ sxx = p.sxx;
+  const T *__restrict__ syy = p.syy;
+  const T *__restrict__ sxy = p.sxy;
+  T *__restrict__ vx = p.vx;
+  T *__restrict__ vy = p.vy;
+  T vxn = vx[c], vyn = vy[c];
   // vx: Fortran i = 2..NX, j = 2..NY (2D-visco2:839-851)
   if (i >= 1 && j >= 1) {
-    T dsxx_dx = dv<ORDER>(p.sxx[c], p.sxx[c - 1], p.sxx[c - 2], p.sxx[c + 1], v.odx, v.n8x,
-                          v.o24x);
-    T dsxy_dy = dv<ORDER>(p.sxy[c], p.sxy[c - ld], p.sxy[c - 2 * ld], p.sxy[c + ld], v.ody,
-                          v.n8y, v.o24y);
+    T dsxx_dx = dv<ORDER>(sxx[c], sxx[c - 1], sxx[c - 2], sxx[c + 1], v.odx, v.n8x, v.o24x);
+    T dsxy_dy = dv<ORDER>(sxy[c], sxy[c - ld], sxy[c - 2 * ld], sxy[c + ld], v.ody, v.n8y,
+                          v.o24y);
     if (inx) dsxx_dx = cpml(dsxx_dx, p.mx[2] + mxo, p.px.a[i], p.px.b[i], p.px.K[i]);
     if (iny) dsxy_dy = cpml(dsxy_dy, p.my[2] + myo, p.py.a[j], p.py.b[j], p.py.K[j]);
     vxn = vxn + (dsxx_dx + dsxy_dy) * p.dt / p.rho[c];
@@ -124,10 +136,9 @@ __global__ __launch_bounds__(BX *VTY) void k_velocity2dve(Visco2D<T> v) {
   // vy: Fortran i = 1..NX-1, j = 1..NY-1 (2D-visco2:853-870)
   if (i <= p.nx - 2 && j <= p.ny - 2) {
     const T rhh = T(0.25) * (p.rho[c] + p.rho[c + 1] + p.rho[c + 1 + ld] + p.rho[c + ld]);
-    T dsxy_dx = dv<ORDER>(p.sxy[c + 1], p.sxy[c], p.sxy[c - 1], p.sxy[c + 2], v.odx, v.n8x,
-                          v.o24x);
-    T dsyy_dy = dv<ORDER>(p.syy[c + ld], p.syy[c], p.syy[c - ld], p.syy[c + 2 * ld], v.ody,
-                          v.n8y, v.o24y);
+    T dsxy_dx = dv<ORDER>(sxy[c + 1], sxy[c], sxy[c - 1], sxy[c + 2], v.odx, v.n8x, v.o24x);
+    T dsyy_dy = dv<ORDER>(syy[c + ld], syy[c], syy[c - ld], syy[c + 2 * ld], v.ody, v.n8y,
+                          v.o24y);
     if (inx) dsxy_dx = cpml(dsxy_dx, p.mx[3] + mxo, p.px.ah[i], p.px.bh[i], p.px.Kh[i]);
     if (iny) dsyy_dy = cpml(dsyy_dy, p.my[3] + myo, p.py.ah[j], p.py.bh[j], p.py.Kh[j]);
     vyn = vyn + (dsxy_dx + dsyy_dy) * p.dt / rhh;
@@ -144,8 +155,8 @@ __global__ __launch_bounds__(BX *VTY) void k_velocity2dve(Visco2D<T> v) {
     vxn = T(0);
     vyn = T(0);
   }
-  p.vx[c] = vxn;
-  p.vy[c] = vyn;
+  vx[c] = vxn;
+  vy[c] = vyn;
 }
 
 // Pressure at the receivers from sigma_xx, sigma_yy in plane strain
