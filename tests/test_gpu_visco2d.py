@@ -73,3 +73,36 @@ def test_visco2d_graph_equals_eager():
     b = run_gpu(d, "f32", nstep=150, graph_steps=0)
     for k in KEYS:
         np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+def test_attenuation_and_order4_error_paths():
+    """Refusals of the new entry points, reported before any launch: the
+    attenuation on a 3D handle, non-positive or missing tau values, more than
+    8 SLS, pressure without attenuation; the fused step and the learned
+    boundary on a 3D 4th-order handle.  Each leaves the handle usable."""
+    from seismic_cpml_amd import CpmlError, CpmlGpu, solver
+    t = np.array([0.02, 0.004, 0.0009])
+    with CpmlGpu(3, 4, 4, 32, 32, 32, 5.0, 5.0, 5.0, 1e-4, 4, 10) as s3:
+        with pytest.raises(CpmlError) as e:
+            s3.set_attenuation(t, t, t, t)
+        assert e.value.code == solver.CPML_ERR_ARG
+        with pytest.raises(CpmlError) as e:
+            s3.set_fused(True)
+        assert e.value.code == solver.CPML_ERR_ARG
+    with CpmlGpu(2, 2, 8, 32, 32, 1, 5.0, 5.0, 1.0, 1e-4, 4, 10) as s2:
+        with pytest.raises(CpmlError) as e:
+            s2.pressure()
+        assert e.value.code == solver.CPML_ERR_STATE
+        with pytest.raises(CpmlError) as e:
+            s2.set_attenuation(t, -t, t, t)
+        assert e.value.code == solver.CPML_ERR_ARG
+        nine = np.full(9, 0.01)
+        with pytest.raises(CpmlError) as e:
+            s2.set_attenuation(nine, nine, nine, nine)
+        assert e.value.code == solver.CPML_ERR_ARG
+        s2.set_attenuation(t, t * 0.9, t, t * 0.9)  # a valid call still succeeds
+        s2.set_attenuation([], [], [], [])          # and back to elastic
+    with CpmlGpu(2, 8, 8, 32, 32, 1, 5.0, 5.0, 1.0, 1e-4, 4, 10) as s8:
+        with pytest.raises(CpmlError) as e:
+            s8.set_attenuation(t, t, t, t)
+        assert e.value.code == solver.CPML_ERR_ARG
