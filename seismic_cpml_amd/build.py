@@ -82,6 +82,10 @@ DRIVERS = [
     ("seismic_cpml_3d_isotropic_gpu.F90", "seismic_cpml_3d_small_f32",
      {"WP": 4, "NX_": 40, "NY_": 44, "NZ_": 36, "NSTEP_": 300, "NPML_": 8, "YDEB_": 300,
       "YFIN_": 100}),
+    ("seismic_cpml_3d4_gpu.F90", "seismic_cpml_3d4", {"NSTEP_": 2000}),
+    ("seismic_cpml_3d4_gpu.F90", "seismic_cpml_3d4_small",
+     {"NX_": 40, "NY_": 60, "NZ_": 40, "NSTEP_": 400, "NPML_": 8, "RX1_": 20, "RY1_": 40,
+      "RY2_": 120}),
     ("seismic_cpml_2d_visco_gpu.F90", "seismic_cpml_2d_visco", {}),
     ("seismic_cpml_2d_visco_gpu.F90", "seismic_cpml_2d_visco_small",
      {"NX_": 161, "NY_": 161, "NSTEP_": 700, "XSRC_": 120, "YSRC_": 120, "NREC_": 3,

@@ -8,7 +8,9 @@ values, and their PNM snapshots the reference's image files byte for byte.
 The reduced 3D build matches the 3D golden case within the 3D tolerances
 (1e-12 float64, 1e-5 float32 rel-L2), and its timestamp files and cut-plane
 snapshots the reference's.  The 2D viscoelastic driver's seismograms (vx, vy,
-pressure) and final fields equal the reference's bit for bit.
+pressure) and final fields equal the reference's bit for bit.  The 3D
+4th-order driver's profiles and receivers equal the reference's, and its
+seismograms the oracle's single-domain run.
 """
 import subprocess
 from pathlib import Path
@@ -156,3 +158,34 @@ def test_visco2d_driver_matches_reference(tmp_path):
         off += n * n
         assert field_digest(a) == str(g[f"sha_{k}"]), k
     assert off == raw.size
+
+
+def test_3d4_driver_matches_oracle_and_reference_setup(tmp_path):
+    """The 4th-order 3D driver on the reduced case (40 x 60 x 40, 400 steps):
+    its profiles (K_MAX_PML 7, Rcoef 1e-4) and receiver indices equal the
+    reference's (tests/golden/3d4_small_f64.npz), and its seismograms and
+    final cut plane equal the oracle's single-domain run bit for bit."""
+    from case_inputs import run_oracle
+    run_driver("seismic_cpml_3d4_small", tmp_path)
+    g = fixture("3d4_small", "f64")
+    raw = (tmp_path / "gpu_dump.bin").read_bytes()
+    off = 0
+
+    def take(n, t=np.float64):
+        nonlocal off
+        a = np.frombuffer(raw, t, count=n, offset=off)
+        off += a.nbytes
+        return a
+
+    for key, n in (("prof_x", 40), ("prof_y", 60), ("prof_z", 40)):
+        np.testing.assert_array_equal(take(6 * n).reshape(6, n), g[key], err_msg=key)
+    np.testing.assert_array_equal(take(3, np.int32), g["ix_rec"])
+    np.testing.assert_array_equal(take(3, np.int32), g["iy_rec"])
+    sisvx = take(400 * 3).reshape(3, 400)
+    sisvy = take(400 * 3).reshape(3, 400)
+    plane = take(40 * 60).reshape(60, 40)
+    assert off == len(raw)
+    o = run_oracle("3d4_small", "f64", nthreads=8, energy=False)
+    np.testing.assert_array_equal(sisvx, o["sisvx"])
+    np.testing.assert_array_equal(sisvy, o["sisvy"])
+    np.testing.assert_array_equal(plane, o["vx"][40 // 2 - 1])
