@@ -112,6 +112,11 @@ VARIANTS["ref2dve_f64"] = dict(src=SRC_2DVE, prec="f64", ndim=2, ghost=0, visco2
                                        "xdeb": "60.d0", "ydeb": "190.d0", "xfin": "190.d0",
                                        "yfin": "60.d0"})
 VARIANTS["ref2dve4_f64"] = dict(VARIANTS["ref2dve_f64"], src=SRC_2DVE4, ghost=1)
+# ADE8 (RK4, ADE-PML, 8th order) on 81 x 81 for 300 steps with a binary dump
+# after its time loop (and its SU files, for which it needs its folders).
+VARIANTS["refade8_f64"] = dict(src=SRC_ADE8, prec="f64", ndim=2, ghost=0, ade8=True,
+                               params={"NX": "81", "NY": "81", "NSTEP": "300",
+                                       "IT_DISPLAY": "100"})
 # ADE8 on a 61 x 61 grid for 60 steps, snapshots every 30: run only for the SU
 # files it writes (seismograms ADE8:987-1003, snapshots ADE8:935-953); no dump.
 VARIANTS["refade8_su_f64"] = dict(src=SRC_ADE8, prec="f64", ndim=2, ghost=0, dump=False,
@@ -263,6 +268,27 @@ _DUMP_2DVE = (
     "  close(77)\n")
 
 
+_DUMP_ADE8 = (
+    "  open(unit=77,file='ref_dump.bin',access='stream',form='unformatted',status='replace')\n"
+    "  write(77) a_x,b_x,K_x,a_x_half,b_x_half,K_x_half\n"
+    "  write(77) a_y,b_y,K_y,a_y_half,b_y_half,K_y_half\n"
+    "  write(77) sisvx,sisvy\n"
+    "  write(77) vx,vy,sigmaxx,sigmayy,sigmaxy\n"
+    "  write(77) memory_dvx_dx,memory_dvx_dy,memory_dvy_dx,memory_dvy_dy\n"
+    "  write(77) memory_dsigmaxx_dx,memory_dsigmayy_dy,memory_dsigmaxy_dx,memory_dsigmaxy_dy\n"
+    "  write(77) lambda,mu,rho\n"
+    "  write(77) ix_rec,iy_rec\n"
+    "  close(77)\n")
+
+
+def _patch_ade8(text: str) -> str:
+    """Binary dump after the time loop (ADE8:983)."""
+    lines = text.split("\n")
+    mark = next(n for n, l in enumerate(lines) if "end of time loop" in l)
+    lines.insert(mark + 1, _DUMP_ADE8.rstrip("\n"))
+    return "\n".join(lines)
+
+
 def _patch_visco2d(text: str) -> str:
     """Images and the file-removal shell call off, binary dump after the loop."""
     text = text.replace("call create_color_image(", "if (.false.) call create_color_image(")
@@ -283,6 +309,8 @@ def patch(text: str, v: dict) -> str:
         return _patch_visco(text)
     if v.get("visco2d"):
         return _patch_visco2d(text)
+    if v.get("ade8"):
+        return _patch_ade8(text)
     if v.get("model"):
         text, k = _MODEL_LOOP.subn(_MODEL_READ, text)
         if k != 1:
@@ -371,6 +399,8 @@ def run(name: str, workdir: Path, model: tuple | None = None, timeout: int = 600
     workdir.mkdir(parents=True, exist_ok=True)
     if v.get("visco2d"):
         return _run_visco2d(v, exe, workdir, timeout)
+    if v.get("ade8"):
+        return _run_ade8(v, exe, workdir, timeout)
     if not v.get("dump", True):  # ADE8 writes into these folders (ADE8:942, 994)
         for sub in ("seismograms", "snapshots"):
             (workdir / sub).mkdir(exist_ok=True)
@@ -463,6 +493,49 @@ def run(name: str, workdir: Path, model: tuple | None = None, timeout: int = 600
             assert off == len(raw), (rank, off, len(raw))
         for key, parts in fields.items():
             out[key] = np.concatenate(parts, axis=0)
+    return out
+
+
+def _run_ade8(v: dict, exe: Path, workdir: Path, timeout: int) -> dict:
+    """Run the ADE8 variant.  Arrays keep the program's 4-cell ghost border:
+    fields, memories, media (ny+9, nx+9); profiles a, b (ny|nx+9, 4 stages)
+    as [stage][index], K (n+9)."""
+    nx, ny, nstep = (int(v["params"][k]) for k in ("NX", "NY", "NSTEP"))
+    nrec = nx
+    for sub in ("seismograms", "snapshots"):
+        (workdir / sub).mkdir(exist_ok=True)
+    with open(workdir / "stdout.txt", "w") as log:
+        r = subprocess.run([str(exe)], cwd=workdir, stdout=log, stderr=subprocess.STDOUT,
+                           timeout=timeout)
+    if r.returncode != 0:
+        raise RuntimeError(f"exited with {r.returncode}; see {workdir}/stdout.txt")
+    raw = (workdir / "ref_dump.bin").read_bytes()
+    off = 0
+
+    def take(count, typ=np.float64):
+        nonlocal off
+        a = np.frombuffer(raw, dtype=typ, count=count, offset=off)
+        off += a.nbytes
+        return a.copy()
+
+    out = {}
+    for ax, n in (("x", nx), ("y", ny)):
+        g = n + 9
+        out[f"a_{ax}"] = take(4 * g).reshape(g, 4).T.copy()
+        out[f"b_{ax}"] = take(4 * g).reshape(g, 4).T.copy()
+        out[f"K_{ax}"] = take(g)
+        out[f"ah_{ax}"] = take(4 * g).reshape(g, 4).T.copy()
+        out[f"bh_{ax}"] = take(4 * g).reshape(g, 4).T.copy()
+        out[f"Kh_{ax}"] = take(g)
+    out["sisvx"] = take(nstep * nrec).reshape(nrec, nstep)
+    out["sisvy"] = take(nstep * nrec).reshape(nrec, nstep)
+    shp = (ny + 9, nx + 9)
+    for k in ("vx", "vy", "sxx", "syy", "sxy", "m_dvx_dx", "m_dvx_dy", "m_dvy_dx", "m_dvy_dy",
+              "m_dsxx_dx", "m_dsyy_dy", "m_dsxy_dx", "m_dsxy_dy", "lam", "mu", "rho"):
+        out[k] = take(shp[0] * shp[1]).reshape(shp)
+    out["ix_rec"] = take(nrec, np.int32)
+    out["iy_rec"] = take(nrec, np.int32)
+    assert off == len(raw), (off, len(raw))
     return out
 
 

@@ -7,6 +7,7 @@
 
 #include <math.h>
 #include <stdlib.h>
+#include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif

@@ -74,7 +74,16 @@ extern "C" {
                            const REAL *tau_eps2, const REAL *tau_sig2, int isrc, int jsrc, \
                            const REAL *fx, const REAL *fy, int nrec, const int *ixr,      \
                            const int *iyr, int ldseis, REAL *sisvx, REAL *sisvy,          \
-                           REAL *sisp, REAL *state, int nthreads);
+                           REAL *sisp, REAL *state, int nthreads);                      \
+  int oracle_runade8_##SFX(int nx, int ny, REAL dx, REAL dy, REAL dt, int it_begin,       \
+                           int it_end, int npml, const REAL *lam, const REAL *mu,         \
+                           const REAL *rho, const REAL *ax, const REAL *bx, const REAL *Kx, \
+                           const REAL *axh, const REAL *bxh, const REAL *Kxh,             \
+                           const REAL *ay, const REAL *by, const REAL *Ky,                \
+                           const REAL *ayh, const REAL *byh, const REAL *Kyh, int isrc,   \
+                           int jsrc, REAL f0, REAL t0, REAL factor, REAL angle_deg,       \
+                           int nrec, const int *ixr, const int *iyr, int ldseis,          \
+                           REAL *sisvx, REAL *sisvy, REAL *state, int nthreads);
 
 ORACLE_DECLS(double, d)
 ORACLE_DECLS(float, f)

@@ -1061,6 +1061,188 @@ int ORC(run2dve)(int nx, int ny, int order, REAL dx, REAL dy, REAL dt, int it_be
   return 0;
 }
 
+/* --------------------------------------------------------------------------
+ * ADE-PML, RK4 in time, 8th-order (Holberg) staggered operator:
+ * seismic_ADEPML_2D_elastic_RK4_eighth_order.f90 (ADE8) time loop 692-906.
+ * Per step: dX(2) = dX(3) = X; for each of the 4 stages: dX(1) = dX(3) +
+ * rk41 * dX(2) * DELTAT (all arrays), velocity rates from dsigma(1) with the
+ * ADE memory update inside the PML band (i <= NPML+2, i >= NX-NPML-2, same
+ * in j) using the stage coefficients a(inc), b(inc) (ADE8:718-796), the
+ * force at t = (it-1 + rk41) * DELTAT added to the rates (797-802), the rate
+ * arrays' borders zeroed at every level (803-810), stress rates from dv(1)
+ * (811-861), X += rk42 * dX(2) * DELTAT and the velocity borders zeroed
+ * (863-874); then vx is recorded as the mean of 4 nodes, vy at the node
+ * (877-884).  Arrays keep the 4-cell ghost border of the program
+ * (-4:N+4): index (i+4) + (j+4)*(nx+9) for Fortran (i, j); profiles
+ * a, b as [stage][i+4], K as [i+4].  state: vx vy sxx syy sxy and the 8
+ * memory arrays, each (ny+9)*(nx+9).
+ * ------------------------------------------------------------------------ */
+int ORC(runade8)(int nx, int ny, REAL dx, REAL dy, REAL dt, int it_begin, int it_end, int npml,
+                 const REAL *lam, const REAL *mu, const REAL *rho, const REAL *ax,
+                 const REAL *bx, const REAL *Kx, const REAL *axh, const REAL *bxh,
+                 const REAL *Kxh, const REAL *ay, const REAL *by, const REAL *Ky,
+                 const REAL *ayh, const REAL *byh, const REAL *Kyh, int isrc, int jsrc, REAL f0,
+                 REAL t0, REAL factor, REAL angle_deg, int nrec, const int *ixr, const int *iyr,
+                 int ldseis, REAL *sisvx, REAL *sisvy, REAL *state, int nthreads)
+{
+  if (nx < 10 || ny < 10) return 1;
+  const long W = nx + 9, gx = nx + 9, gy = ny + 9;
+  const size_t nn = (size_t)gx * gy;
+#define A(i, j) ((size_t)((i) + 4) + (size_t)((j) + 4) * W)
+  REAL *X[5], *D[3][5];
+  for (int f = 0; f < 5; ++f) X[f] = state + (size_t)f * nn;
+  REAL *m_dvx_dx = state + 5 * nn, *m_dvx_dy = state + 6 * nn, *m_dvy_dx = state + 7 * nn;
+  REAL *m_dvy_dy = state + 8 * nn, *m_dsxx_dx = state + 9 * nn, *m_dsyy_dy = state + 10 * nn;
+  REAL *m_dsxy_dx = state + 11 * nn, *m_dsxy_dy = state + 12 * nn;
+  REAL *buf = (REAL *)calloc(nn * 15, sizeof(REAL));
+  if (!buf) return 2;
+  for (int l = 0; l < 3; ++l)
+    for (int f = 0; f < 5; ++f) D[l][f] = buf + (size_t)(l * 5 + f) * nn;
+  /* ADE8:263-266, 403-410 */
+  const REAL c1 = LIT(1.231666), c2 = LIT(-1.041182e-1), c3 = LIT(2.063707e-2),
+             c4 = LIT(-3.570998e-3);
+  const REAL rk41[4] = {LIT(0.0), LIT(0.5), LIT(0.5), LIT(1.0)};
+  const REAL rk42[4] = {LIT(1.0) / LIT(6.0), LIT(2.0) / LIT(6.0), LIT(2.0) / LIT(6.0),
+                        LIT(1.0) / LIT(6.0)};
+  const REAL PI = LIT(3.141592653589793238462643);
+  const REAL DEG2RAD = PI / LIT(180.0);
+  /* 8th-order difference, (c1 d1 + c2 d2 + c3 d3 + c4 d4) / delta */
+#define H8(d1, d2, d3, d4, del) ((((c1 * (d1) + c2 * (d2)) + c3 * (d3)) + c4 * (d4)) / (del))
+#define BAND(i, j) ((i) <= npml + 2 || (i) >= nx - npml - 2 || (j) <= npml + 2 || (j) >= ny - npml - 2)
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  for (int it = it_begin; it <= it_end; ++it) {
+    for (int f = 0; f < 5; ++f) {
+      memcpy(D[1][f], X[f], nn * sizeof(REAL));
+      memcpy(D[2][f], X[f], nn * sizeof(REAL));
+    }
+    for (int inc = 0; inc < 4; ++inc) {
+      for (int f = 0; f < 5; ++f)
+        for (size_t q = 0; q < nn; ++q) D[0][f][q] = D[2][f][q] + rk41[inc] * D[1][f][q] * dt;
+      REAL *vx1 = D[0][0], *vy1 = D[0][1], *sxx1 = D[0][2], *syy1 = D[0][3], *sxy1 = D[0][4];
+      REAL *dvx2 = D[1][0], *dvy2 = D[1][1], *dsxx2 = D[1][2], *dsyy2 = D[1][3], *dsxy2 = D[1][4];
+      const REAL *bxs = bx + inc * W, *axs = ax + inc * W, *bxhs = bxh + inc * W,
+                 *axhs = axh + inc * W;
+      const REAL *bys = by + inc * (ny + 9), *ays = ay + inc * (ny + 9),
+                 *byhs = byh + inc * (ny + 9), *ayhs = ayh + inc * (ny + 9);
+      /* velocity rates (ADE8:718-796) */
+#pragma omp parallel for schedule(static)
+      for (int j = 2; j <= ny; ++j)
+        for (int i = 2; i <= nx; ++i) {
+          const size_t q = A(i, j);
+          REAL vx_ = H8(sxx1[q] - sxx1[q - 1], sxx1[q + 1] - sxx1[q - 2], sxx1[q + 2] - sxx1[q - 3],
+                        sxx1[q + 3] - sxx1[q - 4], dx);
+          REAL vy_ = H8(sxy1[q] - sxy1[q - W], sxy1[q + W] - sxy1[q - 2 * W],
+                        sxy1[q + 2 * W] - sxy1[q - 3 * W], sxy1[q + 3 * W] - sxy1[q - 4 * W], dy);
+          if (BAND(i, j)) {
+            m_dsxx_dx[q] = bxs[i + 4] * m_dsxx_dx[q] + axs[i + 4] * vx_;
+            m_dsxy_dy[q] = bys[j + 4] * m_dsxy_dy[q] + ays[j + 4] * vy_;
+            vx_ = vx_ / Kx[i + 4] + m_dsxx_dx[q];
+            vy_ = vy_ / Ky[j + 4] + m_dsxy_dy[q];
+          }
+          dvx2[q] = (vx_ + vy_) / rho[q];
+        }
+#pragma omp parallel for schedule(static)
+      for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+          const size_t q = A(i, j);
+          const REAL rh = LIT(0.25) * (rho[q] + rho[q + 1] + rho[q + 1 + W] + rho[q + W]);
+          REAL sx_ = H8(sxy1[q + 1] - sxy1[q], sxy1[q + 2] - sxy1[q - 1], sxy1[q + 3] - sxy1[q - 2],
+                        sxy1[q + 4] - sxy1[q - 3], dx);
+          REAL sy_ = H8(syy1[q + W] - syy1[q], syy1[q + 2 * W] - syy1[q - W],
+                        syy1[q + 3 * W] - syy1[q - 2 * W], syy1[q + 4 * W] - syy1[q - 3 * W], dy);
+          if (BAND(i, j)) {
+            m_dsxy_dx[q] = bxhs[i + 4] * m_dsxy_dx[q] + axhs[i + 4] * sx_;
+            m_dsyy_dy[q] = byhs[j + 4] * m_dsyy_dy[q] + ayhs[j + 4] * sy_;
+            sx_ = sx_ / Kxh[i + 4] + m_dsxy_dx[q];
+            sy_ = sy_ / Kyh[j + 4] + m_dsyy_dy[q];
+          }
+          dvy2[q] = (sx_ + sy_) / rh;
+        }
+      /* force at the stage time (ADE8:786-802) */
+      {
+        const REAL a = PI * PI * f0 * f0;
+        const REAL t = ((REAL)(it - 1) + rk41[inc]) * dt;
+        const REAL tt = t - t0;
+        const REAL st = -(factor * LIT(2.0) * a * tt * EXP(-(a * (tt * tt))));
+        const REAL fx = SIN(angle_deg * DEG2RAD) * st;
+        const REAL fy = COS(angle_deg * DEG2RAD) * st;
+        const size_t q = A(isrc, jsrc);
+        const REAL rh = LIT(0.25) * (rho[q] + rho[q + 1] + rho[q + 1 + W] + rho[q + W]);
+        dvx2[q] = dvx2[q] + fx / rho[q];
+        dvy2[q] = dvy2[q] + fy / rh;
+      }
+      /* rate borders zeroed at all three levels (ADE8:803-810) */
+      for (int l = 0; l < 3; ++l)
+        for (int f = 0; f < 2; ++f)
+          for (int j = -4; j <= ny + 4; ++j)
+            for (int i = -4; i <= nx + 4; ++i)
+              if (i <= 1 || i >= nx || j <= 1 || j >= ny) D[l][f][A(i, j)] = 0;
+      /* stress rates from the stage velocities (ADE8:811-861) */
+#pragma omp parallel for schedule(static)
+      for (int j = 2; j <= ny; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+          const size_t q = A(i, j);
+          const REAL lh = LIT(0.5) * (lam[q + 1] + lam[q]);
+          const REAL mh = LIT(0.5) * (mu[q + 1] + mu[q]);
+          const REAL l2m = lh + LIT(2.0) * mh;
+          REAL dvx_dx = H8(vx1[q + 1] - vx1[q], vx1[q + 2] - vx1[q - 1], vx1[q + 3] - vx1[q - 2],
+                           vx1[q + 4] - vx1[q - 3], dx);
+          REAL dvy_dy = H8(vy1[q] - vy1[q - W], vy1[q + W] - vy1[q - 2 * W],
+                           vy1[q + 2 * W] - vy1[q - 3 * W], vy1[q + 3 * W] - vy1[q - 4 * W], dy);
+          if (BAND(i, j)) {
+            m_dvx_dx[q] = bxhs[i + 4] * m_dvx_dx[q] + axhs[i + 4] * dvx_dx;
+            m_dvy_dy[q] = bys[j + 4] * m_dvy_dy[q] + ays[j + 4] * dvy_dy;
+            dvx_dx = dvx_dx / Kxh[i + 4] + m_dvx_dx[q];
+            dvy_dy = dvy_dy / Ky[j + 4] + m_dvy_dy[q];
+          }
+          dsxx2[q] = l2m * dvx_dx + lh * dvy_dy;
+          dsyy2[q] = lh * dvx_dx + l2m * dvy_dy;
+        }
+#pragma omp parallel for schedule(static)
+      for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 2; i <= nx; ++i) {
+          const size_t q = A(i, j);
+          const REAL mhy = LIT(0.5) * (mu[q + W] + mu[q]);
+          REAL dvx_dy = H8(vx1[q + W] - vx1[q], vx1[q + 2 * W] - vx1[q - W],
+                           vx1[q + 3 * W] - vx1[q - 2 * W], vx1[q + 4 * W] - vx1[q - 3 * W], dy);
+          REAL dvy_dx = H8(vy1[q] - vy1[q - 1], vy1[q + 1] - vy1[q - 2], vy1[q + 2] - vy1[q - 3],
+                           vy1[q + 3] - vy1[q - 4], dx);
+          if (BAND(i, j)) {
+            m_dvy_dx[q] = bxs[i + 4] * m_dvy_dx[q] + axs[i + 4] * dvy_dx;
+            m_dvx_dy[q] = byhs[j + 4] * m_dvx_dy[q] + ayhs[j + 4] * dvx_dy;
+            dvy_dx = dvy_dx / Kx[i + 4] + m_dvy_dx[q];
+            dvx_dy = dvx_dy / Kyh[j + 4] + m_dvx_dy[q];
+          }
+          dsxy2[q] = mhy * (dvy_dx + dvx_dy);
+        }
+      /* accumulate and zero the velocity borders (ADE8:863-874) */
+      for (int f = 0; f < 5; ++f)
+        for (size_t q = 0; q < nn; ++q) X[f][q] = X[f][q] + rk42[inc] * D[1][f][q] * dt;
+      for (int f = 0; f < 2; ++f)
+        for (int j = -4; j <= ny + 4; ++j)
+          for (int i = -4; i <= nx + 4; ++i)
+            if (i <= 1 || i >= nx || j <= 1 || j >= ny) X[f][A(i, j)] = 0;
+    }
+    /* seismograms (ADE8:877-884) */
+    for (int r = 0; r < nrec; ++r) {
+      const size_t q = A(ixr[r], iyr[r]);
+      sisvx[(size_t)(it - 1) + (size_t)r * ldseis] =
+          (X[0][q] + X[0][q + 1] + X[0][q + W] + X[0][q + 1 + W]) / LIT(4.0);
+      sisvy[(size_t)(it - 1) + (size_t)r * ldseis] = X[1][q];
+    }
+  }
+#undef BAND
+#undef H8
+#undef A
+  free(buf);
+  (void)gy;
+  return 0;
+}
+
 #undef ORC
 #undef CAT
 #undef CAT_

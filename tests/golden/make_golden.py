@@ -6,6 +6,7 @@ reference programs (built by oracle/build_ref.py from /root/reference).
     python tests/golden/make_golden.py display3d  # only the 3D display-step files
     python tests/golden/make_golden.py su         # only the ADE8 SU files
     python tests/golden/make_golden.py visco2d    # only the 2D viscoelastic case
+    python tests/golden/make_golden.py ade8       # only the ADE-PML RK4 case
 
 Each fixture ``<case>_<prec>.npz`` holds the reference's own outputs at full
 precision: damping profiles, receiver indices, seismograms, energy curve(s),
@@ -21,7 +22,8 @@ that the shipped 2D2 and 2D4 programs write at steps 5, 100 and 200;
 the reduced 3D program at its display steps; ``su_ade8.npz`` the SU
 seismogram and snapshot files of a reduced run of the 8th-order program;
 ``2dve_small_f64.npz`` / ``2dve4_small_f64.npz`` the reduced 2D
-viscoelastic (attenuation) case of the 2nd- / 4th-order program.
+viscoelastic (attenuation) case of the 2nd- / 4th-order program;
+``ade8_small_f64.npz`` the reduced ADE-PML RK4 8th-order case.
 This script needs /root/reference and amdflang; the fixtures it writes do not.
 """
 from __future__ import annotations
@@ -167,6 +169,24 @@ def make_visco2d(workroot: Path, order: int = 2) -> Path:
     return path
 
 
+def make_ade8(workroot: Path) -> Path:
+    """ade8_small_f64.npz: the ADE-PML RK4 8th-order program on 81 x 81 for
+    300 steps: its stage-dependent profiles, medium and receivers (with the
+    4-cell ghost border), the seismograms of its 81 receivers, and the digest
+    and every-4th-point subsample of each final field and memory array."""
+    out = build_ref.run("refade8_f64", workroot / "refade8_f64")
+    keys = ("a_x", "b_x", "K_x", "ah_x", "bh_x", "Kh_x", "a_y", "b_y", "K_y", "ah_y", "bh_y",
+            "Kh_y", "sisvx", "sisvy", "lam", "mu", "rho", "ix_rec", "iy_rec")
+    save = {k: out[k] for k in keys}
+    for k in ("vx", "vy", "sxx", "syy", "sxy", "m_dvx_dx", "m_dvx_dy", "m_dvy_dx", "m_dvy_dy",
+              "m_dsxx_dx", "m_dsyy_dy", "m_dsxy_dx", "m_dsxy_dy"):
+        save[f"sha_{k}"] = np.array(field_digest(out[k]))
+        save[f"sub_{k}"] = out[k][::4, ::4]
+    path = HERE / "ade8_small_f64.npz"
+    np.savez_compressed(path, **save)
+    return path
+
+
 SU_FILES = ("seismograms/seisVx.su", "seismograms/seisVy.su", "snapshots/00030snapVx.su",
             "snapshots/00060snapVy.su")
 
@@ -185,6 +205,12 @@ def make_su(workroot: Path) -> Path:
 
 
 def main(argv):
+    if argv == ["ade8"]:
+        build_ref.build(["refade8_f64"])
+        with tempfile.TemporaryDirectory(prefix="cpml_golden_") as tmp:
+            path = make_ade8(Path(tmp))
+            print(f"{path.name}: {path.stat().st_size / 1024:.0f} KiB")
+        return
     if argv == ["visco2d"]:
         build_ref.build(["ref2dve_f64", "ref2dve4_f64"])
         with tempfile.TemporaryDirectory(prefix="cpml_golden_") as tmp:
@@ -237,6 +263,9 @@ def main(argv):
             for order in (2, 4):
                 path = make_visco2d(Path(tmp), order)
                 print(f"{path.name}: {path.stat().st_size / 1024:.0f} KiB")
+            build_ref.build(["refade8_f64"])
+            path = make_ade8(Path(tmp))
+            print(f"{path.name}: {path.stat().st_size / 1024:.0f} KiB")
 
 
 if __name__ == "__main__":

@@ -207,3 +207,33 @@ def run2dve(nx, ny, dx, dy, deltat, nstep, lam, mu, rho, prof_x, prof_y, taus, i
     out.update(state=state, sisvx=sis[0][:nrec], sisvy=sis[1][:nrec],
                sispressure=sis[2][:nrec])
     return out
+
+
+def runade8(nx, ny, dx, dy, deltat, nstep, npml, lam, mu, rho, prof, isrc, jsrc, f0, t0,
+            factor, angle, ixr, iyr, prec="f64", state=None, nthreads=0):
+    """ADE-PML RK4 8th-order loop (seismic_ADEPML_2D_elastic_RK4_eighth_order.f90).
+    Arrays keep the program's 4-cell ghost border: (ny+9, nx+9).  prof: dict
+    with a_x, b_x, ah_x, bh_x (4, nx+9), K_x, Kh_x (nx+9) and the same for y."""
+    dt, cr, s = _real(prec)
+    nrec = len(ixr)
+    shp = (ny + 9, nx + 9)
+    if state is None:
+        state = np.zeros((13,) + shp, dt)
+    state = np.ascontiguousarray(state, dt)
+    sis = [np.zeros((max(nrec, 1), nstep), dt) for _ in range(2)]
+    lam, mu, rho = (np.ascontiguousarray(a, dt) for a in (lam, mu, rho))
+    P = {k: np.ascontiguousarray(v, dt) for k, v in prof.items()}
+    ixr, iyr = (np.ascontiguousarray(a, np.int32) for a in (ixr, iyr))
+    rc = getattr(lib(), f"oracle_runade8_{s}")(
+        C.c_int(nx), C.c_int(ny), cr(dx), cr(dy), cr(deltat), C.c_int(1), C.c_int(nstep),
+        C.c_int(npml), _ptr(lam), _ptr(mu), _ptr(rho),
+        *[_ptr(P[k]) for k in ("a_x", "b_x", "K_x", "ah_x", "bh_x", "Kh_x",
+                               "a_y", "b_y", "K_y", "ah_y", "bh_y", "Kh_y")],
+        C.c_int(isrc), C.c_int(jsrc), cr(f0), cr(t0), cr(factor), cr(angle), C.c_int(nrec),
+        _ptr(ixr), _ptr(iyr), C.c_int(nstep), _ptr(sis[0]), _ptr(sis[1]), _ptr(state),
+        C.c_int(nthreads))
+    if rc != 0:
+        raise RuntimeError(f"oracle_runade8 returned {rc}")
+    out = {k: state[q] for q, k in enumerate(STATE2DVE)}
+    out.update(state=state, sisvx=sis[0][:nrec], sisvy=sis[1][:nrec])
+    return out
