@@ -305,6 +305,23 @@ int cpml_gpu_get_attenuation_memory(cpml_solver *h, int which, void *out);
 /* sispressure(NSTEP, NREC) of 2D-visco2:297, 990 (viscoelastic step only). */
 int cpml_gpu_get_pressure(cpml_solver *h, void *sispressure);
 
+/* ---- ADE-PML RK4, 8th order (ADE8 = seismic_ADEPML_2D_elastic_RK4_eighth_order.f90) */
+
+/* Switches a 2D order-8 handle to the time scheme of ADE8 (ADE8:692-906):
+ * four Runge-Kutta stages per step (rk41 = 0, 1/2, 1/2, 1; rk42 = 1/6, 2/6,
+ * 2/6, 1/6), the Holberg operator divided by DELTA, the auxiliary-
+ * differential-equation PML with the stage tables a_x(4, NX), b_x(4, NX),
+ * a_x_half, b_x_half, a_y(4, NY) ... of ADE8:410-640 (Fortran layout, stage
+ * fastest; the K profiles are those given to cpml_gpu_set_pml) applied where
+ * i <= NPOINTS_PML+2 or i >= NX-NPOINTS_PML-2 (same in j), the force at each
+ * stage time force_x4(4, NSTEP) / force_y4(4, NSTEP) (ADE8:786-796), and vx
+ * recorded as the mean of the four nodes (i..i+1, j..j+1) (ADE8:877-884).
+ * All stage and memory arrays start at zero. */
+int cpml_gpu_set_ade_rk4(cpml_solver *h, const void *a_x, const void *b_x,
+                         const void *a_x_half, const void *b_x_half, const void *a_y,
+                         const void *b_y, const void *a_y_half, const void *b_y_half,
+                         const void *force_x4, const void *force_y4);
+
 /* ---- PNM snapshots (2D2:716-731, 861-999; 3D:1231-1234) ----------------- */
 
 /* Colour image of the current Vx (field_number 1) or Vy (2), computed on the

@@ -75,6 +75,8 @@ extern "C" {
                            const REAL *fx, const REAL *fy, int nrec, const int *ixr,      \
                            const int *iyr, int ldseis, REAL *sisvx, REAL *sisvy,          \
                            REAL *sisp, REAL *state, int nthreads);                      \
+  void oracle_stf_at_##SFX(int n, const REAL *t, REAL f0, REAL t0, REAL factor,           \
+                           REAL angle_deg, int kind, REAL *fx, REAL *fy);                 \
   int oracle_runade8_##SFX(int nx, int ny, REAL dx, REAL dy, REAL dt, int it_begin,       \
                            int it_end, int npml, const REAL *lam, const REAL *mu,         \
                            const REAL *rho, const REAL *ax, const REAL *bx, const REAL *Kx, \

@@ -127,6 +127,30 @@ void ORC(stf)(int nstep, REAL deltat, REAL f0, REAL t0, REAL factor, REAL angle_
   }
 }
 
+/* Same source time function at given times t[0..n-1] (the RK4 stage times
+ * of ADE8:786-796); kind as above. */
+void ORC(stf_at)(int n, const REAL *t, REAL f0, REAL t0, REAL factor, REAL angle_deg, int kind,
+                 REAL *fx, REAL *fy)
+{
+  const REAL PI = LIT(3.141592653589793238462643);
+  const REAL DEG2RAD = PI / LIT(180.0);
+  const REAL a = PI * PI * f0 * f0;
+  const REAL sa = SIN(angle_deg * DEG2RAD);
+  const REAL ca = COS(angle_deg * DEG2RAD);
+  for (int q = 0; q < n; ++q) {
+    const REAL tt = t[q] - t0;
+    REAL st;
+    if (kind == 0)
+      st = factor * EXP(-(a * (tt * tt)));
+    else if (kind == 1)
+      st = -(factor * LIT(2.0) * a * tt * EXP(-(a * (tt * tt))));
+    else
+      st = factor * (LIT(1.0) - LIT(2.0) * a * (tt * tt)) * EXP(-(a * (tt * tt)));
+    fx[q] = sa * st;
+    fy[q] = ca * st;
+  }
+}
+
 /* --------------------------------------------------------------------------
  * a-8  Receiver placement: nearest grid node to points evenly spaced on the
  * segment (xdeb,ydeb)-(xfin,yfin); first minimum in j-outer/i-inner order.

@@ -1,0 +1,209 @@
+// ADE-PML RK4 step of seismic_ADEPML_2D_elastic_RK4_eighth_order.f90 (ADE8)
+// for gfx950: the 8th-order (Holberg) staggered operator, four Runge-Kutta
+// stages per time step and the auxiliary-differential-equation PML whose
+// coefficients depend on the stage (ADE8:692-906):
+//   per step   d2 = d3 = X
+//   per stage  d1 = d3 + rk41 * d2 * DELTAT                 (k_ade_stage)
+//              velocity rates from d1(sigma), ADE memories in the PML band,
+//              force at t = (it-1 + rk41) * DELTAT, rate borders zeroed at
+//              all three levels                            (k_ade_vel)
+//              stress rates from d1(v)                      (k_ade_str)
+//              X += rk42 * d2 * DELTAT, velocity borders 0  (k_ade_acc)
+// Arithmetic statement by statement as there (8th-order difference divided
+// by DELTA; -ffp-contract=off), so float64 results equal the reference.
+// One point per thread over the interior, 64 x 4 blocks; the ghost cells of
+// every array stay zero, as the program's 4-cell borders do.
+#include <hip/hip_runtime.h>
+
+#include "cpml_device.h"
+#include "cpml_internal.h"
+
+namespace cpml {
+namespace {
+
+constexpr int ATY = 4;
+
+template <typename T>
+__device__ __forceinline__ bool pt(const Ade8<T> &a, int &i, int &j) {
+  i = blockIdx.x * BX + threadIdx.x;
+  j = blockIdx.y * ATY + threadIdx.y;
+  return i < a.s.nx && j < a.s.ny;
+}
+
+// (c1 d1 + c2 d2 + c3 d3 + c4 d4) / delta, ADE8:263-266, 720-723
+template <typename T>
+__device__ __forceinline__ T h8(T d1, T d2, T d3, T d4, T del) {
+  return (((T(kHolberg1) * d1 + T(kHolberg2) * d2) + T(kHolberg3) * d3) + T(kHolberg4) * d4) / del;
+}
+
+__device__ __forceinline__ bool band(int I, int J, int nx, int ny, int npml) {  // Fortran indices
+  return I <= npml + 2 || I >= nx - npml - 2 || J <= npml + 2 || J >= ny - npml - 2;
+}
+
+template <typename T>
+__device__ __forceinline__ T rk41(int inc) {
+  return inc == 0 ? T(0) : inc == 3 ? T(1) : T(0.5);
+}
+template <typename T>
+__device__ __forceinline__ T rk42(int inc) {
+  return (inc == 0 || inc == 3) ? T(1) / T(6) : T(2) / T(6);
+}
+
+template <typename T>
+__global__ __launch_bounds__(BX *ATY) void k_ade_copy(Ade8<T> a) {
+  int i, j;
+  if (!pt(a, i, j)) return;
+  const long c = i + j * a.s.ld;
+  T *X[5] = {a.s.vx, a.s.vy, a.s.sxx, a.s.syy, a.s.sxy};
+  for (int f = 0; f < 5; ++f) {
+    const T x = X[f][c];
+    a.d[1][f][c] = x;
+    a.d[2][f][c] = x;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(BX *ATY) void k_ade_stage(Ade8<T> a, int inc) {
+  int i, j;
+  if (!pt(a, i, j)) return;
+  const long c = i + j * a.s.ld;
+  const T r = rk41<T>(inc);
+  for (int f = 0; f < 5; ++f) a.d[0][f][c] = a.d[2][f][c] + r * a.d[1][f][c] * a.s.dt;
+}
+
+template <typename T>
+__global__ __launch_bounds__(BX *ATY) void k_ade_vel(Ade8<T> a, int inc) {
+  int i, j;
+  if (!pt(a, i, j)) return;
+  const Step2D<T> &p = a.s;
+  const long ld = p.ld, c = i + j * ld;
+  const int nx = p.nx, ny = p.ny, I = i + 1, J = j + 1;
+  const bool bd = band(I, J, nx, ny, a.npml);
+  const T *sxx = a.d[0][2], *syy = a.d[0][3], *sxy = a.d[0][4];
+  // vx rates: Fortran i = 2..NX, j = 2..NY (ADE8:718-744)
+  if (i >= 1 && j >= 1) {
+    T vx_ = h8(sxx[c] - sxx[c - 1], sxx[c + 1] - sxx[c - 2], sxx[c + 2] - sxx[c - 3],
+               sxx[c + 3] - sxx[c - 4], a.dx);
+    T vy_ = h8(sxy[c] - sxy[c - ld], sxy[c + ld] - sxy[c - 2 * ld], sxy[c + 2 * ld] - sxy[c - 3 * ld],
+               sxy[c + 3 * ld] - sxy[c - 4 * ld], a.dy);
+    if (bd) {
+      T *m4 = a.mem[4] + c, *m7 = a.mem[7] + c;
+      *m4 = a.bx[inc * nx + i] * *m4 + a.ax[inc * nx + i] * vx_;
+      *m7 = a.by[inc * ny + j] * *m7 + a.ay[inc * ny + j] * vy_;
+      vx_ = vx_ / p.px.K[i] + *m4;
+      vy_ = vy_ / p.py.K[j] + *m7;
+    }
+    a.d[1][0][c] = (vx_ + vy_) / p.rho[c];
+  }
+  // vy rates: Fortran i = 1..NX-1, j = 1..NY-1 (ADE8:746-768)
+  if (i <= nx - 2 && j <= ny - 2) {
+    const T rh = T(0.25) * (p.rho[c] + p.rho[c + 1] + p.rho[c + 1 + ld] + p.rho[c + ld]);
+    T sx_ = h8(sxy[c + 1] - sxy[c], sxy[c + 2] - sxy[c - 1], sxy[c + 3] - sxy[c - 2],
+               sxy[c + 4] - sxy[c - 3], a.dx);
+    T sy_ = h8(syy[c + ld] - syy[c], syy[c + 2 * ld] - syy[c - ld], syy[c + 3 * ld] - syy[c - 2 * ld],
+               syy[c + 4 * ld] - syy[c - 3 * ld], a.dy);
+    if (bd) {
+      T *m6 = a.mem[6] + c, *m5 = a.mem[5] + c;
+      *m6 = a.bxh[inc * nx + i] * *m6 + a.axh[inc * nx + i] * sx_;
+      *m5 = a.byh[inc * ny + j] * *m5 + a.ayh[inc * ny + j] * sy_;
+      sx_ = sx_ / p.px.Kh[i] + *m6;
+      sy_ = sy_ / p.py.Kh[j] + *m5;
+    }
+    a.d[1][1][c] = (sx_ + sy_) / rh;
+  }
+  // force at the stage time (ADE8:786-802)
+  if (i == p.isrc && j == p.jsrc) {
+    const int it = *p.step;
+    const T rh = T(0.25) * (p.rho[c] + p.rho[c + 1] + p.rho[c + 1 + ld] + p.rho[c + ld]);
+    a.d[1][0][c] = a.d[1][0][c] + a.f4x[(long)(it - 1) * 4 + inc] / p.rho[c];
+    a.d[1][1][c] = a.d[1][1][c] + a.f4y[(long)(it - 1) * 4 + inc] / rh;
+  }
+  // rate borders at all three levels (ADE8:803-810)
+  if (i == 0 || i == nx - 1 || j == 0 || j == ny - 1)
+    for (int l = 0; l < 3; ++l) {
+      a.d[l][0][c] = T(0);
+      a.d[l][1][c] = T(0);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(BX *ATY) void k_ade_str(Ade8<T> a, int inc) {
+  int i, j;
+  if (!pt(a, i, j)) return;
+  const Step2D<T> &p = a.s;
+  const long ld = p.ld, c = i + j * ld;
+  const int nx = p.nx, ny = p.ny, I = i + 1, J = j + 1;
+  const bool bd = band(I, J, nx, ny, a.npml);
+  const T *vx = a.d[0][0], *vy = a.d[0][1];
+  // sigma_xx, sigma_yy rates: Fortran i = 1..NX-1, j = 2..NY (ADE8:811-834)
+  if (i <= nx - 2 && j >= 1) {
+    const T lh = T(0.5) * (p.lam[c + 1] + p.lam[c]);
+    const T mh = T(0.5) * (p.mu[c + 1] + p.mu[c]);
+    const T l2m = lh + T(2) * mh;
+    T dvx_dx = h8(vx[c + 1] - vx[c], vx[c + 2] - vx[c - 1], vx[c + 3] - vx[c - 2],
+                  vx[c + 4] - vx[c - 3], a.dx);
+    T dvy_dy = h8(vy[c] - vy[c - ld], vy[c + ld] - vy[c - 2 * ld], vy[c + 2 * ld] - vy[c - 3 * ld],
+                  vy[c + 3 * ld] - vy[c - 4 * ld], a.dy);
+    if (bd) {
+      T *m0 = a.mem[0] + c, *m3 = a.mem[3] + c;
+      *m0 = a.bxh[inc * nx + i] * *m0 + a.axh[inc * nx + i] * dvx_dx;
+      *m3 = a.by[inc * ny + j] * *m3 + a.ay[inc * ny + j] * dvy_dy;
+      dvx_dx = dvx_dx / p.px.Kh[i] + *m0;
+      dvy_dy = dvy_dy / p.py.K[j] + *m3;
+    }
+    a.d[1][2][c] = l2m * dvx_dx + lh * dvy_dy;
+    a.d[1][3][c] = lh * dvx_dx + l2m * dvy_dy;
+  }
+  // sigma_xy rate: Fortran i = 2..NX, j = 1..NY-1 (ADE8:836-861)
+  if (i >= 1 && j <= ny - 2) {
+    const T mhy = T(0.5) * (p.mu[c + ld] + p.mu[c]);
+    T dvx_dy = h8(vx[c + ld] - vx[c], vx[c + 2 * ld] - vx[c - ld], vx[c + 3 * ld] - vx[c - 2 * ld],
+                  vx[c + 4 * ld] - vx[c - 3 * ld], a.dy);
+    T dvy_dx = h8(vy[c] - vy[c - 1], vy[c + 1] - vy[c - 2], vy[c + 2] - vy[c - 3],
+                  vy[c + 3] - vy[c - 4], a.dx);
+    if (bd) {
+      T *m2 = a.mem[2] + c, *m1 = a.mem[1] + c;
+      *m2 = a.bx[inc * nx + i] * *m2 + a.ax[inc * nx + i] * dvy_dx;
+      *m1 = a.byh[inc * ny + j] * *m1 + a.ayh[inc * ny + j] * dvx_dy;
+      dvy_dx = dvy_dx / p.px.K[i] + *m2;
+      dvx_dy = dvx_dy / p.py.Kh[j] + *m1;
+    }
+    a.d[1][4][c] = mhy * (dvy_dx + dvx_dy);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(BX *ATY) void k_ade_acc(Ade8<T> a, int inc) {
+  int i, j;
+  if (!pt(a, i, j)) return;
+  const Step2D<T> &p = a.s;
+  const long c = i + j * p.ld;
+  const T r = rk42<T>(inc);
+  T *X[5] = {p.vx, p.vy, p.sxx, p.syy, p.sxy};
+  for (int f = 0; f < 5; ++f) X[f][c] = X[f][c] + r * a.d[1][f][c] * p.dt;
+  // velocity borders (ADE8:865-872)
+  if (i == 0 || i == p.nx - 1 || j == 0 || j == p.ny - 1) {
+    p.vx[c] = T(0);
+    p.vy[c] = T(0);
+  }
+}
+
+}  // namespace
+
+template <typename T>
+hipError_t launch_ade8_step(const Ade8<T> &a, hipStream_t s) {
+  const dim3 g((a.s.nx + BX - 1) / BX, (a.s.ny + ATY - 1) / ATY), b(BX, ATY);
+  hipLaunchKernelGGL(k_ade_copy<T>, g, b, 0, s, a);
+  for (int inc = 0; inc < 4; ++inc) {
+    hipLaunchKernelGGL(k_ade_stage<T>, g, b, 0, s, a, inc);
+    hipLaunchKernelGGL(k_ade_vel<T>, g, b, 0, s, a, inc);
+    hipLaunchKernelGGL(k_ade_str<T>, g, b, 0, s, a, inc);
+    hipLaunchKernelGGL(k_ade_acc<T>, g, b, 0, s, a, inc);
+  }
+  return hipGetLastError();
+}
+
+template hipError_t launch_ade8_step<float>(const Ade8<float> &, hipStream_t);
+template hipError_t launch_ade8_step<double>(const Ade8<double> &, hipStream_t);
+
+}  // namespace cpml

@@ -163,6 +163,12 @@ struct cpml_solver {
   int nsls = 0;                 // 0: elastic
   double tau[4][cpml::kMaxSls] = {};  // tau_epsilon_nu1, tau_sigma_nu1, _nu2, _nu2
   void *re[3] = {};             // R_e1, R_e11, R_e13: nsls dense nx*ny planes each
+  // ADE-PML RK4 step of the 8th-order program (cpml_gpu_set_ade_rk4)
+  bool ade = false;
+  void *ade_d[15] = {};         // stage copies [level][field], padded like the fields
+  void *ade_mem[8] = {};        // memory arrays, padded like the fields
+  void *ade_tab[8] = {};        // a_x b_x a_x_half b_x_half a_y b_y a_y_half b_y_half: [4][n]
+  void *ade_f4[2] = {};         // force at each stage time: [NSTEP][4]
   std::string err;
 };
 
@@ -418,7 +424,25 @@ cpml::Record<T> record_of(cpml_solver *h) {
   r.sy = static_cast<T *>(h->seis[1]);
   r.sz = h->ndim == 3 ? static_cast<T *>(h->seis[2]) : nullptr;
   r.step = h->step;
+  r.avg4 = h->ade ? 1 : 0;
   return r;
+}
+
+template <typename T>
+cpml::Ade8<T> ade_of(cpml_solver *h) {
+  cpml::Ade8<T> a{};
+  a.s = step2d<T>(h);
+  for (int l = 0; l < 3; ++l)
+    for (int f = 0; f < 5; ++f) a.d[l][f] = static_cast<T *>(h->ade_d[l * 5 + f]) + h->base;
+  for (int q = 0; q < 8; ++q) a.mem[q] = static_cast<T *>(h->ade_mem[q]) + h->base;
+  const T **tab[8] = {&a.ax, &a.bx, &a.axh, &a.bxh, &a.ay, &a.by, &a.ayh, &a.byh};
+  for (int q = 0; q < 8; ++q) *tab[q] = static_cast<const T *>(h->ade_tab[q]);
+  a.f4x = static_cast<const T *>(h->ade_f4[0]);
+  a.f4y = static_cast<const T *>(h->ade_f4[1]);
+  a.dx = T(h->dx);  // ADE8 divides by DELTAX, DELTAY (ADE8:721-723)
+  a.dy = T(h->dy);
+  a.npml = h->npml;
+  return a;
 }
 
 template <typename T>
@@ -530,7 +554,9 @@ hipError_t enqueue_boundary(cpml_solver *h) {
 template <typename T>
 hipError_t enqueue_step(cpml_solver *h) {
   hipError_t e;
-  if (h->fused) {
+  if (h->ade) {
+    e = cpml::launch_ade8_step<T>(ade_of<T>(h), h->stream);
+  } else if (h->fused) {
     e = enqueue_fused<T>(h);
   } else {
     e = enqueue_stress<T>(h);
@@ -1419,6 +1445,56 @@ int cpml_gpu_set_attenuation(cpml_solver *h, int N_SLS, const double *tau_epsilo
     HIPCHK(h, hipMemsetAsync(h->re[k], 0, bytes, h->stream));
   }
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->graph_dirty = true;
+  return CPML_OK;
+}
+
+int cpml_gpu_set_ade_rk4(cpml_solver *h, const void *a_x, const void *b_x,
+                         const void *a_x_half, const void *b_x_half, const void *a_y,
+                         const void *b_y, const void *a_y_half, const void *b_y_half,
+                         const void *force_x4, const void *force_y4) {
+  if (!h) return CPML_ERR_ARG;
+  if (h->ndim != 2 || h->order != 8)
+    return fail(h, CPML_ERR_ARG, "the ADE-PML RK4 step is for 2D order-8 handles");
+  if (!h->have_pml) return fail(h, CPML_ERR_STATE, "call set_pml first (K profiles)");
+  const void *tab[8] = {a_x, b_x, a_x_half, b_x_half, a_y, b_y, a_y_half, b_y_half};
+  for (auto t : tab)
+    if (!t) return fail(h, CPML_ERR_ARG, "null ADE coefficient table");
+  if (!force_x4 || !force_y4) return fail(h, CPML_ERR_ARG, "null stage force table");
+  int rc = set_device(h);
+  if (rc) return rc;
+  const size_t fb = (size_t)h->total * h->esize;
+  for (int q = 0; q < 15; ++q)
+    if (!h->ade_d[q]) {
+      if ((rc = dev_alloc(h, &h->ade_d[q], fb))) return rc;
+    }
+  for (int q = 0; q < 8; ++q)
+    if (!h->ade_mem[q]) {
+      if ((rc = dev_alloc(h, &h->ade_mem[q], fb))) return rc;
+    }
+  for (int q = 0; q < 15; ++q) HIPCHK(h, hipMemsetAsync(h->ade_d[q], 0, fb, h->stream));
+  for (int q = 0; q < 8; ++q) HIPCHK(h, hipMemsetAsync(h->ade_mem[q], 0, fb, h->stream));
+  // tables: Fortran a(4, N) (stage fastest) stored as [stage][i]
+  for (int q = 0; q < 8; ++q) {
+    const int n = q < 4 ? h->nx : h->ny;
+    const size_t bytes = (size_t)4 * n * h->esize;
+    if (!h->ade_tab[q] && (rc = dev_alloc(h, &h->ade_tab[q], bytes))) return rc;
+    std::vector<char> t(bytes);
+    const char *src = static_cast<const char *>(tab[q]);
+    for (int i = 0; i < n; ++i)
+      for (int s = 0; s < 4; ++s)
+        std::memcpy(t.data() + ((size_t)s * n + i) * h->esize,
+                    src + ((size_t)i * 4 + s) * h->esize, h->esize);
+    HIPCHK(h, hipMemcpy(h->ade_tab[q], t.data(), bytes, hipMemcpyHostToDevice));
+  }
+  const size_t f4 = (size_t)4 * h->nstep * h->esize;
+  const void *fsrc[2] = {force_x4, force_y4};
+  for (int q = 0; q < 2; ++q) {
+    if (!h->ade_f4[q] && (rc = dev_alloc(h, &h->ade_f4[q], f4))) return rc;
+    HIPCHK(h, hipMemcpy(h->ade_f4[q], fsrc[q], f4, hipMemcpyHostToDevice));
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->ade = true;
   h->graph_dirty = true;
   return CPML_OK;
 }

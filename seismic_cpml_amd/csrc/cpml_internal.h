@@ -146,6 +146,7 @@ struct Record {
   const T *vx, *vy, *vz;  // interior origin
   long ld, ps;
   int nrec;
+  int avg4;        // 1: vx as the mean of (i,j), (i+1,j), (i,j+1), (i+1,j+1) (ADE8:879-882)
   const int *idx;  // nrec offsets (i + j*ld + k*ps)
   T *sx, *sy, *sz;  // [nstep][nrec]
   int *step;
@@ -192,6 +193,23 @@ template <typename T>
 hipError_t launch_stress3d(const Step3D<T> &p, hipStream_t s);
 template <typename T>
 hipError_t launch_velocity3d(const Step3D<T> &p, hipStream_t s);
+// ADE-PML RK4 step of the 8th-order program: the 2D fields (s), three stage
+// copies of vx vy sxx syy sxy (d[level][field], padded like the fields), the
+// eight memory arrays (padded), the stage tables a, b ([4][n]: x, x_half, y,
+// y_half) and the force at each stage time ([NSTEP][4]).
+template <typename T>
+struct Ade8 {
+  Step2D<T> s;
+  T *d[3][5];
+  T *mem[8];  // dvx_dx dvx_dy dvy_dx dvy_dy dsxx_dx dsyy_dy dsxy_dx dsxy_dy
+  const T *ax, *bx, *axh, *bxh, *ay, *by, *ayh, *byh;
+  const T *f4x, *f4y;
+  T dx, dy;
+  int npml;
+};
+template <typename T>
+hipError_t launch_ade8_step(const Ade8<T> &a, hipStream_t s);
+
 // 2D viscoelastic step: the 2D step plus the attenuation memory variables
 // (dense nx*ny per SLS plane) and their per-SLS coefficients.
 constexpr int kMaxSls = 8;

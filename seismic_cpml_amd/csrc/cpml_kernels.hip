@@ -388,7 +388,8 @@ __global__ __launch_bounds__(kMaxRecBlock) void k_record(Record<T> p) {
   const long row = (long)(it - 1) * p.nrec;
   for (int r = threadIdx.x; r < p.nrec; r += blockDim.x) {
     const long q = p.idx[r];
-    p.sx[row + r] = p.vx[q];
+    p.sx[row + r] = p.avg4 ? (p.vx[q] + p.vx[q + 1] + p.vx[q + p.ld] + p.vx[q + 1 + p.ld]) / T(4)
+                           : p.vx[q];
     p.sy[row + r] = p.vy[q];
     if (p.sz) p.sz[row + r] = p.vz[q];
   }

@@ -61,6 +61,7 @@ SIGNATURES = {
     "cpml_gpu_set_attenuation": [_VP, _I, _VP, _VP, _VP, _VP],
     "cpml_gpu_get_attenuation_memory": [_VP, _I, _VP],
     "cpml_gpu_get_pressure": [_VP, _VP],
+    "cpml_gpu_set_ade_rk4": [_VP] + [_VP] * 10,
     "cpml_gpu_set_graph_steps": [_VP, _I],
     "cpml_gpu_set_kchunk": [_VP, _I],
     "cpml_gpu_set_tile": [_VP, _I],
@@ -367,6 +368,21 @@ class CpmlGpu:
                                                              tau_epsilon_nu2, tau_sigma_nu2)]
         self.nsls = len(t[0])
         self._call("cpml_gpu_set_attenuation", self.nsls, *[_ptr(a) for a in t])
+
+    def set_ade_rk4(self, tables, force_x4, force_y4):
+        """2D order 8: the ADE-PML RK4 step of
+        seismic_ADEPML_2D_elastic_RK4_eighth_order.f90.  tables: dict of
+        a_x, b_x, ah_x, bh_x (4, NX) and a_y, b_y, ah_y, bh_y (4, NY), stage
+        first; force_x4, force_y4: (NSTEP, 4) force at each stage time."""
+        def fort(a, n):  # Fortran a(4, n): stage fastest = C (n, 4)
+            a = np.asarray(a)
+            assert a.shape == (4, n)
+            return np.ascontiguousarray(a.T, self.dtype)
+        t = [fort(tables[k], self.nx) for k in ("a_x", "b_x", "ah_x", "bh_x")]
+        t += [fort(tables[k], self.ny) for k in ("a_y", "b_y", "ah_y", "bh_y")]
+        f = [np.ascontiguousarray(np.asarray(a).reshape(self.nstep, 4), self.dtype)
+             for a in (force_x4, force_y4)]
+        self._call("cpml_gpu_set_ade_rk4", *[_ptr(a) for a in t + f])
 
     def attenuation_memory(self, which):
         """R_e1 (0), R_e11 (1) or R_e13 (2) as (N_SLS, ny, nx)."""

@@ -237,3 +237,14 @@ def runade8(nx, ny, dx, dy, deltat, nstep, npml, lam, mu, rho, prof, isrc, jsrc,
     out = {k: state[q] for q, k in enumerate(STATE2DVE)}
     out.update(state=state, sisvx=sis[0][:nrec], sisvy=sis[1][:nrec])
     return out
+
+
+def stf_at(t, f0, t0, factor, angle, kind, prec="f64"):
+    """Source time function at the given times (e.g. the RK4 stage times)."""
+    dt, cr, s = _real(prec)
+    t = np.ascontiguousarray(t, dt)
+    fx = np.zeros(t.size, dt)
+    fy = np.zeros(t.size, dt)
+    getattr(lib(), f"oracle_stf_at_{s}")(C.c_int(t.size), _ptr(t), cr(f0), cr(t0), cr(factor),
+                                         cr(angle), C.c_int(kind), _ptr(fx), _ptr(fy))
+    return fx, fy
