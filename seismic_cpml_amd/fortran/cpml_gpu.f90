@@ -30,7 +30,7 @@ module cpml_gpu
        cpml_gpu_set_stability, cpml_gpu_run, cpml_gpu_get_seismograms, cpml_gpu_get_energy, &
        cpml_gpu_get_field, cpml_gpu_set_field, cpml_gpu_velocity_norm, cpml_gpu_elapsed_ms, &
        cpml_gpu_error_message, cpml_gpu_create_color_image, cpml_gpu_write_su, &
-       cpml_gpu_set_attenuation, cpml_gpu_get_pressure
+       cpml_gpu_set_attenuation, cpml_gpu_get_pressure, cpml_gpu_set_ade_rk4
   ! many 2D shots in one launch per phase (no reference counterpart: a survey
   ! is a loop of reference runs)
   public :: cpml_gpu_batch_init, cpml_gpu_batch_finalize, cpml_gpu_batch_set_medium, &
@@ -126,6 +126,12 @@ module cpml_gpu
       type(c_ptr), value :: h
       integer(c_int), value :: nsls
       real(c_double), dimension(*), intent(in) :: te1, ts1, te2, ts2
+    end function
+    integer(c_int) function c_set_ade_rk4(h, ax, bx, axh, bxh, ay, by, ayh, byh, fx4, fy4) &
+         bind(C, name="cpml_gpu_set_ade_rk4")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      type(*), dimension(*), intent(in) :: ax, bx, axh, bxh, ay, by, ayh, byh, fx4, fy4
     end function
     integer(c_int) function c_get_pressure(h, sp) bind(C, name="cpml_gpu_get_pressure")
       import :: c_ptr, c_int
@@ -406,6 +412,19 @@ contains
     integer, intent(out) :: ierr
     ierr = c_set_attenuation(h, N_SLS, tau_epsilon_nu1, tau_sigma_nu1, tau_epsilon_nu2, &
                              tau_sigma_nu2)
+  end subroutine
+
+  ! ADE-PML RK4 scheme of seismic_ADEPML_2D_elastic_RK4_eighth_order.f90 (2D,
+  ! order 8): stage tables a_x(4, NX) ... b_y_half(4, NY) (the interior of its
+  ! (4, -4:N+4) arrays) and the force at each stage time force_x4(4, NSTEP)
+  subroutine cpml_gpu_set_ade_rk4(h, a_x, b_x, a_x_half, b_x_half, a_y, b_y, a_y_half, &
+                                  b_y_half, force_x4, force_y4, ierr)
+    type(c_ptr), intent(in) :: h
+    type(*), dimension(*), intent(in) :: a_x, b_x, a_x_half, b_x_half, a_y, b_y, a_y_half, &
+         b_y_half, force_x4, force_y4
+    integer, intent(out) :: ierr
+    ierr = c_set_ade_rk4(h, a_x, b_x, a_x_half, b_x_half, a_y, b_y, a_y_half, b_y_half, &
+                         force_x4, force_y4)
   end subroutine
 
   ! sispressure(NSTEP, NREC) of the viscoelastic step (its kind is the handle's)
