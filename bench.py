@@ -20,7 +20,8 @@ uneven grids 512..2048 per side, 3000 steps, batched solver against 128
 sequential plain-solver runs; ``shipped`` -- 2D2 as shipped (101 x 641, FP64,
 2000 steps); ``3d4`` -- the default 3D workload with the 4th-order step (the
 elastic update of seismic_CPML_3D_viscoelastic_MPI.f90); ``2dve`` -- the 2D
-workload in 2nd order with viscoelastic attenuation (3 SLS).
+workload in 2nd order with viscoelastic attenuation (3 SLS); ``ade8`` -- the
+2D workload in 8th order stepped by the ADE-PML RK4 scheme.
 """
 from __future__ import annotations
 
@@ -329,6 +330,53 @@ def bench_2dve(args):
                  steps, wall / steps * 1e3, "f32", config, _roofline(nb, avg, f"2dve_{n}"), cpu)
 
 
+def bench_ade8(args):
+    """ADE-PML RK4 8th-order step (the 8th-order program's scheme) on the
+    layered 2D workload in 8th order.  The stage tables are the C-PML
+    profiles of the workload repeated over the four stages (the timing does
+    not depend on their values); the force at each stage time is the
+    workload's Ricker."""
+    import numpy as np
+    import torch
+
+    from seismic_cpml_amd import workloads
+
+    steps = args.steps or 500
+    nstep = args.warmup + steps
+    n = args.n or 4096
+    w = workloads.layered_2d(n=n, nstep=nstep, order=8)
+    s = w.make_solver()
+    px, py = w.prof
+    tab = {"a_x": np.repeat(px[0][None], 4, 0), "b_x": np.repeat(px[1][None], 4, 0),
+           "ah_x": np.repeat(px[3][None], 4, 0), "bh_x": np.repeat(px[4][None], 4, 0),
+           "a_y": np.repeat(py[0][None], 4, 0), "b_y": np.repeat(py[1][None], 4, 0),
+           "ah_y": np.repeat(py[3][None], 4, 0), "bh_y": np.repeat(py[4][None], 4, 0)}
+    fx, fy = w.force
+    s.set_ade_rk4(tab, np.repeat(np.asarray(fx)[:, None], 4, 1),
+                  np.repeat(np.asarray(fy)[:, None], 4, 1))
+    wall = _timed_run(s, args.warmup, nstep)
+    torch.cuda.synchronize()
+    s.close()
+    value = w.cells * steps / wall / 1e9
+    # words per cell and step: copy 15; per stage: stage 15, velocity rates 6,
+    # stress rates 7, accumulate 15 (memory arrays in the band not counted)
+    words = 15 + 4 * (15 + 6 + 7 + 15)
+    step_gbs = 4 * words * w.cells / (wall / steps) / 1e9
+    config = {
+        "workload": f"2D ADE-PML RK4 {n}^2, 8th order (Holberg), NPML 20, 12 seeded layers, "
+                    "Ricker 10 Hz at the centre, 256 receivers",
+        "grid": [w.nx, w.ny], "npml": w.npml, "deltat": w.dt,
+        "bytes_per_cell_update": 4 * words,
+        "step_hbm_gbs": round(step_gbs, 1), "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+        "launches_per_step": 18,
+    }
+    roof = {"bound": "hbm", "kernel": "step (18 launches)", "achieved": round(step_gbs, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4),
+            "traffic": None, "algorithmic_bytes": 4 * words * w.cells}
+    return _line("Gcell-updates/s (2D 4096^2 ADE-PML RK4 8th-order FP32 time loop)", value, args,
+                 steps, wall / steps * 1e3, "f32", config, roof, None)
+
+
 def bench_batch(args):
     import numpy as np
     import torch
@@ -479,7 +527,8 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload",
-                    choices=("3d", "3d4", "2d", "2dve", "batch", "shipped", "abc"), default="3d")
+                    choices=("3d", "3d4", "2d", "2dve", "ade8", "batch", "shipped", "abc"),
+                    default="3d")
     ap.add_argument("--steps", type=int, default=0, help="timed steps (0: the workload's own)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=0, help="grid points per axis (3d, 2d)")
@@ -518,7 +567,7 @@ def main():
         torch.cuda.synchronize()
         del a, b
     fn = {"3d": bench_3d, "3d4": lambda a: bench_3d(a, order=4), "2d": bench_2d,
-          "2dve": bench_2dve,
+          "2dve": bench_2dve, "ade8": bench_ade8,
           "batch": bench_batch, "shipped": bench_shipped, "abc": bench_abc}
     print(json.dumps(fn[args.workload](args)), flush=True)
 

@@ -29,7 +29,7 @@ def inputs(prec):
 
 
 # the program's arrays run from -4 to N+4: Fortran index 1 is element 5
-def run_gpu(f, d, prec, graph_steps=16):
+def run_gpu(f, d, prec, graph_steps=16, timing=False):
     from seismic_cpml_amd import CpmlGpu
     dt = d["dtype"]
     sl = slice(5, 5 + N)
@@ -43,6 +43,8 @@ def run_gpu(f, d, prec, graph_steps=16):
         s.set_source(40, 40, 1, np.zeros(NSTEP, dt), np.zeros(NSTEP, dt))
         s.set_receivers(f["ix_rec"], f["iy_rec"])
         s.set_graph_steps(graph_steps)
+        if timing:
+            s.set_kernel_timing(True)
         s.run()
         g = {k: s.field(n) for k, n in (("vx", "vx"), ("vy", "vy"), ("sxx", "sigmaxx"),
                                         ("syy", "sigmayy"), ("sxy", "sigmaxy"))}
@@ -79,5 +81,15 @@ def test_ade8_graph_equals_eager():
     f, d = inputs("f32")
     a = run_gpu(f, d, "f32", graph_steps=16)
     b = run_gpu(f, d, "f32", graph_steps=0)
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+def test_ade8_timed_run_is_the_same_step():
+    """With per-kernel timing on (events around every step) the handle still
+    takes the RK4 step: results equal the untimed run bit for bit."""
+    f, d = inputs("f32")
+    a = run_gpu(f, d, "f32")
+    b = run_gpu(f, d, "f32", timing=True)
     for k in a:
         np.testing.assert_array_equal(a[k], b[k], err_msg=k)
