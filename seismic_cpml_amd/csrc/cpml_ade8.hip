@@ -2,13 +2,13 @@
 // for gfx950: the 8th-order (Holberg) staggered operator, four Runge-Kutta
 // stages per time step and the auxiliary-differential-equation PML whose
 // coefficients depend on the stage (ADE8:692-906):
-//   per step   d2 = d3 = X
-//   per stage  d1 = d3 + rk41 * d2 * DELTAT                 (k_ade_stage)
-//              velocity rates from d1(sigma), ADE memories in the PML band,
+//   per step   d2 = d3 = X and stage 1's d1                 (k_ade_copy)
+//   per stage  velocity rates from d1(sigma), ADE memories in the PML band,
 //              force at t = (it-1 + rk41) * DELTAT, rate borders zeroed at
 //              all three levels                            (k_ade_vel)
 //              stress rates from d1(v)                      (k_ade_str)
-//              X += rk42 * d2 * DELTAT, velocity borders 0  (k_ade_acc)
+//              X += rk42 * d2 * DELTAT, velocity borders 0, and the next
+//              stage's d1 = d3 + rk41 * d2 * DELTAT          (k_ade_acc)
 // Arithmetic statement by statement as there (8th-order difference divided
 // by DELTA; -ffp-contract=off), so float64 results equal the reference.
 // One point per thread over the interior, 64 x 4 blocks; the ghost cells of
@@ -49,26 +49,20 @@ __device__ __forceinline__ T rk42(int inc) {
   return (inc == 0 || inc == 3) ? T(1) / T(6) : T(2) / T(6);
 }
 
+// d2 = d3 = X, then the first stage state d1 = d3 + rk41(1) * d2 * DELTAT
 template <typename T>
 __global__ __launch_bounds__(BX *ATY) void k_ade_copy(Ade8<T> a) {
   int i, j;
   if (!pt(a, i, j)) return;
   const long c = i + j * a.s.ld;
   T *X[5] = {a.s.vx, a.s.vy, a.s.sxx, a.s.syy, a.s.sxy};
+  const T r = rk41<T>(0);
   for (int f = 0; f < 5; ++f) {
     const T x = X[f][c];
     a.d[1][f][c] = x;
     a.d[2][f][c] = x;
+    a.d[0][f][c] = x + r * x * a.s.dt;
   }
-}
-
-template <typename T>
-__global__ __launch_bounds__(BX *ATY) void k_ade_stage(Ade8<T> a, int inc) {
-  int i, j;
-  if (!pt(a, i, j)) return;
-  const long c = i + j * a.s.ld;
-  const T r = rk41<T>(inc);
-  for (int f = 0; f < 5; ++f) a.d[0][f][c] = a.d[2][f][c] + r * a.d[1][f][c] * a.s.dt;
 }
 
 template <typename T>
@@ -172,6 +166,9 @@ __global__ __launch_bounds__(BX *ATY) void k_ade_str(Ade8<T> a, int inc) {
   }
 }
 
+// X += rk42(inc) * d2 * DELTAT, velocity borders zeroed (ADE8:863-874), and
+// the next stage's state d1 = d3 + rk41(inc+1) * d2 * DELTAT from the same d2
+// (inc = 3: no next stage)
 template <typename T>
 __global__ __launch_bounds__(BX *ATY) void k_ade_acc(Ade8<T> a, int inc) {
   int i, j;
@@ -180,8 +177,13 @@ __global__ __launch_bounds__(BX *ATY) void k_ade_acc(Ade8<T> a, int inc) {
   const long c = i + j * p.ld;
   const T r = rk42<T>(inc);
   T *X[5] = {p.vx, p.vy, p.sxx, p.syy, p.sxy};
-  for (int f = 0; f < 5; ++f) X[f][c] = X[f][c] + r * a.d[1][f][c] * p.dt;
-  // velocity borders (ADE8:865-872)
+  const bool next = inc < 3;
+  const T rn = next ? rk41<T>(inc + 1) : T(0);
+  for (int f = 0; f < 5; ++f) {
+    const T d2 = a.d[1][f][c];
+    X[f][c] = X[f][c] + r * d2 * p.dt;
+    if (next) a.d[0][f][c] = a.d[2][f][c] + rn * d2 * p.dt;
+  }
   if (i == 0 || i == p.nx - 1 || j == 0 || j == p.ny - 1) {
     p.vx[c] = T(0);
     p.vy[c] = T(0);
@@ -193,9 +195,8 @@ __global__ __launch_bounds__(BX *ATY) void k_ade_acc(Ade8<T> a, int inc) {
 template <typename T>
 hipError_t launch_ade8_step(const Ade8<T> &a, hipStream_t s) {
   const dim3 g((a.s.nx + BX - 1) / BX, (a.s.ny + ATY - 1) / ATY), b(BX, ATY);
-  hipLaunchKernelGGL(k_ade_copy<T>, g, b, 0, s, a);
+  hipLaunchKernelGGL(k_ade_copy<T>, g, b, 0, s, a);  // includes stage 1's state
   for (int inc = 0; inc < 4; ++inc) {
-    hipLaunchKernelGGL(k_ade_stage<T>, g, b, 0, s, a, inc);
     hipLaunchKernelGGL(k_ade_vel<T>, g, b, 0, s, a, inc);
     hipLaunchKernelGGL(k_ade_str<T>, g, b, 0, s, a, inc);
     hipLaunchKernelGGL(k_ade_acc<T>, g, b, 0, s, a, inc);
