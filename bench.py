@@ -43,6 +43,26 @@ def _orc():
     return orc
 
 
+def host_threads():
+    """Host threads given to this GPU's process: OMP_NUM_THREADS where the box
+    sets it (its CPU share per GPU), else every logical CPU."""
+    try:
+        n = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        n = 0
+    return n if n > 0 else (os.cpu_count() or 1)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline_2d(nx, ny, order, dx, dt, lam, mu, rho, px, py, fx, fy, seconds=15.0,
                     threads=None, what=""):
     """Time the oracle's 2D loop (reference structure, full-grid memory
@@ -50,7 +70,7 @@ def cpu_baseline_2d(nx, ny, order, dx, dt, lam, mu, rho, px, py, fx, fy, seconds
     import numpy as np
 
     orc = _orc()
-    threads = threads or min(16, os.cpu_count() or 1)
+    threads = threads or host_threads()
     prec = "f32" if lam.dtype == np.float32 else "f64"
     ix = np.array([nx // 2], np.int32)
     iy = np.array([ny // 2], np.int32)
@@ -67,10 +87,17 @@ def cpu_baseline_2d(nx, ny, order, dx, dt, lam, mu, rho, px, py, fx, fy, seconds
     t = time.perf_counter()
     run(steps, out["state"])
     el = time.perf_counter() - t
-    return {"value": round(nx * ny * steps / el / 1e9, 4), "unit": "Gcell/s", "cores": threads,
-            "kind": "port",
-            "sample": f"oracle C 2D loop (reference structure, OpenMP) on {what}{nx}x{ny}, "
-                      f"{steps} steps, {el:.1f} s"}
+    out = {"value": round(nx * ny * steps / el / 1e9, 4), "unit": "Gcell/s", "cores": threads,
+           "kind": "port", "cpu_model": cpu_model(),
+           "sample": f"oracle C 2D loop (reference structure, OpenMP) on {what}{nx}x{ny}, "
+                     f"{steps} steps, {el:.1f} s"}
+    if threads > 1 and ny >= 64:  # the shipped programs are serial: one thread on fewer rows
+        r = max(64, ny // 16)
+        one = cpu_baseline_2d(nx, r, order, dx, dt, lam[:r], mu[:r], rho[:r], px, py[:, :r], fx,
+                              fy, seconds=seconds / 3, threads=1, what=what)
+        out["value_1thread"] = one["value"]
+        out["sample_1thread"] = one["sample"]
+    return out
 
 
 def cpu_baseline(w, lam, mu, rho, nzs=128, seconds=15.0, threads=None):
@@ -80,7 +107,7 @@ def cpu_baseline(w, lam, mu, rho, nzs=128, seconds=15.0, threads=None):
     import numpy as np
 
     orc = _orc()
-    threads = threads or min(16, os.cpu_count() or 1)
+    threads = threads or host_threads()
     n = w.nx
     z0 = w.nz - nzs
     lam_h = lam[z0:].cpu().numpy()
@@ -112,10 +139,15 @@ def cpu_baseline(w, lam, mu, rho, nzs=128, seconds=15.0, threads=None):
     run(steps, out["state"])
     el = time.perf_counter() - t
     cells = n * n * nzs
-    return {"value": round(cells * steps / el / 1e9, 4), "unit": "Gcell/s", "cores": threads,
-            "kind": "port",
-            "sample": f"oracle C loop (reference structure, OpenMP) on the top {n}x{n}x{nzs} "
-                      f"planes of the same medium, {steps} steps, {el:.1f} s"}
+    out = {"value": round(cells * steps / el / 1e9, 4), "unit": "Gcell/s", "cores": threads,
+           "kind": "port", "cpu_model": cpu_model(),
+           "sample": f"oracle C loop (reference structure, OpenMP) on the top {n}x{n}x{nzs} "
+                     f"planes of the same medium, {steps} steps, {el:.1f} s"}
+    if threads > 1:  # the shipped programs are serial: the same loop on one thread
+        one = cpu_baseline(w, lam, mu, rho, nzs=8, seconds=seconds / 3, threads=1)
+        out["value_1thread"] = one["value"]
+        out["sample_1thread"] = one["sample"]
+    return out
 
 
 def _timed_run(s, warmup, nstep):
@@ -164,9 +196,12 @@ def bench_3d(args, order=2):
     from seismic_cpml_amd import workloads
 
     steps = args.steps or 2000
+    nt = args.time_block
+    if nt > 0:
+        steps = max(nt, steps // nt * nt)  # whole blocks only in the timed region
     nstep = args.warmup + steps
-    if order == 4 and args.fused:
-        raise SystemExit("bench.py: the fused step is second order only")
+    if order == 4 and (args.fused or nt):
+        raise SystemExit("bench.py: the fused and time-blocked steps are second order only")
     w = workloads.layered_3d(n=args.n or 768, nstep=nstep, order=order)
     s = w.make_solver()
     if args.fused:  # the fused step's own defaults: 14-row blocks, 48-plane chunks
@@ -178,6 +213,8 @@ def bench_3d(args, order=2):
         s.set_fused(True)
     if args.tile:
         s.set_tile(args.tile)
+    if nt:
+        s.set_time_block(nt, args.strip, args.chunk_rows)
     slabs = s.slab_widths()
     wall = _timed_run(s, args.warmup, nstep)
     torch.cuda.synchronize()
@@ -185,8 +222,9 @@ def bench_3d(args, order=2):
     cells = w.cells
     value = cells * steps / wall / 1e9
     fused = args.fused
-    nb = workloads.bytes_per_step(w, slabs, fused=fused)
-    names = ((0, "fused"),) if fused else ((0, "stress"), (1, "velocity"))
+    nb = workloads.bytes_per_step(w, slabs, fused=fused, time_block=nt)
+    names = (((0, "time_block"),) if nt else ((0, "fused"),) if fused
+             else ((0, "stress"), (1, "velocity")))
     kt = {name: s.kernel_time(k) for k, name in names}
     avg = {name: ms / max(n, 1) for name, (n, ms) in kt.items()}
     step_gbs = nb["step"] / (wall / steps) / 1e9
@@ -203,7 +241,10 @@ def bench_3d(args, order=2):
                     f"{'2nd' if order == 2 else '4th'} order, layered "
                     "+-5% smoothed medium, Ricker 8 Hz, 1024 receivers",
         "grid": [w.nx, w.ny, w.nz], "npml": w.npml, "deltat": w.dt,
-        "kernel_path": "fused" if fused else "two-phase",
+        "kernel_path": f"time-blocked ({nt} steps per launch)" if nt else
+        "fused" if fused else "two-phase",
+        "time_block": {"steps": nt, "strip": args.strip, "chunk_planes": args.chunk_rows}
+        if nt else None,
         "slab_widths": slabs,
         "bytes_per_cell_update": round(nb["step"] / cells, 3),
         "step_hbm_gbs": round(step_gbs, 1),
@@ -218,7 +259,8 @@ def bench_3d(args, order=2):
         key = f"3d4_{w.nx}_kchunk{args.kchunk or 8}"
         return _line("Gcell-updates/s (3D 768^3 FP32 4th-order C-PML time loop)", value, args,
                      steps, wall / steps * 1e3, "f32", config, _roofline(nb, avg, key), cpu)
-    key = (f"3d_{w.nx}_{'fused' if fused else 'two-phase'}_tile{args.tile or 4}_"
+    key = (f"3d_{w.nx}_tb{nt}" if nt else
+           f"3d_{w.nx}_{'fused' if fused else 'two-phase'}_tile{args.tile or 4}_"
            f"kchunk{args.kchunk or 8}")
     return _line("Gcell-updates/s (3D 768^3 FP32 C-PML time loop)", value, args, steps,
                  wall / steps * 1e3, "f32", config, _roofline(nb, avg, key), cpu)
@@ -230,19 +272,28 @@ def bench_2d(args):
     from seismic_cpml_amd import workloads
 
     steps = args.steps or 4000
+    nt = args.time_block
+    if nt > 0:
+        steps = max(nt, steps // nt * nt)  # whole blocks only in the timed region
     nstep = args.warmup + steps
     n = args.n or 4096
     w = workloads.layered_2d(n=n, nstep=nstep)
     s = w.make_solver()
     if args.tile:
         s.set_tile(args.tile)
+    if nt > 0:
+        s.set_time_block(nt, args.strip, args.chunk_rows)
     slabs = s.slab_widths()
     wall = _timed_run(s, args.warmup, nstep)
     torch.cuda.synchronize()
     value = w.cells * steps / wall / 1e9
-    nb = workloads.bytes_per_step(w, slabs)
-    kt = {name: s.kernel_time(k) for k, name in ((0, "stress"), (1, "velocity"))}
-    avg = {name: ms / max(c, 1) for name, (c, ms) in kt.items()}
+    nb = workloads.bytes_per_step(w, slabs, time_block=nt)
+    if nt > 0:
+        c, ms = s.kernel_time(0)
+        avg = {"time_block": ms / max(c, 1)}
+    else:
+        kt = {name: s.kernel_time(k) for k, name in ((0, "stress"), (1, "velocity"))}
+        avg = {name: ms / max(c, 1) for name, (c, ms) in kt.items()}
     step_gbs = nb["step"] / (wall / steps) / 1e9
     s.close()
     cpu = None
@@ -258,13 +309,16 @@ def bench_2d(args):
                     "Ricker 10 Hz at the centre, 256 receivers on the top interior row",
         "grid": [w.nx, w.ny], "npml": w.npml, "deltat": w.dt, "slab_widths": slabs,
         "tile_rows": args.tile or 4,
+        "time_block": {"steps": nt, "strip": args.strip, "chunk_rows": args.chunk_rows}
+        if nt else None,
         "bytes_per_cell_update": round(nb["step"] / w.cells, 3),
         "step_hbm_gbs": round(step_gbs, 1), "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 4),
         "kernel_ms": {k: round(v, 4) for k, v in avg.items()},
     }
     return _line("Gcell-updates/s (2D 4096^2 4th-order FP32 C-PML time loop)", value, args,
                   steps, wall / steps * 1e3, "f32", config,
-                  _roofline(nb, avg, f"2d_{n}_tile{args.tile or 4}"), cpu)
+                  _roofline(nb, avg, f"2d_{n}_tb{nt}" if nt else f"2d_{n}_tile{args.tile or 4}"),
+                  cpu)
 
 
 def bench_2dve(args):
@@ -297,7 +351,7 @@ def bench_2dve(args):
         rows = 2048
         px, py = w.prof
         fx, fy = w.force
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         ixr = np.array([n // 2], np.int32)
         iyr = np.array([rows // 2], np.int32)
 
@@ -535,6 +589,10 @@ def main():
     ap.add_argument("--n", type=int, default=0, help="grid points per axis (3d, 2d)")
     ap.add_argument("--kchunk", type=int, default=0)
     ap.add_argument("--tile", type=int, default=0, help="rows per tile (4, 8, 16)")
+    ap.add_argument("--time-block", type=int, default=0,
+                    help="2d: leapfrog steps per launch (time blocking; 0: off)")
+    ap.add_argument("--strip", type=int, default=0, help="2d time blocking: strip width")
+    ap.add_argument("--chunk-rows", type=int, default=0, help="2d time blocking: chunk rows")
     ap.add_argument("--fused", action="store_true",
                     help="3d: one fused kernel per step instead of the stress and velocity "
                          "kernels")

@@ -147,6 +147,21 @@ int cpml_gpu_set_tile(cpml_solver *h, int tile_y);
  * single pass over HBM, state kept in two ping-pong buffers: 9 more field
  * arrays).  Results are identical to the two-kernel path. */
 int cpml_gpu_set_fused(cpml_solver *h, int on);
+/* Advance `steps` full leapfrog steps per launch and per pass over HBM
+ * (time blocking of the loops 2D2:550-686, 2D4:551-687, 3D:836-1121: every
+ * intermediate time level stays in registers (and, in 3D, LDS); the state
+ * ping-pongs between two buffers: one more set of field arrays and memory
+ * variables).  steps 0 turns it off.  2D (orders 2, 4, 8): at most 4 / 3 / 2
+ * steps in float, 2 in double; one 64-lane wavefront per strip of
+ * strip_width columns marching through chunks of chunk_rows rows (0: 128).
+ * 3D (order 2): at most 2 steps; a workgroup owns strip_width columns x
+ * 12 (float; double: 4) rows for 2 steps (14 / 6 for 1) and marches through
+ * z chunks of chunk_rows planes (0: 64).  strip_width 0 = the widest
+ * multiple of 8 the ring of steps*(order-1) columns per side leaves.  Used
+ * while energy is off (set_energy cadence 0) on the elastic step without a
+ * learned boundary; run() ends a block at every stability check.  Results
+ * are identical to the single-step kernels. */
+int cpml_gpu_set_time_block(cpml_solver *h, int steps, int strip_width, int chunk_rows);
 /* When on, run() launches eagerly with a HIP event pair around every stress
  * and velocity kernel on the handle's stream; see cpml_gpu_kernel_time. */
 int cpml_gpu_set_kernel_timing(cpml_solver *h, int on);

@@ -33,7 +33,7 @@ HIPFLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas",
 ]
 HIP_SOURCES = ["cpml_kernels.hip", "cpml_3d4.hip", "cpml_visco2d.hip", "cpml_ade8.hip",
-               "cpml_fused3d.hip",
+               "cpml_fused3d.hip", "cpml_tb2d.hip", "cpml_tb3d.hip",
                "cpml_batch.hip", "cpml_abc.hip", "cpml_api.hip"]
 
 

@@ -124,6 +124,12 @@ struct cpml_solver {
   void *field2[9] = {}; // second state buffer of the fused 3D step
   int cur = 0;          // 1: the state lives in field2 / mem2
   bool fused = false;   // 3D: one fused kernel per step (ping-pong state)
+  // 2D time blocking: tb steps per launch (0: off), strips of tb_ow columns
+  // (0: the widest multiple of 8 the ring allows), chunks of tb_jc rows
+  int tb = 0, tb_ow = 0, tb_jc = 128;
+  int *tb_rec = nullptr;        // per-strip receiver lists (device)
+  int tb_rec_ow = -1;           // strip width they were built for (-1: stale)
+  std::vector<int> rec_ix, rec_iy, rec_iz;  // receiver grid indices (0-based), host copy
   void *media[6] = {};  // lambda mu rho, and in 3D: DELTAT*lambda DELTAT*mu DELTAT/rho
   void *prof = nullptr; // [3 axes][6][n]
   unsigned char *img = nullptr;  // PNM snapshot pixels (2D), allocated on first use
@@ -151,6 +157,7 @@ struct cpml_solver {
   int tile_y = 4;
   bool ktiming = false;
   std::vector<hipEvent_t> evpool;
+  std::vector<int> ev_tb;  // event triples of this run that bracket a time-blocked launch
   double ktime[2] = {0, 0};
   int kcount[2] = {0, 0};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -180,6 +187,15 @@ int fail(cpml_solver *h, int code, const std::string &msg) {
   else
     g_init_error = msg;
   return code;
+}
+
+// The 3D kernels address a z chunk through one buffer resource with unsigned
+// 32-bit byte offsets: the chunk's planes plus up to 8 halo and pipeline planes
+// must stay below 4 GiB.
+bool chunk_fits(const cpml_solver *h, int kchunk) {
+  if (h->ndim != 3) return true;
+  const double planes = (double)std::min(kchunk, h->nz) + 8.0;
+  return planes * (double)h->ps * (double)h->esize <= 4294967295.0;
 }
 
 #define HIPCHK(h, call)                                                                  \
@@ -551,6 +567,135 @@ hipError_t enqueue_boundary(cpml_solver *h) {
   return cpml::abc_enqueue(h->abc, static_cast<float *>(h->abc_corr), view_of(h));
 }
 
+// 2D time blocking is used when set and the step has nothing it does not
+// cover (energy every step, attenuation, the RK4 scheme).
+bool tb_active(const cpml_solver *h) {
+  if (h->tb <= 0 || h->cadence != 0) return false;
+  if (h->ndim == 3) return h->order == 2 && !h->abc;
+  return !h->ade && h->nsls == 0;
+}
+// rows of a 3D time-blocked tile (wavefronts per workgroup - 2 tb)
+int tb3d_rows(const cpml_solver *h) { return (h->esize == 4 ? 16 : 8) - 2 * h->tb; }
+// steps advanced by one launch unit
+int unit_steps(const cpml_solver *h) { return tb_active(h) ? h->tb : 1; }
+
+// Per-strip receiver lists of the time-blocked kernel, sorted by row.
+// 2D: per strip of tb_ow columns, sorted by row; 3D: per tile of tb_ow
+// columns x tb3d_rows rows, sorted by plane.  Layout: start[ntiles + 1], then
+// i, j, slot and (3D) k arrays of nrec entries each.
+int tb_tiles(const cpml_solver *h) {
+  const int ns = (h->nx + h->tb_ow - 1) / h->tb_ow;
+  return h->ndim == 3 ? ns * ((h->ny + tb3d_rows(h) - 1) / tb3d_rows(h)) : ns;
+}
+int build_tb_receivers(cpml_solver *h) {
+  const int key = h->tb_ow * 64 + h->tb;
+  if (h->tb_rec_ow == key) return CPML_OK;
+  const int ns = (h->nx + h->tb_ow - 1) / h->tb_ow;
+  const int nt = tb_tiles(h);
+  const int rows = h->ndim == 3 ? tb3d_rows(h) : 1;
+  auto tile = [&](int r) {
+    return h->ndim == 3 ? (h->rec_iy[r] / rows) * ns + h->rec_ix[r] / h->tb_ow
+                        : h->rec_ix[r] / h->tb_ow;
+  };
+  const int n = h->nrec;
+  std::vector<int> order(n);
+  for (int r = 0; r < n; ++r) order[r] = r;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    const int sa = tile(a), sb = tile(b);
+    if (sa != sb) return sa < sb;
+    return h->ndim == 3 ? h->rec_iz[a] < h->rec_iz[b] : h->rec_iy[a] < h->rec_iy[b];
+  });
+  const size_t m = (size_t)std::max(n, 1);
+  std::vector<int> buf((size_t)nt + 1 + 4 * m, 0);
+  int *start = buf.data(), *ri = start + nt + 1, *rj = ri + m, *rs = rj + m, *rk = rs + m;
+  for (int k = 0; k < n; ++k) {
+    const int r = order[k];
+    ri[k] = h->rec_ix[r];
+    rj[k] = h->rec_iy[r];
+    rk[k] = h->rec_iz[r];
+    rs[k] = r;
+    start[tile(r) + 1]++;
+  }
+  for (int t = 0; t < nt; ++t) start[t + 1] += start[t];
+  dev_free(h, *(void **)&h->tb_rec);
+  int rc = dev_alloc(h, (void **)&h->tb_rec, buf.size() * sizeof(int));
+  if (rc) return rc;
+  HIPCHK(h, hipMemcpy(h->tb_rec, buf.data(), buf.size() * sizeof(int), hipMemcpyHostToDevice));
+  h->tb_rec_ow = key;
+  return CPML_OK;
+}
+
+template <typename T>
+cpml::TB2D<T> tb_of(cpml_solver *h) {
+  cpml::TB2D<T> P{};
+  P.a = step2d<T>(h);
+  void **nf = h->cur ? h->field : h->field2;
+  void **nm = h->cur ? h->mem : h->mem2;
+  const int fq[5] = {CPML_VX, CPML_VY, CPML_SIGMAXX, CPML_SIGMAYY, CPML_SIGMAXY};
+  T **out[5] = {&P.vx, &P.vy, &P.sxx, &P.syy, &P.sxy};
+  for (int q = 0; q < 5; ++q) *out[q] = static_cast<T *>(nf[fq[q]]) + h->base;
+  for (int q = 0; q < 4; ++q) {
+    P.mx[q] = static_cast<T *>(nm[kSlot2Dx[q]]);
+    P.my[q] = static_cast<T *>(nm[kSlot2Dy[q]]);
+  }
+  P.ow = h->tb_ow;
+  P.jc = h->tb_jc;
+  const int nt = tb_tiles(h);
+  P.rec_start = h->tb_rec;
+  P.rec_i = h->tb_rec + nt + 1;
+  P.rec_j = P.rec_i + std::max(h->nrec, 1);
+  P.rec_slot = P.rec_j + std::max(h->nrec, 1);
+  P.sisx = static_cast<T *>(h->seis[0]);
+  P.sisy = static_cast<T *>(h->seis[1]);
+  P.nrec = h->nrec;
+  return P;
+}
+
+template <typename T>
+cpml::TB3D<T> tb3_of(cpml_solver *h) {
+  cpml::TB3D<T> P{};
+  P.a = step3d<T>(h);
+  P.a.kchunk = h->tb_jc;  // z planes per chunk of the time-blocked march
+  void **nf = h->cur ? h->field : h->field2;
+  void **nm = h->cur ? h->mem : h->mem2;
+  for (int q = 0; q < 9; ++q) (&P.vx)[q] = static_cast<T *>(nf[q]) + h->base;
+  for (int q = 0; q < 6; ++q) {
+    P.mx[q] = static_cast<T *>(nm[kSlot3Dx[q]]);
+    P.my[q] = static_cast<T *>(nm[kSlot3Dy[q]]);
+    P.mz[q] = static_cast<T *>(nm[kSlot3Dz[q]]);
+  }
+  P.ow = h->tb_ow;
+  const int nt = tb_tiles(h);
+  const int m = std::max(h->nrec, 1);
+  P.rec_start = h->tb_rec;
+  P.rec_i = h->tb_rec + nt + 1;
+  P.rec_j = P.rec_i + m;
+  P.rec_slot = P.rec_j + m;
+  P.rec_k = P.rec_slot + m;
+  P.sisx = static_cast<T *>(h->seis[0]);
+  P.sisy = static_cast<T *>(h->seis[1]);
+  P.sisz = static_cast<T *>(h->seis[2]);
+  P.nrec = h->nrec;
+  return P;
+}
+
+// One time-blocked launch (tb steps): the kernel writes the other state
+// buffer and records the receivers of its first tb - 1 steps; the record
+// kernel takes the last one and advances the step counter by tb.
+template <typename T>
+hipError_t enqueue_tb(cpml_solver *h) {
+  hipError_t e = h->ndim == 3 ? cpml::launch_tb3d<T>(h->tb, tb3_of<T>(h), h->stream)
+                              : cpml::launch_tb2d<T>(h->order, h->tb, tb_of<T>(h), h->stream);
+  h->cur ^= 1;
+  return e;
+}
+template <typename T>
+hipError_t enqueue_tb_tail(cpml_solver *h) {
+  cpml::Record<T> r = record_of<T>(h);
+  r.skip = h->tb - 1;
+  return cpml::launch_record<T>(r, h->stream);
+}
+
 template <typename T>
 hipError_t enqueue_step(cpml_solver *h) {
   hipError_t e;
@@ -567,13 +712,25 @@ hipError_t enqueue_step(cpml_solver *h) {
   return e;
 }
 
-int graph_len(const cpml_solver *h) {
-  // an even number of fused steps returns the state to the buffer it started in
-  return h->fused ? (h->graph_steps + 1) / 2 * 2 : h->graph_steps;
+// launch units per captured graph: an even number of ping-pong launches
+// returns the state to the buffer it started in
+int graph_units(const cpml_solver *h) {
+  const int u = unit_steps(h);
+  if (h->graph_steps <= 0) return 0;
+  if (!h->fused && !tb_active(h)) return h->graph_steps;
+  const int n = (h->graph_steps + u - 1) / u;
+  return (n + 1) / 2 * 2;
 }
 
 hipError_t enqueue_step_any(cpml_solver *h) {
   return h->prec == 4 ? enqueue_step<float>(h) : enqueue_step<double>(h);
+}
+// one launch unit: unit_steps(h) steps
+hipError_t enqueue_unit_any(cpml_solver *h) {
+  if (!tb_active(h)) return enqueue_step_any(h);
+  hipError_t e = h->prec == 4 ? enqueue_tb<float>(h) : enqueue_tb<double>(h);
+  if (e == hipSuccess) e = h->prec == 4 ? enqueue_tb_tail<float>(h) : enqueue_tb_tail<double>(h);
+  return e;
 }
 
 void drop_graphs(cpml_solver *h) {
@@ -590,7 +747,7 @@ int build_graph(cpml_solver *h) {
   hipGraph_t g = nullptr;
   HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   hipError_t e = hipSuccess;
-  for (int s = 0; s < graph_len(h) && e == hipSuccess; ++s) e = enqueue_step_any(h);
+  for (int s = 0; s < graph_units(h) && e == hipSuccess; ++s) e = enqueue_unit_any(h);
   h->cur = slot;  // capture does not execute: the state is still where it was
   hipError_t e2 = hipStreamEndCapture(h->stream, &g);
   if (e != hipSuccess || e2 != hipSuccess) {
@@ -617,9 +774,24 @@ int ensure_events(cpml_solver *h, size_t n) {
 // Enqueue n steps; in timing mode event triples bracket each stress and
 // velocity kernel (evpool[3*s .. 3*s+2] for step s of this segment).
 int enqueue_steps(cpml_solver *h, int n, int *ev_used) {
+  if (tb_active(h)) {
+    int rc = build_tb_receivers(h);
+    if (rc) return rc;
+  }
+  const int u = unit_steps(h);
   if (h->ktiming) {
     int rc = ensure_events(h, (size_t)(*ev_used + 3 * n));
     if (rc) return rc;
+    for (; n >= u && u > 1; n -= u) {  // time-blocked launches count as the stress kernel
+      hipEvent_t *e = &h->evpool[*ev_used];
+      HIPCHK(h, hipEventRecord(e[0], h->stream));
+      HIPCHK(h, h->prec == 4 ? enqueue_tb<float>(h) : enqueue_tb<double>(h));
+      HIPCHK(h, hipEventRecord(e[1], h->stream));
+      HIPCHK(h, hipEventRecord(e[2], h->stream));
+      HIPCHK(h, h->prec == 4 ? enqueue_tb_tail<float>(h) : enqueue_tb_tail<double>(h));
+      h->ev_tb.push_back(*ev_used);
+      *ev_used += 3;
+    }
     for (int s = 0; s < n; ++s) {
       hipEvent_t *e = &h->evpool[*ev_used];
       HIPCHK(h, hipEventRecord(e[0], h->stream));
@@ -642,7 +814,7 @@ int enqueue_steps(cpml_solver *h, int n, int *ev_used) {
     }
     return CPML_OK;
   }
-  const int gl = graph_len(h);
+  const int gl = graph_units(h) * u;  // steps per graph replay
   if (gl > 0 && n >= gl) {
     if (h->graph_dirty) {
       drop_graphs(h);
@@ -654,9 +826,10 @@ int enqueue_steps(cpml_solver *h, int n, int *ev_used) {
     }
     while (n >= gl) {
       HIPCHK(h, hipGraphLaunch(h->gexec[h->cur], h->stream));
-      n -= gl;  // gl is even when fused: the state returns to the same buffer
+      n -= gl;  // an even number of ping-pong launches: the state is back in its buffer
     }
   }
+  for (; n >= u && u > 1; n -= u) HIPCHK(h, enqueue_unit_any(h));
   for (int s = 0; s < n; ++s) HIPCHK(h, enqueue_step_any(h));
   return CPML_OK;
 }
@@ -886,6 +1059,12 @@ int cpml_gpu_init(cpml_solver **hp, int ndim, int order, int prec, int NX, int N
   h->base = h->xpad + h->R * h->ld + (ndim == 3 ? h->R * h->ps : 0);
   h->total = h->ps * (ndim == 3 ? NZ + 2 * h->R : 1);
   h->mld = round_up(NX, align);
+  // default chunk depth, reduced where a chunk's buffer would pass 4 GiB
+  while (h->kchunk > 1 && !chunk_fits(h, h->kchunk)) --h->kchunk;
+  if (!chunk_fits(h, 1)) {
+    delete h;
+    return fail(nullptr, CPML_ERR_OOM, "3D grid planes too large for 32-bit buffer offsets");
+  }
   h->sx.n = NX;
   h->sy.n = NY;
   h->sz.n = h->nz;
@@ -1088,7 +1267,7 @@ int cpml_gpu_set_pml(cpml_solver *h, const void *a_x, const void *b_x, const voi
     dev_free(h, h->mem2[w]);
     h->mem_elems[w] = elems;
     if ((rc = dev_alloc(h, &h->mem[w], elems * h->esize))) return rc;
-    if (h->fused && (rc = dev_alloc(h, &h->mem2[w], elems * h->esize))) return rc;
+    if ((h->fused || h->tb) && (rc = dev_alloc(h, &h->mem2[w], elems * h->esize))) return rc;
   }
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->have_pml = true;
@@ -1148,6 +1327,12 @@ int cpml_gpu_set_receivers(cpml_solver *h, int NREC, const int *ix_rec, const in
       return rc;
   HIPCHK(h, hipMemcpy(h->recidx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
   h->nrec = NREC;
+  h->rec_ix.assign(ix_rec, ix_rec + NREC);
+  h->rec_iy.assign(iy_rec, iy_rec + NREC);
+  h->rec_iz.assign((size_t)NREC, 1);
+  if (h->ndim == 3) h->rec_iz.assign(iz_rec, iz_rec + NREC);
+  for (int r = 0; r < NREC; ++r) --h->rec_ix[r], --h->rec_iy[r], --h->rec_iz[r];
+  h->tb_rec_ow = -1;
   h->graph_dirty = true;
   return CPML_OK;
 }
@@ -1209,6 +1394,7 @@ int cpml_gpu_run(cpml_solver *h, int it_begin, int it_end) {
                            h->stream));
   HIPCHK(h, hipEventRecord(h->ev0, h->stream));
   int ev_used = 0;
+  h->ev_tb.clear();
   int it = it_begin;
   while (it <= it_end) {
     int seg_end = it_end;
@@ -1241,12 +1427,15 @@ int cpml_gpu_run(cpml_solver *h, int it_begin, int it_end) {
   float ms = 0;
   HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
   h->last_ms = ms;
+  size_t tbk = 0;
   for (int s = 0; s + 2 < ev_used; s += 3) {
     float a = 0, b = 0;
     HIPCHK(h, hipEventElapsedTime(&a, h->evpool[s], h->evpool[s + 1]));
     h->ktime[0] += a;
     h->kcount[0] += 1;
-    if (!h->fused) {
+    const bool tbl = tbk < h->ev_tb.size() && h->ev_tb[tbk] == s;
+    if (tbl) ++tbk;
+    if (!h->fused && !tbl) {
       HIPCHK(h, hipEventElapsedTime(&b, h->evpool[s + 1], h->evpool[s + 2]));
       h->ktime[1] += b;
       h->kcount[1] += 1;
@@ -1550,6 +1739,10 @@ int cpml_gpu_set_graph_steps(cpml_solver *h, int steps) {
 int cpml_gpu_set_kchunk(cpml_solver *h, int kchunk) {
   if (!h) return CPML_ERR_ARG;
   if (kchunk < 1) return fail(h, CPML_ERR_ARG, "kchunk must be >= 1");
+  if (!chunk_fits(h, kchunk))
+    return fail(h, CPML_ERR_ARG,
+                "kchunk too large: a chunk's planes (with their halo planes) must stay below "
+                "4 GiB, the reach of the kernels' 32-bit buffer offsets");
   h->kchunk = kchunk;
   h->graph_dirty = true;
   return CPML_OK;
@@ -1583,39 +1776,83 @@ int cpml_gpu_set_boundary(cpml_solver *h, int mode, cpml_abc_net *net) {
   return CPML_OK;
 }
 
+// Second state buffer (fields and memory variables) of the ping-pong steps.
+int ensure_second_state(cpml_solver *h) {
+  if (h->field2[0]) return CPML_OK;
+  const int nf = h->ndim == 3 ? 9 : 5;
+  size_t free_b = 0, tot_b = 0;
+  if (hipMemGetInfo(&free_b, &tot_b) == hipSuccess &&
+      (double)h->total * h->esize * nf > 0.95 * (double)free_b)
+    return fail(h, CPML_ERR_OOM, "not enough device memory for the second state buffer");
+  bool want[9];
+  for (int q = 0; q < 9; ++q) want[q] = h->field[q] != nullptr;
+  int rc = dev_alloc_group(h, h->field2, 9, (size_t)h->total * h->esize, want);
+  if (rc) return rc;
+  for (int w = 0; w < 18 && h->have_pml; ++w)
+    if (mem_used(h->ndim, w) && !h->mem2[w] &&
+        (rc = dev_alloc(h, &h->mem2[w], h->mem_elems[w] * h->esize)))
+      return rc;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return CPML_OK;
+}
+
+// Bring the state back to the first buffer.
+int restore_first_state(cpml_solver *h) {
+  if (!h->cur) return CPML_OK;
+  for (int q = 0; q < 9; ++q)
+    if (h->field[q])
+      HIPCHK(h, hipMemcpyAsync(h->field[q], h->field2[q], (size_t)h->total * h->esize,
+                               hipMemcpyDeviceToDevice, h->stream));
+  for (int w = 0; w < 18; ++w)
+    if (h->mem[w] && h->mem_elems[w])
+      HIPCHK(h, hipMemcpyAsync(h->mem[w], h->mem2[w], h->mem_elems[w] * h->esize,
+                               hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->cur = 0;
+  return CPML_OK;
+}
+
 int cpml_gpu_set_fused(cpml_solver *h, int on) {
   if (!h) return CPML_ERR_ARG;
   if (on && (h->ndim != 3 || h->order != 2))
     return fail(h, CPML_ERR_ARG, "the fused step is 3D second order only");
   int rc = set_device(h);
   if (rc) return rc;
-  if (on && !h->fused) {
-    size_t free_b = 0, tot_b = 0;
-    if (hipMemGetInfo(&free_b, &tot_b) == hipSuccess &&
-        (double)h->total * h->esize * 9 > 0.95 * (double)free_b)
-      return fail(h, CPML_ERR_OOM, "not enough device memory for the second state buffer");
-    if (!h->field2[0] &&
-        (rc = dev_alloc_group(h, h->field2, 9, (size_t)h->total * h->esize, nullptr)))
-      return rc;
-    for (int w = 0; w < 18 && h->have_pml; ++w)
-      if (mem_used(h->ndim, w) && !h->mem2[w] &&
-          (rc = dev_alloc(h, &h->mem2[w], h->mem_elems[w] * h->esize)))
-        return rc;
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-  }
-  if (!on && h->cur) {  // bring the state back to the first buffer
-    for (int q = 0; q < 9; ++q)
-      if (h->field[q])
-        HIPCHK(h, hipMemcpyAsync(h->field[q], h->field2[q], (size_t)h->total * h->esize,
-                                 hipMemcpyDeviceToDevice, h->stream));
-    for (int w = 0; w < 18; ++w)
-      if (h->mem[w] && h->mem_elems[w])
-        HIPCHK(h, hipMemcpyAsync(h->mem[w], h->mem2[w], h->mem_elems[w] * h->esize,
-                                 hipMemcpyDeviceToDevice, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    h->cur = 0;
-  }
+  if (on && (rc = ensure_second_state(h))) return rc;
+  if (!on && (rc = restore_first_state(h))) return rc;
   h->fused = on != 0;
+  h->graph_dirty = true;
+  return CPML_OK;
+}
+
+int cpml_gpu_set_time_block(cpml_solver *h, int steps, int strip_width, int chunk_rows) {
+  if (!h) return CPML_ERR_ARG;
+  if (steps < 0) return fail(h, CPML_ERR_ARG, "steps must be >= 0");
+  if (steps > 0 && h->ndim == 3 && h->order != 2)
+    return fail(h, CPML_ERR_ARG, "3D time blocking is built for the second-order step");
+  const int nmax = h->ndim == 3 ? cpml::tb3d_max_nt((int)h->esize)
+                                : cpml::tb2d_max_nt(h->order, (int)h->esize);
+  if (steps > nmax)
+    return fail(h, CPML_ERR_ARG, "at most " + std::to_string(nmax) +
+                                     " steps per launch for this order and precision");
+  const int r = h->order / 2;
+  const int g = steps * (2 * r - 1);
+  if (strip_width < 0 || (steps > 0 && strip_width > 64 - 2 * g))
+    return fail(h, CPML_ERR_ARG, "strip width must be 0 (default) or 1.." +
+                                     std::to_string(64 - 2 * g) + " for this order and step count");
+  if (chunk_rows < 0) return fail(h, CPML_ERR_ARG, "chunk rows must be >= 0");
+  const int jc = chunk_rows > 0 ? chunk_rows : (h->ndim == 3 ? 64 : 128);
+  if (!chunk_fits(h, jc))
+    return fail(h, CPML_ERR_ARG, "chunk too deep for the kernels' 32-bit buffer offsets");
+  int rc = set_device(h);
+  if (rc) return rc;
+  if (steps > 0 && (rc = ensure_second_state(h))) return rc;
+  if (steps == 0 && (rc = restore_first_state(h))) return rc;
+  h->tb = steps;
+  h->tb_ow = strip_width > 0 ? strip_width : std::max(8, (64 - 2 * g) / 8 * 8);
+  if (h->tb_ow > 64 - 2 * g) h->tb_ow = 64 - 2 * g;
+  h->tb_jc = jc;
+  h->tb_rec_ow = -1;
   h->graph_dirty = true;
   return CPML_OK;
 }

@@ -150,6 +150,51 @@ struct Record {
   const int *idx;  // nrec offsets (i + j*ld + k*ps)
   T *sx, *sy, *sz;  // [nstep][nrec]
   int *step;
+  // the state is that of step it + skip (a launch that advanced skip + 1 steps
+  // and recorded the first skip itself): recorded as step it + skip, and the
+  // counter advances by skip + 1
+  int skip;
+};
+
+// Time-blocked 3D step: nt leapfrog steps per launch, state a -> the arrays
+// below.  A workgroup is a tile of `ow` columns x (its wavefronts - 2 nt)
+// rows marching through a z chunk; tiles whose points and ring avoid the x
+// and y slabs form the box [box[0], box[1]) x [box[2], box[3]) of tile
+// coordinates, and z chunks are cut per region as in Fused3D (z0, z1, zn).
+template <typename T>
+struct TB3D {
+  Step3D<T> a;                                          // state in
+  T *vx, *vy, *vz, *sxx, *syy, *szz, *sxy, *sxz, *syz;  // state out, interior origin
+  T *mx[6], *my[6], *mz[6];                             // memory variables out
+  int ow, ty;          // owned columns and rows of a tile
+  int box[4];
+  int z0, z1, zn[3];
+  // receivers of each x-y tile (t = ty * ntx + tx) sorted by plane: entries
+  // [rec_start[t], rec_start[t+1]) of rec_i, rec_j, rec_k (0-based) and rec_slot
+  const int *rec_start, *rec_i, *rec_j, *rec_k, *rec_slot;
+  T *sisx, *sisy, *sisz;  // [nstep][nrec]
+  int nrec;
+};
+
+// Time-blocked 2D step: nt leapfrog steps per launch, state a -> the arrays
+// below.  Work items are (strip, chunk): 64-lane wavefronts owning `ow`
+// columns of a strip and the rows [jb, jb + jc) of a chunk.  Items whose
+// columns and rows (with their rings) avoid the absorbing slabs form the box
+// of strips [sx0, sx1) x chunks [cy0, cy1); the others run in the launch built
+// with the C-PML code.
+template <typename T>
+struct TB2D {
+  Step2D<T> a;                   // state in (fields, memory variables, coefficients)
+  T *vx, *vy, *sxx, *syy, *sxy;  // state out, interior origin
+  T *mx[4], *my[4];              // memory variables out (layout of a.mx, a.my)
+  int ow, jc, nstrips, nchunks;
+  int sx0, sx1, cy0, cy1;
+  // receivers of each strip sorted by row: strip t owns entries
+  // [rec_start[t], rec_start[t+1]) of rec_i, rec_j (0-based) and rec_slot
+  // (the receiver's index in the seismogram rows)
+  const int *rec_start, *rec_i, *rec_j, *rec_slot;
+  T *sisx, *sisy;  // [nstep][nrec]
+  int nrec;
 };
 
 // Energy over a window, per step, accumulated in double.
@@ -241,6 +286,16 @@ template <typename T>
 hipError_t launch_fused3d(const Fused3D<T> &p, hipStream_t s);
 template <typename T>
 hipError_t launch_record(const Record<T> &p, hipStream_t s);
+// nt steps per launch (3D, second order); tiles of ow columns x (nw - 2 nt)
+// rows marching through z chunks of at most kchunk planes
+template <typename T>
+hipError_t launch_tb3d(int nt, const TB3D<T> &p, hipStream_t s);
+int tb3d_max_nt(int prec);
+// nt steps per launch (2D, plain elastic); clean = items without C-PML code
+template <typename T>
+hipError_t launch_tb2d(int order, int nt, const TB2D<T> &p, hipStream_t s);
+// largest nt launch_tb2d has a kernel for, per order and precision
+int tb2d_max_nt(int order, int prec);
 template <typename T>
 hipError_t launch_energy(const Energy<T> &p, int nblocks, hipStream_t s);
 template <typename T>

@@ -69,10 +69,11 @@ __global__ __launch_bounds__(BX *TY) void k_velocity2d(Step2D<T> p) {
 //
 // A 3D tile is 64 points along x (one wavefront per row) by TY3 rows, and
 // marches through a chunk of z planes carrying the z neighbours in
-// registers.  All offsets are unsigned 32-bit from the start of the
-// allocation (the host checks that every array fits), added to uniform base
-// pointers, and the loads of plane k+1 are issued before the arithmetic and
-// stores of plane k (register double buffering).
+// registers.  Field offsets are unsigned 32-bit byte offsets into a buffer
+// rebased at the tile's first halo plane (the host keeps a chunk's planes plus
+// 8 halo and pipeline planes below 4 GiB: chunk_fits() in cpml_api.hip),
+// added to uniform base pointers, and the loads of plane k+1 are issued
+// before the arithmetic and stores of plane k (register double buffering).
 
 struct Tile3 {
   int i, j, kb, ke;
@@ -384,7 +385,7 @@ __global__ __launch_bounds__(BX *TY3) void k_velocity3d(Step3D<T> p) {
 
 template <typename T>
 __global__ __launch_bounds__(kMaxRecBlock) void k_record(Record<T> p) {
-  const int it = *p.step;
+  const int it = *p.step + p.skip;
   const long row = (long)(it - 1) * p.nrec;
   for (int r = threadIdx.x; r < p.nrec; r += blockDim.x) {
     const long q = p.idx[r];

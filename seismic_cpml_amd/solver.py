@@ -66,6 +66,7 @@ SIGNATURES = {
     "cpml_gpu_set_kchunk": [_VP, _I],
     "cpml_gpu_set_tile": [_VP, _I],
     "cpml_gpu_set_fused": [_VP, _I],
+    "cpml_gpu_set_time_block": [_VP, _I, _I, _I],
     "cpml_gpu_set_kernel_timing": [_VP, _I],
     "cpml_gpu_kernel_time": [_VP, _I, _IP, _DP],
     "cpml_gpu_elapsed_ms": [_VP, _DP],
@@ -312,6 +313,9 @@ class CpmlGpu:
 
     def set_tile(self, rows):
         self._call("cpml_gpu_set_tile", rows)
+
+    def set_time_block(self, steps, strip_width=0, chunk_rows=0):
+        self._call("cpml_gpu_set_time_block", steps, strip_width, chunk_rows)
 
     def set_fused(self, on=True):
         self._call("cpml_gpu_set_fused", int(on))

@@ -193,7 +193,8 @@ def layered_3d(n=768, nstep=2000, seed=3, prec=4, courant=0.5, npml=20, dx=10.0,
                     medium_dev=(lam, mu, rho), cp_max=cp_max)
 
 
-def bytes_per_step(w: Workload, slabs, fused: bool = False, nsls: int = 0) -> dict:
+def bytes_per_step(w: Workload, slabs, fused: bool = False, nsls: int = 0,
+                   time_block: int = 0) -> dict:
     """Algorithmic HBM bytes of one time step, per kernel: every array touched
     once per kernel.  3D stress: read v(3), sigma(6), DELTAT*lambda, DELTAT*mu,
     write sigma(6) = 17 words; velocity: read sigma(6), v(3), DELTAT/rho, write
@@ -204,7 +205,12 @@ def bytes_per_step(w: Workload, slabs, fused: bool = False, nsls: int = 0) -> di
     writes = 10 words; velocity 5 + rho + 2 writes = 8 words; 2 memory
     variables per slab point per phase (4 words); the 2D viscoelastic stress
     kernel also reads and writes R_e1, R_e11, R_e13 per SLS (6 words per
-    SLS)."""
+    SLS).  A 2D time-blocked launch of `time_block` steps reads v(2), sigma(3),
+    lambda, mu, rho and writes v(2), sigma(3) once (13 words) and reads and
+    writes the 4 memory variables of each slab axis once (8 words per slab
+    point); "time_block" is those bytes per launch, "step" per time step.  A
+    3D time-blocked launch moves the fused step's 21 words per point and 12
+    per slab point once per `time_block` steps."""
     es = 4 if w.prec == 4 else 8
     wx = slabs[0] + slabs[1]
     wy = slabs[2] + slabs[3]
@@ -212,6 +218,9 @@ def bytes_per_step(w: Workload, slabs, fused: bool = False, nsls: int = 0) -> di
     n = w.cells
     if w.ndim == 3:
         slab_pts = wx * w.ny * w.nz + w.nx * wy * w.nz + w.nx * w.ny * wz
+        if time_block:
+            f = es * (21 * n + 12 * slab_pts)
+            return {"time_block": f, "step": f / time_block}
         if fused:
             f = es * (21 * n + 12 * slab_pts)
             return {"fused": f, "step": f}
@@ -219,6 +228,9 @@ def bytes_per_step(w: Workload, slabs, fused: bool = False, nsls: int = 0) -> di
         vel = es * (13 * n + 6 * slab_pts)
     else:
         slab_pts = wx * w.ny + w.nx * wy
+        if time_block:
+            f = es * (13 * n + 8 * slab_pts)
+            return {"time_block": f, "step": f / time_block}
         stress = es * ((10 + 6 * nsls) * n + 4 * slab_pts)
         vel = es * (8 * n + 4 * slab_pts)
     return {"stress": stress, "velocity": vel, "step": stress + vel}

@@ -66,6 +66,9 @@ CASES = {
 OTHER_CASES = {
     "3d_wide": dict(CASES["3d_small"], NX=160, NY=48, NZ=40, NSTEP=200, ISOURCE=91,
                     JSOURCE=33),
+    # the 3D benchmark medium at test size (layered along z, +-5% smoothed
+    # perturbation, Courant 0.5): inputs from tests/layered_media.py only
+    "3d_layered": dict(order=2, NX=96, NY=80, NZ=72, NPML=10, NSTEP=240),
 }
 # 3D 4th order: the elastic limit of the 3D viscoelastic program
 # (oracle/build_ref.py _SMALL_3DV; 3D-visco:152-222), 2 MPI ranks, float64

@@ -212,3 +212,4 @@ def test_su_writer_rounds_and_refuses(tmp_path):
     for bad in ((np.zeros((1, 32768)), 1.0), (x, 32768.0), (x, -1.0)):
         with pytest.raises(solver.CpmlError):
             write_su(tmp_path / "c.su", *bad)
+

@@ -92,3 +92,28 @@ def test_3d4_reference_halo_reads_matter():
         for k in FIELDS_3D:
             diff |= set(np.nonzero((a[k] != b[k]).any(axis=(1, 2)))[0].tolist())
         assert sorted(diff) == planes, (n, sorted(diff))
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_reference_f32_build_against_its_f64_build(case):
+    """The north-star bar for single precision is 1e-4 relative L2 against the
+    reference's double-precision build.  The reference's own float32 build
+    (the textual variant its authors prescribe, 2D2:131-133) meets it on the
+    seismograms of every case; its final wavefields meet it where the wave is
+    still in the grid (the layered cases) but not after the wave has left
+    (shipped 2D2/2D4: up to 4.1e-3 on sigma_xx; reduced 3D: 1.5e-3 on vz),
+    where the field left is rounding noise of the absorbed wave."""
+    a, b = fixture(case, "f32"), fixture(case, "f64")
+
+    def rel(x, y):
+        x, y = np.asarray(x, np.float64), np.asarray(y, np.float64)
+        return np.linalg.norm(x - y) / np.linalg.norm(y)
+
+    for k in ("sisvx", "sisvy"):
+        assert rel(a[k], b[k]) <= 1e-4, k
+    fields = [k for k in a.keys() if k.startswith("sub_")]
+    worst = max(rel(a[k], b[k]) for k in fields)
+    if "layered" in case:
+        assert worst <= 1e-4
+    else:
+        assert worst > 1e-4
