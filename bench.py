@@ -196,7 +196,8 @@ def bench_3d(args, order=2):
     from seismic_cpml_amd import workloads
 
     steps = args.steps or 2000
-    nt = args.time_block
+    # default: 2 steps per launch (time-blocked) for the 2nd-order step
+    nt = args.time_block if args.time_block >= 0 else (2 if order == 2 and not args.fused else 0)
     if nt > 0:
         steps = max(nt, steps // nt * nt)  # whole blocks only in the timed region
     nstep = args.warmup + steps
@@ -272,7 +273,7 @@ def bench_2d(args):
     from seismic_cpml_amd import workloads
 
     steps = args.steps or 4000
-    nt = args.time_block
+    nt = max(args.time_block, 0)
     if nt > 0:
         steps = max(nt, steps // nt * nt)  # whole blocks only in the timed region
     nstep = args.warmup + steps
@@ -447,6 +448,7 @@ def bench_batch(args):
     Path(cache).parent.mkdir(parents=True, exist_ok=True)
     # the autotuner leaves the batch at rest with the fastest launch shape set
     ty, bpc, tune_ms = b.autotune(trial_steps=min(30, steps), cache_path=str(cache))
+    _, _, tb, jc = b.launch_shape()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     b.run(1, steps)
@@ -456,41 +458,57 @@ def bench_batch(args):
     value = cells * steps / wall / 1e9
     seis = [b.seismograms(q) for q in range(len(w.shots))]
     b.close()
-    # the same shots one after another with the plain solver
-    seq_run, seq_total, same = 0.0, 0.0, 0
+    # the same shots one after another with the plain solver: its default
+    # single-step kernels, and time-blocked as the batch is
     nseq = len(w.shots) if args.seq_shots is None else min(args.seq_shots, len(w.shots))
-    for q in range(nseq):
-        t0 = time.perf_counter()
-        h = w.make_solver(q)
-        t1 = time.perf_counter()
-        h.run()
-        torch.cuda.synchronize()
-        t2 = time.perf_counter()
-        sx, sy = h.seismograms()
-        h.close()
-        seq_run += t2 - t1
-        seq_total += t2 - t0
-        same += int(np.array_equal(sx, seis[q][0]) and np.array_equal(sy, seis[q][1]))
     seq_cells = sum(s["NX"] * s["NY"] for s in w.shots[:nseq])
-    seq_value = seq_cells * steps / seq_run / 1e9 if nseq else None
+
+    def sequential(block):
+        run, total, same = 0.0, 0.0, 0
+        for q in range(nseq):
+            t0 = time.perf_counter()
+            h = w.make_solver(q)
+            if block:
+                h.set_time_block(block, 0, jc)
+            t1 = time.perf_counter()
+            h.run()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            sx, sy = h.seismograms()
+            h.close()
+            run += t2 - t1
+            total += t2 - t0
+            same += int(np.array_equal(sx, seis[q][0]) and np.array_equal(sy, seis[q][1]))
+        value = seq_cells * steps / run / 1e9 if nseq else None
+        return {"shots": nseq, "time_block": block, "gcell_s": value and round(value, 3),
+                "run_s": round(run, 3), "with_setup_s": round(total, 3),
+                "bitwise_equal_seismograms": same}
+
+    seq = sequential(0)
+    seq_tb = sequential(tb) if tb else None
+    seq_value = seq["gcell_s"]
     es = 4 if w.prec == 4 else 8
-    bytes_per = es * 18  # two-phase 2D interior words; slabs add under 1%
+    # algorithmic bytes per cell-update: single steps 18 words (2D two-phase
+    # interior), time-blocked 13 words per tb steps; slabs add under 1%
+    bytes_per = es * (13.0 / tb if tb else 18)
     config = {
         "workload": f"{len(w.shots)} shots of 2D 4th-order C-PML, NX, NY ~ U{{512..2048}} "
                     "(seeded), NPML 20, per-shot layered model, source and Ricker f0 in "
                     "[5, 15] Hz, 64 receivers each",
         "cells_per_step": cells, "setup_s": round(setup, 2),
-        "autotune": {"tile_rows": ty, "blocks_per_cu": bpc, "trial_ms": tune_ms},
-        "sequential": {"shots": nseq, "gcell_s": seq_value and round(seq_value, 3),
-                       "run_s": round(seq_run, 3), "with_setup_s": round(seq_total, 3),
-                       "bitwise_equal_seismograms": same},
+        "autotune": {"tile_rows": ty, "blocks_per_cu": bpc, "time_block": tb,
+                     "chunk_rows": jc, "trial_ms": tune_ms},
+        "sequential": seq,
+        "sequential_time_blocked": seq_tb,
         "batched_vs_sequential": seq_value and round(value / seq_value, 3),
+        "batched_vs_sequential_time_blocked":
+            seq_tb and seq_tb["gcell_s"] and round(value / seq_tb["gcell_s"], 3),
         "step_hbm_frac": round(bytes_per * value / HBM_PEAK_GBS, 4),
     }
     roof = {"bound": "hbm", "kernel": "batch step", "achieved": round(bytes_per * value, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(bytes_per * value / HBM_PEAK_GBS, 4),
-            "traffic": _traffic(f"batch_{len(w.shots)}shots_tile{ty}_bpc{bpc}", "step")}
+            "traffic": _traffic(f"batch_{len(w.shots)}shots_tile{ty}_bpc{bpc}_tb{tb}", "step")}
     cpu = None
     if not args.no_cpu_baseline:
         s0 = w.shots[0]
@@ -589,8 +607,9 @@ def main():
     ap.add_argument("--n", type=int, default=0, help="grid points per axis (3d, 2d)")
     ap.add_argument("--kchunk", type=int, default=0)
     ap.add_argument("--tile", type=int, default=0, help="rows per tile (4, 8, 16)")
-    ap.add_argument("--time-block", type=int, default=0,
-                    help="2d: leapfrog steps per launch (time blocking; 0: off)")
+    ap.add_argument("--time-block", type=int, default=-1,
+                    help="leapfrog steps per launch (time blocking; 0: off; -1: the "
+                         "workload's default: 2 for 3d, off for 2d)")
     ap.add_argument("--strip", type=int, default=0, help="2d time blocking: strip width")
     ap.add_argument("--chunk-rows", type=int, default=0, help="2d time blocking: chunk rows")
     ap.add_argument("--fused", action="store_true",

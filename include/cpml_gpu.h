@@ -13,14 +13,17 @@
  *  - All functions return a status code (CPML_OK == 0).  No C++ exception
  *    crosses the ABI.  cpml_gpu_last_error() returns the message.
  *  - Grid arrays are passed in Fortran order (x fastest), interior points only
- *    (NX*NY or NX*NY*NZ values), in the handle's precision (prec = 4: float,
- *    8: double).  Host arrays are borrowed for the duration of the call.
+ *    (NX*NY or NX*NY*NZ values) unless cpml_gpu_set_host_layout declares
+ *    ghost cells, in the handle's precision (prec = 4: float, 8: double).
+ *    Host arrays are borrowed for the duration of the call.
  *  - Indices (ISOURCE, ix_rec, ...) are 1-based as in the reference.
  *  - A handle is single-threaded; it owns one HIP stream, its device buffers
  *    and its captured step graphs.  Distinct handles may run concurrently.
  */
 #ifndef CPML_GPU_H
 #define CPML_GPU_H
+
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -130,6 +133,12 @@ int cpml_gpu_set_field(cpml_solver *h, int which, const void *in);
 /* One memory variable (CPML_MEM_...) expanded to the full grid (zero outside
  * its absorbing slab), as the reference stores it. */
 int cpml_gpu_get_memory(cpml_solver *h, int which, void *out);
+/* The inverse: a full-grid memory variable (its slab points are stored, the
+ * rest is the reference's exact zero).  With cpml_gpu_set_field, restores a
+ * state saved by get_field / get_memory exactly, so that a run resumed from
+ * a checkpoint continues as the uninterrupted run does, bit for bit (the
+ * reference keeps these arrays as plain program variables, 3D:222-240). */
+int cpml_gpu_set_memory(cpml_solver *h, int which, const void *in);
 /* max over the grid of sqrt(vx^2 + vy^2 [+ vz^2]) (2D2:719, 3D:1185). */
 int cpml_gpu_velocity_norm(cpml_solver *h, double *vmax);
 
@@ -174,6 +183,40 @@ int cpml_gpu_device_bytes(cpml_solver *h, double *bytes);
  * y min/max, z min/max. */
 int cpml_gpu_slab_widths(cpml_solver *h, int *w);
 
+/* CPML_OK when arrays of `kind` bytes per value (4 or 8) match the handle's
+ * precision, else CPML_ERR_ARG with the message; the Fortran module calls it
+ * before handing over real(4) or real(8) arrays. */
+int cpml_gpu_check_kind(cpml_solver *h, int kind);
+/* Grid of the handle (NZ = 1 in 2D). */
+int cpml_gpu_grid(cpml_solver *h, int *NX, int *NY, int *NZ);
+/* Host layout of the grid arrays handed to set_medium, get_field, set_field,
+ * get_memory and set_memory: dimensioned (1-lo : N+hi) along every grid axis,
+ * x fastest -- the reference's explicit-lower-bound arrays, e.g.
+ * 2D4:205-225 `(0:NX+1,0:NY+1)` is lo = hi = 1 and ADE8:272-301
+ * `(-4:NX+4,-4:NY+4)` is lo = 5, hi = 4.  Only the interior 1..N is read or
+ * written (the ghost cells of the host array are left as they are), straight
+ * from the caller's array: no interior copy on the host.  Default 0, 0;
+ * 0 <= lo, hi <= 8.  The 1D profile, source and seismogram arrays keep their
+ * own bounds (the reference declares its profiles (NX), 2D4:244). */
+int cpml_gpu_set_host_layout(cpml_solver *h, int lo, int hi);
+/* CPML_OK when a host grid array of `kind` bytes per value, rank `rank` and
+ * extents shape[0..rank-1] fits the handle's grid and host layout (rank ==
+ * ndim: every extent; other ranks: the number of values), else CPML_ERR_ARG
+ * naming both shapes; the Fortran module calls it on every grid array. */
+int cpml_gpu_check_grid_array(cpml_solver *h, int kind, int rank, const int64_t *shape);
+
+/* Debug aid against out-of-grid accesses.  Guard cells are the words of the
+ * device field, medium and memory-variable arrays that belong neither to the
+ * grid nor to its ghost points (the row padding that aligns every row to 256
+ * bytes).  op 1 fills every guard cell with quiet NaN and sets *count to
+ * their number; op 0 sets *count to the number of guard cells that no longer
+ * hold a NaN (a store outside the grid).  A run after op 1 whose fields and
+ * seismograms are unchanged shows that no kernel lets a guard cell reach a
+ * result.  The guard cells of arrays allocated later (set_pml, the second
+ * state of set_fused / set_time_block) are not poisoned: call it after the
+ * set-up. */
+int cpml_gpu_guard_cells(cpml_solver *h, int op, long long *count);
+
 /* Message of the last error on this handle (h == NULL: of the last failed
  * cpml_gpu_init on this thread).  Always NUL-terminated. */
 int cpml_gpu_last_error(cpml_solver *h, char *buf, int len);
@@ -187,7 +230,8 @@ int cpml_gpu_last_error(cpml_solver *h, char *buf, int len);
  * time step, NPOINTS_PML, medium, profiles, source and receivers, and its
  * results equal a cpml_solver run of that shot alone, bit for bit.  Per-shot
  * arrays are indexed 0..nshots-1; every other convention is cpml_solver's.
- * Energy and the stability check of cpml_gpu_run are not computed here. */
+ * Energy and the stability check are per shot (cpml_batch_set_energy,
+ * cpml_batch_set_stability). */
 typedef struct cpml_batch cpml_batch;
 
 int cpml_batch_init(cpml_batch **b, int order, int prec, int nshots, const int *NX,
@@ -210,19 +254,45 @@ int cpml_batch_set_receivers(cpml_batch *b, int shot, int NREC, const int *ix_re
  * grid of blocks_per_cu * CUs workgroups; steps per captured graph (0: no
  * graph). */
 int cpml_batch_set_tiling(cpml_batch *b, int tile_y, int blocks_per_cu, int graph_steps);
-/* Times every launch shape over trial_steps steps of this batch, keeps the
- * fastest, then returns the batch to rest (fields, memory variables and
- * seismograms zero).  cache_path (may be NULL) is a text file of earlier
- * choices keyed by device, order, precision and batch shape; a hit skips the
- * trials (best_ms = -1). */
+/* Times every launch shape (single steps: rows per tile x persistent grid;
+ * time-blocked: steps per launch x chunk rows x persistent grid) over about
+ * trial_steps steps of this batch, keeps the fastest per step, then returns
+ * the batch to rest (fields, memory variables and seismograms zero); the
+ * whole choice is read back with cpml_batch_launch_shape.  cache_path (may
+ * be NULL) is a text file of earlier choices keyed by device, order,
+ * precision and batch shape; a hit skips the trials (best_ms = -1). */
 int cpml_batch_autotune(cpml_batch *b, int trial_steps, const char *cache_path, int *tile_y,
                         int *blocks_per_cu, double *best_ms);
 /* Steps it_begin..it_end of every shot (2D2:545-735 per shot). */
+/* Time blocking of the whole batch: `steps` leapfrog steps of every shot per
+ * launch (the kernel of cpml_gpu_set_time_block over a flat list of (shot,
+ * strip, chunk) items; slab items beside the others on a second stream;
+ * state ping-ponging between two pools), 0 = single steps.  strip_width,
+ * chunk_rows: as cpml_gpu_set_time_block (0 = defaults).  With energy on,
+ * blocks are used when the cadence is a multiple of `steps`. */
+int cpml_batch_set_time_block(cpml_batch *b, int steps, int strip_width, int chunk_rows);
+/* The launch shape in use (after set_tiling / set_time_block / autotune). */
+int cpml_batch_launch_shape(cpml_batch *b, int *tile_y, int *blocks_per_cu, int *time_block,
+                            int *chunk_rows);
+/* Per-shot energy every `cadence` steps (0 = off) over each shot's reference
+ * window (2D2:695 / 2D4:696), 2D2:694-713; kinetic(NSTEP), potential(NSTEP)
+ * of one shot (double). */
+int cpml_batch_set_energy(cpml_batch *b, int cadence);
+int cpml_batch_get_energy(cpml_batch *b, int shot, double *kinetic, double *potential);
+/* Per-shot stability check inside run (2D2:716-726): at it % IT_DISPLAY == 0
+ * and it == 5 the max norm of v of every shot is compared with threshold; the
+ * first shot above it (or NaN) fails the run with CPML_ERR_BLOWUP and its
+ * index in the message.  IT_DISPLAY <= 0 disables it. */
+int cpml_batch_set_stability(cpml_batch *b, int IT_DISPLAY, double threshold);
 int cpml_batch_run(cpml_batch *b, int it_begin, int it_end);
 int cpml_batch_get_seismograms(cpml_batch *b, int shot, void *sisvx, void *sisvy);
 /* which: CPML_VX, CPML_VY, CPML_SIGMAXX, CPML_SIGMAYY or CPML_SIGMAXY */
 int cpml_batch_get_field(cpml_batch *b, int shot, int which, void *out);
 int cpml_batch_elapsed_ms(cpml_batch *b, double *ms);
+/* as cpml_gpu_check_kind for a batch */
+int cpml_batch_check_kind(cpml_batch *b, int kind);
+/* grid of one shot */
+int cpml_batch_grid(cpml_batch *b, int shot, int *NX, int *NY);
 int cpml_batch_last_error(cpml_batch *b, char *buf, int len);
 
 /* ---- learned thin absorbing boundary (3D, FP32) ------------------------ */

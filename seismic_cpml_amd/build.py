@@ -12,6 +12,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
@@ -50,22 +51,57 @@ def _run(cmd, cwd=None):
     return r
 
 
-def build_hip(verbose: bool = False) -> Path:
-    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+def _hipcc():
+    return shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+
+
+def _compile_all(flags, suffix, verbose):
+    """Compile every HIP source with `flags` in parallel; returns the objects."""
     BUILD.mkdir(exist_ok=True)
-    objs = []
+    jobs = []
     for src in HIP_SOURCES:
-        obj = BUILD / (Path(src).stem + ".o")
-        _run([hipcc, *HIPFLAGS, "-c", str(CSRC / src), "-o", str(obj)])
-        objs.append(str(obj))
-        if verbose:
-            print("compiled", src)
+        obj = BUILD / (Path(src).stem + suffix)
+        jobs.append(([_hipcc(), *flags, "-c", str(CSRC / src), "-o", str(obj)], obj, src))
+    workers = max(1, min(len(jobs), os.cpu_count() or 1, 8))
+    with ThreadPoolExecutor(workers) as pool:
+        for (_, obj, src), _r in zip(jobs, pool.map(lambda j: _run(j[0]), jobs)):
+            if verbose:
+                print("compiled", src, "->", obj.name)
+    return [str(j[1]) for j in jobs]
+
+
+def build_hip(verbose: bool = False) -> Path:
+    objs = _compile_all(HIPFLAGS, ".o", verbose)
     tmp = LIB.with_suffix(".so.tmp")
-    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-o", str(tmp), *objs])
+    _run([_hipcc(), f"--offload-arch={ARCH}", "-shared", "-o", str(tmp), *objs])
     os.replace(tmp, LIB)
     if verbose:
         print("linked", LIB)
     return LIB
+
+
+# AddressSanitizer build of the host code (the device code is the ordinary
+# gfx950 build), linked with the C-ABI exercise tests/native/cpml_host_check.cpp
+# into one executable; the sanitizer applies to the host pass only.
+ASAN_HOST = ["-g", "-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fno-omit-frame-pointer"]
+ASAN_EXE = BUILD / "cpml_host_check_asan"
+
+
+def build_asan(verbose: bool = False) -> Path:
+    objs = _compile_all(HIPFLAGS + ASAN_HOST, ".asan.o", verbose)
+    main = ROOT / "tests" / "native" / "cpml_host_check.cpp"
+    main_obj = BUILD / "cpml_host_check.asan.o"
+    gxx = shutil.which("g++") or "g++"
+    _run([gxx, "-std=c++17", "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer", "-c",
+          str(main), "-o", str(main_obj)])
+    rocm = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
+    tmp = ASAN_EXE.with_suffix(".tmp")
+    _run([gxx, "-fsanitize=address", "-o", str(tmp), str(main_obj), *objs,
+          f"-L{rocm / 'lib'}", "-lamdhip64", f"-Wl,-rpath,{rocm / 'lib'}"])
+    os.replace(tmp, ASAN_EXE)
+    if verbose:
+        print("linked", ASAN_EXE)
+    return ASAN_EXE
 
 
 # Driver programs: (source, executable, preprocessor macros).  WP is the real
@@ -75,6 +111,11 @@ DRIVERS = [
     ("seismic_cpml_2d_isotropic_gpu.F90", "seismic_cpml_2d_second_f32", {"ORDER": 2, "WP": 4}),
     ("seismic_cpml_2d_isotropic_gpu.F90", "seismic_cpml_2d_fourth_f64", {"ORDER": 4, "WP": 8}),
     ("seismic_cpml_2d_isotropic_gpu.F90", "seismic_cpml_2d_fourth_f32", {"ORDER": 4, "WP": 4}),
+    # checkpoint after a display step, a new handle, restore, go on
+    ("seismic_cpml_2d_isotropic_gpu.F90", "seismic_cpml_2d_second_f64_resume",
+     {"ORDER": 2, "WP": 8, "RESUME_AT": 1000}),
+    ("seismic_cpml_2d_isotropic_gpu.F90", "seismic_cpml_2d_fourth_f32_resume",
+     {"ORDER": 4, "WP": 4, "RESUME_AT": 2200}),
     ("seismic_cpml_3d_isotropic_gpu.F90", "seismic_cpml_3d_f64", {"WP": 8}),
     ("seismic_cpml_3d_isotropic_gpu.F90", "seismic_cpml_3d_f32", {"WP": 4}),
     ("seismic_cpml_3d_isotropic_gpu.F90", "seismic_cpml_3d_small_f64",
@@ -91,6 +132,8 @@ DRIVERS = [
     ("seismic_cpml_2d_visco_gpu.F90", "seismic_cpml_2d_visco_small",
      {"NX_": 161, "NY_": 161, "NSTEP_": 700, "XSRC_": 120, "YSRC_": 120, "NREC_": 3,
       "XDEB_": 60, "YDEB_": 190, "XFIN_": 190, "YFIN_": 60}),
+    # kind / shape checks and the ghost-padded host layout of the binding
+    ("cpml_gpu_binding_check.F90", "cpml_gpu_binding_check", {}),
 ]
 
 
@@ -120,6 +163,7 @@ def build_fortran(verbose: bool = False) -> list[Path]:
 def build(verbose: bool = False):
     build_hip(verbose)
     build_fortran(verbose)
+    build_asan(verbose)
 
 
 if __name__ == "__main__":

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -104,6 +105,23 @@ __global__ void k_medium_check(const T *lam, const T *mu, const T *rho, int nx, 
   atomicMax(vmax, (unsigned long long)__double_as_longlong(m));
 }
 
+// Guard cells of a row-padded array: the columns c of every row (length ld)
+// with c < lo or c >= hi.  op 1: fill them with quiet NaN; op 0: count those
+// that no longer hold a NaN.
+template <typename T>
+__global__ void k_guard(T *a, long n, long ld, long lo, long hi, int op,
+                        unsigned long long *changed) {
+  unsigned long long c = 0;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n;
+       t += (long)gridDim.x * blockDim.x) {
+    const long col = t % ld;
+    if (col >= lo && col < hi) continue;
+    if (op) a[t] = std::numeric_limits<T>::quiet_NaN();
+    else if (a[t] == a[t]) ++c;
+  }
+  if (c) atomicAdd(changed, c);
+}
+
 long round_up(long v, long m) { return (v + m - 1) / m * m; }
 
 }  // namespace
@@ -115,6 +133,8 @@ struct cpml_solver {
   size_t esize = 8;
   int R = 1;
   long xpad = 0, ld = 0, ps = 0, base = 0, total = 0, mld = 0;
+  // host grid arrays span (1-hlo : N+hhi) along each grid axis
+  int hlo = 0, hhi = 0;
   Slab sx, sy, sz;
   hipStream_t stream = nullptr;
   std::vector<void *> allocs;
@@ -127,6 +147,9 @@ struct cpml_solver {
   // 2D time blocking: tb steps per launch (0: off), strips of tb_ow columns
   // (0: the widest multiple of 8 the ring allows), chunks of tb_jc rows
   int tb = 0, tb_ow = 0, tb_jc = 128;
+  // side stream of the time-blocked launches (slab classes beside the rest)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int *tb_rec = nullptr;        // per-strip receiver lists (device)
   int tb_rec_ow = -1;           // strip width they were built for (-1: stale)
   std::vector<int> rec_ix, rec_iy, rec_iz;  // receiver grid indices (0-based), host copy
@@ -258,16 +281,27 @@ T *M(cpml_solver *h, int q) {
   return static_cast<T *>(h->media[q]) + h->base;
 }
 
-// Copy an interior-only host array into / out of a padded device array.
+// Host grid arrays: x fastest, (1-hlo : N+hhi) along x, y and (3D) z.
+// host_x / host_y are the padded extents; host_at is the element offset of
+// the 0-based interior point (i, j, k).
+size_t host_x(const cpml_solver *h) { return (size_t)h->nx + h->hlo + h->hhi; }
+size_t host_y(const cpml_solver *h) { return (size_t)h->ny + h->hlo + h->hhi; }
+size_t host_at(const cpml_solver *h, long i, long j, long k) {
+  const size_t kk = h->ndim == 3 ? (size_t)(k + h->hlo) : 0;
+  return (size_t)(i + h->hlo) + host_x(h) * ((size_t)(j + h->hlo) + host_y(h) * kk);
+}
+
+// Copy the interior of a host grid array into / out of a padded device array
+// (ghost cells of the host array are neither read nor written).
 int upload_padded(cpml_solver *h, void *dst_alloc, const void *src) {
   const size_t es = h->esize;
   char *dst = static_cast<char *>(dst_alloc) + h->base * es;
   const char *s = static_cast<const char *>(src);
   const int nzz = h->ndim == 3 ? h->nz : 1;
   for (int k = 0; k < nzz; ++k)
-    HIPCHK(h, hipMemcpy2DAsync(dst + k * h->ps * es, h->ld * es,
-                               s + (size_t)k * h->nx * h->ny * es, h->nx * es, h->nx * es, h->ny,
-                               hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpy2DAsync(dst + k * h->ps * es, h->ld * es, s + host_at(h, 0, 0, k) * es,
+                               host_x(h) * es, h->nx * es, h->ny, hipMemcpyHostToDevice,
+                               h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return CPML_OK;
 }
@@ -278,7 +312,7 @@ int download_padded(cpml_solver *h, const void *src_alloc, void *dst) {
   char *d = static_cast<char *>(dst);
   const int nzz = h->ndim == 3 ? h->nz : 1;
   for (int k = 0; k < nzz; ++k)
-    HIPCHK(h, hipMemcpy2DAsync(d + (size_t)k * h->nx * h->ny * es, h->nx * es,
+    HIPCHK(h, hipMemcpy2DAsync(d + host_at(h, 0, 0, k) * es, host_x(h) * es,
                                src + k * h->ps * es, h->ld * es, h->nx * es, h->ny,
                                hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -684,8 +718,13 @@ cpml::TB3D<T> tb3_of(cpml_solver *h) {
 // kernel takes the last one and advances the step counter by tb.
 template <typename T>
 hipError_t enqueue_tb(cpml_solver *h) {
-  hipError_t e = h->ndim == 3 ? cpml::launch_tb3d<T>(h->tb, tb3_of<T>(h), h->stream)
-                              : cpml::launch_tb2d<T>(h->order, h->tb, tb_of<T>(h), h->stream);
+  hipError_t e = hipEventRecord(h->ev_fork, h->stream);  // fork: side waits for main
+  if (e == hipSuccess) e = hipStreamWaitEvent(h->side, h->ev_fork, 0);
+  if (e == hipSuccess)
+    e = h->ndim == 3 ? cpml::launch_tb3d<T>(h->tb, tb3_of<T>(h), h->stream, h->side)
+                     : cpml::launch_tb2d<T>(h->order, h->tb, tb_of<T>(h), h->stream, h->side);
+  if (e == hipSuccess) e = hipEventRecord(h->ev_join, h->side);  // join: main waits for side
+  if (e == hipSuccess) e = hipStreamWaitEvent(h->stream, h->ev_join, 0);
   h->cur ^= 1;
   return e;
 }
@@ -858,16 +897,22 @@ int velocity_norm(cpml_solver *h, double *out) {
 
 template <typename T>
 int check_medium_host(cpml_solver *h, const T *lam, const T *mu, const T *rho, double *cpmax) {
-  const size_t n = (size_t)h->nx * h->ny * (h->ndim == 3 ? h->nz : 1);
+  const int nzz = h->ndim == 3 ? h->nz : 1;
   double m = 0.0;
-  for (size_t q = 0; q < n; ++q) {
-    const double r = rho[q], u = mu[q], l = lam[q];
-    if (!(r > 0.0) || !(u > 0.0) || !(l + 2.0 * u > 0.0) || !std::isfinite(l))
-      return fail(h, CPML_ERR_ARG,
-                  "medium: rho and mu must be positive and lambda+2mu > 0 at every point "
-                  "(first bad point " + std::to_string(q) + ")");
-    m = std::max(m, (l + 2.0 * u) / r);
-  }
+  for (int k = 0; k < nzz; ++k)
+    for (int j = 0; j < h->ny; ++j) {
+      const size_t row = host_at(h, 0, j, k);
+      for (int i = 0; i < h->nx; ++i) {
+        const size_t q = row + i;
+        const double r = rho[q], u = mu[q], l = lam[q];
+        if (!(r > 0.0) || !(u > 0.0) || !(l + 2.0 * u > 0.0) || !std::isfinite(l))
+          return fail(h, CPML_ERR_ARG,
+                      "medium: rho and mu must be positive and lambda+2mu > 0 at every point "
+                      "(first bad point i,j,k = " + std::to_string(i + 1) + "," +
+                          std::to_string(j + 1) + "," + std::to_string(k + 1) + ")");
+        m = std::max(m, (l + 2.0 * u) / r);
+      }
+    }
   *cpmax = std::sqrt(m);
   return CPML_OK;
 }
@@ -1136,9 +1181,13 @@ int cpml_gpu_finalize(cpml_solver *h) {
   for (auto g : h->gexec)
     if (g) (void)hipGraphExecDestroy(g);
   for (hipEvent_t e : h->evpool) (void)hipEventDestroy(e);
+  if (h->side) (void)hipStreamSynchronize(h->side);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+  if (h->ev_join) (void)hipEventDestroy(h->ev_join);
   for (void *p : h->allocs) (void)hipFree(p);
+  if (h->side) (void)hipStreamDestroy(h->side);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return CPML_OK;
@@ -1505,7 +1554,8 @@ int cpml_gpu_get_memory(cpml_solver *h, int which, void *out) {
                         hipMemcpyDeviceToHost));
   char *o = static_cast<char *>(out);
   const int nx = h->nx, ny = h->ny, nz = h->nz;
-  std::memset(o, 0, (size_t)nx * ny * nz * es);
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j) std::memset(o + host_at(h, 0, j, k) * es, 0, (size_t)nx * es);
   for (int k = 0; k < nz; ++k)
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < nx; ++i) {
@@ -1521,10 +1571,137 @@ int cpml_gpu_get_memory(cpml_solver *h, int which, void *out) {
             if (h->sz.in(k)) src = i + h->mld * (j + (long)ny * h->sz.s(k));
             break;
         }
-        if (src >= 0)
-          std::memcpy(o + ((size_t)i + (size_t)j * nx + (size_t)k * nx * ny) * es,
-                      tmp.data() + src * es, es);
+        if (src >= 0) std::memcpy(o + host_at(h, i, j, k) * es, tmp.data() + src * es, es);
       }
+  return CPML_OK;
+}
+
+int cpml_gpu_check_kind(cpml_solver *h, int kind) {
+  if (!h) return CPML_ERR_ARG;
+  if (kind != h->prec)
+    return fail(h, CPML_ERR_ARG, "arrays of real(" + std::to_string(kind) +
+                                     ") given to a handle of precision " +
+                                     std::to_string(h->prec));
+  return CPML_OK;
+}
+
+int cpml_gpu_guard_cells(cpml_solver *h, int op, long long *count) {
+  if (!h || !count || (op != 0 && op != 1)) return CPML_ERR_ARG;
+  int rc = set_device(h);
+  if (rc) return rc;
+  struct Arr {
+    void *p;
+    long n, ld, lo, hi;
+  };
+  std::vector<Arr> arrs;
+  // fields and media: x halo of xpad columns, of which the R next to the
+  // grid are its ghost points
+  const long flo = h->xpad - h->R, fhi = h->xpad + h->nx + h->R;
+  for (int q = 0; q < 9; ++q)
+    for (void *p : {h->field[q], h->field2[q]})
+      if (p) arrs.push_back({p, h->total, h->ld, flo, fhi});
+  for (void *p : h->media)
+    if (p) arrs.push_back({p, h->total, h->ld, flo, fhi});
+  // y- and z-slab memory variables: rows of mld >= nx columns
+  for (int w = 0; w < 18; ++w)
+    if (kMemAxis[w] != 0 && h->mem_elems[w])
+      for (void *p : {h->mem[w], h->mem2[w]})
+        if (p) arrs.push_back({p, (long)h->mem_elems[w], h->mld, 0, h->nx});
+  long long guard = 0;
+  for (const Arr &a : arrs) guard += (a.n / a.ld) * (a.ld - (a.hi - a.lo));
+  HIPCHK(h, hipMemsetAsync(h->scratch_max, 0, sizeof(unsigned long long), h->stream));
+  for (const Arr &a : arrs) {
+    const int blocks = (int)std::min<long>(4096, (a.n + 255) / 256);
+    if (h->esize == 4)
+      hipLaunchKernelGGL(k_guard<float>, dim3(blocks), dim3(256), 0, h->stream, (float *)a.p,
+                         a.n, a.ld, a.lo, a.hi, op, h->scratch_max);
+    else
+      hipLaunchKernelGGL(k_guard<double>, dim3(blocks), dim3(256), 0, h->stream, (double *)a.p,
+                         a.n, a.ld, a.lo, a.hi, op, h->scratch_max);
+    HIPCHK(h, hipGetLastError());
+  }
+  unsigned long long changed = 0;
+  HIPCHK(h, hipMemcpyAsync(&changed, h->scratch_max, sizeof(changed), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  *count = op ? guard : (long long)changed;
+  return CPML_OK;
+}
+
+int cpml_gpu_set_host_layout(cpml_solver *h, int lo, int hi) {
+  if (!h) return CPML_ERR_ARG;
+  if (lo < 0 || hi < 0 || lo > 8 || hi > 8)
+    return fail(h, CPML_ERR_ARG, "host ghost widths must be in 0..8");
+  h->hlo = lo;
+  h->hhi = hi;
+  return CPML_OK;
+}
+
+int cpml_gpu_check_grid_array(cpml_solver *h, int kind, int rank, const int64_t *shape) {
+  if (!h) return CPML_ERR_ARG;
+  int rc = cpml_gpu_check_kind(h, kind);
+  if (rc) return rc;
+  if (rank < 1 || !shape) return fail(h, CPML_ERR_ARG, "grid array: rank must be >= 1");
+  const int nd = h->ndim;
+  const int64_t want[3] = {(int64_t)host_x(h), (int64_t)host_y(h),
+                           (int64_t)h->nz + h->hlo + h->hhi};
+  int64_t total = 1, expect = 1;
+  for (int d = 0; d < rank; ++d) total *= shape[d];
+  for (int d = 0; d < nd; ++d) expect *= want[d];
+  bool ok = total == expect;
+  if (rank == nd)
+    for (int d = 0; d < nd; ++d) ok = ok && shape[d] == want[d];
+  if (!ok) {
+    std::string got, exp;
+    for (int d = 0; d < rank; ++d) got += (d ? "," : "") + std::to_string(shape[d]);
+    for (int d = 0; d < nd; ++d) exp += (d ? "," : "") + std::to_string(want[d]);
+    return fail(h, CPML_ERR_ARG, "grid array of shape (" + got + ") given where (" + exp +
+                                     ") is expected (grid plus host ghost cells " +
+                                     std::to_string(h->hlo) + " below, " +
+                                     std::to_string(h->hhi) + " above)");
+  }
+  return CPML_OK;
+}
+
+int cpml_gpu_grid(cpml_solver *h, int *NX, int *NY, int *NZ) {
+  if (!h || !NX || !NY || !NZ) return CPML_ERR_ARG;
+  *NX = h->nx;
+  *NY = h->ny;
+  *NZ = h->nz;
+  return CPML_OK;
+}
+
+int cpml_gpu_set_memory(cpml_solver *h, int which, const void *in) {
+  if (!h) return CPML_ERR_ARG;
+  if (!mem_used(h->ndim, which) || !in) return fail(h, CPML_ERR_ARG, "bad memory selector");
+  if (!h->have_pml) return fail(h, CPML_ERR_STATE, "set_pml first");
+  int rc = set_device(h);
+  if (rc) return rc;
+  if (!h->mem_elems[which]) return CPML_OK;  // no slab along this axis
+  const size_t es = h->esize;
+  std::vector<char> tmp(h->mem_elems[which] * es, 0);
+  const char *src = static_cast<const char *>(in);
+  const int nx = h->nx, ny = h->ny, nz = h->nz;
+  // the slab points of the full-grid array (the rest is the reference's
+  // exact zero outside the slab and is not stored)
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        long dst = -1;
+        switch (kMemAxis[which]) {
+          case 0:
+            if (h->sx.in(i)) dst = h->sx.s(i) + (long)h->sx.w * (j + (long)ny * k);
+            break;
+          case 1:
+            if (h->sy.in(j)) dst = i + h->mld * (h->sy.s(j) + (long)h->sy.w * k);
+            break;
+          default:
+            if (h->sz.in(k)) dst = i + h->mld * (j + (long)ny * h->sz.s(k));
+            break;
+        }
+        if (dst >= 0) std::memcpy(tmp.data() + dst * es, src + host_at(h, i, j, k) * es, es);
+      }
+  HIPCHK(h, hipMemcpy(mbuf(h, which), tmp.data(), tmp.size(), hipMemcpyHostToDevice));
   return CPML_OK;
 }
 
@@ -1848,6 +2025,11 @@ int cpml_gpu_set_time_block(cpml_solver *h, int steps, int strip_width, int chun
   if (rc) return rc;
   if (steps > 0 && (rc = ensure_second_state(h))) return rc;
   if (steps == 0 && (rc = restore_first_state(h))) return rc;
+  if (steps > 0 && !h->side) {
+    HIPCHK(h, hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
+    HIPCHK(h, hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+    HIPCHK(h, hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
+  }
   h->tb = steps;
   h->tb_ow = strip_width > 0 ? strip_width : std::max(8, (64 - 2 * g) / 8 * 8);
   if (h->tb_ow > 64 - 2 * g) h->tb_ow = 64 - 2 * g;

@@ -288,14 +288,24 @@ template <typename T>
 hipError_t launch_record(const Record<T> &p, hipStream_t s);
 // nt steps per launch (3D, second order); tiles of ow columns x (nw - 2 nt)
 // rows marching through z chunks of at most kchunk planes
+// (the launches of tiles that touch a slab go to `side` when given, so they
+// run beside the slab-free tiles; the caller joins the two streams)
 template <typename T>
-hipError_t launch_tb3d(int nt, const TB3D<T> &p, hipStream_t s);
+hipError_t launch_tb3d(int nt, const TB3D<T> &p, hipStream_t s, hipStream_t side);
 int tb3d_max_nt(int prec);
 // nt steps per launch (2D, plain elastic); clean = items without C-PML code
 template <typename T>
-hipError_t launch_tb2d(int order, int nt, const TB2D<T> &p, hipStream_t s);
+hipError_t launch_tb2d(int order, int nt, const TB2D<T> &p, hipStream_t s, hipStream_t side);
 // largest nt launch_tb2d has a kernel for, per order and precision
 int tb2d_max_nt(int order, int prec);
+// many grids: items (grid, strip, chunk) of one class (slab: with C-PML code)
+// from the descriptor table `grids`, on `blocks` workgroups of 4 wavefronts
+template <typename T>
+hipError_t launch_tb2d_multi(int order, int nt, bool slab, const TB2D<T> *grids,
+                             const int4 *items, int nitems, int blocks, hipStream_t s);
+// strips [box[0], box[1]) x chunks [box[2], box[3]) of one grid need no C-PML code
+void tb2d_clean_box(int order, int nt, int nx, int ny, int ow, int jc, const Slab &sx,
+                    const Slab &sy, int box[4]);
 template <typename T>
 hipError_t launch_energy(const Energy<T> &p, int nblocks, hipStream_t s);
 template <typename T>

@@ -9,8 +9,8 @@
 // i0 - G + l of a strip (G = NT (2r - 1) ring columns on each side, r = ORDER
 // / 2) and the wave marches up the rows of one chunk.  At row iteration s the
 // wave
-//   * loads row s + r of v, row s of sigma and row s + 1 of the medium (one
-//     iteration ahead of their use),
+//   * loads row s + r of v, row s of sigma and row s + 1 of the medium (two
+//     iterations ahead of their use),
 //   * for each time level m = 1..NT computes sigma^m at row p = s - 2(m-1)r
 //     from the row window of v^{m-1} (2r + 1 rows) and v^m at row p - r from
 //     the row window of sigma^m, x neighbours moving across lanes,
@@ -81,8 +81,10 @@ __device__ __forceinline__ void push(T (&w)[N], T x) {
   w[N - 1] = x;
 }
 
+// One work item: strip ts (columns from ts * ow) and chunk tc (rows from
+// tc * jc) of the grid described by P, advanced NT steps.
 template <typename T, int ORDER, int NT, bool SLAB>
-__global__ __launch_bounds__(64 * WPB) void k_tb2d(TB2D<T> P) {
+__device__ __forceinline__ void tb2d_item(const TB2D<T> &P, int ts, int tc) {
   constexpr int r = ORDER / 2;
   constexpr int W = 2 * r + 1;                  // row window of one level
   constexpr int G = NT * (2 * r - 1);           // ring lanes on each side
@@ -91,42 +93,7 @@ __global__ __launch_bounds__(64 * WPB) void k_tb2d(TB2D<T> P) {
   const Step2D<T> &p = P.a;
   const int nx = p.nx, ny = p.ny;
   const long ld = p.ld;
-
-  // ---- work item
   const int lane = threadIdx.x & 63;
-  const int wv = threadIdx.x >> 6;
-  int nitem;
-  const int bx = P.sx1 - P.sx0, cy = P.cy1 - P.cy0;
-  if (!SLAB) nitem = bx * cy;
-  else nitem = P.nstrips * P.nchunks - bx * cy;
-  // XCD-aware order: each XCD takes one contiguous band of items (strip fastest)
-  const int nblk = (nitem + WPB - 1) / WPB;
-  const int xcd = blockIdx.x & 7, loc = blockIdx.x >> 3;
-  const int q8 = nblk >> 3, rem = nblk & 7;
-  const int blk = xcd * q8 + min(xcd, rem) + loc;
-  const int item = __builtin_amdgcn_readfirstlane(blk * WPB + wv);
-  if (item >= nitem) return;
-  int ts, tc;
-  if (!SLAB) {
-    ts = P.sx0 + item % bx;
-    tc = P.cy0 + item / bx;
-  } else {
-    // chunks below the box, the box's chunks outside its strips, chunks above
-    const int n1 = P.cy0 * P.nstrips, wdt = P.nstrips - bx, n2 = cy * wdt;
-    if (item < n1) {
-      tc = item / P.nstrips;
-      ts = item % P.nstrips;
-    } else if (item < n1 + n2) {
-      const int t = item - n1;
-      tc = P.cy0 + t / wdt;
-      const int c = t % wdt;
-      ts = c < P.sx0 ? c : c - P.sx0 + P.sx1;
-    } else {
-      const int t = item - n1 - n2;
-      tc = P.cy1 + t / P.nstrips;
-      ts = t % P.nstrips;
-    }
-  }
   const int i0 = ts * P.ow;
   const int jb = tc * P.jc, je = min(jb + P.jc, ny);
   const int i = i0 - G + lane;
@@ -235,8 +202,11 @@ __global__ __launch_bounds__(64 * WPB) void k_tb2d(TB2D<T> P) {
       }
     }
   };
-  Row C, N;
+  // loads run two iterations ahead of their use (the march is a chain of
+  // dependent rows: the second row in flight hides the memory latency)
+  Row C, N, N2;
   load(C, s0);
+  if (s0 + 1 <= s1) load(N, s0 + 1);
 
   // receivers of this strip: a cursor per recorded level (rows only increase)
   int rc0 = 0, rc1 = 0;
@@ -250,7 +220,7 @@ __global__ __launch_bounds__(64 * WPB) void k_tb2d(TB2D<T> P) {
   const int it0 = *p.step;
 
   for (int s = s0; s <= s1; ++s) {
-    if (s < s1) load(N, s + 1);
+    if (s + 2 <= s1) load(N2, s + 2);
     push(V[0][0], C.vx);
     push(V[0][1], C.vy);
     push(Mx[0], C.lam);
@@ -455,6 +425,65 @@ __global__ __launch_bounds__(64 * WPB) void k_tb2d(TB2D<T> P) {
       }
     }
     C = N;
+    N = N2;
+  }
+}
+
+template <typename T, int ORDER, int NT, bool SLAB>
+__global__ __launch_bounds__(64 * WPB) void k_tb2d(TB2D<T> P) {
+  const int wv = threadIdx.x >> 6;
+  int nitem;
+  const int bx = P.sx1 - P.sx0, cy = P.cy1 - P.cy0;
+  if (!SLAB) nitem = bx * cy;
+  else nitem = P.nstrips * P.nchunks - bx * cy;
+  // XCD-aware order: each XCD takes one contiguous band of items (strip fastest)
+  const int nblk = (nitem + WPB - 1) / WPB;
+  const int xcd = blockIdx.x & 7, loc = blockIdx.x >> 3;
+  const int q8 = nblk >> 3, rem = nblk & 7;
+  const int blk = xcd * q8 + min(xcd, rem) + loc;
+  const int item = __builtin_amdgcn_readfirstlane(blk * WPB + wv);
+  if (item >= nitem) return;
+  int ts, tc;
+  if (!SLAB) {
+    ts = P.sx0 + item % bx;
+    tc = P.cy0 + item / bx;
+  } else {
+    // chunks below the box, the box's chunks outside its strips, chunks above
+    const int n1 = P.cy0 * P.nstrips, wdt = P.nstrips - bx, n2 = cy * wdt;
+    if (item < n1) {
+      tc = item / P.nstrips;
+      ts = item % P.nstrips;
+    } else if (item < n1 + n2) {
+      const int t = item - n1;
+      tc = P.cy0 + t / wdt;
+      const int c = t % wdt;
+      ts = c < P.sx0 ? c : c - P.sx0 + P.sx1;
+    } else {
+      const int t = item - n1 - n2;
+      tc = P.cy1 + t / P.nstrips;
+      ts = t % P.nstrips;
+    }
+  }
+  tb2d_item<T, ORDER, NT, SLAB>(P, ts, tc);
+}
+
+// Many grids in one launch: items[k] = (grid, strip, chunk) of one class,
+// split into 8 contiguous bands (one per XCD); the wavefronts of the blocks
+// an XCD receives stride through its band (one item each when the grid has
+// one wavefront per item, several when it is a persistent grid).
+template <typename T, int ORDER, int NT, bool SLAB>
+__global__ __launch_bounds__(64 * WPB) void k_tb2d_multi(const TB2D<T> *__restrict__ grids,
+                                                         const int4 *__restrict__ items,
+                                                         int nitems) {
+  const int G = gridDim.x, h = blockIdx.x;
+  const int xcd = h & 7;
+  const int nb = (G - xcd + 7) >> 3;  // blocks this XCD receives
+  const int q = nitems >> 3, rem = nitems & 7;
+  const int start = xcd * q + min(xcd, rem), count = q + (xcd < rem ? 1 : 0);
+  const int wv = threadIdx.x >> 6;
+  for (int u = __builtin_amdgcn_readfirstlane((h >> 3) * WPB + wv); u < count; u += nb * WPB) {
+    const int4 e = items[start + u];
+    tb2d_item<T, ORDER, NT, SLAB>(grids[e.x], e.y, e.z);
   }
 }
 
@@ -473,7 +502,7 @@ void clean_range(int n, int B, int g, int wl, int wr, int nt, int &t0, int &t1) 
 }
 
 template <typename T, int ORDER, int NT>
-hipError_t tb2d_launch(const TB2D<T> &P0, hipStream_t s) {
+hipError_t tb2d_launch(const TB2D<T> &P0, hipStream_t s, hipStream_t side) {
   constexpr int r = ORDER / 2;
   constexpr int G = NT * (2 * r - 1);
   TB2D<T> P = P0;
@@ -487,11 +516,11 @@ hipError_t tb2d_launch(const TB2D<T> &P0, hipStream_t s) {
   const int nclean = (P.sx1 - P.sx0) * (P.cy1 - P.cy0);
   const int nslab = P.nstrips * P.nchunks - nclean;
   const dim3 blk(64 * WPB);
+  if (nslab)  // on the side stream when given: beside the slab-free items
+    hipLaunchKernelGGL((k_tb2d<T, ORDER, NT, true>), dim3((nslab + WPB - 1) / WPB), blk, 0,
+                       side ? side : s, P);
   if (nclean)
     hipLaunchKernelGGL((k_tb2d<T, ORDER, NT, false>), dim3((nclean + WPB - 1) / WPB), blk, 0, s,
-                       P);
-  if (nslab)
-    hipLaunchKernelGGL((k_tb2d<T, ORDER, NT, true>), dim3((nslab + WPB - 1) / WPB), blk, 0, s,
                        P);
   return hipGetLastError();
 }
@@ -504,29 +533,81 @@ int tb2d_max_nt(int order, int prec) {
 }
 
 template <typename T>
-hipError_t launch_tb2d(int order, int nt, const TB2D<T> &p, hipStream_t s) {
+hipError_t launch_tb2d(int order, int nt, const TB2D<T> &p, hipStream_t s, hipStream_t side) {
   if (nt < 1 || nt > tb2d_max_nt(order, sizeof(T))) return hipErrorInvalidValue;
   switch (order * 10 + nt) {
-    case 21: return tb2d_launch<T, 2, 1>(p, s);
-    case 22: return tb2d_launch<T, 2, 2>(p, s);
-    case 41: return tb2d_launch<T, 4, 1>(p, s);
-    case 42: return tb2d_launch<T, 4, 2>(p, s);
-    case 81: return tb2d_launch<T, 8, 1>(p, s);
-    case 82: return tb2d_launch<T, 8, 2>(p, s);
+    case 21: return tb2d_launch<T, 2, 1>(p, s, side);
+    case 22: return tb2d_launch<T, 2, 2>(p, s, side);
+    case 41: return tb2d_launch<T, 4, 1>(p, s, side);
+    case 42: return tb2d_launch<T, 4, 2>(p, s, side);
+    case 81: return tb2d_launch<T, 8, 1>(p, s, side);
+    case 82: return tb2d_launch<T, 8, 2>(p, s, side);
     default: break;
   }
   if constexpr (sizeof(T) == 4) {
     switch (order * 10 + nt) {
-      case 23: return tb2d_launch<T, 2, 3>(p, s);
-      case 24: return tb2d_launch<T, 2, 4>(p, s);
-      case 43: return tb2d_launch<T, 4, 3>(p, s);
+      case 23: return tb2d_launch<T, 2, 3>(p, s, side);
+      case 24: return tb2d_launch<T, 2, 4>(p, s, side);
+      case 43: return tb2d_launch<T, 4, 3>(p, s, side);
       default: break;
     }
   }
   return hipErrorInvalidValue;
 }
 
-template hipError_t launch_tb2d<float>(int, int, const TB2D<float> &, hipStream_t);
-template hipError_t launch_tb2d<double>(int, int, const TB2D<double> &, hipStream_t);
+template hipError_t launch_tb2d<float>(int, int, const TB2D<float> &, hipStream_t, hipStream_t);
+template hipError_t launch_tb2d<double>(int, int, const TB2D<double> &, hipStream_t,
+                                        hipStream_t);
+
+namespace {
+template <typename T, int ORDER, int NT>
+void tb2d_multi(bool slab, const TB2D<T> *grids, const int4 *items, int nitems, int blocks,
+                hipStream_t s) {
+  const dim3 g(blocks), b(64 * WPB);
+  if (slab) hipLaunchKernelGGL((k_tb2d_multi<T, ORDER, NT, true>), g, b, 0, s, grids, items, nitems);
+  else hipLaunchKernelGGL((k_tb2d_multi<T, ORDER, NT, false>), g, b, 0, s, grids, items, nitems);
+}
+}  // namespace
+
+template <typename T>
+hipError_t launch_tb2d_multi(int order, int nt, bool slab, const TB2D<T> *grids,
+                             const int4 *items, int nitems, int blocks, hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
+  if (nt < 1 || nt > tb2d_max_nt(order, sizeof(T)) || blocks < 1) return hipErrorInvalidValue;
+  switch (order * 10 + nt) {
+    case 21: tb2d_multi<T, 2, 1>(slab, grids, items, nitems, blocks, s); break;
+    case 22: tb2d_multi<T, 2, 2>(slab, grids, items, nitems, blocks, s); break;
+    case 41: tb2d_multi<T, 4, 1>(slab, grids, items, nitems, blocks, s); break;
+    case 42: tb2d_multi<T, 4, 2>(slab, grids, items, nitems, blocks, s); break;
+    case 81: tb2d_multi<T, 8, 1>(slab, grids, items, nitems, blocks, s); break;
+    case 82: tb2d_multi<T, 8, 2>(slab, grids, items, nitems, blocks, s); break;
+    default:
+      if constexpr (sizeof(T) == 4) {
+        if (order * 10 + nt == 23) tb2d_multi<T, 2, 3>(slab, grids, items, nitems, blocks, s);
+        else if (order * 10 + nt == 24) tb2d_multi<T, 2, 4>(slab, grids, items, nitems, blocks, s);
+        else if (order * 10 + nt == 43) tb2d_multi<T, 4, 3>(slab, grids, items, nitems, blocks, s);
+        else return hipErrorInvalidValue;
+      } else {
+        return hipErrorInvalidValue;
+      }
+  }
+  return hipGetLastError();
+}
+
+template hipError_t launch_tb2d_multi<float>(int, int, bool, const TB2D<float> *, const int4 *,
+                                             int, int, hipStream_t);
+template hipError_t launch_tb2d_multi<double>(int, int, bool, const TB2D<double> *, const int4 *,
+                                              int, int, hipStream_t);
+
+// Strips / chunks of one grid whose ring avoids the slabs (the box of
+// launch_tb2d's clean class), for building item lists on the host.
+void tb2d_clean_box(int order, int nt, int nx, int ny, int ow, int jc, const Slab &sx,
+                    const Slab &sy, int box[4]) {
+  const int r = order / 2, g = nt * (2 * r - 1);
+  const int ns = (nx + ow - 1) / ow, nc = (ny + jc - 1) / jc;
+  clean_range(nx, ow, g, sx.wl, sx.wr, ns, box[0], box[1]);
+  clean_range(ny, jc, (2 * nt - 1) * r + r, sy.wl, sy.wr, nc, box[2], box[3]);
+  if (box[1] <= box[0] || box[3] <= box[2]) box[0] = box[1] = box[2] = box[3] = 0;
+}
 
 }  // namespace cpml

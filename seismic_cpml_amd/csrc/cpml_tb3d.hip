@@ -270,10 +270,10 @@ __global__ __launch_bounds__(64 * NW) void k_tb3d(TB3D<T> P) {
     // iteration (which reads buffer (s0 & 1) ^ 1)
     for (int q = threadIdx.x; q < 2 * NS * NW * 64; q += 64 * NW) (&xl[0][0][0][0])[q] = T(0);
     __syncthreads();
-    const int b0 = (s0 & 1) ^ 1;
-    xl[b0][0][w][lane] = V[0][0][1];
-    xl[b0][1][w][lane] = V[0][1][1];
-    xl[b0][2][w][lane] = V[0][2][1];
+    const int b0 = (s0 & 1) ^ 1;  // plane s0 is the newest of the prefilled window
+    xl[b0][0][w][lane] = V[0][0][2];
+    xl[b0][1][w][lane] = V[0][1][2];
+    xl[b0][2][w][lane] = V[0][2][2];
   }
   Ld C, N;
   load(C, s0);
@@ -570,7 +570,7 @@ void tb3d_class(const TB3D<T> &P, const int (&n)[3], hipStream_t s) {
 }
 
 template <typename T, int NT, int NW>
-hipError_t tb3d_launch(const TB3D<T> &P0, hipStream_t s) {
+hipError_t tb3d_launch(const TB3D<T> &P0, hipStream_t s, hipStream_t side) {
   TB3D<T> P = P0;
   const Step3D<T> &p = P.a;
   P.ty = NW - 2 * NT;
@@ -591,14 +591,17 @@ hipError_t tb3d_launch(const TB3D<T> &P0, hipStream_t s) {
   if (P.box[3] < P.box[2]) P.box[2] = P.box[3] = 0;
   const int n[3] = {(p.nx + P.ow - 1) / P.ow, (p.ny + P.ty - 1) / P.ty,
                     P.zn[0] + P.zn[1] + P.zn[2]};
+  // slab classes first on the side stream (the corners, with the most code,
+  // start earliest), the slab-free class on the main stream
+  hipStream_t t = side ? side : s;
+  tb3d_class<T, NT, NW, true, true, true>(P, n, t);
+  tb3d_class<T, NT, NW, true, true, false>(P, n, t);
+  tb3d_class<T, NT, NW, true, false, true>(P, n, t);
+  tb3d_class<T, NT, NW, false, true, true>(P, n, t);
+  tb3d_class<T, NT, NW, true, false, false>(P, n, t);
+  tb3d_class<T, NT, NW, false, true, false>(P, n, t);
+  tb3d_class<T, NT, NW, false, false, true>(P, n, t);
   tb3d_class<T, NT, NW, false, false, false>(P, n, s);
-  tb3d_class<T, NT, NW, true, false, false>(P, n, s);
-  tb3d_class<T, NT, NW, false, true, false>(P, n, s);
-  tb3d_class<T, NT, NW, false, false, true>(P, n, s);
-  tb3d_class<T, NT, NW, true, true, false>(P, n, s);
-  tb3d_class<T, NT, NW, true, false, true>(P, n, s);
-  tb3d_class<T, NT, NW, false, true, true>(P, n, s);
-  tb3d_class<T, NT, NW, true, true, true>(P, n, s);
   return hipGetLastError();
 }
 
@@ -609,14 +612,14 @@ int tb3d_max_nt(int) { return 2; }
 // rows per workgroup: 16 wavefronts in float, 8 in double (LDS: 6 NT values
 // per point and buffer)
 template <typename T>
-hipError_t launch_tb3d(int nt, const TB3D<T> &p, hipStream_t s) {
+hipError_t launch_tb3d(int nt, const TB3D<T> &p, hipStream_t s, hipStream_t side) {
   constexpr int NW = sizeof(T) == 4 ? 16 : 8;
-  if (nt == 1) return tb3d_launch<T, 1, NW>(p, s);
-  if (nt == 2) return tb3d_launch<T, 2, NW>(p, s);
+  if (nt == 1) return tb3d_launch<T, 1, NW>(p, s, side);
+  if (nt == 2) return tb3d_launch<T, 2, NW>(p, s, side);
   return hipErrorInvalidValue;
 }
 
-template hipError_t launch_tb3d<float>(int, const TB3D<float> &, hipStream_t);
-template hipError_t launch_tb3d<double>(int, const TB3D<double> &, hipStream_t);
+template hipError_t launch_tb3d<float>(int, const TB3D<float> &, hipStream_t, hipStream_t);
+template hipError_t launch_tb3d<double>(int, const TB3D<double> &, hipStream_t, hipStream_t);
 
 }  // namespace cpml

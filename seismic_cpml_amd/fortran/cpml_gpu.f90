@@ -12,8 +12,15 @@
 !
 ! Every routine returns ierr (0 = ok, see the CPML_* constants); the message
 ! of a failure is cpml_gpu_error_message(h).  Array arguments take real(4) or
-! real(8) arrays matching the precision given to cpml_gpu_init, in the
-! reference's Fortran layout (x fastest), interior points only.
+! real(8) arrays (generic names over kind-specific procedures) that must match
+! the precision given to cpml_gpu_init (checked: CPML_ERR_ARG otherwise), in
+! the reference's Fortran layout (x fastest).  Grid arrays (medium, fields,
+! memory variables) may have any rank and are checked against the grid: by
+! default they hold the interior points only, (NX,NY[,NZ]); after
+! cpml_gpu_set_host_layout(h, lo, hi) they are the reference's
+! explicit-lower-bound arrays (1-lo:NX+hi, 1-lo:NY+hi[, 1-lo:NZ+hi]), e.g.
+! (0:NX+1,0:NY+1) of 2D4:205 or (-4:NX+4,-4:NY+4) of ADE8:272, handed over
+! in place (a contiguous actual argument is not copied).
 module cpml_gpu
   use, intrinsic :: iso_c_binding
   implicit none
@@ -24,11 +31,20 @@ module cpml_gpu
        CPML_ERR_STATE = 7, CPML_ERR_NODEVICE = 8
   integer, parameter, public :: CPML_VX = 0, CPML_VY = 1, CPML_VZ = 2, CPML_SIGMAXX = 3, &
        CPML_SIGMAYY = 4, CPML_SIGMAZZ = 5, CPML_SIGMAXY = 6, CPML_SIGMAXZ = 7, CPML_SIGMAYZ = 8
+  ! memory variables, named as in the reference (3D:222-240)
+  integer, parameter, public :: CPML_MEM_DVX_DX = 0, CPML_MEM_DVX_DY = 1, CPML_MEM_DVX_DZ = 2, &
+       CPML_MEM_DVY_DX = 3, CPML_MEM_DVY_DY = 4, CPML_MEM_DVY_DZ = 5, CPML_MEM_DVZ_DX = 6, &
+       CPML_MEM_DVZ_DY = 7, CPML_MEM_DVZ_DZ = 8, CPML_MEM_DSIGMAXX_DX = 9, &
+       CPML_MEM_DSIGMAYY_DY = 10, CPML_MEM_DSIGMAZZ_DZ = 11, CPML_MEM_DSIGMAXY_DX = 12, &
+       CPML_MEM_DSIGMAXY_DY = 13, CPML_MEM_DSIGMAXZ_DX = 14, CPML_MEM_DSIGMAXZ_DZ = 15, &
+       CPML_MEM_DSIGMAYZ_DY = 16, CPML_MEM_DSIGMAYZ_DZ = 17
 
   public :: cpml_gpu_init, cpml_gpu_finalize, cpml_gpu_set_medium, cpml_gpu_set_pml_2d, &
        cpml_gpu_set_pml_3d, cpml_gpu_set_source, cpml_gpu_set_receivers, cpml_gpu_set_energy, &
        cpml_gpu_set_stability, cpml_gpu_run, cpml_gpu_get_seismograms, cpml_gpu_get_energy, &
-       cpml_gpu_get_field, cpml_gpu_set_field, cpml_gpu_velocity_norm, cpml_gpu_elapsed_ms, &
+       cpml_gpu_get_field, cpml_gpu_set_field, cpml_gpu_get_memory, cpml_gpu_set_memory, &
+       cpml_gpu_set_time_block, cpml_gpu_velocity_norm, cpml_gpu_elapsed_ms, &
+       cpml_gpu_set_host_layout, &
        cpml_gpu_error_message, cpml_gpu_create_color_image, cpml_gpu_write_su, &
        cpml_gpu_set_attenuation, cpml_gpu_get_pressure, cpml_gpu_set_ade_rk4
   ! many 2D shots in one launch per phase (no reference counterpart: a survey
@@ -54,8 +70,7 @@ module cpml_gpu
     end function
     integer(c_int) function c_set_medium(h, lam, mu, rho) bind(C, name="cpml_gpu_set_medium")
       import :: c_ptr, c_int
-      type(c_ptr), value :: h
-      type(*), dimension(*), intent(in) :: lam, mu, rho
+      type(c_ptr), value :: h, lam, mu, rho
     end function
     integer(c_int) function c_set_pml(h, ax, bx, kx, axh, bxh, kxh, ay, by, ky, ayh, byh, kyh, &
                                       az, bz, kz, azh, bzh, kzh) bind(C, name="cpml_gpu_set_pml")
@@ -99,9 +114,7 @@ module cpml_gpu
     integer(c_int) function c_get_seismograms(h, sx, sy, sz) &
          bind(C, name="cpml_gpu_get_seismograms")
       import :: c_ptr, c_int
-      type(c_ptr), value :: h
-      type(*), dimension(*) :: sx, sy
-      type(*), dimension(*), optional :: sz
+      type(c_ptr), value :: h, sx, sy, sz
     end function
     integer(c_int) function c_get_energy(h, kin, pot) bind(C, name="cpml_gpu_get_energy")
       import :: c_ptr, c_int, c_double
@@ -110,15 +123,29 @@ module cpml_gpu
     end function
     integer(c_int) function c_get_field(h, which, out) bind(C, name="cpml_gpu_get_field")
       import :: c_ptr, c_int
-      type(c_ptr), value :: h
+      type(c_ptr), value :: h, out
       integer(c_int), value :: which
-      type(*), dimension(*) :: out
     end function
     integer(c_int) function c_set_field(h, which, in) bind(C, name="cpml_gpu_set_field")
       import :: c_ptr, c_int
-      type(c_ptr), value :: h
+      type(c_ptr), value :: h, in
       integer(c_int), value :: which
-      type(*), dimension(*), intent(in) :: in
+    end function
+    integer(c_int) function c_get_memory(h, which, out) bind(C, name="cpml_gpu_get_memory")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h, out
+      integer(c_int), value :: which
+    end function
+    integer(c_int) function c_set_memory(h, which, in) bind(C, name="cpml_gpu_set_memory")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h, in
+      integer(c_int), value :: which
+    end function
+    integer(c_int) function c_set_time_block(h, steps, width, rows) &
+         bind(C, name="cpml_gpu_set_time_block")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      integer(c_int), value :: steps, width, rows
     end function
     integer(c_int) function c_set_attenuation(h, nsls, te1, ts1, te2, ts2) &
          bind(C, name="cpml_gpu_set_attenuation")
@@ -135,8 +162,7 @@ module cpml_gpu
     end function
     integer(c_int) function c_get_pressure(h, sp) bind(C, name="cpml_gpu_get_pressure")
       import :: c_ptr, c_int
-      type(c_ptr), value :: h
-      type(*), dimension(*) :: sp
+      type(c_ptr), value :: h, sp
     end function
     integer(c_int) function c_velocity_norm(h, v) bind(C, name="cpml_gpu_velocity_norm")
       import :: c_ptr, c_int, c_double
@@ -157,11 +183,11 @@ module cpml_gpu
     end function
     integer(c_int) function c_su_write(path, ntr, ns, dt_value, traces, prec) &
          bind(C, name="cpml_su_write")
-      import :: c_int, c_char, c_double
+      import :: c_ptr, c_int, c_char, c_double
       character(kind=c_char), dimension(*), intent(in) :: path
       integer(c_int), value :: ntr, ns, prec
       real(c_double), value :: dt_value
-      type(*), dimension(*), intent(in) :: traces
+      type(c_ptr), value :: traces
     end function
     integer(c_int) function c_elapsed_ms(h, v) bind(C, name="cpml_gpu_elapsed_ms")
       import :: c_ptr, c_int, c_double
@@ -183,9 +209,8 @@ module cpml_gpu
     integer(c_int) function c_batch_set_medium(b, shot, lam, mu, rho) &
          bind(C, name="cpml_batch_set_medium")
       import :: c_ptr, c_int
-      type(c_ptr), value :: b
+      type(c_ptr), value :: b, lam, mu, rho
       integer(c_int), value :: shot
-      type(*), dimension(*), intent(in) :: lam, mu, rho
     end function
     integer(c_int) function c_batch_set_pml(b, shot, ax, bx, kx, axh, bxh, kxh, ay, by, ky, &
                                             ayh, byh, kyh) bind(C, name="cpml_batch_set_pml")
@@ -225,9 +250,8 @@ module cpml_gpu
     integer(c_int) function c_batch_get_seismograms(b, shot, sx, sy) &
          bind(C, name="cpml_batch_get_seismograms")
       import :: c_ptr, c_int
-      type(c_ptr), value :: b
+      type(c_ptr), value :: b, sx, sy
       integer(c_int), value :: shot
-      type(*), dimension(*) :: sx, sy
     end function
     integer(c_int) function c_abc_create(net, w, scales) bind(C, name="cpml_abc_net_create")
       import :: c_ptr, c_int, c_float
@@ -249,6 +273,86 @@ module cpml_gpu
       character(kind=c_char), dimension(*) :: buf
       integer(c_int), value :: n
     end function
+    integer(c_int) function c_check_kind(h, k) bind(C, name="cpml_gpu_check_kind")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      integer(c_int), value :: k
+    end function
+    integer(c_int) function c_batch_check_kind(b, k) bind(C, name="cpml_batch_check_kind")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: b
+      integer(c_int), value :: k
+    end function
+    integer(c_int) function c_check_grid_array(h, k, r, shp) &
+         bind(C, name="cpml_gpu_check_grid_array")
+      import :: c_ptr, c_int, c_int64_t
+      type(c_ptr), value :: h
+      integer(c_int), value :: k, r
+      integer(c_int64_t), dimension(*), intent(in) :: shp
+    end function
+    integer(c_int) function c_set_host_layout(h, lo, hi) bind(C, name="cpml_gpu_set_host_layout")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      integer(c_int), value :: lo, hi
+    end function
+    integer(c_int) function c_batch_grid(b, shot, nx, ny) bind(C, name="cpml_batch_grid")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: b
+      integer(c_int), value :: shot
+      integer(c_int), intent(out) :: nx, ny
+    end function
+  end interface
+
+  ! real(4) / real(8) specific procedures under the generic names: the
+  ! compiler picks the kind of the arrays given, and each checks it against
+  ! the precision of the handle (cpml_gpu_check_kind)
+  interface cpml_gpu_set_medium
+    module procedure cpml_gpu_set_medium_r4, cpml_gpu_set_medium_r8
+  end interface
+  interface cpml_gpu_set_pml_2d
+    module procedure cpml_gpu_set_pml_2d_r4, cpml_gpu_set_pml_2d_r8
+  end interface
+  interface cpml_gpu_set_pml_3d
+    module procedure cpml_gpu_set_pml_3d_r4, cpml_gpu_set_pml_3d_r8
+  end interface
+  interface cpml_gpu_set_source
+    module procedure cpml_gpu_set_source_r4, cpml_gpu_set_source_r8
+  end interface
+  interface cpml_gpu_get_seismograms
+    module procedure cpml_gpu_get_seismograms_r4, cpml_gpu_get_seismograms_r8
+  end interface
+  interface cpml_gpu_get_field
+    module procedure cpml_gpu_get_field_r4, cpml_gpu_get_field_r8
+  end interface
+  interface cpml_gpu_set_field
+    module procedure cpml_gpu_set_field_r4, cpml_gpu_set_field_r8
+  end interface
+  interface cpml_gpu_get_memory
+    module procedure cpml_gpu_get_memory_r4, cpml_gpu_get_memory_r8
+  end interface
+  interface cpml_gpu_set_memory
+    module procedure cpml_gpu_set_memory_r4, cpml_gpu_set_memory_r8
+  end interface
+  interface cpml_gpu_set_ade_rk4
+    module procedure cpml_gpu_set_ade_rk4_r4, cpml_gpu_set_ade_rk4_r8
+  end interface
+  interface cpml_gpu_get_pressure
+    module procedure cpml_gpu_get_pressure_r4, cpml_gpu_get_pressure_r8
+  end interface
+  interface cpml_gpu_write_su
+    module procedure cpml_gpu_write_su_r4, cpml_gpu_write_su_r8
+  end interface
+  interface cpml_gpu_batch_set_medium
+    module procedure cpml_gpu_batch_set_medium_r4, cpml_gpu_batch_set_medium_r8
+  end interface
+  interface cpml_gpu_batch_set_pml
+    module procedure cpml_gpu_batch_set_pml_r4, cpml_gpu_batch_set_pml_r8
+  end interface
+  interface cpml_gpu_batch_set_source
+    module procedure cpml_gpu_batch_set_source_r4, cpml_gpu_batch_set_source_r8
+  end interface
+  interface cpml_gpu_batch_get_seismograms
+    module procedure cpml_gpu_batch_get_seismograms_r4, cpml_gpu_batch_get_seismograms_r8
   end interface
 
 contains
@@ -273,42 +377,102 @@ contains
   end subroutine
 
   ! lambda, mu, rho (2D2:204, 468-474); checks the Courant number (2D2:513-516)
-  subroutine cpml_gpu_set_medium(h, lambda, mu, rho, ierr)
+  subroutine cpml_gpu_set_medium_r4(h, lambda, mu, rho, ierr)
     type(c_ptr), intent(in) :: h
-    type(*), dimension(*), intent(in) :: lambda, mu, rho
+    real(4), dimension(..), contiguous, intent(in), target :: lambda, mu, rho
     integer, intent(out) :: ierr
-    ierr = c_set_medium(h, lambda, mu, rho)
+    ierr = c_check_grid_array(h, 4, rank(lambda), int(shape(lambda), c_int64_t))
+    if (ierr /= CPML_OK) return
+    ierr = c_check_grid_array(h, 4, rank(mu), int(shape(mu), c_int64_t))
+    if (ierr /= CPML_OK) return
+    ierr = c_check_grid_array(h, 4, rank(rho), int(shape(rho), c_int64_t))
+    if (ierr /= CPML_OK) return
+    ierr = c_set_medium(h, c_loc(lambda), c_loc(mu), c_loc(rho))
+  end subroutine
+
+  subroutine cpml_gpu_set_medium_r8(h, lambda, mu, rho, ierr)
+    type(c_ptr), intent(in) :: h
+    real(8), dimension(..), contiguous, intent(in), target :: lambda, mu, rho
+    integer, intent(out) :: ierr
+    ierr = c_check_grid_array(h, 8, rank(lambda), int(shape(lambda), c_int64_t))
+    if (ierr /= CPML_OK) return
+    ierr = c_check_grid_array(h, 8, rank(mu), int(shape(mu), c_int64_t))
+    if (ierr /= CPML_OK) return
+    ierr = c_check_grid_array(h, 8, rank(rho), int(shape(rho), c_int64_t))
+    if (ierr /= CPML_OK) return
+    ierr = c_set_medium(h, c_loc(lambda), c_loc(mu), c_loc(rho))
   end subroutine
 
   ! damping profiles of 2D2:283-465 (x and y)
-  subroutine cpml_gpu_set_pml_2d(h, a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, &
+  subroutine cpml_gpu_set_pml_2d_r4(h, a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, &
                                  a_y, b_y, K_y, a_y_half, b_y_half, K_y_half, ierr)
     type(c_ptr), intent(in) :: h
-    type(*), dimension(*), intent(in) :: a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, &
+    real(4), dimension(*), intent(in) :: a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, &
          a_y, b_y, K_y, a_y_half, b_y_half, K_y_half
     integer, intent(out) :: ierr
+    ierr = c_check_kind(h, 4)
+    if (ierr /= CPML_OK) return
+    ierr = c_set_pml(h, a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, a_y, b_y, K_y, &
+                     a_y_half, b_y_half, K_y_half, a_x, b_x, K_x, a_x_half, b_x_half, K_x_half)
+  end subroutine
+
+  subroutine cpml_gpu_set_pml_2d_r8(h, a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, &
+                                 a_y, b_y, K_y, a_y_half, b_y_half, K_y_half, ierr)
+    type(c_ptr), intent(in) :: h
+    real(8), dimension(*), intent(in) :: a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, &
+         a_y, b_y, K_y, a_y_half, b_y_half, K_y_half
+    integer, intent(out) :: ierr
+    ierr = c_check_kind(h, 8)
+    if (ierr /= CPML_OK) return
     ierr = c_set_pml(h, a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, a_y, b_y, K_y, &
                      a_y_half, b_y_half, K_y_half, a_x, b_x, K_x, a_x_half, b_x_half, K_x_half)
   end subroutine
 
   ! damping profiles of 3D:399-667 (x, y and z)
-  subroutine cpml_gpu_set_pml_3d(h, a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, &
+  subroutine cpml_gpu_set_pml_3d_r4(h, a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, &
                                  a_y, b_y, K_y, a_y_half, b_y_half, K_y_half, &
                                  a_z, b_z, K_z, a_z_half, b_z_half, K_z_half, ierr)
     type(c_ptr), intent(in) :: h
-    type(*), dimension(*), intent(in) :: a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, &
+    real(4), dimension(*), intent(in) :: a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, &
          a_y, b_y, K_y, a_y_half, b_y_half, K_y_half, a_z, b_z, K_z, a_z_half, b_z_half, K_z_half
     integer, intent(out) :: ierr
+    ierr = c_check_kind(h, 4)
+    if (ierr /= CPML_OK) return
+    ierr = c_set_pml(h, a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, a_y, b_y, K_y, &
+                     a_y_half, b_y_half, K_y_half, a_z, b_z, K_z, a_z_half, b_z_half, K_z_half)
+  end subroutine
+
+  subroutine cpml_gpu_set_pml_3d_r8(h, a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, &
+                                 a_y, b_y, K_y, a_y_half, b_y_half, K_y_half, &
+                                 a_z, b_z, K_z, a_z_half, b_z_half, K_z_half, ierr)
+    type(c_ptr), intent(in) :: h
+    real(8), dimension(*), intent(in) :: a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, &
+         a_y, b_y, K_y, a_y_half, b_y_half, K_y_half, a_z, b_z, K_z, a_z_half, b_z_half, K_z_half
+    integer, intent(out) :: ierr
+    ierr = c_check_kind(h, 8)
+    if (ierr /= CPML_OK) return
     ierr = c_set_pml(h, a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, a_y, b_y, K_y, &
                      a_y_half, b_y_half, K_y_half, a_z, b_z, K_z, a_z_half, b_z_half, K_z_half)
   end subroutine
 
   ! force_x(it), force_y(it) at (ISOURCE, JSOURCE[, KSOURCE]) (2D2:644-667)
-  subroutine cpml_gpu_set_source(h, ISOURCE, JSOURCE, KSOURCE, force_x, force_y, ierr)
+  subroutine cpml_gpu_set_source_r4(h, ISOURCE, JSOURCE, KSOURCE, force_x, force_y, ierr)
     type(c_ptr), intent(in) :: h
     integer, intent(in) :: ISOURCE, JSOURCE, KSOURCE
-    type(*), dimension(*), intent(in) :: force_x, force_y
+    real(4), dimension(*), intent(in) :: force_x, force_y
     integer, intent(out) :: ierr
+    ierr = c_check_kind(h, 4)
+    if (ierr /= CPML_OK) return
+    ierr = c_set_source(h, ISOURCE, JSOURCE, KSOURCE, force_x, force_y)
+  end subroutine
+
+  subroutine cpml_gpu_set_source_r8(h, ISOURCE, JSOURCE, KSOURCE, force_x, force_y, ierr)
+    type(c_ptr), intent(in) :: h
+    integer, intent(in) :: ISOURCE, JSOURCE, KSOURCE
+    real(8), dimension(*), intent(in) :: force_x, force_y
+    integer, intent(out) :: ierr
+    ierr = c_check_kind(h, 8)
+    if (ierr /= CPML_OK) return
     ierr = c_set_source(h, ISOURCE, JSOURCE, KSOURCE, force_x, force_y)
   end subroutine
 
@@ -360,12 +524,30 @@ contains
   end subroutine
 
   ! sisvx(NSTEP,NREC), sisvy(NSTEP,NREC)[, sisvz]
-  subroutine cpml_gpu_get_seismograms(h, sisvx, sisvy, ierr, sisvz)
+  subroutine cpml_gpu_get_seismograms_r4(h, sisvx, sisvy, ierr, sisvz)
     type(c_ptr), intent(in) :: h
-    type(*), dimension(*) :: sisvx, sisvy
+    real(4), dimension(..), contiguous, intent(inout), target :: sisvx, sisvy
     integer, intent(out) :: ierr
-    type(*), dimension(*), optional :: sisvz
-    ierr = c_get_seismograms(h, sisvx, sisvy, sisvz)
+    real(4), dimension(..), contiguous, intent(inout), target, optional :: sisvz
+    type(c_ptr) :: pz
+    ierr = c_check_kind(h, 4)
+    if (ierr /= CPML_OK) return
+    pz = c_null_ptr
+    if (present(sisvz)) pz = c_loc(sisvz)
+    ierr = c_get_seismograms(h, c_loc(sisvx), c_loc(sisvy), pz)
+  end subroutine
+
+  subroutine cpml_gpu_get_seismograms_r8(h, sisvx, sisvy, ierr, sisvz)
+    type(c_ptr), intent(in) :: h
+    real(8), dimension(..), contiguous, intent(inout), target :: sisvx, sisvy
+    integer, intent(out) :: ierr
+    real(8), dimension(..), contiguous, intent(inout), target, optional :: sisvz
+    type(c_ptr) :: pz
+    ierr = c_check_kind(h, 8)
+    if (ierr /= CPML_OK) return
+    pz = c_null_ptr
+    if (present(sisvz)) pz = c_loc(sisvz)
+    ierr = c_get_seismograms(h, c_loc(sisvx), c_loc(sisvy), pz)
   end subroutine
 
   ! total_energy_kinetic(NSTEP), total_energy_potential(NSTEP) (double precision)
@@ -376,20 +558,111 @@ contains
     ierr = c_get_energy(h, total_energy_kinetic, total_energy_potential)
   end subroutine
 
-  subroutine cpml_gpu_get_field(h, which, field, ierr)
+  subroutine cpml_gpu_get_field_r4(h, which, field, ierr)
     type(c_ptr), intent(in) :: h
     integer, intent(in) :: which
-    type(*), dimension(*) :: field
+    real(4), dimension(..), contiguous, intent(inout), target :: field
     integer, intent(out) :: ierr
-    ierr = c_get_field(h, which, field)
+    ierr = c_check_grid_array(h, 4, rank(field), int(shape(field), c_int64_t))
+    if (ierr /= CPML_OK) return
+    ierr = c_get_field(h, which, c_loc(field))
   end subroutine
 
-  subroutine cpml_gpu_set_field(h, which, field, ierr)
+  subroutine cpml_gpu_get_field_r8(h, which, field, ierr)
     type(c_ptr), intent(in) :: h
     integer, intent(in) :: which
-    type(*), dimension(*), intent(in) :: field
+    real(8), dimension(..), contiguous, intent(inout), target :: field
     integer, intent(out) :: ierr
-    ierr = c_set_field(h, which, field)
+    ierr = c_check_grid_array(h, 8, rank(field), int(shape(field), c_int64_t))
+    if (ierr /= CPML_OK) return
+    ierr = c_get_field(h, which, c_loc(field))
+  end subroutine
+
+  subroutine cpml_gpu_set_field_r4(h, which, field, ierr)
+    type(c_ptr), intent(in) :: h
+    integer, intent(in) :: which
+    real(4), dimension(..), contiguous, intent(in), target :: field
+    integer, intent(out) :: ierr
+    ierr = c_check_grid_array(h, 4, rank(field), int(shape(field), c_int64_t))
+    if (ierr /= CPML_OK) return
+    ierr = c_set_field(h, which, c_loc(field))
+  end subroutine
+
+  subroutine cpml_gpu_set_field_r8(h, which, field, ierr)
+    type(c_ptr), intent(in) :: h
+    integer, intent(in) :: which
+    real(8), dimension(..), contiguous, intent(in), target :: field
+    integer, intent(out) :: ierr
+    ierr = c_check_grid_array(h, 8, rank(field), int(shape(field), c_int64_t))
+    if (ierr /= CPML_OK) return
+    ierr = c_set_field(h, which, c_loc(field))
+  end subroutine
+
+  ! one C-PML memory variable (CPML_MEM_...) as the reference's full-grid array
+  ! (zero outside its absorbing slab); with get_field / set_field a run can be
+  ! checkpointed and resumed exactly
+  subroutine cpml_gpu_get_memory_r4(h, which, memory, ierr)
+    type(c_ptr), intent(in) :: h
+    integer, intent(in) :: which
+    real(4), dimension(..), contiguous, intent(inout), target :: memory
+    integer, intent(out) :: ierr
+    ierr = c_check_grid_array(h, 4, rank(memory), int(shape(memory), c_int64_t))
+    if (ierr /= CPML_OK) return
+    ierr = c_get_memory(h, which, c_loc(memory))
+  end subroutine
+
+  subroutine cpml_gpu_get_memory_r8(h, which, memory, ierr)
+    type(c_ptr), intent(in) :: h
+    integer, intent(in) :: which
+    real(8), dimension(..), contiguous, intent(inout), target :: memory
+    integer, intent(out) :: ierr
+    ierr = c_check_grid_array(h, 8, rank(memory), int(shape(memory), c_int64_t))
+    if (ierr /= CPML_OK) return
+    ierr = c_get_memory(h, which, c_loc(memory))
+  end subroutine
+
+  subroutine cpml_gpu_set_memory_r4(h, which, memory, ierr)
+    type(c_ptr), intent(in) :: h
+    integer, intent(in) :: which
+    real(4), dimension(..), contiguous, intent(in), target :: memory
+    integer, intent(out) :: ierr
+    ierr = c_check_grid_array(h, 4, rank(memory), int(shape(memory), c_int64_t))
+    if (ierr /= CPML_OK) return
+    ierr = c_set_memory(h, which, c_loc(memory))
+  end subroutine
+
+  subroutine cpml_gpu_set_memory_r8(h, which, memory, ierr)
+    type(c_ptr), intent(in) :: h
+    integer, intent(in) :: which
+    real(8), dimension(..), contiguous, intent(in), target :: memory
+    integer, intent(out) :: ierr
+    ierr = c_check_grid_array(h, 8, rank(memory), int(shape(memory), c_int64_t))
+    if (ierr /= CPML_OK) return
+    ierr = c_set_memory(h, which, c_loc(memory))
+  end subroutine
+
+  ! `steps` leapfrog steps per launch (time blocking; 0 = off); optional strip
+  ! width (columns per wavefront) and chunk depth (rows in 2D, planes in 3D)
+  ! grid arrays span (1-lo : N+hi) along each axis (lo = hi = 1 for the
+  ! (0:NX+1,0:NY+1) arrays of 2D4:205-225; lo = 5, hi = 4 for ADE8:272-301)
+  subroutine cpml_gpu_set_host_layout(h, lo, hi, ierr)
+    type(c_ptr), intent(in) :: h
+    integer, intent(in) :: lo, hi
+    integer, intent(out) :: ierr
+    ierr = c_set_host_layout(h, lo, hi)
+  end subroutine
+
+  subroutine cpml_gpu_set_time_block(h, steps, ierr, strip_width, chunk)
+    type(c_ptr), intent(in) :: h
+    integer, intent(in) :: steps
+    integer, intent(out) :: ierr
+    integer, intent(in), optional :: strip_width, chunk
+    integer :: w, c
+    w = 0
+    c = 0
+    if (present(strip_width)) w = strip_width
+    if (present(chunk)) c = chunk
+    ierr = c_set_time_block(h, steps, w, c)
   end subroutine
 
   ! maxval(sqrt(vx**2 + vy**2 [+ vz**2])) (2D2:719, 3D:1185)
@@ -417,22 +690,47 @@ contains
   ! ADE-PML RK4 scheme of seismic_ADEPML_2D_elastic_RK4_eighth_order.f90 (2D,
   ! order 8): stage tables a_x(4, NX) ... b_y_half(4, NY) (the interior of its
   ! (4, -4:N+4) arrays) and the force at each stage time force_x4(4, NSTEP)
-  subroutine cpml_gpu_set_ade_rk4(h, a_x, b_x, a_x_half, b_x_half, a_y, b_y, a_y_half, &
+  subroutine cpml_gpu_set_ade_rk4_r4(h, a_x, b_x, a_x_half, b_x_half, a_y, b_y, a_y_half, &
                                   b_y_half, force_x4, force_y4, ierr)
     type(c_ptr), intent(in) :: h
-    type(*), dimension(*), intent(in) :: a_x, b_x, a_x_half, b_x_half, a_y, b_y, a_y_half, &
+    real(4), dimension(*), intent(in) :: a_x, b_x, a_x_half, b_x_half, a_y, b_y, a_y_half, &
          b_y_half, force_x4, force_y4
     integer, intent(out) :: ierr
+    ierr = c_check_kind(h, 4)
+    if (ierr /= CPML_OK) return
+    ierr = c_set_ade_rk4(h, a_x, b_x, a_x_half, b_x_half, a_y, b_y, a_y_half, b_y_half, &
+                         force_x4, force_y4)
+  end subroutine
+
+  subroutine cpml_gpu_set_ade_rk4_r8(h, a_x, b_x, a_x_half, b_x_half, a_y, b_y, a_y_half, &
+                                  b_y_half, force_x4, force_y4, ierr)
+    type(c_ptr), intent(in) :: h
+    real(8), dimension(*), intent(in) :: a_x, b_x, a_x_half, b_x_half, a_y, b_y, a_y_half, &
+         b_y_half, force_x4, force_y4
+    integer, intent(out) :: ierr
+    ierr = c_check_kind(h, 8)
+    if (ierr /= CPML_OK) return
     ierr = c_set_ade_rk4(h, a_x, b_x, a_x_half, b_x_half, a_y, b_y, a_y_half, b_y_half, &
                          force_x4, force_y4)
   end subroutine
 
   ! sispressure(NSTEP, NREC) of the viscoelastic step (its kind is the handle's)
-  subroutine cpml_gpu_get_pressure(h, sispressure, ierr)
+  subroutine cpml_gpu_get_pressure_r4(h, sispressure, ierr)
     type(c_ptr), intent(in) :: h
-    type(*), dimension(*) :: sispressure
+    real(4), dimension(..), contiguous, intent(inout), target :: sispressure
     integer, intent(out) :: ierr
-    ierr = c_get_pressure(h, sispressure)
+    ierr = c_check_kind(h, 4)
+    if (ierr /= CPML_OK) return
+    ierr = c_get_pressure(h, c_loc(sispressure))
+  end subroutine
+
+  subroutine cpml_gpu_get_pressure_r8(h, sispressure, ierr)
+    type(c_ptr), intent(in) :: h
+    real(8), dimension(..), contiguous, intent(inout), target :: sispressure
+    integer, intent(out) :: ierr
+    ierr = c_check_kind(h, 8)
+    if (ierr /= CPML_OK) return
+    ierr = c_get_pressure(h, c_loc(sispressure))
   end subroutine
 
   ! create_color_image of 2D2:861-999 (2D4:729-731 for order 4, ghost-padded;
@@ -463,13 +761,26 @@ contains
   ! SU file of ADE8:987-1003 (seismograms: dt_value = DELTAT*1e6) and
   ! ADE8:935-953 (snapshots: dt_value = DELTAY*1e3): traces(ns, ntr) of kind
   ! prec (4 or 8), each written as float32 after a 120-word int16 header
-  subroutine cpml_gpu_write_su(file_name, traces, ns, ntr, prec, dt_value, ierr)
+  subroutine cpml_gpu_write_su_r4(file_name, traces, ns, ntr, prec, dt_value, ierr)
     character(len=*), intent(in) :: file_name
-    type(*), dimension(*), intent(in) :: traces
+    real(4), dimension(..), contiguous, intent(in), target :: traces
     integer, intent(in) :: ns, ntr, prec
     double precision, intent(in) :: dt_value
     integer, intent(out) :: ierr
-    ierr = c_su_write(trim(file_name) // c_null_char, ntr, ns, dt_value, traces, prec)
+    ierr = CPML_ERR_ARG
+    if (prec /= 4 .or. size(traces) < int(ns, 8) * ntr) return
+    ierr = c_su_write(trim(file_name) // c_null_char, ntr, ns, dt_value, c_loc(traces), prec)
+  end subroutine
+
+  subroutine cpml_gpu_write_su_r8(file_name, traces, ns, ntr, prec, dt_value, ierr)
+    character(len=*), intent(in) :: file_name
+    real(8), dimension(..), contiguous, intent(in), target :: traces
+    integer, intent(in) :: ns, ntr, prec
+    double precision, intent(in) :: dt_value
+    integer, intent(out) :: ierr
+    ierr = CPML_ERR_ARG
+    if (prec /= 8 .or. size(traces) < int(ns, 8) * ntr) return
+    ierr = c_su_write(trim(file_name) // c_null_char, ntr, ns, dt_value, c_loc(traces), prec)
   end subroutine
 
   subroutine cpml_gpu_elapsed_ms(h, ms, ierr)
@@ -498,30 +809,83 @@ contains
     b = c_null_ptr
   end subroutine
 
-  subroutine cpml_gpu_batch_set_medium(b, shot, lambda, mu, rho, ierr)
+  subroutine cpml_gpu_batch_set_medium_r4(b, shot, lambda, mu, rho, ierr)
     type(c_ptr), intent(in) :: b
     integer, intent(in) :: shot
-    type(*), dimension(*), intent(in) :: lambda, mu, rho
+    real(4), dimension(..), contiguous, intent(in), target :: lambda, mu, rho
     integer, intent(out) :: ierr
-    ierr = c_batch_set_medium(b, shot - 1, lambda, mu, rho)
+    integer(c_int) :: nx, ny
+    ierr = c_batch_check_kind(b, 4)
+    if (ierr /= CPML_OK) return
+    ierr = c_batch_grid(b, shot - 1, nx, ny)
+    if (ierr /= CPML_OK) return
+    if (size(lambda) /= int(nx, 8) * ny) ierr = CPML_ERR_ARG
+    if (size(mu) /= int(nx, 8) * ny) ierr = CPML_ERR_ARG
+    if (size(rho) /= int(nx, 8) * ny) ierr = CPML_ERR_ARG
+    if (ierr /= CPML_OK) return
+    ierr = c_batch_set_medium(b, shot - 1, c_loc(lambda), c_loc(mu), c_loc(rho))
   end subroutine
 
-  subroutine cpml_gpu_batch_set_pml(b, shot, a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, &
+  subroutine cpml_gpu_batch_set_medium_r8(b, shot, lambda, mu, rho, ierr)
+    type(c_ptr), intent(in) :: b
+    integer, intent(in) :: shot
+    real(8), dimension(..), contiguous, intent(in), target :: lambda, mu, rho
+    integer, intent(out) :: ierr
+    integer(c_int) :: nx, ny
+    ierr = c_batch_check_kind(b, 8)
+    if (ierr /= CPML_OK) return
+    ierr = c_batch_grid(b, shot - 1, nx, ny)
+    if (ierr /= CPML_OK) return
+    if (size(lambda) /= int(nx, 8) * ny) ierr = CPML_ERR_ARG
+    if (size(mu) /= int(nx, 8) * ny) ierr = CPML_ERR_ARG
+    if (size(rho) /= int(nx, 8) * ny) ierr = CPML_ERR_ARG
+    if (ierr /= CPML_OK) return
+    ierr = c_batch_set_medium(b, shot - 1, c_loc(lambda), c_loc(mu), c_loc(rho))
+  end subroutine
+
+  subroutine cpml_gpu_batch_set_pml_r4(b, shot, a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, &
                                     a_y, b_y, K_y, a_y_half, b_y_half, K_y_half, ierr)
     type(c_ptr), intent(in) :: b
     integer, intent(in) :: shot
-    type(*), dimension(*), intent(in) :: a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, &
+    real(4), dimension(*), intent(in) :: a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, &
          a_y, b_y, K_y, a_y_half, b_y_half, K_y_half
     integer, intent(out) :: ierr
+    ierr = c_batch_check_kind(b, 4)
+    if (ierr /= CPML_OK) return
     ierr = c_batch_set_pml(b, shot - 1, a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, a_y, b_y, &
                            K_y, a_y_half, b_y_half, K_y_half)
   end subroutine
 
-  subroutine cpml_gpu_batch_set_source(b, shot, ISOURCE, JSOURCE, force_x, force_y, ierr)
+  subroutine cpml_gpu_batch_set_pml_r8(b, shot, a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, &
+                                    a_y, b_y, K_y, a_y_half, b_y_half, K_y_half, ierr)
+    type(c_ptr), intent(in) :: b
+    integer, intent(in) :: shot
+    real(8), dimension(*), intent(in) :: a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, &
+         a_y, b_y, K_y, a_y_half, b_y_half, K_y_half
+    integer, intent(out) :: ierr
+    ierr = c_batch_check_kind(b, 8)
+    if (ierr /= CPML_OK) return
+    ierr = c_batch_set_pml(b, shot - 1, a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, a_y, b_y, &
+                           K_y, a_y_half, b_y_half, K_y_half)
+  end subroutine
+
+  subroutine cpml_gpu_batch_set_source_r4(b, shot, ISOURCE, JSOURCE, force_x, force_y, ierr)
     type(c_ptr), intent(in) :: b
     integer, intent(in) :: shot, ISOURCE, JSOURCE
-    type(*), dimension(*), intent(in) :: force_x, force_y
+    real(4), dimension(*), intent(in) :: force_x, force_y
     integer, intent(out) :: ierr
+    ierr = c_batch_check_kind(b, 4)
+    if (ierr /= CPML_OK) return
+    ierr = c_batch_set_source(b, shot - 1, ISOURCE, JSOURCE, force_x, force_y)
+  end subroutine
+
+  subroutine cpml_gpu_batch_set_source_r8(b, shot, ISOURCE, JSOURCE, force_x, force_y, ierr)
+    type(c_ptr), intent(in) :: b
+    integer, intent(in) :: shot, ISOURCE, JSOURCE
+    real(8), dimension(*), intent(in) :: force_x, force_y
+    integer, intent(out) :: ierr
+    ierr = c_batch_check_kind(b, 8)
+    if (ierr /= CPML_OK) return
     ierr = c_batch_set_source(b, shot - 1, ISOURCE, JSOURCE, force_x, force_y)
   end subroutine
 
@@ -551,12 +915,24 @@ contains
     ierr = c_batch_run(b, it_begin, it_end)
   end subroutine
 
-  subroutine cpml_gpu_batch_get_seismograms(b, shot, sisvx, sisvy, ierr)
+  subroutine cpml_gpu_batch_get_seismograms_r4(b, shot, sisvx, sisvy, ierr)
     type(c_ptr), intent(in) :: b
     integer, intent(in) :: shot
-    type(*), dimension(*) :: sisvx, sisvy
+    real(4), dimension(..), contiguous, intent(inout), target :: sisvx, sisvy
     integer, intent(out) :: ierr
-    ierr = c_batch_get_seismograms(b, shot - 1, sisvx, sisvy)
+    ierr = c_batch_check_kind(b, 4)
+    if (ierr /= CPML_OK) return
+    ierr = c_batch_get_seismograms(b, shot - 1, c_loc(sisvx), c_loc(sisvy))
+  end subroutine
+
+  subroutine cpml_gpu_batch_get_seismograms_r8(b, shot, sisvx, sisvy, ierr)
+    type(c_ptr), intent(in) :: b
+    integer, intent(in) :: shot
+    real(8), dimension(..), contiguous, intent(inout), target :: sisvx, sisvy
+    integer, intent(out) :: ierr
+    ierr = c_batch_check_kind(b, 8)
+    if (ierr /= CPML_OK) return
+    ierr = c_batch_get_seismograms(b, shot - 1, c_loc(sisvx), c_loc(sisvy))
   end subroutine
 
   ! ---- learned thin boundary: weights(6*3*27*16*16), scales(9), real(4)

@@ -7,9 +7,18 @@
 ! WP=8 (double precision, as shipped) and WP=4 (the single-precision form
 ! the reference recommends at 2D2:131-133).
 !
+! The grid arrays keep the reference's bounds: (NX,NY) at order 2 and the
+! ghost-padded (0:NX+1,0:NY+1) of 2D4:205 at order 4, handed over in place
+! after cpml_gpu_set_host_layout(h, 1, 1).
+!
 ! Outputs: Vx_file_NNN.dat, Vy_file_NNN.dat, energy.dat, imageNNNNNN_V[xy].pnm
 ! (reference formats) and gpu_dump.bin (seismograms, energy and final vx, vy
 ! at full precision).
+!
+! Built with RESUME_AT=n, the driver checkpoints after step n (fields and
+! C-PML memory variables, with get_field / get_memory), closes the handle,
+! opens a new one and restores the state (set_field / set_memory) before it
+! goes on: the outputs equal the uninterrupted run's.
 #ifndef WP
 #define WP 8
 #endif
@@ -58,7 +67,9 @@ program seismic_cpml_2d_isotropic_gpu
   real(wp), parameter :: yfin = 300._wp
   double precision, parameter :: STABILITY_THRESHOLD = 1.d+25
 
-  real(wp), dimension(NX,NY) :: lambda, mu, rho, vx, vy
+  ! ghost cells per side of the grid arrays (2D4:205 declares 0:NX+1,0:NY+1)
+  integer, parameter :: G = ORDER / 4
+  real(wp), dimension(1-G:NX+G,1-G:NY+G) :: lambda, mu, rho, vx, vy
   real(wp), dimension(NX) :: a_x, b_x, K_x, a_x_half, b_x_half, K_x_half
   real(wp), dimension(NY) :: a_y, b_y, K_y, a_y_half, b_y_half, K_y_half
   real(wp), dimension(NSTEP) :: force_x, force_y
@@ -68,6 +79,18 @@ program seismic_cpml_2d_isotropic_gpu
   type(c_ptr) :: h
   integer :: i, j, it, it_end, ierr
   double precision :: velocnorm, courant_number, ms
+#ifdef RESUME_AT
+  ! the state of the 2D loop: 5 fields and 8 memory variables (2D2:222-230)
+  integer, parameter :: FIELDS(5) = [CPML_VX, CPML_VY, CPML_SIGMAXX, CPML_SIGMAYY, &
+       CPML_SIGMAXY]
+  integer, parameter :: MEMS(8) = [CPML_MEM_DVX_DX, CPML_MEM_DVX_DY, CPML_MEM_DVY_DX, &
+       CPML_MEM_DVY_DY, CPML_MEM_DSIGMAXX_DX, CPML_MEM_DSIGMAYY_DY, CPML_MEM_DSIGMAXY_DX, &
+       CPML_MEM_DSIGMAXY_DY]
+  real(wp), dimension(1-G:NX+G,1-G:NY+G,13) :: saved
+  real(wp), dimension(NSTEP,NREC) :: sisvx0, sisvy0
+  double precision, dimension(NSTEP) :: ekin0, epot0
+  integer :: q
+#endif
 
   print *, '2D elastic finite-difference code in velocity and stress formulation with C-PML'
   print *, 'time loop on the GPU (cpml_gpu), order ', ORDER, ', real kind ', wp
@@ -79,8 +102,8 @@ program seismic_cpml_2d_isotropic_gpu
   call pml_profile(NY, DELTAY, DELTAT, NPOINTS_PML, cp, f0, USE_PML_YMIN, USE_PML_YMAX, &
                    YORIGIN_N, .false., a_y, b_y, K_y, a_y_half, b_y_half, K_y_half)
   ! Lame parameters and density (2D2:468-474)
-  do j = 1, NY
-    do i = 1, NX
+  do j = 1-G, NY+G
+    do i = 1-G, NX+G
       rho(i,j) = density
       mu(i,j) = density*cs*cs
       lambda(i,j) = density*(cp*cp - 2._wp*cs*cs)
@@ -95,22 +118,7 @@ program seismic_cpml_2d_isotropic_gpu
   call source_forces(NSTEP, DELTAT, f0, t0, factor, ANGLE_FORCE, 1, force_x, force_y)
 
   ! hand the arrays to the GPU library
-  call cpml_gpu_init(h, 2, ORDER, wp, NX, NY, 1, dble(DELTAX), dble(DELTAY), 1.d0, &
-                     dble(DELTAT), NPOINTS_PML, NSTEP, ierr)
-  call check(h, ierr, 'init')
-  call cpml_gpu_set_medium(h, lambda, mu, rho, ierr)
-  call check(h, ierr, 'set_medium')
-  call cpml_gpu_set_pml_2d(h, a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, &
-                           a_y, b_y, K_y, a_y_half, b_y_half, K_y_half, ierr)
-  call check(h, ierr, 'set_pml')
-  call cpml_gpu_set_source(h, ISOURCE, JSOURCE, 1, force_x, force_y, ierr)
-  call check(h, ierr, 'set_source')
-  call cpml_gpu_set_receivers(h, NREC, ix_rec, iy_rec, ierr=ierr)
-  call check(h, ierr, 'set_receivers')
-  call cpml_gpu_set_energy(h, 1, ierr)
-  call check(h, ierr, 'set_energy')
-  call cpml_gpu_set_stability(h, IT_DISPLAY, STABILITY_THRESHOLD, ierr)
-  call check(h, ierr, 'set_stability')
+  call open_handle()
 
   ! time loop, in the display chunks of 2D2:716 (it == 5 and every IT_DISPLAY)
   it = 1
@@ -133,6 +141,30 @@ program seismic_cpml_2d_isotropic_gpu
       call cpml_gpu_create_color_image(h, NX, NY, ORDER, it_end, 2, ierr)
       call check(h, ierr, 'color_image')
     endif
+#ifdef RESUME_AT
+    if (it_end == RESUME_AT) then
+      ! checkpoint, a new handle, restart from the saved state
+      call cpml_gpu_get_seismograms(h, sisvx0, sisvy0, ierr)
+      call cpml_gpu_get_energy(h, ekin0, epot0, ierr)
+      do q = 1, 5
+        call cpml_gpu_get_field(h, FIELDS(q), saved(:,:,q), ierr)
+      enddo
+      do q = 1, 8
+        call cpml_gpu_get_memory(h, MEMS(q), saved(:,:,5+q), ierr)
+      enddo
+      call check(h, ierr, 'checkpoint')
+      call cpml_gpu_finalize(h, ierr)
+      call open_handle()
+      do q = 1, 5
+        call cpml_gpu_set_field(h, FIELDS(q), saved(:,:,q), ierr)
+      enddo
+      do q = 1, 8
+        call cpml_gpu_set_memory(h, MEMS(q), saved(:,:,5+q), ierr)
+      enddo
+      call check(h, ierr, 'restore')
+      print *, 'resumed from the checkpoint of step ', RESUME_AT
+    endif
+#endif
     it = it_end + 1
   enddo
 
@@ -140,6 +172,14 @@ program seismic_cpml_2d_isotropic_gpu
   call check(h, ierr, 'get_seismograms')
   call cpml_gpu_get_energy(h, total_energy_kinetic, total_energy_potential, ierr)
   call check(h, ierr, 'get_energy')
+#ifdef RESUME_AT
+  sisvx(1:RESUME_AT,:) = sisvx0(1:RESUME_AT,:)
+  sisvy(1:RESUME_AT,:) = sisvy0(1:RESUME_AT,:)
+  total_energy_kinetic(1:RESUME_AT) = ekin0(1:RESUME_AT)
+  total_energy_potential(1:RESUME_AT) = epot0(1:RESUME_AT)
+#endif
+  vx = 0._wp
+  vy = 0._wp
   call cpml_gpu_get_field(h, CPML_VX, vx, ierr)
   call cpml_gpu_get_field(h, CPML_VY, vy, ierr)
   call check(h, ierr, 'get_field')
@@ -155,12 +195,36 @@ program seismic_cpml_2d_isotropic_gpu
   enddo
   close(20)
   open(unit=21, file='gpu_dump.bin', access='stream', form='unformatted', status='replace')
-  write(21) sisvx, sisvy, total_energy_kinetic, total_energy_potential, vx, vy
+  write(21) sisvx, sisvy, total_energy_kinetic, total_energy_potential, vx(1:NX,1:NY), &
+       vy(1:NX,1:NY)
   close(21)
   print *, 'End of the simulation'
 
 contains
 
   include 'cpml_setup.inc'
+
+  ! handle with the program's medium, profiles, source, receivers, energy and
+  ! stability check
+  subroutine open_handle()
+    call cpml_gpu_init(h, 2, ORDER, wp, NX, NY, 1, dble(DELTAX), dble(DELTAY), 1.d0, &
+                       dble(DELTAT), NPOINTS_PML, NSTEP, ierr)
+    call check(h, ierr, 'init')
+    call cpml_gpu_set_host_layout(h, G, G, ierr)
+    call check(h, ierr, 'set_host_layout')
+    call cpml_gpu_set_medium(h, lambda, mu, rho, ierr)
+    call check(h, ierr, 'set_medium')
+    call cpml_gpu_set_pml_2d(h, a_x, b_x, K_x, a_x_half, b_x_half, K_x_half, &
+                             a_y, b_y, K_y, a_y_half, b_y_half, K_y_half, ierr)
+    call check(h, ierr, 'set_pml')
+    call cpml_gpu_set_source(h, ISOURCE, JSOURCE, 1, force_x, force_y, ierr)
+    call check(h, ierr, 'set_source')
+    call cpml_gpu_set_receivers(h, NREC, ix_rec, iy_rec, ierr=ierr)
+    call check(h, ierr, 'set_receivers')
+    call cpml_gpu_set_energy(h, 1, ierr)
+    call check(h, ierr, 'set_energy')
+    call cpml_gpu_set_stability(h, IT_DISPLAY, STABILITY_THRESHOLD, ierr)
+    call check(h, ierr, 'set_stability')
+  end subroutine
 
 end program seismic_cpml_2d_isotropic_gpu

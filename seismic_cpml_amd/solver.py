@@ -54,6 +54,7 @@ SIGNATURES = {
     "cpml_gpu_get_field": [_VP, _I, _VP],
     "cpml_gpu_set_field": [_VP, _I, _VP],
     "cpml_gpu_get_memory": [_VP, _I, _VP],
+    "cpml_gpu_set_memory": [_VP, _I, _VP],
     "cpml_gpu_velocity_norm": [_VP, _DP],
     "cpml_gpu_color_image": [_VP, C.c_int, C.c_void_p],
     "cpml_pnm_write": [C.c_char_p, C.c_int, C.c_int, C.c_void_p],
@@ -73,6 +74,13 @@ SIGNATURES = {
     "cpml_gpu_device_bytes": [_VP, _DP],
     "cpml_gpu_slab_widths": [_VP, _IP],
     "cpml_gpu_last_error": [_VP, C.c_char_p, _I],
+    "cpml_gpu_check_kind": [_VP, _I],
+    "cpml_batch_check_kind": [_VP, _I],
+    "cpml_gpu_grid": [_VP, _IP, _IP, _IP],
+    "cpml_gpu_set_host_layout": [_VP, _I, _I],
+    "cpml_gpu_guard_cells": [_VP, _I, C.POINTER(C.c_longlong)],
+    "cpml_gpu_check_grid_array": [_VP, _I, _I, C.POINTER(C.c_int64)],
+    "cpml_batch_grid": [_VP, _I, _IP, _IP],
     "cpml_batch_init": [C.POINTER(_VP), _I, _I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _I],
     "cpml_batch_finalize": [_VP],
     "cpml_batch_set_medium": [_VP, _I, _VP, _VP, _VP],
@@ -81,6 +89,11 @@ SIGNATURES = {
     "cpml_batch_set_receivers": [_VP, _I, _I, _VP, _VP],
     "cpml_batch_set_tiling": [_VP, _I, _I, _I],
     "cpml_batch_autotune": [_VP, _I, C.c_char_p, _IP, _IP, _DP],
+    "cpml_batch_set_time_block": [_VP, _I, _I, _I],
+    "cpml_batch_launch_shape": [_VP, _IP, _IP, _IP, _IP],
+    "cpml_batch_set_energy": [_VP, _I],
+    "cpml_batch_get_energy": [_VP, _I, _VP, _VP],
+    "cpml_batch_set_stability": [_VP, _I, _D],
     "cpml_batch_run": [_VP, _I, _I],
     "cpml_batch_get_seismograms": [_VP, _I, _VP, _VP],
     "cpml_batch_get_field": [_VP, _I, _I, _VP],
@@ -359,6 +372,10 @@ class CpmlGpu:
         a = self._grid(a)
         self._call("cpml_gpu_set_field", FIELDS.index(name), _ptr(a))
 
+    def set_memory(self, name, a):
+        a = self._grid(a)
+        self._call("cpml_gpu_set_memory", MEMORY.index(name), _ptr(a))
+
     def memory(self, name):
         out = np.zeros(self.shape, self.dtype)
         self._call("cpml_gpu_get_memory", MEMORY.index(name), _ptr(out))
@@ -404,6 +421,19 @@ class CpmlGpu:
         v = C.c_double()
         self._call("cpml_gpu_velocity_norm", C.byref(v))
         return v.value
+
+    def poison_guards(self):
+        """Fill the guard cells (row padding outside the grid and its ghost
+        points) of the device arrays with NaN; returns their number."""
+        n = C.c_longlong()
+        self._call("cpml_gpu_guard_cells", 1, C.byref(n))
+        return n.value
+
+    def guards_changed(self):
+        """Number of guard cells that no longer hold a NaN."""
+        n = C.c_longlong()
+        self._call("cpml_gpu_guard_cells", 0, C.byref(n))
+        return n.value
 
     def color_image(self, field_number):
         """PNM snapshot of Vx (1) or Vy (2), computed on the device
@@ -640,6 +670,27 @@ class CpmlBatch:
         self._call("cpml_batch_autotune", trial_steps, path, C.byref(ty), C.byref(bpc),
                    C.byref(ms))
         return ty.value, bpc.value, ms.value
+
+    def set_time_block(self, steps, strip_width=0, chunk_rows=0):
+        self._call("cpml_batch_set_time_block", steps, strip_width, chunk_rows)
+
+    def launch_shape(self):
+        """(tile_y, blocks_per_cu, time_block, chunk_rows) in use."""
+        v = [C.c_int() for _ in range(4)]
+        self._call("cpml_batch_launch_shape", *[C.byref(x) for x in v])
+        return tuple(x.value for x in v)
+
+    def set_energy(self, cadence=1):
+        self._call("cpml_batch_set_energy", cadence)
+
+    def energy(self, q):
+        kin = np.zeros(self.nstep)
+        pot = np.zeros(self.nstep)
+        self._call("cpml_batch_get_energy", q, _ptr(kin), _ptr(pot))
+        return kin, pot
+
+    def set_stability(self, it_display=100, threshold=1e25):
+        self._call("cpml_batch_set_stability", it_display, threshold)
 
     def run(self, it_begin=1, it_end=None):
         self._call("cpml_batch_run", it_begin, self.nstep if it_end is None else it_end)

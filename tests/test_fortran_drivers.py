@@ -189,3 +189,36 @@ def test_3d4_driver_matches_oracle_and_reference_setup(tmp_path):
     np.testing.assert_array_equal(sisvx, o["sisvx"])
     np.testing.assert_array_equal(sisvy, o["sisvy"])
     np.testing.assert_array_equal(plane, o["vx"][40 // 2 - 1])
+
+
+@pytest.mark.parametrize("order,prec", [(2, "f64"), (4, "f32")])
+def test_2d_driver_resumes_from_checkpoint_exactly(order, prec, tmp_path):
+    """The RESUME_AT build saves the fields and C-PML memory variables after a
+    display step (cpml_gpu_get_field / cpml_gpu_get_memory), closes the
+    handle, opens a new one and restores them (set_field / set_memory): every
+    output file equals the uninterrupted run's byte for byte."""
+    base = f"seismic_cpml_2d_{'second' if order == 2 else 'fourth'}_{prec}"
+    (tmp_path / "a").mkdir()
+    (tmp_path / "b").mkdir()
+    run_driver(base, tmp_path / "a")
+    out = run_driver(base + "_resume", tmp_path / "b")
+    assert "resumed from the checkpoint" in out
+    names = sorted(p.name for p in (tmp_path / "a").iterdir())
+    assert "gpu_dump.bin" in names and "energy.dat" in names
+    assert names == sorted(p.name for p in (tmp_path / "b").iterdir())
+    for n in names:
+        assert (tmp_path / "a" / n).read_bytes() == (tmp_path / "b" / n).read_bytes(), n
+
+
+
+def test_binding_checks_kinds_shapes_and_ghost_layout(tmp_path):
+    """The generic wrappers refuse real(4) arrays on a double handle and grid
+    arrays of the wrong shape (CPML_ERR_ARG with a message), and the
+    reference's explicit-lower-bound arrays (ADE8 -4:NX+4, 2D4 0:NX+1) go
+    through set/get of fields and memory variables in place, interior only,
+    ghost cells untouched (cpml_gpu_binding_check.F90)."""
+    exe = BIN / "cpml_gpu_binding_check"
+    assert exe.is_file(), f"{exe} not built (python -m seismic_cpml_amd.build)"
+    r = subprocess.run([str(exe)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "BINDING OK" in r.stdout, r.stdout[-3000:] + r.stderr[-2000:]
+    assert "real(4)" in r.stdout and "(41,30)" in r.stdout

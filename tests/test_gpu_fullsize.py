@@ -2,11 +2,16 @@
 smoothed medium, Ricker 8 Hz, 1024 receivers) checked through properties
 that hold at any size, since the oracle cannot run it in seconds:
 
-* the fused step equals the two-phase kernels bit for bit on every field and
-  seismogram sample after 20 steps (two independent kernel designs, with
-  the 32-bit per-chunk buffer offsets at 1.8 GB per field);
+* the fused step and the time-blocked step (2 steps per launch) equal the
+  two-phase kernels bit for bit on every field and seismogram sample after
+  20 steps (three independent kernel designs, with the 32-bit per-chunk
+  buffer offsets at 1.8 GB per field); fields are compared through 128-bit
+  digests of their bytes;
 * linearity: doubling the force doubles every field exactly (x2 is exact in
-  binary floating point and every update is linear in the fields);
+  binary floating point and every update is linear in the fields) wherever
+  the values are normal numbers; in the far tail of the wave, where values
+  fall below the smallest normal float (1.2e-38), rounding of subnormals
+  may differ by one subnormal step (seen: 6 of 4.5e8 values off by 1.4e-45);
 * a zero force leaves every field exactly zero.
 """
 import numpy as np
@@ -16,6 +21,12 @@ pytestmark = pytest.mark.gpu
 
 FIELDS = ("vx", "vy", "vz", "sigmaxx", "sigmayy", "sigmazz", "sigmaxy", "sigmaxz", "sigmayz")
 NSTEP = 20
+
+
+def _digest(a):
+    import xxhash
+
+    return xxhash.xxh3_128_hexdigest(np.ascontiguousarray(a).view(np.uint8))
 
 
 @pytest.fixture(scope="module")
@@ -30,7 +41,7 @@ def workload():
     torch.cuda.empty_cache()
 
 
-def _solver(w, scale=1.0, fused=False):
+def _solver(w, scale=1.0, fused=False, time_block=0):
     s = w.make_solver()
     if scale != 1.0:
         fx, fy = w.force
@@ -39,32 +50,55 @@ def _solver(w, scale=1.0, fused=False):
         s.set_kchunk(48)
         s.set_tile(16)
         s.set_fused(True)
-    print(f"fullsize: 768^3 scale={scale} fused={fused}", flush=True)
+    if time_block:
+        s.set_time_block(time_block)
     s.run()
     return s
 
 
-def test_fullsize_fused_equals_two_phase_and_linearity(workload):
-    w = workload
-    a = _solver(w)
+@pytest.fixture(scope="module")
+def two_phase(workload):
+    s = _solver(workload)
+    digests = {name: _digest(s.field(name)) for name in FIELDS}
+    yield s, digests, s.seismograms()
+    s.close()
+
+
+@pytest.mark.parametrize("path", ["fused", "time_blocked"])
+def test_fullsize_paths_equal_two_phase(workload, two_phase, path):
+    a, digests, seis = two_phase
     assert a.velocity_norm() > 0
-    b = _solver(w, fused=True)
-    for name in FIELDS:
-        fa = a.field(name)
-        np.testing.assert_array_equal(b.field(name), fa, err_msg=f"fused {name}")
-    for x, y in zip(a.seismograms(), b.seismograms()):
-        np.testing.assert_array_equal(x, y)
-    b.close()
-    c = _solver(w, scale=2.0)
-    for name in FIELDS:
-        np.testing.assert_array_equal(c.field(name), 2 * a.field(name), err_msg=f"x2 {name}")
-    c.close()
-    a.close()
+    b = _solver(workload, fused=path == "fused", time_block=2 if path == "time_blocked" else 0)
+    try:
+        for name in FIELDS:
+            assert _digest(b.field(name)) == digests[name], f"{path} {name}"
+        for x, y in zip(seis, b.seismograms()):
+            np.testing.assert_array_equal(x, y)
+    finally:
+        b.close()
+
+
+def test_fullsize_linearity(workload, two_phase):
+    a = two_phase[0]
+    c = _solver(workload, scale=2.0)
+    tiny = np.finfo(np.float32).tiny
+    try:
+        for name in FIELDS:
+            fa2, fc = 2 * a.field(name), c.field(name)
+            if np.array_equal(fc, fa2):
+                continue
+            normal = np.abs(fa2) >= tiny
+            np.testing.assert_array_equal(fc[normal], fa2[normal], err_msg=f"x2 {name}")
+            assert np.abs(fc[~normal] - fa2[~normal]).max(initial=0.0) <= 2 * tiny, name
+    finally:
+        c.close()
 
 
 def test_fullsize_zero_force_leaves_zero_fields(workload):
     z = _solver(workload, scale=0.0)
-    assert z.velocity_norm() == 0.0
-    for name in FIELDS:
-        assert not np.any(z.field(name)), name
-    z.close()
+    try:
+        assert z.velocity_norm() == 0.0
+        for name in FIELDS:
+            assert not np.any(z.field(name)), name
+    finally:
+        z.close()
