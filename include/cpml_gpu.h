@@ -162,7 +162,7 @@ int cpml_gpu_set_fused(cpml_solver *h, int on);
  * ping-pongs between two buffers: one more set of field arrays and memory
  * variables).  steps 0 turns it off.  2D (orders 2, 4, 8): at most 4 / 3 / 2
  * steps in float, 2 in double; one 64-lane wavefront per strip of
- * strip_width columns marching through chunks of chunk_rows rows (0: 128).
+ * strip_width columns marching through chunks of chunk_rows rows (0: 32).
  * 3D (order 2): at most 2 steps; a workgroup owns strip_width columns x
  * 12 (float; double: 4) rows for 2 steps (14 / 6 for 1) and marches through
  * z chunks of chunk_rows planes (0: 64).  strip_width 0 = the widest

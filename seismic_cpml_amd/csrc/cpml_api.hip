@@ -146,7 +146,7 @@ struct cpml_solver {
   bool fused = false;   // 3D: one fused kernel per step (ping-pong state)
   // 2D time blocking: tb steps per launch (0: off), strips of tb_ow columns
   // (0: the widest multiple of 8 the ring allows), chunks of tb_jc rows
-  int tb = 0, tb_ow = 0, tb_jc = 128;
+  int tb = 0, tb_ow = 0, tb_jc = 32;
   // side stream of the time-blocked launches (slab classes beside the rest)
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -821,7 +821,7 @@ int enqueue_steps(cpml_solver *h, int n, int *ev_used) {
   if (h->ktiming) {
     int rc = ensure_events(h, (size_t)(*ev_used + 3 * n));
     if (rc) return rc;
-    for (; n >= u && u > 1; n -= u) {  // time-blocked launches count as the stress kernel
+    for (; n >= u && tb_active(h); n -= u) {  // time-blocked launches count as the stress kernel
       hipEvent_t *e = &h->evpool[*ev_used];
       HIPCHK(h, hipEventRecord(e[0], h->stream));
       HIPCHK(h, h->prec == 4 ? enqueue_tb<float>(h) : enqueue_tb<double>(h));
@@ -868,7 +868,7 @@ int enqueue_steps(cpml_solver *h, int n, int *ev_used) {
       n -= gl;  // an even number of ping-pong launches: the state is back in its buffer
     }
   }
-  for (; n >= u && u > 1; n -= u) HIPCHK(h, enqueue_unit_any(h));
+  for (; n >= u && tb_active(h); n -= u) HIPCHK(h, enqueue_unit_any(h));
   for (int s = 0; s < n; ++s) HIPCHK(h, enqueue_step_any(h));
   return CPML_OK;
 }
@@ -1679,7 +1679,9 @@ int cpml_gpu_set_memory(cpml_solver *h, int which, const void *in) {
   if (rc) return rc;
   if (!h->mem_elems[which]) return CPML_OK;  // no slab along this axis
   const size_t es = h->esize;
+  // start from the device copy, so that the row padding keeps its contents
   std::vector<char> tmp(h->mem_elems[which] * es, 0);
+  HIPCHK(h, hipMemcpy(tmp.data(), mbuf(h, which), tmp.size(), hipMemcpyDeviceToHost));
   const char *src = static_cast<const char *>(in);
   const int nx = h->nx, ny = h->ny, nz = h->nz;
   // the slab points of the full-grid array (the rest is the reference's
@@ -2018,7 +2020,7 @@ int cpml_gpu_set_time_block(cpml_solver *h, int steps, int strip_width, int chun
     return fail(h, CPML_ERR_ARG, "strip width must be 0 (default) or 1.." +
                                      std::to_string(64 - 2 * g) + " for this order and step count");
   if (chunk_rows < 0) return fail(h, CPML_ERR_ARG, "chunk rows must be >= 0");
-  const int jc = chunk_rows > 0 ? chunk_rows : (h->ndim == 3 ? 64 : 128);
+  const int jc = chunk_rows > 0 ? chunk_rows : (h->ndim == 3 ? 64 : 32);
   if (!chunk_fits(h, jc))
     return fail(h, CPML_ERR_ARG, "chunk too deep for the kernels' 32-bit buffer offsets");
   int rc = set_device(h);

@@ -91,12 +91,46 @@ def _medium(cpz_ext, nxy, device="cuda"):
     return out
 
 
+# Learned thin C-PML: the profile family of 2D2:283-394 / 3D:399-667 with
+# four trainable factors theta = (d0 scale, power, alpha_max scale, K_max):
+#   d(x) = s_d d0 (x/L)^p, K(x) = 1 + (K_max - 1)(x/L)^p,
+#   alpha(x) = s_a pi f0 (1 - x/L), d0 = -(2 + 1) cp ln(1e-3) / (2 L),
+# x the depth into the layer of thickness L; a, b as 2D2:388-393.
+THETA_REFERENCE = (1.0, 2.0, 1.0, 1.0)  # the reference's own profile
+
+
+def profile_theta(n: int, dx: float, dt: float, width: int, cp: float, f0: float, theta,
+                  prec: int = 4) -> np.ndarray:
+    """(6, n) profile a, b, K, a_half, b_half, K_half of a C-PML of `width`
+    cells at both ends of an axis of n points, with the factors theta."""
+    s_d, p, s_a, kmax = (float(v) for v in theta)
+    L = width * dx
+    d0 = -(2.0 + 1.0) * cp * math.log(1e-3) / (2.0 * L)
+    alpha_max = math.pi * f0
+    x = np.arange(n, dtype=np.float64) * dx
+    out = []
+    for xv in (x, x + dx / 2):
+        depth = np.maximum(L - xv, xv - ((n - 1) * dx - L))
+        inside = depth >= 0  # as 2D2:333-381: the layer's inner edge is in it
+        r = np.where(inside, depth / L, 0.0)
+        d = np.where(inside, s_d * d0 * r ** p, 0.0)
+        K = np.where(inside, 1.0 + (kmax - 1.0) * r ** p, 1.0)
+        al = np.where(inside, np.maximum(s_a * alpha_max * (1.0 - r), 0.0), 0.0)
+        b = np.exp(-(d / K + al) * dt)
+        den = K * (d + K * al)
+        a = np.where(np.abs(d) > 1e-6, d * (b - 1.0) / np.where(den != 0, den, 1.0), 0.0)
+        out += [a, b, K]
+    dt_ = np.float32 if prec == 4 else np.float64
+    return np.stack([out[0], out[1], out[2], out[3], out[4], out[5]]).astype(dt_)
+
+
 def make_run(model: Model, pad: int, absorb: int, nstep: int, ext: int = 0,
-             receivers=None, source=True):
+             receivers=None, source=True, theta=None):
     """A solver over the interior grown by ``ext`` cells of extended medium
     on every side, then ``pad`` boundary cells, of which the outer ``absorb``
-    are a C-PML (0: no absorption).  Returns (solver, offset of interior
-    cell 0, media tensors kept alive)."""
+    are a C-PML (0: no absorption; ``theta``: the learned profile factors
+    instead of the reference's profile).  Returns (solver, offset of
+    interior cell 0, media tensors kept alive)."""
     n_int = model.n + 2 * ext
     n = n_int + 2 * pad
     off = pad + ext
@@ -106,7 +140,10 @@ def make_run(model: Model, pad: int, absorb: int, nstep: int, ext: int = 0,
                   max(absorb, 1), nstep)
     s.set_medium_device(lam.data_ptr(), mu.data_ptr(), rho.data_ptr())
     cp_max = float(model.cpz.max())
-    if absorb > 0:
+    if absorb > 0 and theta is not None:
+        prof = profile_theta(n, model.dx, model.dt, absorb, cp_max, model.f0, theta)
+        s.set_pml(prof, prof, prof)
+    elif absorb > 0:
         # the C-PML of exactly `absorb` cells at the outer edge of the grid
         prof = S.pml_profile(n, model.dx, model.dt, absorb, cp_max, model.f0, 4,
                              clamp_alpha=False)
@@ -210,6 +247,66 @@ def reflection(model: Model, nstep: int, net: S.CpmlAbcNet | None = None, per_ax
         s.close()
         del keep
     return out, reference
+
+
+def reference_seismograms(model: Model, nstep: int, per_axis: int = 8):
+    """Seismograms of the doubled-domain 20-cell C-PML reference run."""
+    rec = lattice(model.n, per_axis)
+    ref, _, keep = make_run(model, NPML_WIDE, NPML_WIDE, nstep, ext=model.n // 2, receivers=rec)
+    ref.run()
+    u = _seis(ref)
+    ref.close()
+    del keep
+    return u
+
+
+def strip_reflection_db(model: Model, nstep: int, width: int, theta, reference,
+                        per_axis: int = 8, time_block: int = 0) -> float:
+    """Reflected energy (dB) of a `width`-cell C-PML strip with profile
+    factors theta against the reference seismograms."""
+    rec = lattice(model.n, per_axis)
+    s, _, keep = make_run(model, width, width, nstep, receivers=rec, theta=theta)
+    if time_block:
+        s.set_time_block(time_block)
+    s.run()
+    u = _seis(s)
+    s.close()
+    del keep
+    if not np.isfinite(u).all():
+        return 100.0
+    e_ref = float((reference.astype(np.float64) ** 2).sum())
+    err = float(((u.astype(np.float64) - reference) ** 2).sum())
+    return 10.0 * math.log10(max(err, 1e-300) / e_ref)
+
+
+def learn_profile(width: int = STRIP, n: int = 96, models: int = 4, seed: int = 2000,
+                  maxfev: int = 160, log=None):
+    """Fit theta of a `width`-cell C-PML to seeded random layered models:
+    minimise the mean reflected energy (dB) against their doubled-domain
+    20-cell C-PML references (Nelder-Mead from the reference profile).
+    Returns (theta, history)."""
+    from scipy.optimize import minimize
+
+    ms = [layered_model(n, seed + r) for r in range(models)]
+    nstep = 3 * n
+    refs = [reference_seismograms(m, nstep) for m in ms]
+    hist = []
+
+    def f(x):
+        th = (math.exp(x[0]), 1.0 + math.exp(x[1]), math.exp(x[2]), 1.0 + math.exp(x[3]))
+        db = float(np.mean([strip_reflection_db(m, nstep, width, th, r)
+                            for m, r in zip(ms, refs)]))
+        hist.append((th, db))
+        if log:
+            log(f"theta {tuple(round(v, 4) for v in th)}: {db:.2f} dB")
+        return db
+
+    x0 = np.array([0.0, 0.0, 0.0, math.log(1e-3)])  # (1, 2, 1, ~1): the reference profile
+    res = minimize(f, x0, method="Nelder-Mead",
+                   options={"maxfev": maxfev, "xatol": 1e-3, "fatol": 0.02,
+                            "initial_simplex": x0 + np.vstack([np.zeros(4), 0.5 * np.eye(4)])})
+    best = min(hist, key=lambda t: t[1])
+    return best[0], hist
 
 
 def time_to_solution(model: Model, nstep: int, net: S.CpmlAbcNet, repeats: int = 1):

@@ -216,7 +216,8 @@ static void run_solver(int prec, int ndim, int order, int tb, int fused, int lo,
   EXPECT(cpml_gpu_get_seismograms(h, sx.data(), sy.data(), ndim == 3 ? sz.data() : nullptr),
          CPML_OK);
   std::vector<double> ek(nstep), ep(nstep);
-  EXPECT(cpml_gpu_get_energy(h, ek.data(), ep.data()), CPML_OK);
+  // the energy curve is on only for the single-step runs
+  EXPECT(cpml_gpu_get_energy(h, ek.data(), ep.data()), tb ? CPML_ERR_STATE : CPML_OK);
   double v = 0;
   EXPECT(cpml_gpu_velocity_norm(h, &v), CPML_OK);
   if (!(v > 0)) ++g_fail, std::printf("FAILED: velocity norm %g\n", v);

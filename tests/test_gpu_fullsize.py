@@ -8,10 +8,13 @@ that hold at any size, since the oracle cannot run it in seconds:
   buffer offsets at 1.8 GB per field); fields are compared through 128-bit
   digests of their bytes;
 * linearity: doubling the force doubles every field exactly (x2 is exact in
-  binary floating point and every update is linear in the fields) wherever
-  the values are normal numbers; in the far tail of the wave, where values
-  fall below the smallest normal float (1.2e-38), rounding of subnormals
-  may differ by one subnormal step (seen: 6 of 4.5e8 values off by 1.4e-45);
+  binary floating point and every update is linear in the fields) except
+  where a product or sum inside an update falls below the smallest normal
+  float (1.2e-38) in one run and not in the other: there the two roundings
+  differ by a subnormal step (1.4e-45), and such absolute errors, carried
+  by the later steps, stay far below 1e-30 (seen: a handful of 4.5e8 values
+  off by 1.4e-45, one of them next to a normal value).  Values above 1e-30
+  must double bit for bit; the rest must agree to 2 x 1.2e-38;
 * a zero force leaves every field exactly zero.
 """
 import numpy as np
@@ -87,9 +90,9 @@ def test_fullsize_linearity(workload, two_phase):
             fa2, fc = 2 * a.field(name), c.field(name)
             if np.array_equal(fc, fa2):
                 continue
-            normal = np.abs(fa2) >= tiny
-            np.testing.assert_array_equal(fc[normal], fa2[normal], err_msg=f"x2 {name}")
-            assert np.abs(fc[~normal] - fa2[~normal]).max(initial=0.0) <= 2 * tiny, name
+            big = np.abs(fa2) >= 1e-30
+            np.testing.assert_array_equal(fc[big], fa2[big], err_msg=f"x2 {name}")
+            assert np.abs(fc[~big] - fa2[~big]).max(initial=0.0) <= 2 * tiny, name
     finally:
         c.close()
 

@@ -31,7 +31,9 @@ def _run(mode, tmp_path, leaks):
 
 
 def test_host_layer_under_asan_cpu(tmp_path):
-    out = _run("cpu", tmp_path, leaks=True)
+    # leak checks where no GPU driver is present (the HIP runtime keeps its
+    # own allocations until exit on a GPU machine)
+    out = _run("cpu", tmp_path, leaks=not Path("/dev/kfd").exists())
     assert (tmp_path / "x.pnm").is_file() and (tmp_path / "x.su").is_file()
     assert "0 failed" in out
 
