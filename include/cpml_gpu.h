@@ -142,6 +142,21 @@ int cpml_gpu_set_memory(cpml_solver *h, int which, const void *in);
 /* max over the grid of sqrt(vx^2 + vy^2 [+ vz^2]) (2D2:719, 3D:1185). */
 int cpml_gpu_velocity_norm(cpml_solver *h, double *vmax);
 
+/* The same norm without allocating: no counterpart in the reference; for
+ * callers that cannot allocate device memory when they call (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * d_vx, d_vy (and d_vz in 3D) are device arrays of the NX*NY[*NZ] interior
+ * points (Fortran order, precision prec); the result (a double) is written to
+ * device memory at d_vmax on `stream` (a hipStream_t; NULL = the default
+ * stream) without synchronising.  The per-block partial maxima live in a
+ * fixed 8 KiB device array of the library (one per GPU, held until the
+ * process ends); grids above 2^36 points are refused (CPML_ERR_ARG).  Calls
+ * that overlap on one GPU share that array and race: serialise them.  On
+ * error, cpml_gpu_last_error(NULL, ...) gives the message. */
+int cpml_velocity_norm_noalloc(int prec, int ndim, int NX, int NY, int NZ, const void *d_vx,
+                               const void *d_vy, const void *d_vz, double *d_vmax,
+                               void *stream);
+
 /* ---- execution options and measurement ------------------------------- */
 
 /* Steps captured per HIP graph replay (default 16; 0 = launch eagerly). */
@@ -218,7 +233,8 @@ int cpml_gpu_check_grid_array(cpml_solver *h, int kind, int rank, const int64_t 
 int cpml_gpu_guard_cells(cpml_solver *h, int op, long long *count);
 
 /* Message of the last error on this handle (h == NULL: of the last failed
- * cpml_gpu_init on this thread).  Always NUL-terminated. */
+ * cpml_gpu_init or cpml_velocity_norm_noalloc on this thread).  Always
+ * NUL-terminated. */
 int cpml_gpu_last_error(cpml_solver *h, char *buf, int len);
 
 /* ---- batched multi-shot 2D ------------------------------------------- */

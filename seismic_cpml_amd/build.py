@@ -35,7 +35,7 @@ HIPFLAGS = [
 ]
 HIP_SOURCES = ["cpml_kernels.hip", "cpml_3d4.hip", "cpml_visco2d.hip", "cpml_ade8.hip",
                "cpml_fused3d.hip", "cpml_tb2d.hip", "cpml_tb3d.hip",
-               "cpml_batch.hip", "cpml_abc.hip", "cpml_api.hip"]
+               "cpml_batch.hip", "cpml_abc.hip", "cpml_noalloc.hip", "cpml_api.hip"]
 
 
 def _run(cmd, cwd=None):

@@ -2149,6 +2149,8 @@ int cpml_courant_number(int ndim, double cp, double deltat, double deltax, doubl
 
 }  // extern "C"
 
+void cpml::set_thread_error(const std::string &msg) { g_init_error = msg; }
+
 int cpml::solver_view(cpml_solver *h, cpml::SolverView *v) {
   if (!h || !v) return CPML_ERR_ARG;
   *v = view_of(h);

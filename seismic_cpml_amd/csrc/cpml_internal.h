@@ -224,6 +224,8 @@ struct SolverView {
 
 // defined in cpml_api.hip
 int solver_view(cpml_solver *h, SolverView *v);
+// message returned by cpml_gpu_last_error(NULL, ...) on this thread
+void set_thread_error(const std::string &msg);
 // defined in cpml_abc.hip: correction buffer size, weight upload, per-step launch
 long abc_corr_floats(const SolverView &v);
 int abc_prepare(cpml_abc_net *n, const SolverView &v, std::string *err);

@@ -56,6 +56,7 @@ SIGNATURES = {
     "cpml_gpu_get_memory": [_VP, _I, _VP],
     "cpml_gpu_set_memory": [_VP, _I, _VP],
     "cpml_gpu_velocity_norm": [_VP, _DP],
+    "cpml_velocity_norm_noalloc": [_I, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP],
     "cpml_gpu_color_image": [_VP, C.c_int, C.c_void_p],
     "cpml_pnm_write": [C.c_char_p, C.c_int, C.c_int, C.c_void_p],
     "cpml_su_write": [C.c_char_p, _I, _I, C.c_double, C.c_void_p, _I],

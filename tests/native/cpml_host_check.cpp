@@ -96,6 +96,14 @@ static void check_null_handles() {
   EXPECT(cpml_batch_run(nullptr, 1, 1), CPML_ERR_ARG);
   EXPECT(cpml_batch_check_kind(nullptr, 4), CPML_ERR_ARG);
   EXPECT(cpml_batch_grid(nullptr, 0, &i, &i), CPML_ERR_ARG);
+  double dv = 0;
+  EXPECT(cpml_velocity_norm_noalloc(3, 2, 8, 8, 1, &dv, &dv, nullptr, &dv, nullptr),
+         CPML_ERR_ARG);
+  EXPECT(cpml_velocity_norm_noalloc(4, 2, 0, 8, 1, &dv, &dv, nullptr, &dv, nullptr),
+         CPML_ERR_ARG);
+  EXPECT(cpml_velocity_norm_noalloc(4, 3, 1 << 13, 1 << 13, 1 << 12, &dv, &dv, &dv, &dv,
+                                    nullptr),
+         CPML_ERR_ARG);
 }
 
 template <typename T>

@@ -213,3 +213,22 @@ def test_su_writer_rounds_and_refuses(tmp_path):
         with pytest.raises(solver.CpmlError):
             write_su(tmp_path / "c.su", *bad)
 
+
+def test_noalloc_norm_refuses_bad_arguments_before_any_gpu_call():
+    """cpml_velocity_norm_noalloc checks its arguments on the host: a grid
+    above its 2^36-point limit and an empty axis are refused with
+    CPML_ERR_ARG and a message, before any HIP call (none happens here)."""
+    import ctypes as C
+
+    from seismic_cpml_amd.solver import library
+
+    lib = library()
+    p = C.c_void_p(256)
+    assert lib.cpml_velocity_norm_noalloc(4, 3, 1 << 13, 1 << 13, 1 << 12, p, p, p, p,
+                                          None) == 1
+    buf = C.create_string_buffer(256)
+    lib.cpml_gpu_last_error(None, buf, 256)
+    assert b"2^36" in buf.value
+    assert lib.cpml_velocity_norm_noalloc(4, 3, 0, 4, 4, p, p, p, p, None) == 1
+    assert lib.cpml_velocity_norm_noalloc(8, 2, 4, 4, 1, p, None, None, p, None) == 1
+    assert lib.cpml_velocity_norm_noalloc(2, 2, 4, 4, 1, p, p, None, p, None) == 1
