@@ -177,11 +177,12 @@ int cpml_gpu_set_fused(cpml_solver *h, int on);
  * ping-pongs between two buffers: one more set of field arrays and memory
  * variables).  steps 0 turns it off.  2D (orders 2, 4, 8): at most 4 / 3 / 2
  * steps in float, 2 in double; one 64-lane wavefront per strip of
- * strip_width columns marching through chunks of chunk_rows rows (0: 32).
+ * strip_width columns marching through chunks of chunk_rows rows (0: 40).
  * 3D (order 2): at most 2 steps; a workgroup owns strip_width columns x
  * 12 (float; double: 4) rows for 2 steps (14 / 6 for 1) and marches through
- * z chunks of chunk_rows planes (0: 64).  strip_width 0 = the widest
- * multiple of 8 the ring of steps*(order-1) columns per side leaves.  Used
+ * z chunks of chunk_rows planes (0: 64).  strip_width 0 = every column the
+ * ring of steps*(order-1) columns per side leaves (3D: rounded down to a
+ * multiple of 8).  Used
  * while energy is off (set_energy cadence 0) on the elastic step without a
  * learned boundary; run() ends a block at every stability check.  Results
  * are identical to the single-step kernels. */
@@ -279,7 +280,6 @@ int cpml_batch_set_tiling(cpml_batch *b, int tile_y, int blocks_per_cu, int grap
  * precision and batch shape; a hit skips the trials (best_ms = -1). */
 int cpml_batch_autotune(cpml_batch *b, int trial_steps, const char *cache_path, int *tile_y,
                         int *blocks_per_cu, double *best_ms);
-/* Steps it_begin..it_end of every shot (2D2:545-735 per shot). */
 /* Time blocking of the whole batch: `steps` leapfrog steps of every shot per
  * launch (the kernel of cpml_gpu_set_time_block over a flat list of (shot,
  * strip, chunk) items; slab items beside the others on a second stream;
@@ -300,6 +300,7 @@ int cpml_batch_get_energy(cpml_batch *b, int shot, double *kinetic, double *pote
  * first shot above it (or NaN) fails the run with CPML_ERR_BLOWUP and its
  * index in the message.  IT_DISPLAY <= 0 disables it. */
 int cpml_batch_set_stability(cpml_batch *b, int IT_DISPLAY, double threshold);
+/* Steps it_begin..it_end of every shot (2D2:545-735 per shot). */
 int cpml_batch_run(cpml_batch *b, int it_begin, int it_end);
 int cpml_batch_get_seismograms(cpml_batch *b, int shot, void *sisvx, void *sisvy);
 /* which: CPML_VX, CPML_VY, CPML_SIGMAXX, CPML_SIGMAYY or CPML_SIGMAXY */

@@ -1172,7 +1172,7 @@ int cpml_batch_autotune(cpml_batch *b, int trial_steps, const char *cache_path, 
       for (int bpc : {0, 2}) trials.push_back(Shape{ty, bpc, 0, 128});
     const int maxnt = cpml::tb2d_max_nt(b->order, b->prec);
     for (int nt = 1; nt <= maxnt; ++nt)  // time-blocked
-      for (int jc : {16, 32, 64, 128})
+      for (int jc : {24, 40, 64, 128})
         for (int bpc : {0, 2}) trials.push_back(Shape{4, bpc, nt, jc});
     for (const Shape &c : trials) {
       int rc = cpml_batch_set_time_block(b, c.tb, 0, c.jc);
@@ -1250,9 +1250,9 @@ int cpml_batch_set_time_block(cpml_batch *b, int steps, int strip_width, int chu
     b->cur = 0;
   }
   b->tb = steps;
-  b->tb_ow = strip_width > 0 ? strip_width : std::max(8, (64 - 2 * g) / 8 * 8);
+  b->tb_ow = strip_width > 0 ? strip_width : 64 - 2 * g;  // every lane the ring leaves
   if (steps > 0 && b->tb_ow > 64 - 2 * g) b->tb_ow = 64 - 2 * g;
-  b->tb_jc = chunk_rows > 0 ? chunk_rows : 32;
+  b->tb_jc = chunk_rows > 0 ? chunk_rows : 40;
   b->dirty = true;
   return CPML_OK;
 }
