@@ -2,13 +2,13 @@
 // for gfx950: the 8th-order (Holberg) staggered operator, four Runge-Kutta
 // stages per time step and the auxiliary-differential-equation PML whose
 // coefficients depend on the stage (ADE8:692-906):
-//   per step   d2 = d3 = X and stage 1's d1                 (k_ade_copy)
-//   per stage  velocity rates from d1(sigma), ADE memories in the PML band,
-//              force at t = (it-1 + rk41) * DELTAT, rate borders zeroed at
-//              all three levels                            (k_ade_vel)
-//              stress rates from d1(v)                      (k_ade_str)
-//              X += rk42 * d2 * DELTAT, velocity borders 0, and the next
-//              stage's d1 = d3 + rk41 * d2 * DELTAT          (k_ade_acc)
+//   per step   X0 = X and stage 1's state                    (k_ade_copy)
+//   per stage  velocity rates from the stage state's sigma, ADE memories in
+//              the PML band, force at t = (it-1 + rk41) * DELTAT, rate
+//              borders zeroed; stress rates from its v; then at once
+//              X += rk42 * rate * DELTAT, velocity borders 0, and the next
+//              stage's state X0 + rk41 * rate * DELTAT         (k_ade_stage)
+// so the rates stay in registers: 5 launches per step instead of 13.
 // Arithmetic statement by statement as there (8th-order difference divided
 // by DELTA; -ffp-contract=off), so float64 results equal the reference.
 // One point per thread over the interior, 64 x 4 blocks; the ghost cells of
@@ -49,7 +49,8 @@ __device__ __forceinline__ T rk42(int inc) {
   return (inc == 0 || inc == 3) ? T(1) / T(6) : T(2) / T(6);
 }
 
-// d2 = d3 = X, then the first stage state d1 = d3 + rk41(1) * d2 * DELTAT
+// Step start: X0 = X (the program's d3, ADE8:697-703, and the level its d2
+// takes before stage 1) and stage 1's state S = X0 + rk41(1) * X * DELTAT
 template <typename T>
 __global__ __launch_bounds__(BX *ATY) void k_ade_copy(Ade8<T> a) {
   int i, j;
@@ -59,21 +60,34 @@ __global__ __launch_bounds__(BX *ATY) void k_ade_copy(Ade8<T> a) {
   const T r = rk41<T>(0);
   for (int f = 0; f < 5; ++f) {
     const T x = X[f][c];
-    a.d[1][f][c] = x;
     a.d[2][f][c] = x;
     a.d[0][f][c] = x + r * x * a.s.dt;
   }
 }
 
+// One Runge-Kutta stage in one pass (ADE8:715-874): the rates of the stage
+// state S = d[in] at this point -- velocity rates from S's stresses with
+// the ADE memories in the PML band and the stage-time force, rate borders
+// zeroed (ADE8:718-810); stress rates from S's velocities, whose border
+// values the program has zeroed by then (ADE8:803-861) -- and at once the
+// accumulation X += rk42 * rate * DELTAT with the velocity borders zeroed,
+// and the next stage's state d[out] = X0 + rk41(inc+1) * rate * DELTAT
+// (ADE8:863-874; X0's velocity borders count as zero, as the program's d3
+// after ADE8:803-810).  The rates never go to memory, and S is read from one
+// buffer while the next state goes to the other, so neighbours always see
+// the stage's own state.
 template <typename T>
-__global__ __launch_bounds__(BX *ATY) void k_ade_vel(Ade8<T> a, int inc) {
+__global__ __launch_bounds__(BX *ATY) void k_ade_stage(Ade8<T> a, int inc, int in, int out) {
   int i, j;
   if (!pt(a, i, j)) return;
   const Step2D<T> &p = a.s;
   const long ld = p.ld, c = i + j * ld;
   const int nx = p.nx, ny = p.ny, I = i + 1, J = j + 1;
   const bool bd = band(I, J, nx, ny, a.npml);
-  const T *sxx = a.d[0][2], *syy = a.d[0][3], *sxy = a.d[0][4];
+  const bool edge = i == 0 || i == nx - 1 || j == 0 || j == ny - 1;
+  const T *sxx = a.d[in][2], *syy = a.d[in][3], *sxy = a.d[in][4];
+  const T *vxs = a.d[in][0], *vys = a.d[in][1];
+  T rate[5] = {T(0), T(0), T(0), T(0), T(0)};
   // vx rates: Fortran i = 2..NX, j = 2..NY (ADE8:718-744)
   if (i >= 1 && j >= 1) {
     T vx_ = h8(sxx[c] - sxx[c - 1], sxx[c + 1] - sxx[c - 2], sxx[c + 2] - sxx[c - 3],
@@ -87,7 +101,7 @@ __global__ __launch_bounds__(BX *ATY) void k_ade_vel(Ade8<T> a, int inc) {
       vx_ = vx_ / p.px.K[i] + *m4;
       vy_ = vy_ / p.py.K[j] + *m7;
     }
-    a.d[1][0][c] = (vx_ + vy_) / p.rho[c];
+    rate[0] = (vx_ + vy_) / p.rho[c];
   }
   // vy rates: Fortran i = 1..NX-1, j = 1..NY-1 (ADE8:746-768)
   if (i <= nx - 2 && j <= ny - 2) {
@@ -103,41 +117,31 @@ __global__ __launch_bounds__(BX *ATY) void k_ade_vel(Ade8<T> a, int inc) {
       sx_ = sx_ / p.px.Kh[i] + *m6;
       sy_ = sy_ / p.py.Kh[j] + *m5;
     }
-    a.d[1][1][c] = (sx_ + sy_) / rh;
+    rate[1] = (sx_ + sy_) / rh;
   }
   // force at the stage time (ADE8:786-802)
   if (i == p.isrc && j == p.jsrc) {
     const int it = *p.step;
     const T rh = T(0.25) * (p.rho[c] + p.rho[c + 1] + p.rho[c + 1 + ld] + p.rho[c + ld]);
-    a.d[1][0][c] = a.d[1][0][c] + a.f4x[(long)(it - 1) * 4 + inc] / p.rho[c];
-    a.d[1][1][c] = a.d[1][1][c] + a.f4y[(long)(it - 1) * 4 + inc] / rh;
+    rate[0] = rate[0] + a.f4x[(long)(it - 1) * 4 + inc] / p.rho[c];
+    rate[1] = rate[1] + a.f4y[(long)(it - 1) * 4 + inc] / rh;
   }
-  // rate borders at all three levels (ADE8:803-810)
-  if (i == 0 || i == nx - 1 || j == 0 || j == ny - 1)
-    for (int l = 0; l < 3; ++l) {
-      a.d[l][0][c] = T(0);
-      a.d[l][1][c] = T(0);
-    }
-}
-
-template <typename T>
-__global__ __launch_bounds__(BX *ATY) void k_ade_str(Ade8<T> a, int inc) {
-  int i, j;
-  if (!pt(a, i, j)) return;
-  const Step2D<T> &p = a.s;
-  const long ld = p.ld, c = i + j * ld;
-  const int nx = p.nx, ny = p.ny, I = i + 1, J = j + 1;
-  const bool bd = band(I, J, nx, ny, a.npml);
-  const T *vx = a.d[0][0], *vy = a.d[0][1];
+  if (edge) rate[0] = rate[1] = T(0);  // ADE8:803-810
+  // S's velocities with the border points zero (ADE8:803-810 ran before)
+  auto V = [&](const T *f, int di, int dj) -> T {
+    const int ii = i + di, jj = j + dj;
+    if (ii == 0 || ii == nx - 1 || jj == 0 || jj == ny - 1) return T(0);
+    return f[c + di + (long)dj * ld];
+  };
   // sigma_xx, sigma_yy rates: Fortran i = 1..NX-1, j = 2..NY (ADE8:811-834)
   if (i <= nx - 2 && j >= 1) {
     const T lh = T(0.5) * (p.lam[c + 1] + p.lam[c]);
     const T mh = T(0.5) * (p.mu[c + 1] + p.mu[c]);
     const T l2m = lh + T(2) * mh;
-    T dvx_dx = h8(vx[c + 1] - vx[c], vx[c + 2] - vx[c - 1], vx[c + 3] - vx[c - 2],
-                  vx[c + 4] - vx[c - 3], a.dx);
-    T dvy_dy = h8(vy[c] - vy[c - ld], vy[c + ld] - vy[c - 2 * ld], vy[c + 2 * ld] - vy[c - 3 * ld],
-                  vy[c + 3 * ld] - vy[c - 4 * ld], a.dy);
+    T dvx_dx = h8(V(vxs, 1, 0) - V(vxs, 0, 0), V(vxs, 2, 0) - V(vxs, -1, 0),
+                  V(vxs, 3, 0) - V(vxs, -2, 0), V(vxs, 4, 0) - V(vxs, -3, 0), a.dx);
+    T dvy_dy = h8(V(vys, 0, 0) - V(vys, 0, -1), V(vys, 0, 1) - V(vys, 0, -2),
+                  V(vys, 0, 2) - V(vys, 0, -3), V(vys, 0, 3) - V(vys, 0, -4), a.dy);
     if (bd) {
       T *m0 = a.mem[0] + c, *m3 = a.mem[3] + c;
       *m0 = a.bxh[inc * nx + i] * *m0 + a.axh[inc * nx + i] * dvx_dx;
@@ -145,16 +149,16 @@ __global__ __launch_bounds__(BX *ATY) void k_ade_str(Ade8<T> a, int inc) {
       dvx_dx = dvx_dx / p.px.Kh[i] + *m0;
       dvy_dy = dvy_dy / p.py.K[j] + *m3;
     }
-    a.d[1][2][c] = l2m * dvx_dx + lh * dvy_dy;
-    a.d[1][3][c] = lh * dvx_dx + l2m * dvy_dy;
+    rate[2] = l2m * dvx_dx + lh * dvy_dy;
+    rate[3] = lh * dvx_dx + l2m * dvy_dy;
   }
   // sigma_xy rate: Fortran i = 2..NX, j = 1..NY-1 (ADE8:836-861)
   if (i >= 1 && j <= ny - 2) {
     const T mhy = T(0.5) * (p.mu[c + ld] + p.mu[c]);
-    T dvx_dy = h8(vx[c + ld] - vx[c], vx[c + 2 * ld] - vx[c - ld], vx[c + 3 * ld] - vx[c - 2 * ld],
-                  vx[c + 4 * ld] - vx[c - 3 * ld], a.dy);
-    T dvy_dx = h8(vy[c] - vy[c - 1], vy[c + 1] - vy[c - 2], vy[c + 2] - vy[c - 3],
-                  vy[c + 3] - vy[c - 4], a.dx);
+    T dvx_dy = h8(V(vxs, 0, 1) - V(vxs, 0, 0), V(vxs, 0, 2) - V(vxs, 0, -1),
+                  V(vxs, 0, 3) - V(vxs, 0, -2), V(vxs, 0, 4) - V(vxs, 0, -3), a.dy);
+    T dvy_dx = h8(V(vys, 0, 0) - V(vys, -1, 0), V(vys, 1, 0) - V(vys, -2, 0),
+                  V(vys, 2, 0) - V(vys, -3, 0), V(vys, 3, 0) - V(vys, -4, 0), a.dx);
     if (bd) {
       T *m2 = a.mem[2] + c, *m1 = a.mem[1] + c;
       *m2 = a.bx[inc * nx + i] * *m2 + a.ax[inc * nx + i] * dvy_dx;
@@ -162,29 +166,22 @@ __global__ __launch_bounds__(BX *ATY) void k_ade_str(Ade8<T> a, int inc) {
       dvy_dx = dvy_dx / p.px.K[i] + *m2;
       dvx_dy = dvx_dy / p.py.Kh[j] + *m1;
     }
-    a.d[1][4][c] = mhy * (dvy_dx + dvx_dy);
+    rate[4] = mhy * (dvy_dx + dvx_dy);
   }
-}
-
-// X += rk42(inc) * d2 * DELTAT, velocity borders zeroed (ADE8:863-874), and
-// the next stage's state d1 = d3 + rk41(inc+1) * d2 * DELTAT from the same d2
-// (inc = 3: no next stage)
-template <typename T>
-__global__ __launch_bounds__(BX *ATY) void k_ade_acc(Ade8<T> a, int inc) {
-  int i, j;
-  if (!pt(a, i, j)) return;
-  const Step2D<T> &p = a.s;
-  const long c = i + j * p.ld;
+  // accumulation and the next stage's state (ADE8:863-874)
   const T r = rk42<T>(inc);
   T *X[5] = {p.vx, p.vy, p.sxx, p.syy, p.sxy};
   const bool next = inc < 3;
   const T rn = next ? rk41<T>(inc + 1) : T(0);
   for (int f = 0; f < 5; ++f) {
-    const T d2 = a.d[1][f][c];
+    const T d2 = rate[f];
     X[f][c] = X[f][c] + r * d2 * p.dt;
-    if (next) a.d[0][f][c] = a.d[2][f][c] + rn * d2 * p.dt;
+    if (next) {
+      const T x0 = (f < 2 && edge) ? T(0) : a.d[2][f][c];
+      a.d[out][f][c] = x0 + rn * d2 * p.dt;
+    }
   }
-  if (i == 0 || i == p.nx - 1 || j == 0 || j == p.ny - 1) {
+  if (edge) {
     p.vx[c] = T(0);
     p.vy[c] = T(0);
   }
@@ -196,11 +193,9 @@ template <typename T>
 hipError_t launch_ade8_step(const Ade8<T> &a, hipStream_t s) {
   const dim3 g((a.s.nx + BX - 1) / BX, (a.s.ny + ATY - 1) / ATY), b(BX, ATY);
   hipLaunchKernelGGL(k_ade_copy<T>, g, b, 0, s, a);  // includes stage 1's state
-  for (int inc = 0; inc < 4; ++inc) {
-    hipLaunchKernelGGL(k_ade_vel<T>, g, b, 0, s, a, inc);
-    hipLaunchKernelGGL(k_ade_str<T>, g, b, 0, s, a, inc);
-    hipLaunchKernelGGL(k_ade_acc<T>, g, b, 0, s, a, inc);
-  }
+  // stage states alternate between d[0] and d[1]; d[2] holds X0
+  for (int inc = 0; inc < 4; ++inc)
+    hipLaunchKernelGGL(k_ade_stage<T>, g, b, 0, s, a, inc, inc & 1, (inc & 1) ^ 1);
   return hipGetLastError();
 }
 

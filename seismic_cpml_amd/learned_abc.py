@@ -280,15 +280,16 @@ def strip_reflection_db(model: Model, nstep: int, width: int, theta, reference,
 
 
 def learn_profile(width: int = STRIP, n: int = 96, models: int = 4, seed: int = 2000,
-                  maxfev: int = 160, log=None):
+                  maxfev: int = 160, nstep: int = 0, log=None):
     """Fit theta of a `width`-cell C-PML to seeded random layered models:
     minimise the mean reflected energy (dB) against their doubled-domain
     20-cell C-PML references (Nelder-Mead from the reference profile).
-    Returns (theta, history)."""
+    nstep (default 6n) must let the reflections come back to the receivers
+    through the slow layers.  Returns (theta, history)."""
     from scipy.optimize import minimize
 
     ms = [layered_model(n, seed + r) for r in range(models)]
-    nstep = 3 * n
+    nstep = nstep or 6 * n
     refs = [reference_seismograms(m, nstep) for m in ms]
     hist = []
 
@@ -309,20 +310,26 @@ def learn_profile(width: int = STRIP, n: int = 96, models: int = 4, seed: int = 
     return best[0], hist
 
 
-def time_to_solution(model: Model, nstep: int, net: S.CpmlAbcNet, repeats: int = 1):
+def time_to_solution(model: Model, nstep: int, net: S.CpmlAbcNet, repeats: int = 1,
+                     time_block: int = 2):
     """Wall time of nstep steps: learned strip ((n+8)^3 + corrector), the
     strip as a 4-cell C-PML and the strip without absorption ((n+8)^3)
-    against the 20-cell C-PML ((n+40)^3), all with the same step kernels and
-    graphs, no receivers."""
+    against the 20-cell C-PML ((n+40)^3), no receivers.  Every form runs its
+    fastest path: the C-PML and rigid grids `time_block` steps per launch,
+    the corrector (which acts after every step) single steps."""
     import torch
 
     res = {}
-    for name, pad, absorb, nn in (("cpml20", NPML_WIDE, NPML_WIDE, None),
-                                  ("learned4", STRIP, net.strip_cpml, net),
-                                  ("cpml4", STRIP, STRIP, None), ("rigid4", STRIP, 0, None)):
+    for name, pad, absorb, nn, tb in (("cpml20", NPML_WIDE, NPML_WIDE, None, time_block),
+                                      ("learned4", STRIP, net.strip_cpml, net, 0),
+                                      ("strip_single", STRIP, net.strip_cpml, None, 0),
+                                      ("cpml4", STRIP, STRIP, None, time_block),
+                                      ("rigid4", STRIP, 0, None, time_block)):
         s, _, keep = make_run(model, pad, absorb, nstep)
         if nn is not None:
             s.set_boundary(nn)
+        elif tb:
+            s.set_time_block(tb)
         s.run(1, min(16, nstep))  # build graphs
         best = None
         for _ in range(repeats):
