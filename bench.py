@@ -100,6 +100,51 @@ def cpu_baseline_2d(nx, ny, order, dx, dt, lam, mu, rho, px, py, fx, fy, seconds
     return out
 
 
+def cpu_baseline_ade8(w, tab, rows=256, seconds=15.0, threads=None):
+    """Time the oracle's ADE-PML RK4 loop (the 8th-order program's
+    structure: rates, stage copies and ADE memories as full arrays, OpenMP)
+    on the first `rows` rows of the same medium and stage tables."""
+    import numpy as np
+
+    orc = _orc()
+    threads = threads or host_threads()
+    nx, ny = w.nx, min(rows, w.ny)
+    lam, mu, rho = (np.pad(np.asarray(a[:ny], np.float32), ((5, 4), (5, 4)), mode="edge")
+                    for a in w.medium_host)
+    px, py = w.prof
+
+    def ghost(a):  # (4, n) or (n,) -> the program's -4..n+4 layout
+        a = np.asarray(a, np.float32)
+        return np.pad(a, ((0, 0), (5, 4)) if a.ndim == 2 else (5, 4), mode="edge")
+
+    prof = {"a_x": ghost(tab["a_x"]), "b_x": ghost(tab["b_x"]), "ah_x": ghost(tab["ah_x"]),
+            "bh_x": ghost(tab["bh_x"]), "K_x": ghost(px[2]), "Kh_x": ghost(px[5]),
+            "a_y": ghost(tab["a_y"][:, :ny]), "b_y": ghost(tab["b_y"][:, :ny]),
+            "ah_y": ghost(tab["ah_y"][:, :ny]), "bh_y": ghost(tab["bh_y"][:, :ny]),
+            "K_y": ghost(py[2][:ny]), "Kh_y": ghost(py[5][:ny])}
+    ix = np.array([nx // 2], np.int32)
+    iy = np.array([ny // 2], np.int32)
+
+    def run(nst, st):
+        return orc.runade8(nx, ny, w.dx, w.dx, w.dt, nst, w.npml, lam, mu, rho, prof, nx // 2,
+                           ny // 2, 10.0, 0.12, 1e7, 0.0, ix, iy, prec="f32", state=st,
+                           nthreads=threads)
+
+    out = run(1, None)
+    t = time.perf_counter()
+    out = run(2, out["state"])
+    t1 = (time.perf_counter() - t) / 2
+    steps = max(1, int(seconds / max(t1, 1e-4)))
+    t = time.perf_counter()
+    run(steps, out["state"])
+    el = time.perf_counter() - t
+    return {"value": round(nx * ny * steps / el / 1e9, 4), "unit": "Gcell/s", "cores": threads,
+            "kind": "port", "cpu_model": cpu_model(),
+            "sample": f"oracle C ADE-PML RK4 loop (the program's structure, OpenMP) on the first "
+                      f"{nx}x{ny} rows of the same medium and stage tables, {steps} steps, "
+                      f"{el:.1f} s"}
+
+
 def cpu_baseline(w, lam, mu, rho, nzs=128, seconds=15.0, threads=None):
     """Time the oracle's 3D loop (reference structure, full-grid memory
     variables; the 4th-order loop for an order-4 workload) on the top nzs
@@ -458,8 +503,9 @@ def bench_ade8(args):
             "bytes_model": "the program's array structure (rates stored between phases)",
             "own_algorithmic_bytes": 4 * own_words * w.cells, "own_achieved": round(own_gbs, 1),
             "own_frac": round(own_gbs / HBM_PEAK_GBS, 4)}
+    cpu = None if args.no_cpu_baseline else cpu_baseline_ade8(w, tab, seconds=args.cpu_seconds)
     return _line("Gcell-updates/s (2D 4096^2 ADE-PML RK4 8th-order FP32 time loop)", value, args,
-                 steps, wall / steps * 1e3, "f32", config, roof, None)
+                 steps, wall / steps * 1e3, "f32", config, roof, cpu)
 
 
 def bench_batch(args):
