@@ -332,7 +332,8 @@ struct cpml_batch {
   void *d_rec2 = nullptr;    // Record<T>[nshots] over pool2
   void *d_tbd[2] = {};       // TB2D<T>[nshots]: state in pool (0) / pool2 (1)
   int4 *d_items = nullptr;   // time-blocked items: the slab-free ones, then the others
-  int items_cap = 0, nitems[2] = {0, 0};
+  // time-blocked items by class: slab-free, x slabs only, y slabs only, both
+  int items_cap = 0, nitems[4] = {0, 0, 0, 0};
   int tb = 0, tb_ow = 0, tb_jc = 128;
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -576,8 +577,8 @@ int upload_tables(cpml_batch *b) {
                         hipMemcpyHostToDevice));
     }
   }
-  if (b->tb > 0) {  // items: the slab-free (strip, chunk) of every shot, then the others
-    std::vector<int4> it[2];
+  if (b->tb > 0) {  // items (strip, chunk) of every shot by class: slab-free, x, y, both
+    std::vector<int4> it[4];
     for (int q = 0; q < b->nshots; ++q) {
       const Shot &s = b->shots[q];
       int box[4];
@@ -585,20 +586,19 @@ int upload_tables(cpml_batch *b) {
       const int ns = (s.nx + b->tb_ow - 1) / b->tb_ow, nc = (s.ny + b->tb_jc - 1) / b->tb_jc;
       for (int c = 0; c < nc; ++c)
         for (int t = 0; t < ns; ++t) {
-          const bool clean = t >= box[0] && t < box[1] && c >= box[2] && c < box[3];
-          it[clean ? 0 : 1].push_back(make_int4(q, t, c, 0));
+          const int cls = (t >= box[0] && t < box[1] ? 0 : 1) | (c >= box[2] && c < box[3] ? 0 : 2);
+          it[cls].push_back(make_int4(q, t, c, 0));
         }
     }
-    b->nitems[0] = (int)it[0].size();
-    b->nitems[1] = (int)it[1].size();
-    const int n = b->nitems[0] + b->nitems[1];
+    int n = 0;
+    for (int k = 0; k < 4; ++k) n += b->nitems[k] = (int)it[k].size();
     if (n > b->items_cap) {
       bfree(b, *(void **)&b->d_items);
       int rc = balloc(b, (void **)&b->d_items, (size_t)n * sizeof(int4));
       if (rc) return rc;
       b->items_cap = n;
     }
-    it[0].insert(it[0].end(), it[1].begin(), it[1].end());
+    for (int k = 1; k < 4; ++k) it[0].insert(it[0].end(), it[k].begin(), it[k].end());
     BCHK(b, hipMemcpy(b->d_items, it[0].data(), (size_t)n * sizeof(int4), hipMemcpyHostToDevice));
   }
   if (b->cadence > 0) {  // energy tiles of 64 x 16 points over every shot's window
@@ -663,18 +663,22 @@ hipError_t bstep_t(cpml_batch *b) {
 }
 
 // NT = tb steps of every shot: the slab-free items on the main stream, the
-// others beside them on the side stream, then energy and receivers of the
-// last step (the kernel recorded the first tb - 1)
+// slab classes (both axes first, then x, then y) beside them on the side
+// stream, then energy and receivers of the last step (the kernel recorded
+// the first tb - 1)
 template <typename T>
 hipError_t bunit_t(cpml_batch *b) {
   const auto *g = static_cast<const cpml::TB2D<T> *>(b->d_tbd[b->cur]);
+  int off[4] = {0, 0, 0, 0};  // start of each class in the item list
+  for (int k = 1; k < 4; ++k) off[k] = off[k - 1] + b->nitems[k - 1];
   hipError_t e = hipEventRecord(b->ev_fork, b->stream);
   if (e == hipSuccess) e = hipStreamWaitEvent(b->side, b->ev_fork, 0);
+  for (int k : {3, 1, 2})
+    if (e == hipSuccess && b->nitems[k])
+      e = cpml::launch_tb2d_multi<T>(b->order, b->tb, k, g, b->d_items + off[k], b->nitems[k],
+                                     tb_blocks(b, b->nitems[k]), b->side);
   if (e == hipSuccess)
-    e = cpml::launch_tb2d_multi<T>(b->order, b->tb, true, g, b->d_items + b->nitems[0],
-                                   b->nitems[1], tb_blocks(b, b->nitems[1]), b->side);
-  if (e == hipSuccess)
-    e = cpml::launch_tb2d_multi<T>(b->order, b->tb, false, g, b->d_items, b->nitems[0],
+    e = cpml::launch_tb2d_multi<T>(b->order, b->tb, 0, g, b->d_items, b->nitems[0],
                                    tb_blocks(b, b->nitems[0]), b->stream);
   if (e == hipSuccess) e = hipEventRecord(b->ev_join, b->side);
   if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->ev_join, 0);

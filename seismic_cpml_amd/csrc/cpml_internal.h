@@ -300,10 +300,11 @@ template <typename T>
 hipError_t launch_tb2d(int order, int nt, const TB2D<T> &p, hipStream_t s, hipStream_t side);
 // largest nt launch_tb2d has a kernel for, per order and precision
 int tb2d_max_nt(int order, int prec);
-// many grids: items (grid, strip, chunk) of one class (slab: with C-PML code)
-// from the descriptor table `grids`, on `blocks` workgroups of 4 wavefronts
+// many grids: items (grid, strip, chunk) of one class (cls bit 0: the x
+// slabs' C-PML code, bit 1: the y slabs') from the descriptor table `grids`,
+// on `blocks` workgroups of 4 wavefronts
 template <typename T>
-hipError_t launch_tb2d_multi(int order, int nt, bool slab, const TB2D<T> *grids,
+hipError_t launch_tb2d_multi(int order, int nt, int cls, const TB2D<T> *grids,
                              const int4 *items, int nitems, int blocks, hipStream_t s);
 // strips [box[0], box[1]) x chunks [box[2], box[3]) of one grid need no C-PML code
 void tb2d_clean_box(int order, int nt, int nx, int ny, int ow, int jc, const Slab &sx,

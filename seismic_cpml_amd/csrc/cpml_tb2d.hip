@@ -83,8 +83,9 @@ __device__ __forceinline__ void push(T (&w)[N], T x) {
 
 // One work item: strip ts (columns from ts * ow) and chunk tc (rows from
 // tc * jc) of the grid described by P, advanced NT steps.
-template <typename T, int ORDER, int NT, bool SLAB>
+template <typename T, int ORDER, int NT, int CLS>
 __device__ __forceinline__ void tb2d_item(const TB2D<T> &P, int ts, int tc) {
+  constexpr bool SX = (CLS & 1) != 0, SY = (CLS & 2) != 0;  // slabs this class touches
   constexpr int r = ORDER / 2;
   constexpr int W = 2 * r + 1;                  // row window of one level
   constexpr int G = NT * (2 * r - 1);           // ring lanes on each side
@@ -104,7 +105,7 @@ __device__ __forceinline__ void tb2d_item(const TB2D<T> &P, int ts, int tc) {
   for (int k = 0; k < 9; ++k) a4[k] = ((lane + k - 4) & 63) * 4;
   auto xs = [&](T x, int k) { return k == 0 ? x : lane_of(x, a4[k + 4]); };
 
-  const bool inx = SLAB && p.sx.in(i);
+  const bool inx = SX && p.sx.in(i);
   const int sxi = inx ? p.sx.s(i) : 0;
   T pxa = T(0), pxb = T(0), pxK = T(1), pxah = T(0), pxbh = T(0), pxKh = T(1);
   if (inx) {
@@ -177,25 +178,25 @@ __device__ __forceinline__ void tb2d_item(const TB2D<T> &P, int ts, int tc) {
     L.lam = okm ? p.lam[at(jm)] : T(0);
     L.mu = okm ? p.mu[at(jm)] : T(0);
     L.rho = okm ? p.rho[at(jm)] : T(0);
-    if (SLAB) {
+    {
 #pragma unroll
       for (int q = 0; q < 4; ++q) L.px[q] = L.py[q] = T(0);
-      if (inx && row_in(s)) {
+      if (SX && inx && row_in(s)) {
         const long o = sxi + (long)p.sx.w * s;
         L.px[0] = p.mx[0][o];
         L.px[1] = p.mx[1][o];
       }
-      if (inx && row_in(jq)) {
+      if (SX && inx && row_in(jq)) {
         const long o = sxi + (long)p.sx.w * jq;
         L.px[2] = p.mx[2][o];
         L.px[3] = p.mx[3][o];
       }
-      if (col_in && p.sy.in(s)) {
+      if (SY && col_in && p.sy.in(s)) {
         const long o = i + p.mld * p.sy.s(s);
         L.py[0] = p.my[0][o];
         L.py[1] = p.my[1][o];
       }
-      if (col_in && p.sy.in(jq)) {
+      if (SY && col_in && p.sy.in(jq)) {
         const long o = i + p.mld * p.sy.s(jq);
         L.py[2] = p.my[2][o];
         L.py[3] = p.my[3][o];
@@ -246,15 +247,15 @@ __device__ __forceinline__ void tb2d_item(const TB2D<T> &P, int ts, int tc) {
         const T(&wy)[W] = V[m - 1][1];
         const int j = pr;
         const bool inside = col_in && row_in(j);
-        const bool iny = SLAB && p.sy.in(j);
+        const bool iny = SY && p.sy.in(j);
         T ox0 = T(0), ox1 = T(0), oy0 = T(0), oy1 = T(0);
-        if (SLAB) {
-          if (m == 1) {
-            ox0 = C.px[0], ox1 = C.px[1], oy0 = C.py[0], oy1 = C.py[1];
-          } else {
-            ox0 = PX[mb][0][0], ox1 = PX[mb][1][0];
-            oy0 = PY[mb][0][0], oy1 = PY[mb][1][0];
-          }
+        if (SX) {
+          ox0 = m == 1 ? C.px[0] : PX[mb][0][0];
+          ox1 = m == 1 ? C.px[1] : PX[mb][1][0];
+        }
+        if (SY) {
+          oy0 = m == 1 ? C.py[0] : PY[mb][0][0];
+          oy1 = m == 1 ? C.py[1] : PY[mb][1][0];
         }
         npx0 = ox0, npx1 = ox1, npy0 = oy0, npy1 = oy1;
         T pya = T(0), pyb = T(0), pyK = T(1), pyah = T(0), pybh = T(0), pyKh = T(1);
@@ -276,10 +277,8 @@ __device__ __forceinline__ void tb2d_item(const TB2D<T> &P, int ts, int tc) {
           T dvx_dx = ddx_vx / p.den_x;
           // Op(vy, c - ld, ld): rows j-1 .. : g(k) = vy(j - 1 + k) = wy[r - 1 + k]
           T dvy_dy = dop<T, ORDER>([&](int k) { return wy[r - 1 + k]; }) / p.den_y;
-          if (SLAB) {
-            dvx_dx = pml(inx, dvx_dx, ox0, npx0, pxah, pxbh, pxKh);
-            dvy_dy = pml(iny, dvy_dy, oy0, npy0, pya, pyb, pyK);
-          }
+          if (SX) dvx_dx = pml(inx, dvx_dx, ox0, npx0, pxah, pxbh, pxKh);
+          if (SY) dvy_dy = pml(iny, dvy_dy, oy0, npy0, pya, pyb, pyK);
           sxx = sxx + (lp2m * dvx_dx + lhx * dvy_dy) * p.dt;
           syy = syy + (lhx * dvx_dx + lp2m * dvy_dy) * p.dt;
         }
@@ -287,20 +286,20 @@ __device__ __forceinline__ void tb2d_item(const TB2D<T> &P, int ts, int tc) {
           const T mhy = T(0.5) * (mu_n + mu);
           T dvy_dx = ddx_vy / p.den_x;
           T dvx_dy = dop<T, ORDER>([&](int k) { return wx[r + k]; }) / p.den_y;
-          if (SLAB) {
-            dvy_dx = pml(inx, dvy_dx, ox1, npx1, pxa, pxb, pxK);
-            // 2D4:596 divides by K_y(j) where 2D2:595 uses K_y_half(j)
-            dvx_dy = pml(iny, dvx_dy, oy1, npy1, pyah, pybh, ORDER == 4 ? pyK : pyKh);
-          }
+          if (SX) dvy_dx = pml(inx, dvy_dx, ox1, npx1, pxa, pxb, pxK);
+          // 2D4:596 divides by K_y(j) where 2D2:595 uses K_y_half(j)
+          if (SY) dvx_dy = pml(iny, dvx_dy, oy1, npy1, pyah, pybh, ORDER == 4 ? pyK : pyKh);
           sxy = sxy + mhy * (dvy_dx + dvx_dy) * p.dt;
         }
       }
       push(S[m - 1][0], sxx);
       push(S[m - 1][1], syy);
       push(S[m - 1][2], sxy);
-      if (SLAB && m < NT) {
+      if (SX && m < NT) {
         push(PX[m - 1][0], npx0);
         push(PX[m - 1][1], npx1);
+      }
+      if (SY && m < NT) {
         push(PY[m - 1][0], npy0);
         push(PY[m - 1][1], npy1);
       }
@@ -309,13 +308,13 @@ __device__ __forceinline__ void tb2d_item(const TB2D<T> &P, int ts, int tc) {
         P.sxx[c] = sxx;
         P.syy[c] = syy;
         P.sxy[c] = sxy;
-        if (SLAB) {
-          if (inx) {
+        {
+          if (SX && inx) {
             const long o = sxi + (long)p.sx.w * pr;
             P.mx[0][o] = npx0;
             P.mx[1][o] = npx1;
           }
-          if (p.sy.in(pr)) {
+          if (SY && p.sy.in(pr)) {
             const long o = i + p.mld * p.sy.s(pr);
             P.my[0][o] = npy0;
             P.my[1][o] = npy1;
@@ -329,16 +328,16 @@ __device__ __forceinline__ void tb2d_item(const TB2D<T> &P, int ts, int tc) {
         const T(&wxy)[W] = S[m - 1][2];
         const int j = qr;
         const bool inside = col_in && row_in(j);
-        const bool iny = SLAB && p.sy.in(j);
+        const bool iny = SY && p.sy.in(j);
         T vxn = V[m - 1][0][0], vyn = V[m - 1][1][0];
         T ox2 = T(0), ox3 = T(0), oy2 = T(0), oy3 = T(0);
-        if (SLAB) {
-          if (m == 1) {
-            ox2 = C.px[2], ox3 = C.px[3], oy2 = C.py[2], oy3 = C.py[3];
-          } else {
-            ox2 = PX[mb][2][0], ox3 = PX[mb][3][0];
-            oy2 = PY[mb][2][0], oy3 = PY[mb][3][0];
-          }
+        if (SX) {
+          ox2 = m == 1 ? C.px[2] : PX[mb][2][0];
+          ox3 = m == 1 ? C.px[3] : PX[mb][3][0];
+        }
+        if (SY) {
+          oy2 = m == 1 ? C.py[2] : PY[mb][2][0];
+          oy3 = m == 1 ? C.py[3] : PY[mb][3][0];
         }
         T npx2 = ox2, npx3 = ox3, npy2 = oy2, npy3 = oy3;
         T pya = T(0), pyb = T(0), pyK = T(1), pyah = T(0), pybh = T(0), pyKh = T(1);
@@ -354,20 +353,16 @@ __device__ __forceinline__ void tb2d_item(const TB2D<T> &P, int ts, int tc) {
         if (inside && i >= 1 && j >= 1) {
           T dsxx_dx = ddx_sxx / p.den_x;
           T dsxy_dy = dop<T, ORDER>([&](int k) { return wxy[r - 1 + k]; }) / p.den_y;
-          if (SLAB) {
-            dsxx_dx = pml(inx, dsxx_dx, ox2, npx2, pxa, pxb, pxK);
-            dsxy_dy = pml(iny, dsxy_dy, oy2, npy2, pya, pyb, pyK);
-          }
+          if (SX) dsxx_dx = pml(inx, dsxx_dx, ox2, npx2, pxa, pxb, pxK);
+          if (SY) dsxy_dy = pml(iny, dsxy_dy, oy2, npy2, pya, pyb, pyK);
           vxn = vxn + (dsxx_dx + dsxy_dy) * p.dt / rho;
         }
         const T rhh = T(0.25) * (rho + rho_e + rho_ne + rho_n);
         if (inside && i <= nx - 2 && j <= ny - 2) {
           T dsxy_dx = ddx_sxy / p.den_x;
           T dsyy_dy = dop<T, ORDER>([&](int k) { return wyy[r + k]; }) / p.den_y;
-          if (SLAB) {
-            dsxy_dx = pml(inx, dsxy_dx, ox3, npx3, pxah, pxbh, pxKh);
-            dsyy_dy = pml(iny, dsyy_dy, oy3, npy3, pyah, pybh, pyKh);
-          }
+          if (SX) dsxy_dx = pml(inx, dsxy_dx, ox3, npx3, pxah, pxbh, pxKh);
+          if (SY) dsyy_dy = pml(iny, dsyy_dy, oy3, npy3, pyah, pybh, pyKh);
           vyn = vyn + (dsxy_dx + dsyy_dy) * p.dt / rhh;
         }
         // point force at step it0 + m - 1, after the update (2D2:660-667)
@@ -387,9 +382,11 @@ __device__ __forceinline__ void tb2d_item(const TB2D<T> &P, int ts, int tc) {
         if (m < NT) {
           push(V[m][0], vxn);
           push(V[m][1], vyn);
-          if (SLAB) {
+          if (SX) {
             push(PX[m - 1][2], npx2);
             push(PX[m - 1][3], npx3);
+          }
+          if (SY) {
             push(PY[m - 1][2], npy2);
             push(PY[m - 1][3], npy3);
           }
@@ -409,13 +406,13 @@ __device__ __forceinline__ void tb2d_item(const TB2D<T> &P, int ts, int tc) {
           const long c = at(j);
           P.vx[c] = vxn;
           P.vy[c] = vyn;
-          if (SLAB) {
-            if (inx) {
+          {
+            if (SX && inx) {
               const long o = sxi + (long)p.sx.w * j;
               P.mx[2][o] = npx2;
               P.mx[3][o] = npx3;
             }
-            if (p.sy.in(j)) {
+            if (SY && p.sy.in(j)) {
               const long o = i + p.mld * p.sy.s(j);
               P.my[2][o] = npy2;
               P.my[3][o] = npy3;
@@ -429,13 +426,22 @@ __device__ __forceinline__ void tb2d_item(const TB2D<T> &P, int ts, int tc) {
   }
 }
 
-template <typename T, int ORDER, int NT, bool SLAB>
+// Items of class CLS of one grid: strips clean along x ([sx0, sx1)) or not
+// (bit 0), chunks clean along y ([cy0, cy1)) or not (bit 1), x fastest; bit 2:
+// every item outside the clean box, with both slabs' code.
+__device__ __forceinline__ int outside(int u, int c0, int c1) { return u < c0 ? u : u - c0 + c1; }
+template <int CLS>
+__device__ __forceinline__ int tb2d_class_items(int nstrips, int nchunks, int bx, int cy) {
+  if (CLS & 4) return nstrips * nchunks - bx * cy;
+  const int sx = (CLS & 1) ? nstrips - bx : bx, sy = (CLS & 2) ? nchunks - cy : cy;
+  return sx * sy;
+}
+
+template <typename T, int ORDER, int NT, int CLS>
 __global__ __launch_bounds__(64 * WPB) void k_tb2d(TB2D<T> P) {
   const int wv = threadIdx.x >> 6;
-  int nitem;
   const int bx = P.sx1 - P.sx0, cy = P.cy1 - P.cy0;
-  if (!SLAB) nitem = bx * cy;
-  else nitem = P.nstrips * P.nchunks - bx * cy;
+  const int nitem = tb2d_class_items<CLS>(P.nstrips, P.nchunks, bx, cy);
   // XCD-aware order: each XCD takes one contiguous band of items (strip fastest)
   const int nblk = (nitem + WPB - 1) / WPB;
   const int xcd = blockIdx.x & 7, loc = blockIdx.x >> 3;
@@ -444,10 +450,7 @@ __global__ __launch_bounds__(64 * WPB) void k_tb2d(TB2D<T> P) {
   const int item = __builtin_amdgcn_readfirstlane(blk * WPB + wv);
   if (item >= nitem) return;
   int ts, tc;
-  if (!SLAB) {
-    ts = P.sx0 + item % bx;
-    tc = P.cy0 + item / bx;
-  } else {
+  if (CLS & 4) {
     // chunks below the box, the box's chunks outside its strips, chunks above
     const int n1 = P.cy0 * P.nstrips, wdt = P.nstrips - bx, n2 = cy * wdt;
     if (item < n1) {
@@ -456,22 +459,26 @@ __global__ __launch_bounds__(64 * WPB) void k_tb2d(TB2D<T> P) {
     } else if (item < n1 + n2) {
       const int t = item - n1;
       tc = P.cy0 + t / wdt;
-      const int c = t % wdt;
-      ts = c < P.sx0 ? c : c - P.sx0 + P.sx1;
+      ts = outside(t % wdt, P.sx0, P.sx1);
     } else {
       const int t = item - n1 - n2;
       tc = P.cy1 + t / P.nstrips;
       ts = t % P.nstrips;
     }
+  } else {
+    const int wx = (CLS & 1) ? P.nstrips - bx : bx;
+    const int us = item % wx, uc = item / wx;
+    ts = (CLS & 1) ? outside(us, P.sx0, P.sx1) : P.sx0 + us;
+    tc = (CLS & 2) ? outside(uc, P.cy0, P.cy1) : P.cy0 + uc;
   }
-  tb2d_item<T, ORDER, NT, SLAB>(P, ts, tc);
+  tb2d_item<T, ORDER, NT, CLS & 3>(P, ts, tc);
 }
 
 // Many grids in one launch: items[k] = (grid, strip, chunk) of one class,
 // split into 8 contiguous bands (one per XCD); the wavefronts of the blocks
 // an XCD receives stride through its band (one item each when the grid has
 // one wavefront per item, several when it is a persistent grid).
-template <typename T, int ORDER, int NT, bool SLAB>
+template <typename T, int ORDER, int NT, int CLS>
 __global__ __launch_bounds__(64 * WPB) void k_tb2d_multi(const TB2D<T> *__restrict__ grids,
                                                          const int4 *__restrict__ items,
                                                          int nitems) {
@@ -483,7 +490,7 @@ __global__ __launch_bounds__(64 * WPB) void k_tb2d_multi(const TB2D<T> *__restri
   const int wv = threadIdx.x >> 6;
   for (int u = __builtin_amdgcn_readfirstlane((h >> 3) * WPB + wv); u < count; u += nb * WPB) {
     const int4 e = items[start + u];
-    tb2d_item<T, ORDER, NT, SLAB>(grids[e.x], e.y, e.z);
+    tb2d_item<T, ORDER, NT, CLS>(grids[e.x], e.y, e.z);
   }
 }
 
@@ -513,15 +520,17 @@ hipError_t tb2d_launch(const TB2D<T> &P0, hipStream_t s, hipStream_t side) {
   clean_range(p.nx, P.ow, G, p.sx.wl, p.sx.wr, P.nstrips, P.sx0, P.sx1);
   clean_range(p.ny, P.jc, (2 * NT - 1) * r + r, p.sy.wl, p.sy.wr, P.nchunks, P.cy0, P.cy1);
   if (P.sx1 <= P.sx0 || P.cy1 <= P.cy0) P.sx0 = P.sx1 = P.cy0 = P.cy1 = 0;
+  // One grid has few slab items, each a serial march: one launch of all of
+  // them (both slabs' code) beside the slab-free items keeps the slab side
+  // to one march's latency.  (The batch splits its many slab items by axis.)
   const int nclean = (P.sx1 - P.sx0) * (P.cy1 - P.cy0);
   const int nslab = P.nstrips * P.nchunks - nclean;
   const dim3 blk(64 * WPB);
-  if (nslab)  // on the side stream when given: beside the slab-free items
-    hipLaunchKernelGGL((k_tb2d<T, ORDER, NT, true>), dim3((nslab + WPB - 1) / WPB), blk, 0,
+  if (nslab)
+    hipLaunchKernelGGL((k_tb2d<T, ORDER, NT, 7>), dim3((nslab + WPB - 1) / WPB), blk, 0,
                        side ? side : s, P);
   if (nclean)
-    hipLaunchKernelGGL((k_tb2d<T, ORDER, NT, false>), dim3((nclean + WPB - 1) / WPB), blk, 0, s,
-                       P);
+    hipLaunchKernelGGL((k_tb2d<T, ORDER, NT, 0>), dim3((nclean + WPB - 1) / WPB), blk, 0, s, P);
   return hipGetLastError();
 }
 
@@ -561,31 +570,35 @@ template hipError_t launch_tb2d<double>(int, int, const TB2D<double> &, hipStrea
 
 namespace {
 template <typename T, int ORDER, int NT>
-void tb2d_multi(bool slab, const TB2D<T> *grids, const int4 *items, int nitems, int blocks,
+void tb2d_multi(int cls, const TB2D<T> *grids, const int4 *items, int nitems, int blocks,
                 hipStream_t s) {
   const dim3 g(blocks), b(64 * WPB);
-  if (slab) hipLaunchKernelGGL((k_tb2d_multi<T, ORDER, NT, true>), g, b, 0, s, grids, items, nitems);
-  else hipLaunchKernelGGL((k_tb2d_multi<T, ORDER, NT, false>), g, b, 0, s, grids, items, nitems);
+  switch (cls) {
+    case 0: hipLaunchKernelGGL((k_tb2d_multi<T, ORDER, NT, 0>), g, b, 0, s, grids, items, nitems); break;
+    case 1: hipLaunchKernelGGL((k_tb2d_multi<T, ORDER, NT, 1>), g, b, 0, s, grids, items, nitems); break;
+    case 2: hipLaunchKernelGGL((k_tb2d_multi<T, ORDER, NT, 2>), g, b, 0, s, grids, items, nitems); break;
+    default: hipLaunchKernelGGL((k_tb2d_multi<T, ORDER, NT, 3>), g, b, 0, s, grids, items, nitems); break;
+  }
 }
 }  // namespace
 
 template <typename T>
-hipError_t launch_tb2d_multi(int order, int nt, bool slab, const TB2D<T> *grids,
+hipError_t launch_tb2d_multi(int order, int nt, int cls, const TB2D<T> *grids,
                              const int4 *items, int nitems, int blocks, hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
   if (nt < 1 || nt > tb2d_max_nt(order, sizeof(T)) || blocks < 1) return hipErrorInvalidValue;
   switch (order * 10 + nt) {
-    case 21: tb2d_multi<T, 2, 1>(slab, grids, items, nitems, blocks, s); break;
-    case 22: tb2d_multi<T, 2, 2>(slab, grids, items, nitems, blocks, s); break;
-    case 41: tb2d_multi<T, 4, 1>(slab, grids, items, nitems, blocks, s); break;
-    case 42: tb2d_multi<T, 4, 2>(slab, grids, items, nitems, blocks, s); break;
-    case 81: tb2d_multi<T, 8, 1>(slab, grids, items, nitems, blocks, s); break;
-    case 82: tb2d_multi<T, 8, 2>(slab, grids, items, nitems, blocks, s); break;
+    case 21: tb2d_multi<T, 2, 1>(cls, grids, items, nitems, blocks, s); break;
+    case 22: tb2d_multi<T, 2, 2>(cls, grids, items, nitems, blocks, s); break;
+    case 41: tb2d_multi<T, 4, 1>(cls, grids, items, nitems, blocks, s); break;
+    case 42: tb2d_multi<T, 4, 2>(cls, grids, items, nitems, blocks, s); break;
+    case 81: tb2d_multi<T, 8, 1>(cls, grids, items, nitems, blocks, s); break;
+    case 82: tb2d_multi<T, 8, 2>(cls, grids, items, nitems, blocks, s); break;
     default:
       if constexpr (sizeof(T) == 4) {
-        if (order * 10 + nt == 23) tb2d_multi<T, 2, 3>(slab, grids, items, nitems, blocks, s);
-        else if (order * 10 + nt == 24) tb2d_multi<T, 2, 4>(slab, grids, items, nitems, blocks, s);
-        else if (order * 10 + nt == 43) tb2d_multi<T, 4, 3>(slab, grids, items, nitems, blocks, s);
+        if (order * 10 + nt == 23) tb2d_multi<T, 2, 3>(cls, grids, items, nitems, blocks, s);
+        else if (order * 10 + nt == 24) tb2d_multi<T, 2, 4>(cls, grids, items, nitems, blocks, s);
+        else if (order * 10 + nt == 43) tb2d_multi<T, 4, 3>(cls, grids, items, nitems, blocks, s);
         else return hipErrorInvalidValue;
       } else {
         return hipErrorInvalidValue;
@@ -594,9 +607,9 @@ hipError_t launch_tb2d_multi(int order, int nt, bool slab, const TB2D<T> *grids,
   return hipGetLastError();
 }
 
-template hipError_t launch_tb2d_multi<float>(int, int, bool, const TB2D<float> *, const int4 *,
+template hipError_t launch_tb2d_multi<float>(int, int, int, const TB2D<float> *, const int4 *,
                                              int, int, hipStream_t);
-template hipError_t launch_tb2d_multi<double>(int, int, bool, const TB2D<double> *, const int4 *,
+template hipError_t launch_tb2d_multi<double>(int, int, int, const TB2D<double> *, const int4 *,
                                               int, int, hipStream_t);
 
 // Strips / chunks of one grid whose ring avoids the slabs (the box of
