@@ -23,10 +23,18 @@ namespace {
 
 constexpr int ATY = 4;
 
+// XCD-aware tile order (one-dimensional grid): the hardware hands workgroup h
+// to XCD h % 8; each XCD gets one contiguous band of 64 x ATY tiles (x
+// fastest), so the halo columns and rows that neighbouring tiles share are
+// read through the same L2.
 template <typename T>
 __device__ __forceinline__ bool pt(const Ade8<T> &a, int &i, int &j) {
-  i = blockIdx.x * BX + threadIdx.x;
-  j = blockIdx.y * ATY + threadIdx.y;
+  const int ntx = (a.s.nx + BX - 1) / BX, nt = ntx * ((a.s.ny + ATY - 1) / ATY);
+  const int h = blockIdx.x, xcd = h & 7, loc = h >> 3;
+  const int q = nt >> 3, r = nt & 7;
+  const int t = xcd * q + min(xcd, r) + loc;
+  i = (t % ntx) * BX + threadIdx.x;
+  j = (t / ntx) * ATY + threadIdx.y;
   return i < a.s.nx && j < a.s.ny;
 }
 
@@ -191,7 +199,7 @@ __global__ __launch_bounds__(BX *ATY) void k_ade_stage(Ade8<T> a, int inc, int i
 
 template <typename T>
 hipError_t launch_ade8_step(const Ade8<T> &a, hipStream_t s) {
-  const dim3 g((a.s.nx + BX - 1) / BX, (a.s.ny + ATY - 1) / ATY), b(BX, ATY);
+  const dim3 g(((a.s.nx + BX - 1) / BX) * ((a.s.ny + ATY - 1) / ATY)), b(BX, ATY);
   hipLaunchKernelGGL(k_ade_copy<T>, g, b, 0, s, a);  // includes stage 1's state
   // stage states alternate between d[0] and d[1]; d[2] holds X0
   for (int inc = 0; inc < 4; ++inc)
