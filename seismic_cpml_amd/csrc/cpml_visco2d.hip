@@ -31,10 +31,17 @@ namespace {
 
 constexpr int VTY = 4;
 
+// XCD-aware tile order (one-dimensional grid): workgroup h runs on XCD h % 8;
+// each XCD gets one contiguous band of 64 x VTY tiles (x fastest), so the
+// halo rows and columns neighbouring tiles share are read through one L2.
 template <typename T>
 __device__ __forceinline__ bool point2(const Visco2D<T> &v, int &i, int &j) {
-  i = blockIdx.x * BX + threadIdx.x;
-  j = blockIdx.y * VTY + threadIdx.y;
+  const int ntx = (v.s.nx + BX - 1) / BX, nt = ntx * ((v.s.ny + VTY - 1) / VTY);
+  const int h = blockIdx.x, xcd = h & 7, loc = h >> 3;
+  const int q = nt >> 3, r = nt & 7;
+  const int t = xcd * q + min(xcd, r) + loc;
+  i = (t % ntx) * BX + threadIdx.x;
+  j = (t / ntx) * VTY + threadIdx.y;
   return i < v.s.nx && j < v.s.ny;
 }
 
@@ -181,7 +188,7 @@ __global__ void k_pressure2d(Visco2D<T> v) {
 
 template <typename T>
 hipError_t launch_stress2dve(int order, const Visco2D<T> &v, hipStream_t s) {
-  const dim3 g((v.s.nx + BX - 1) / BX, (v.s.ny + VTY - 1) / VTY);
+  const dim3 g(((v.s.nx + BX - 1) / BX) * ((v.s.ny + VTY - 1) / VTY));
   if (order == 4)
     hipLaunchKernelGGL((k_stress2dve<T, 4>), g, dim3(BX, VTY), 0, s, v);
   else
@@ -191,7 +198,7 @@ hipError_t launch_stress2dve(int order, const Visco2D<T> &v, hipStream_t s) {
 
 template <typename T>
 hipError_t launch_velocity2dve(int order, const Visco2D<T> &v, hipStream_t s) {
-  const dim3 g((v.s.nx + BX - 1) / BX, (v.s.ny + VTY - 1) / VTY);
+  const dim3 g(((v.s.nx + BX - 1) / BX) * ((v.s.ny + VTY - 1) / VTY));
   if (order == 4)
     hipLaunchKernelGGL((k_velocity2dve<T, 4>), g, dim3(BX, VTY), 0, s, v);
   else
