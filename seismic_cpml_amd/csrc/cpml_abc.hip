@@ -31,7 +31,8 @@
 // over a mini-batch is sum (y - target)^2 / max(sum target^2, 1% of the
 // input energy).
 // Inference runs the same network with BF16 weights and activations and FP32
-// accumulation on the MFMA units (v_mfma_f32_16x16x16_bf16).
+// accumulation on the MFMA units (v_mfma_f32_16x16x32_bf16: two taps of 16
+// channels per instruction).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -60,6 +61,8 @@ constexpr int WL = TAPS * C * C;  // weights of one layer of one face: [27][16 i
 constexpr int WF = 3 * WL;        // one face
 constexpr int WN = 6 * WF;        // all faces
 constexpr int DMAX1 = SO + 2, DMAX2 = SO + 1, DMAX3 = SO;
+constexpr int PAIRS = (TAPS + 1) / 2;  // MFMA K = 32: taps (2p, 2p + 1); tap 27 is zero padding
+constexpr int WBN = 6 * 3 * PAIRS * 64 * 8;  // BF16 B-operand fragments of all faces and layers
 constexpr int NCHUNK = 64;        // cell chunks of the weight-gradient reduction
 constexpr int TB = 256;
 
@@ -373,14 +376,19 @@ __global__ void k_adam(float *W, const float *G, Scales gscale, float *m, float 
   W[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
 }
 
-// FP32 master weights -> BF16 MFMA B-operand fragments:
-// Wb[face][layer][tap][lane][j] = W[face][layer][tap][k = 4*(lane/16) + j][n = lane%16]
+// FP32 master weights -> BF16 MFMA B-operand fragments of the 16x16x32
+// instruction, one per tap pair: lane l holds output channel n = l%16 and
+// the 8 K values of its lane group g = l/16, i.e. input channels
+// 8*(g%2) .. +7 of tap 2p + g/2 (zero for the padding tap 27):
+// Wb[face][layer][p][lane][j] = W[face][layer][2p + g/2][8*(g%2) + j][n]
 __global__ void k_to_frag(const float *W, unsigned short *Wb) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= WN) return;
-  const int j = e & 3, lane = (e >> 2) & 63, flt = e >> 8;  // flt = (face*3+layer)*27+tap
-  const int k = 4 * (lane >> 4) + j, n = lane & 15;
-  Wb[e] = to_bf16(W[(long)flt * C * C + k * C + n]);
+  if (e >= WBN) return;
+  const int j = e & 7, lane = (e >> 3) & 63, rest = e >> 9;
+  const int p = rest % PAIRS, fl = rest / PAIRS;  // fl = face * 3 + layer
+  const int g = lane >> 4, tap = 2 * p + (g >> 1), k = 8 * (g & 1) + j, n = lane & 15;
+  Wb[e] = tap < TAPS ? to_bf16(W[((long)fl * TAPS + tap) * C * C + k * C + n])
+                     : (unsigned short)0;
 }
 
 // 7-deep layer-3 output -> planar correction [face][field][4-deep cell]
@@ -396,7 +404,7 @@ __global__ __launch_bounds__(TB) void k_y_to_corr(Geo g, const float *Y, float *
 
 // ----------------------------------------------------------- inference --
 
-typedef short s4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int TT = 8;                          // tile: 8 x 8 columns of the face
@@ -408,23 +416,26 @@ static_assert(H2N + 1 <= XN + 1, "h2 reuses the input buffer");
 
 // One layer on the MFMA units.  Output region OA x OA x OD (a fastest, then b,
 // then d) of the tile, centred like the input region IA = OA + 2; input cells
-// at depth -1 read the zero cell `izero`.  Lane l of a wave supplies row
-// m = l%16 (an output cell) and channels 4*(l/16)..+3 of A, and holds
-// D[m = 4*(l/16) + r][n = l%16].  Each wave keeps GPW groups of 16 output
-// cells in flight, so GPW independent MFMA chains hide the MFMA latency.
+// at depth -1 (and the padding tap) read the zero cell `izero`.  One
+// 16x16x32 MFMA covers a tap pair: lane l of a wave supplies row m = l%16 (an
+// output cell) of A with input channels 8*(g%2)..+7 of tap 2p + g/2
+// (g = l/16), one 16-byte LDS read, and holds D[m = 4*g + r][n = l%16].
+// Each wave keeps GPW groups of 16 output cells in flight, so GPW
+// independent MFMA chains hide the MFMA latency.
 template <int OA, int OD, bool LAST>
 __device__ __forceinline__ void mfma_layer(const unsigned short *in, int izero,
-                                           unsigned short *out, const s4 *__restrict__ wf,
+                                           unsigned short *out, const bf8 *__restrict__ wf,
                                            const Face &F, int a0, int b0, float *corr) {
   constexpr int IA = OA + 2;
   constexpr int ON = OA * OA * OD, NG = (ON + 15) / 16;
   constexpr int HALO = (OA - TT) / 2;
   constexpr int NW = TBI / 64, GPW = 4;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  s4 bw[TAPS];
+  bf8 bw[PAIRS];
 #pragma unroll
-  for (int t = 0; t < TAPS; ++t) bw[t] = wf[t * 64 + lane];
-  const int kq = (lane >> 4) * 4;
+  for (int p = 0; p < PAIRS; ++p) bw[p] = wf[p * 64 + lane];
+  const int hi = lane >> 5;             // second tap of the pair
+  const int kq = ((lane >> 4) & 1) * 8;  // channel half
   for (int g0 = wave * GPW; g0 < NG; g0 += NW * GPW) {
     // GPW independent 16-cell groups; a group past the layer's last one
     // recomputes the last group (no branch inside the tap loop) and is not
@@ -441,25 +452,26 @@ __device__ __forceinline__ void mfma_layer(const unsigned short *in, int izero,
       top[u] = d == 0;
       acc[u] = f4{0.f, 0.f, 0.f, 0.f};
     }
-    auto a_of = [&](int u, int t) -> s4 {
+    auto a_of = [&](int u, int p) -> bf8 {
+      const int t = 2 * p + hi;
       const int off = (t / 9 - 1) * IA * IA + (t % 3 - 1) * IA + ((t / 3) % 3 - 1);
-      const int ic = (t < 9 && top[u]) ? izero : base[u] + off;
-      return *reinterpret_cast<const s4 *>(in + ic * C + kq);
+      const int ic = (t >= TAPS || (t < 9 && top[u])) ? izero : base[u] + off;
+      return *reinterpret_cast<const bf8 *>(in + ic * C + kq);
     };
-    // A operands of tap t + 1 are read from LDS while the MFMAs of tap t run
-    s4 cur[GPW], nxt[GPW];
+    // A operands of pair p + 1 are read from LDS while the MFMAs of pair p run
+    bf8 cur[GPW], nxt[GPW];
 #pragma unroll
     for (int u = 0; u < GPW; ++u) cur[u] = a_of(u, 0);
 #pragma unroll
-    for (int t = 0; t < TAPS; ++t) {
-      if (t + 1 < TAPS) {
+    for (int p = 0; p < PAIRS; ++p) {
+      if (p + 1 < PAIRS) {
 #pragma unroll
-        for (int u = 0; u < GPW; ++u) nxt[u] = a_of(u, t + 1);
+        for (int u = 0; u < GPW; ++u) nxt[u] = a_of(u, p + 1);
       }
 #pragma unroll
       for (int u = 0; u < GPW; ++u)
-        acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(cur[u], bw[t], acc[u], 0, 0, 0);
-      if (t + 1 < TAPS) {
+        acc[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur[u], bw[p], acc[u], 0, 0, 0);
+      if (p + 1 < PAIRS) {
 #pragma unroll
         for (int u = 0; u < GPW; ++u) cur[u] = nxt[u];
       }
@@ -528,14 +540,14 @@ __global__ __launch_bounds__(TBI) void k_infer(Geo g, Fields u, Scales inv,
     bh1[H1N * C + threadIdx.x] = 0;
   }
   __syncthreads();
-  const s4 *wf = reinterpret_cast<const s4 *>(Wb) + (long)f * 3 * TAPS * 64;
+  const bf8 *wf = reinterpret_cast<const bf8 *>(Wb) + (long)f * 3 * PAIRS * 64;
   mfma_layer<H1A, D - 1, false>(bx, XN, bh1, wf, F, a0, b0, nullptr);
   __syncthreads();
   unsigned short *bh2 = bx;  // the input box is dead now
   if (threadIdx.x < C) bh2[H2N * C + threadIdx.x] = 0;
-  mfma_layer<H2A, D - 2, false>(bh1, H1N, bh2, wf + TAPS * 64, F, a0, b0, nullptr);
+  mfma_layer<H2A, D - 2, false>(bh1, H1N, bh2, wf + PAIRS * 64, F, a0, b0, nullptr);
   __syncthreads();
-  mfma_layer<TT, SO, true>(bh2, H2N, nullptr, wf + 2 * TAPS * 64, F, a0, b0, corr);
+  mfma_layer<TT, SO, true>(bh2, H2N, nullptr, wf + 2 * PAIRS * 64, F, a0, b0, corr);
 }
 
 // u += correction * scale on the strip cells each face owns
@@ -677,7 +689,7 @@ int check_view(cpml_abc_net *n, const cpml::SolverView &v, const char *who) {
 
 int sync_bf16(cpml_abc_net *n, hipStream_t s) {
   if (!n->wb_dirty) return CPML_OK;
-  hipLaunchKernelGGL(k_to_frag, dim3((WN + 255) / 256), dim3(256), 0, s, n->W, n->Wb);
+  hipLaunchKernelGGL(k_to_frag, dim3((WBN + 255) / 256), dim3(256), 0, s, n->W, n->Wb);
   ACHK(n, hipGetLastError());
   n->wb_dirty = false;
   return CPML_OK;
@@ -801,7 +813,7 @@ int cpml_abc_net_create(cpml_abc_net **np, const float *weights, const float *sc
     n->inv.s[k] = 1.f / scales[k];
   }
   int rc = aalloc(n, (void **)&n->W, WN * sizeof(float));
-  if (!rc) rc = aalloc(n, (void **)&n->Wb, WN * sizeof(unsigned short));
+  if (!rc) rc = aalloc(n, (void **)&n->Wb, WBN * sizeof(unsigned short));
   if (!rc) rc = aalloc(n, (void **)&n->red, 18 * sizeof(double));
   if (!rc) rc = aalloc(n, (void **)&n->part, (size_t)6 * TAPS * NCHUNK * C * C * sizeof(float));
   if (!rc && weights && hipMemcpy(n->W, weights, WN * sizeof(float), hipMemcpyHostToDevice))

@@ -315,33 +315,40 @@ def time_to_solution(model: Model, nstep: int, net: S.CpmlAbcNet, repeats: int =
     """Wall time of nstep steps: learned strip ((n+8)^3 + corrector), the
     strip as a 4-cell C-PML and the strip without absorption ((n+8)^3)
     against the 20-cell C-PML ((n+40)^3), no receivers.  Every form runs its
-    fastest path: the C-PML and rigid grids `time_block` steps per launch,
-    the corrector (which acts after every step) single steps."""
+    fastest path: the C-PML and rigid grids are timed with single steps and
+    with `time_block` steps per launch and the faster is kept (small grids,
+    where most tiles touch a slab, favour single steps); the corrector acts
+    after every step, so its form runs single steps.  Each entry records the
+    steps per launch it used."""
     import torch
 
+    blocks = (0, time_block) if time_block else (0,)
     res = {}
-    for name, pad, absorb, nn, tb in (("cpml20", NPML_WIDE, NPML_WIDE, None, time_block),
-                                      ("learned4", STRIP, net.strip_cpml, net, 0),
-                                      ("strip_single", STRIP, net.strip_cpml, None, 0),
-                                      ("cpml4", STRIP, STRIP, None, time_block),
-                                      ("rigid4", STRIP, 0, None, time_block)):
-        s, _, keep = make_run(model, pad, absorb, nstep)
-        if nn is not None:
-            s.set_boundary(nn)
-        elif tb:
-            s.set_time_block(tb)
-        s.run(1, min(16, nstep))  # build graphs
-        best = None
-        for _ in range(repeats):
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            s.run(1, nstep)
-            torch.cuda.synchronize()
-            el = time.perf_counter() - t0
-            best = el if best is None else min(best, el)
-        res[name] = {"seconds": best, "cells": (model.n + 2 * pad) ** 3}
-        s.close()
-        del keep
+    for name, pad, absorb, nn, tbs in (("cpml20", NPML_WIDE, NPML_WIDE, None, blocks),
+                                       ("learned4", STRIP, net.strip_cpml, net, (0,)),
+                                       ("strip_single", STRIP, net.strip_cpml, None, (0,)),
+                                       ("cpml4", STRIP, STRIP, None, blocks),
+                                       ("rigid4", STRIP, 0, None, blocks)):
+        for tb in tbs:
+            s, _, keep = make_run(model, pad, absorb, nstep)
+            if nn is not None:
+                s.set_boundary(nn)
+            elif tb:
+                s.set_time_block(tb)
+            s.run(1, min(16, nstep))  # build graphs
+            best = None
+            for _ in range(repeats):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                s.run(1, nstep)
+                torch.cuda.synchronize()
+                el = time.perf_counter() - t0
+                best = el if best is None else min(best, el)
+            if name not in res or best < res[name]["seconds"]:
+                res[name] = {"seconds": best, "cells": (model.n + 2 * pad) ** 3,
+                             "time_block": tb}
+            s.close()
+            del keep
     res["speedup"] = res["cpml20"]["seconds"] / res["learned4"]["seconds"]
     res["speedup_cpml4"] = res["cpml20"]["seconds"] / res["cpml4"]["seconds"]
     return res
