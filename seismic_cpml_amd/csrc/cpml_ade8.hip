@@ -1,14 +1,20 @@
 // ADE-PML RK4 step of seismic_ADEPML_2D_elastic_RK4_eighth_order.f90 (ADE8)
 // for gfx950: the 8th-order (Holberg) staggered operator, four Runge-Kutta
 // stages per time step and the auxiliary-differential-equation PML whose
-// coefficients depend on the stage (ADE8:692-906):
-//   per step   X0 = X and stage 1's state                    (k_ade_copy)
-//   per stage  velocity rates from the stage state's sigma, ADE memories in
-//              the PML band, force at t = (it-1 + rk41) * DELTAT, rate
-//              borders zeroed; stress rates from its v; then at once
-//              X += rk42 * rate * DELTAT, velocity borders 0, and the next
-//              stage's state X0 + rk41 * rate * DELTAT         (k_ade_stage)
-// so the rates stay in registers: 5 launches per step instead of 13.
+// coefficients depend on the stage (ADE8:692-906).  Per stage one pass
+// (k_ade_stage): velocity rates from the stage state's sigma, ADE memories in
+// the PML band, force at t = (it-1 + rk41) * DELTAT, rate borders zeroed;
+// stress rates from its v; then at once X += rk42 * rate * DELTAT, velocity
+// borders 0, and the next stage's state X0 + rk41 * rate * DELTAT.  The
+// rates stay in registers: 4 launches per step instead of the program's 13
+// array sweeps.
+//
+// X ping-pongs between two state buffers: the step reads X0 (the program's
+// d3, ADE8:697-703) from the buffer the step starts in, which it never
+// writes, and accumulates into the other one.  Stage 1's state is
+// X0 + rk41(1) * X * DELTAT with rk41(1) = 0, i.e. X0 itself bit for bit
+// (x + 0 * x * dt == x for every finite x, signed zeros included), so stage
+// 1 reads it from X0 and no copy of the state is made.
 // Arithmetic statement by statement as there (8th-order difference divided
 // by DELTA; -ffp-contract=off), so float64 results equal the reference.
 // One point per thread over the interior, 64 x 4 blocks; the ghost cells of
@@ -57,33 +63,18 @@ __device__ __forceinline__ T rk42(int inc) {
   return (inc == 0 || inc == 3) ? T(1) / T(6) : T(2) / T(6);
 }
 
-// Step start: X0 = X (the program's d3, ADE8:697-703, and the level its d2
-// takes before stage 1) and stage 1's state S = X0 + rk41(1) * X * DELTAT
-template <typename T>
-__global__ __launch_bounds__(BX *ATY) void k_ade_copy(Ade8<T> a) {
-  int i, j;
-  if (!pt(a, i, j)) return;
-  const long c = i + j * a.s.ld;
-  T *X[5] = {a.s.vx, a.s.vy, a.s.sxx, a.s.syy, a.s.sxy};
-  const T r = rk41<T>(0);
-  for (int f = 0; f < 5; ++f) {
-    const T x = X[f][c];
-    a.d[2][f][c] = x;
-    a.d[0][f][c] = x + r * x * a.s.dt;
-  }
-}
-
 // One Runge-Kutta stage in one pass (ADE8:715-874): the rates of the stage
-// state S = d[in] at this point -- velocity rates from S's stresses with
-// the ADE memories in the PML band and the stage-time force, rate borders
-// zeroed (ADE8:718-810); stress rates from S's velocities, whose border
-// values the program has zeroed by then (ADE8:803-861) -- and at once the
-// accumulation X += rk42 * rate * DELTAT with the velocity borders zeroed,
-// and the next stage's state d[out] = X0 + rk41(inc+1) * rate * DELTAT
-// (ADE8:863-874; X0's velocity borders count as zero, as the program's d3
-// after ADE8:803-810).  The rates never go to memory, and S is read from one
-// buffer while the next state goes to the other, so neighbours always see
-// the stage's own state.
+// state S (X0 in stage 1, else d[in]) at this point -- velocity rates from
+// S's stresses with the ADE memories in the PML band and the stage-time
+// force, rate borders zeroed (ADE8:718-810); stress rates from S's
+// velocities, whose border values the program has zeroed by then
+// (ADE8:803-861) -- and at once the accumulation X += rk42 * rate * DELTAT
+// into the step's output buffer (stage 1 starts from X0) with the velocity
+// borders zeroed, and the next stage's state d[out] =
+// X0 + rk41(inc+1) * rate * DELTAT (ADE8:863-874; X0's velocity borders
+// count as zero, as the program's d3 after ADE8:803-810).  The rates never
+// go to memory, and S is read from one buffer while the next state goes to
+// another, so neighbours always see the stage's own state.
 template <typename T>
 __global__ __launch_bounds__(BX *ATY) void k_ade_stage(Ade8<T> a, int inc, int in, int out) {
   int i, j;
@@ -93,8 +84,10 @@ __global__ __launch_bounds__(BX *ATY) void k_ade_stage(Ade8<T> a, int inc, int i
   const int nx = p.nx, ny = p.ny, I = i + 1, J = j + 1;
   const bool bd = band(I, J, nx, ny, a.npml);
   const bool edge = i == 0 || i == nx - 1 || j == 0 || j == ny - 1;
-  const T *sxx = a.d[in][2], *syy = a.d[in][3], *sxy = a.d[in][4];
-  const T *vxs = a.d[in][0], *vys = a.d[in][1];
+  const T *const X0[5] = {p.vx, p.vy, p.sxx, p.syy, p.sxy};
+  const T *const *st = inc == 0 ? X0 : a.d[in];
+  const T *sxx = st[2], *syy = st[3], *sxy = st[4];
+  const T *vxs = st[0], *vys = st[1];
   T rate[5] = {T(0), T(0), T(0), T(0), T(0)};
   // vx rates: Fortran i = 2..NX, j = 2..NY (ADE8:718-744)
   if (i >= 1 && j >= 1) {
@@ -178,20 +171,22 @@ __global__ __launch_bounds__(BX *ATY) void k_ade_stage(Ade8<T> a, int inc, int i
   }
   // accumulation and the next stage's state (ADE8:863-874)
   const T r = rk42<T>(inc);
-  T *X[5] = {p.vx, p.vy, p.sxx, p.syy, p.sxy};
   const bool next = inc < 3;
   const T rn = next ? rk41<T>(inc + 1) : T(0);
   for (int f = 0; f < 5; ++f) {
     const T d2 = rate[f];
-    X[f][c] = X[f][c] + r * d2 * p.dt;
+    // X0 is read only where used: stage 1 (accumulation base and next
+    // state) and stages 2-3 (next state)
+    const T xo = inc == 0 ? X0[f][c] : a.xn[f][c];
+    a.xn[f][c] = xo + r * d2 * p.dt;
     if (next) {
-      const T x0 = (f < 2 && edge) ? T(0) : a.d[2][f][c];
+      const T x0 = (f < 2 && edge) ? T(0) : inc == 0 ? xo : X0[f][c];
       a.d[out][f][c] = x0 + rn * d2 * p.dt;
     }
   }
   if (edge) {
-    p.vx[c] = T(0);
-    p.vy[c] = T(0);
+    a.xn[0][c] = T(0);
+    a.xn[1][c] = T(0);
   }
 }
 
@@ -200,8 +195,7 @@ __global__ __launch_bounds__(BX *ATY) void k_ade_stage(Ade8<T> a, int inc, int i
 template <typename T>
 hipError_t launch_ade8_step(const Ade8<T> &a, hipStream_t s) {
   const dim3 g(((a.s.nx + BX - 1) / BX) * ((a.s.ny + ATY - 1) / ATY)), b(BX, ATY);
-  hipLaunchKernelGGL(k_ade_copy<T>, g, b, 0, s, a);  // includes stage 1's state
-  // stage states alternate between d[0] and d[1]; d[2] holds X0
+  // stage 1 reads X0; the states of stages 2..4 alternate between d[1] and d[0]
   for (int inc = 0; inc < 4; ++inc)
     hipLaunchKernelGGL(k_ade_stage<T>, g, b, 0, s, a, inc, inc & 1, (inc & 1) ^ 1);
   return hipGetLastError();

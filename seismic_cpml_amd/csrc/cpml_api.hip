@@ -195,7 +195,7 @@ struct cpml_solver {
   void *re[3] = {};             // R_e1, R_e11, R_e13: nsls dense nx*ny planes each
   // ADE-PML RK4 step of the 8th-order program (cpml_gpu_set_ade_rk4)
   bool ade = false;
-  void *ade_d[15] = {};         // stage copies [level][field], padded like the fields
+  void *ade_d[10] = {};         // stage states [2][field], padded like the fields
   void *ade_mem[8] = {};        // memory arrays, padded like the fields
   void *ade_tab[8] = {};        // a_x b_x a_x_half b_x_half a_y b_y a_y_half b_y_half: [4][n]
   void *ade_f4[2] = {};         // force at each stage time: [NSTEP][4]
@@ -482,8 +482,12 @@ template <typename T>
 cpml::Ade8<T> ade_of(cpml_solver *h) {
   cpml::Ade8<T> a{};
   a.s = step2d<T>(h);
-  for (int l = 0; l < 3; ++l)
+  for (int l = 0; l < 2; ++l)
     for (int f = 0; f < 5; ++f) a.d[l][f] = static_cast<T *>(h->ade_d[l * 5 + f]) + h->base;
+  // the step accumulates X into the other state buffer
+  void **nf = h->cur ? h->field : h->field2;
+  const int slot[5] = {CPML_VX, CPML_VY, CPML_SIGMAXX, CPML_SIGMAYY, CPML_SIGMAXY};
+  for (int f = 0; f < 5; ++f) a.xn[f] = static_cast<T *>(nf[slot[f]]) + h->base;
   for (int q = 0; q < 8; ++q) a.mem[q] = static_cast<T *>(h->ade_mem[q]) + h->base;
   const T **tab[8] = {&a.ax, &a.bx, &a.axh, &a.bxh, &a.ay, &a.by, &a.ayh, &a.byh};
   for (int q = 0; q < 8; ++q) *tab[q] = static_cast<const T *>(h->ade_tab[q]);
@@ -735,11 +739,20 @@ hipError_t enqueue_tb_tail(cpml_solver *h) {
   return cpml::launch_record<T>(r, h->stream);
 }
 
+// The RK4 step writes X to the other state buffer; the host flips `cur` as
+// the step is enqueued (as for the fused step).
+template <typename T>
+hipError_t enqueue_ade(cpml_solver *h) {
+  hipError_t e = cpml::launch_ade8_step<T>(ade_of<T>(h), h->stream);
+  h->cur ^= 1;
+  return e;
+}
+
 template <typename T>
 hipError_t enqueue_step(cpml_solver *h) {
   hipError_t e;
   if (h->ade) {
-    e = cpml::launch_ade8_step<T>(ade_of<T>(h), h->stream);
+    e = enqueue_ade<T>(h);
   } else if (h->fused) {
     e = enqueue_fused<T>(h);
   } else {
@@ -756,7 +769,7 @@ hipError_t enqueue_step(cpml_solver *h) {
 int graph_units(const cpml_solver *h) {
   const int u = unit_steps(h);
   if (h->graph_steps <= 0) return 0;
-  if (!h->fused && !tb_active(h)) return h->graph_steps;
+  if (!h->fused && !h->ade && !tb_active(h)) return h->graph_steps;
   const int n = (h->graph_steps + u - 1) / u;
   return (n + 1) / 2 * 2;
 }
@@ -835,8 +848,7 @@ int enqueue_steps(cpml_solver *h, int n, int *ev_used) {
       hipEvent_t *e = &h->evpool[*ev_used];
       HIPCHK(h, hipEventRecord(e[0], h->stream));
       if (h->ade) {  // the whole RK4 step counts as the stress kernel
-        HIPCHK(h, h->prec == 4 ? cpml::launch_ade8_step<float>(ade_of<float>(h), h->stream)
-                               : cpml::launch_ade8_step<double>(ade_of<double>(h), h->stream));
+        HIPCHK(h, h->prec == 4 ? enqueue_ade<float>(h) : enqueue_ade<double>(h));
         HIPCHK(h, hipEventRecord(e[1], h->stream));
       } else if (h->fused) {
         HIPCHK(h, h->prec == 4 ? enqueue_fused<float>(h) : enqueue_fused<double>(h));
@@ -1821,6 +1833,8 @@ int cpml_gpu_set_attenuation(cpml_solver *h, int N_SLS, const double *tau_epsilo
   return CPML_OK;
 }
 
+static int ensure_second_state(cpml_solver *h);
+
 int cpml_gpu_set_ade_rk4(cpml_solver *h, const void *a_x, const void *b_x,
                          const void *a_x_half, const void *b_x_half, const void *a_y,
                          const void *b_y, const void *a_y_half, const void *b_y_half,
@@ -1836,7 +1850,8 @@ int cpml_gpu_set_ade_rk4(cpml_solver *h, const void *a_x, const void *b_x,
   int rc = set_device(h);
   if (rc) return rc;
   const size_t fb = (size_t)h->total * h->esize;
-  for (int q = 0; q < 15; ++q)
+  if ((rc = ensure_second_state(h))) return rc;  // X ping-pongs between the two
+  for (int q = 0; q < 10; ++q)
     if (!h->ade_d[q]) {
       if ((rc = dev_alloc(h, &h->ade_d[q], fb))) return rc;
     }
@@ -1844,7 +1859,7 @@ int cpml_gpu_set_ade_rk4(cpml_solver *h, const void *a_x, const void *b_x,
     if (!h->ade_mem[q]) {
       if ((rc = dev_alloc(h, &h->ade_mem[q], fb))) return rc;
     }
-  for (int q = 0; q < 15; ++q) HIPCHK(h, hipMemsetAsync(h->ade_d[q], 0, fb, h->stream));
+  for (int q = 0; q < 10; ++q) HIPCHK(h, hipMemsetAsync(h->ade_d[q], 0, fb, h->stream));
   for (int q = 0; q < 8; ++q) HIPCHK(h, hipMemsetAsync(h->ade_mem[q], 0, fb, h->stream));
   // tables: Fortran a(4, N) (stage fastest) stored as [stage][i]
   for (int q = 0; q < 8; ++q) {
@@ -1956,7 +1971,7 @@ int cpml_gpu_set_boundary(cpml_solver *h, int mode, cpml_abc_net *net) {
 }
 
 // Second state buffer (fields and memory variables) of the ping-pong steps.
-int ensure_second_state(cpml_solver *h) {
+static int ensure_second_state(cpml_solver *h) {
   if (h->field2[0]) return CPML_OK;
   const int nf = h->ndim == 3 ? 9 : 5;
   size_t free_b = 0, tot_b = 0;
@@ -1976,7 +1991,7 @@ int ensure_second_state(cpml_solver *h) {
 }
 
 // Bring the state back to the first buffer.
-int restore_first_state(cpml_solver *h) {
+static int restore_first_state(cpml_solver *h) {
   if (!h->cur) return CPML_OK;
   for (int q = 0; q < 9; ++q)
     if (h->field[q])

@@ -246,8 +246,9 @@ hipError_t launch_velocity3d(const Step3D<T> &p, hipStream_t s);
 // y_half) and the force at each stage time ([NSTEP][4]).
 template <typename T>
 struct Ade8 {
-  Step2D<T> s;
-  T *d[3][5];
+  Step2D<T> s;         // X (vx vy sxx syy sxy) at the step's start: X0, stage 1's state
+  T *d[2][5];          // the stage states of stages 2..4, alternating
+  T *xn[5];            // X at the step's end (the other state buffer)
   T *mem[8];  // dvx_dx dvx_dy dvy_dx dvy_dy dsxx_dx dsyy_dy dsxy_dx dsxy_dy
   const T *ax, *bx, *axh, *bxh, *ay, *by, *ayh, *byh;
   const T *f4x, *f4y;

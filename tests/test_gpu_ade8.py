@@ -1,6 +1,7 @@
-"""ADE-PML RK4 8th order on the GPU (cpml_gpu_set_ade_rk4: k_ade_copy,
-k_ade_stage, k_ade_vel, k_ade_str, k_ade_acc and the 4-node vx record)
-against the oracle (runade8), which equals the compiled reference bit for bit
+"""ADE-PML RK4 8th order on the GPU (cpml_gpu_set_ade_rk4: four k_ade_stage
+launches per step, X ping-ponging between the handle's two state buffers,
+and the 4-node vx record) against the oracle (runade8), which equals the
+compiled reference bit for bit
 (tests/test_oracle_ade8.py), on the reduced ADE8 case (81 x 81, 300 steps, 81
 receivers).  Bars: float64 -- every final field and seismogram sample
 bit-exact against the oracle, and the seismograms bit-exact against the
@@ -15,6 +16,7 @@ pytestmark = pytest.mark.gpu
 
 N, NSTEP, DX, DT, F0, FACTOR, ANGLE = 81, 300, 10.0, 3e-3, 10.0, 1e4, 90.0
 TABLES = ("a_x", "b_x", "ah_x", "bh_x", "a_y", "b_y", "ah_y", "bh_y")
+NAMES = ("vx", "vy", "sigmaxx", "sigmayy", "sigmaxy")
 
 
 def inputs(prec):
@@ -29,7 +31,7 @@ def inputs(prec):
 
 
 # the program's arrays run from -4 to N+4: Fortran index 1 is element 5
-def run_gpu(f, d, prec, graph_steps=16, timing=False):
+def run_gpu(f, d, prec, graph_steps=16, timing=False, segments=None, reload_at=None):
     from seismic_cpml_amd import CpmlGpu
     dt = d["dtype"]
     sl = slice(5, 5 + N)
@@ -45,9 +47,12 @@ def run_gpu(f, d, prec, graph_steps=16, timing=False):
         s.set_graph_steps(graph_steps)
         if timing:
             s.set_kernel_timing(True)
-        s.run()
-        g = {k: s.field(n) for k, n in (("vx", "vx"), ("vy", "vy"), ("sxx", "sigmaxx"),
-                                        ("syy", "sigmayy"), ("sxy", "sigmaxy"))}
+        for b, e in segments or [(1, NSTEP)]:
+            s.run(b, e)
+            if reload_at == e:  # read the state back and write it again (checkpoint)
+                for n in NAMES:
+                    s.set_field(n, s.field(n))
+        g = {k: s.field(n) for k, n in zip(("vx", "vy", "sxx", "syy", "sxy"), NAMES)}
         g["sisvx"], g["sisvy"] = s.seismograms()[:2]
     return g
 
@@ -91,5 +96,19 @@ def test_ade8_timed_run_is_the_same_step():
     f, d = inputs("f32")
     a = run_gpu(f, d, "f32")
     b = run_gpu(f, d, "f32", timing=True)
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+@pytest.mark.parametrize("graph_steps", [0, 5])
+def test_ade8_odd_segments_and_checkpoint(graph_steps):
+    """X alternates between two buffers every step: runs split into segments
+    of odd lengths (graphs of an even number of steps, eager remainders),
+    with the state read back and written again after an odd step, equal the
+    single run bit for bit."""
+    f, d = inputs("f32")
+    a = run_gpu(f, d, "f32", graph_steps=16)
+    b = run_gpu(f, d, "f32", graph_steps=graph_steps,
+                segments=[(1, 7), (8, 150), (151, 151), (152, NSTEP)], reload_at=7)
     for k in a:
         np.testing.assert_array_equal(a[k], b[k], err_msg=k)
