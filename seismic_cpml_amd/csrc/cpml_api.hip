@@ -2061,6 +2061,118 @@ int cpml_gpu_set_time_block(cpml_solver *h, int steps, int strip_width, int chun
   return CPML_OK;
 }
 
+// Steps per launch and chunk depth chosen by timing the handle itself: the
+// time-blocked kernels give one wavefront (2D) or workgroup (3D) per strip x
+// chunk item, so a small grid can leave too few of them to hide latency,
+// and single steps win there.  Every field, memory variable, seismogram
+// sample and timer the trial runs touch is saved first and put back after.
+int cpml_gpu_autotune(cpml_solver *h, int trial_steps, int *steps_per_launch, int *chunk,
+                      double *best_ms) {
+  if (!h) return CPML_ERR_ARG;
+  if (!h->have_medium || !h->have_pml)
+    return fail(h, CPML_ERR_STATE, "set_medium and set_pml must be called before autotune");
+  if (trial_steps < 1 || trial_steps > h->nstep)
+    return fail(h, CPML_ERR_ARG, "trial_steps must be in 1..NSTEP");
+  int rc = set_device(h);
+  if (rc) return rc;
+  struct Cand {
+    int nt, jc;
+  };
+  std::vector<Cand> cands{{0, 0}};
+  const bool blockable = h->cadence == 0 && !h->ade && h->nsls == 0 && !h->abc && !h->fused &&
+                         (h->ndim == 2 || h->order == 2);
+  if (blockable) {
+    const int nmax = h->ndim == 3 ? cpml::tb3d_max_nt((int)h->esize)
+                                  : cpml::tb2d_max_nt(h->order, (int)h->esize);
+    const std::vector<int> depths =
+        h->ndim == 3 ? std::vector<int>{32, 64} : std::vector<int>{16, 24, 40, 64};
+    for (int nt = 1; nt <= nmax; ++nt)
+      for (int jc : depths)
+        if (chunk_fits(h, jc)) cands.push_back({nt, jc});
+  }
+  auto report = [&](int nt, int jc, double ms) {
+    if (steps_per_launch) *steps_per_launch = nt;
+    if (chunk) *chunk = nt > 0 ? jc : 0;
+    if (best_ms) *best_ms = ms;
+  };
+  if (cands.size() == 1) {  // nothing to choose: the step has no time-blocked form
+    report(h->tb, h->tb_jc, -1.0);
+    return CPML_OK;
+  }
+
+  // save the state (where `cur` says it is) and what the trials overwrite
+  struct Saved {
+    void *dev;
+    size_t bytes;
+    int kind, idx;  // 0 field, 1 memory variable, 2 seismogram
+  };
+  std::vector<Saved> saved;
+  auto release = [&]() {
+    for (Saved &s : saved) (void)hipFree(s.dev);
+    saved.clear();
+  };
+  auto where = [&](const Saved &s) -> void * {
+    return s.kind == 0 ? fbuf(h, s.idx) : s.kind == 1 ? mbuf(h, s.idx) : h->seis[s.idx];
+  };
+  std::vector<Saved> want;
+  for (int q = 0; q < 9; ++q)
+    if (fbuf(h, q)) want.push_back({nullptr, (size_t)h->total * h->esize, 0, q});
+  for (int w = 0; w < 18; ++w)
+    if (mbuf(h, w) && h->mem_elems[w]) want.push_back({nullptr, h->mem_elems[w] * h->esize, 1, w});
+  if (h->nrec > 0)
+    for (int q = 0; q < 3; ++q)
+      want.push_back({nullptr, (size_t)h->nstep * h->nrec * h->esize, 2, q});
+  for (Saved s : want) {
+    if (hipMalloc(&s.dev, s.bytes) != hipSuccess) {
+      release();
+      return fail(h, CPML_ERR_OOM, "not enough device memory to save the state for autotune");
+    }
+    saved.push_back(s);
+    if (hipMemcpyAsync(s.dev, where(s), s.bytes, hipMemcpyDeviceToDevice, h->stream) !=
+        hipSuccess) {
+      release();
+      return fail(h, CPML_ERR_HIP, "autotune: saving the state failed");
+    }
+  }
+  const double ktime[2] = {h->ktime[0], h->ktime[1]};
+  const int kcount[2] = {h->kcount[0], h->kcount[1]};
+  const double last_ms = h->last_ms, last_vnorm = h->last_vnorm;
+  const int prev_tb = h->tb, prev_ow = h->tb_ow, prev_jc = h->tb_jc;
+
+  Cand best_c{prev_tb, prev_jc};
+  double best = -1;
+  for (const Cand &c : cands) {
+    const int u = c.nt > 0 ? c.nt : 1;
+    const int n = std::max(u, trial_steps / u * u);  // whole launches only
+    if (n > h->nstep) continue;
+    if ((rc = cpml_gpu_set_time_block(h, c.nt, 0, c.jc))) break;
+    if ((rc = cpml_gpu_run(h, 1, n))) break;  // warm: graphs, first launch of a shape
+    if ((rc = cpml_gpu_run(h, 1, n))) break;
+    const double per_step = h->last_ms / n;
+    if (best < 0 || per_step < best) {
+      best = per_step;
+      best_c = c;
+    }
+  }
+  const std::string err = rc ? h->err : std::string();
+  // the choice (or, after a failed trial, the previous setting), then the state
+  int rc2 = rc ? cpml_gpu_set_time_block(h, prev_tb, prev_tb > 0 ? prev_ow : 0, prev_jc)
+               : cpml_gpu_set_time_block(h, best_c.nt, 0, best_c.nt > 0 ? best_c.jc : 0);
+  for (const Saved &s : saved)
+    if (hipMemcpyAsync(where(s), s.dev, s.bytes, hipMemcpyDeviceToDevice, h->stream) !=
+        hipSuccess)
+      rc2 = rc2 ? rc2 : CPML_ERR_HIP;
+  if (hipStreamSynchronize(h->stream) != hipSuccess) rc2 = rc2 ? rc2 : CPML_ERR_HIP;
+  release();
+  h->ktime[0] = ktime[0], h->ktime[1] = ktime[1];
+  h->kcount[0] = kcount[0], h->kcount[1] = kcount[1];
+  h->last_ms = last_ms, h->last_vnorm = last_vnorm;
+  if (rc) return fail(h, rc, "autotune trial: " + err);
+  if (rc2) return fail(h, rc2, "autotune: restoring the state failed");
+  report(best_c.nt, best_c.jc, best * trial_steps);
+  return CPML_OK;
+}
+
 int cpml_gpu_set_tile(cpml_solver *h, int tile_y) {
   if (!h) return CPML_ERR_ARG;
   const bool ok3 = tile_y == 4 || tile_y == 8 || tile_y == 16;

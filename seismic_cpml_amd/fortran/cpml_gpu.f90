@@ -43,8 +43,8 @@ module cpml_gpu
        cpml_gpu_set_pml_3d, cpml_gpu_set_source, cpml_gpu_set_receivers, cpml_gpu_set_energy, &
        cpml_gpu_set_stability, cpml_gpu_run, cpml_gpu_get_seismograms, cpml_gpu_get_energy, &
        cpml_gpu_get_field, cpml_gpu_set_field, cpml_gpu_get_memory, cpml_gpu_set_memory, &
-       cpml_gpu_set_time_block, cpml_gpu_velocity_norm, cpml_gpu_elapsed_ms, &
-       cpml_gpu_set_host_layout, &
+       cpml_gpu_set_time_block, cpml_gpu_autotune, cpml_gpu_velocity_norm, &
+       cpml_gpu_elapsed_ms, cpml_gpu_set_host_layout, &
        cpml_gpu_error_message, cpml_gpu_create_color_image, cpml_gpu_write_su, &
        cpml_gpu_set_attenuation, cpml_gpu_get_pressure, cpml_gpu_set_ade_rk4
   ! many 2D shots in one launch per phase (no reference counterpart: a survey
@@ -146,6 +146,14 @@ module cpml_gpu
       import :: c_ptr, c_int
       type(c_ptr), value :: h
       integer(c_int), value :: steps, width, rows
+    end function
+    integer(c_int) function c_autotune(h, trial_steps, nt, jc, ms) &
+         bind(C, name="cpml_gpu_autotune")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: trial_steps
+      integer(c_int), intent(out) :: nt, jc
+      real(c_double), intent(out) :: ms
     end function
     integer(c_int) function c_set_attenuation(h, nsls, te1, ts1, te2, ts2) &
          bind(C, name="cpml_gpu_set_attenuation")
@@ -663,6 +671,22 @@ contains
     if (present(strip_width)) w = strip_width
     if (present(chunk)) c = chunk
     ierr = c_set_time_block(h, steps, w, c)
+  end subroutine
+
+  ! time single steps against every time-blocked shape on this grid and keep
+  ! the fastest; the state is left as it was (no reference counterpart)
+  subroutine cpml_gpu_autotune(h, trial_steps, ierr, steps, chunk, ms)
+    type(c_ptr), intent(in) :: h
+    integer, intent(in) :: trial_steps
+    integer, intent(out) :: ierr
+    integer, intent(out), optional :: steps, chunk
+    double precision, intent(out), optional :: ms
+    integer(c_int) :: nt, jc
+    real(c_double) :: t
+    ierr = c_autotune(h, trial_steps, nt, jc, t)
+    if (present(steps)) steps = nt
+    if (present(chunk)) chunk = jc
+    if (present(ms)) ms = t
   end subroutine
 
   ! maxval(sqrt(vx**2 + vy**2 [+ vz**2])) (2D2:719, 3D:1185)

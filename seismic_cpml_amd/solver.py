@@ -69,6 +69,7 @@ SIGNATURES = {
     "cpml_gpu_set_tile": [_VP, _I],
     "cpml_gpu_set_fused": [_VP, _I],
     "cpml_gpu_set_time_block": [_VP, _I, _I, _I],
+    "cpml_gpu_autotune": [_VP, _I, _IP, _IP, _DP],
     "cpml_gpu_set_kernel_timing": [_VP, _I],
     "cpml_gpu_kernel_time": [_VP, _I, _IP, _DP],
     "cpml_gpu_elapsed_ms": [_VP, _DP],
@@ -330,6 +331,14 @@ class CpmlGpu:
 
     def set_time_block(self, steps, strip_width=0, chunk_rows=0):
         self._call("cpml_gpu_set_time_block", steps, strip_width, chunk_rows)
+
+    def autotune(self, trial_steps=20):
+        """Time single steps against every time-blocked shape on this grid and
+        keep the fastest (cpml_gpu_autotune); the state is left as it was.
+        Returns (steps_per_launch, chunk, ms of trial_steps steps)."""
+        nt, jc, ms = C.c_int(), C.c_int(), C.c_double()
+        self._call("cpml_gpu_autotune", trial_steps, C.byref(nt), C.byref(jc), C.byref(ms))
+        return nt.value, jc.value, ms.value
 
     def set_fused(self, on=True):
         self._call("cpml_gpu_set_fused", int(on))

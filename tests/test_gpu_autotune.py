@@ -1,0 +1,80 @@
+"""cpml_gpu_autotune: the single-grid autotuner times single steps against
+every time-blocked shape of the handle (steps per launch x chunk depth) and
+keeps the fastest, saving and restoring the state around its trial runs.
+
+Bars: a run interrupted by the autotuner part-way (fields, memory variables
+and seismograms already away from zero) ends bit-exact against the oracle
+(every field, memory variable and seismogram sample), in 2D (orders 2, 4, 8)
+and 3D; the reported choice is consistent (a chunk exactly when time
+blocking is chosen); a step without a time-blocked form is left as it was
+(best_ms -1) and its run is unchanged.
+"""
+import numpy as np
+import pytest
+
+import layered_media as lm
+from case_inputs import run_oracle
+from golden.cases import CASES
+from gpu_runner import F3D, MEM2D, MEM3D, ORACLE_NAME, make_solver, oracle_mem_name
+from test_gpu_timeblock import FIELDS, _case, _check, _inputs, _oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("order,prec", [(4, "f32"), (2, "f64"), (8, "f32")])
+def test_autotune_mid_run_2d_bit_exact(order, prec):
+    d = _inputs(order, prec, 180)
+    c = CASES[_case(order)]
+    saved = c["order"]
+    c["order"] = order
+    try:
+        with make_solver(_case(order), prec, d, energy=False, graph_steps=16) as s:
+            s.run(1, 61)
+            nt, jc, ms = s.autotune(trial_steps=12)
+            assert nt >= 0 and ms > 0
+            assert (jc > 0) == (nt > 0)
+            s.run(62, d["nstep"])
+            g = {k: s.field(n) for n, k in FIELDS}
+            g.update({oracle_mem_name(m): s.memory(m) for m in MEM2D})
+            g["sisvx"], g["sisvy"] = s.seismograms()[:2]
+    finally:
+        c["order"] = saved
+    _check(g, _oracle(order, prec, d))
+
+
+def test_autotune_mid_run_3d_bit_exact():
+    d = lm.layered3d_inputs("f32", 90, kmax=3.0)
+    o = run_oracle("3d_layered", "f32", d=d, nthreads=8, energy=False, derived_lp2m=True)
+    with make_solver("3d_layered", "f32", d, energy=False, graph_steps=16) as s:
+        s.run(1, 33)
+        nt, kc, ms = s.autotune(trial_steps=8)
+        assert nt in (0, 1, 2) and ms > 0
+        assert (kc > 0) == (nt > 0)
+        s.run(34, 90)
+        g = {ORACLE_NAME[n]: s.field(n) for n in F3D}
+        g.update({oracle_mem_name(m): s.memory(m) for m in MEM3D})
+        g["sisvx"], g["sisvy"], g["sisvz"] = s.seismograms()
+    for k in [ORACLE_NAME[n] for n in F3D] + [oracle_mem_name(m) for m in MEM3D] + ["sisvx",
+                                                                                     "sisvy"]:
+        np.testing.assert_array_equal(g[k], o[k], err_msg=k)
+    assert np.abs(g["sisvx"]).max() > 0
+
+
+def test_autotune_leaves_a_step_without_time_blocking_alone():
+    """Energy on: the step has no time-blocked form, so nothing is timed."""
+    d = _inputs(4, "f32", 60)
+    c = CASES[_case(4)]
+    saved = c["order"]
+    c["order"] = 4
+    try:
+        with make_solver(_case(4), "f32", d, energy=True, graph_steps=16) as s:
+            s.run(1, 20)
+            nt, jc, ms = s.autotune(trial_steps=10)
+            assert (nt, jc, ms) == (0, 0, -1.0)
+            s.run(21, 60)
+            a = s.field("vx")
+        with make_solver(_case(4), "f32", d, energy=True, graph_steps=16) as s:
+            s.run()
+            np.testing.assert_array_equal(a, s.field("vx"))
+    finally:
+        c["order"] = saved
