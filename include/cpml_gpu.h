@@ -294,7 +294,8 @@ int cpml_batch_autotune(cpml_batch *b, int trial_steps, const char *cache_path, 
                         int *blocks_per_cu, double *best_ms);
 /* Time blocking of the whole batch: `steps` leapfrog steps of every shot per
  * launch (the kernel of cpml_gpu_set_time_block over a flat list of (shot,
- * strip, chunk) items; slab items beside the others on a second stream;
+ * strip, chunk) items; slab items beside the others on a second stream, each
+ * covering half a chunk when chunks have 48 rows or more;
  * state ping-ponging between two pools), 0 = single steps.  strip_width,
  * chunk_rows: as cpml_gpu_set_time_block (0 = defaults).  With energy on,
  * blocks are used when the cadence is a multiple of `steps`. */

@@ -81,10 +81,10 @@ __device__ __forceinline__ void push(T (&w)[N], T x) {
   w[N - 1] = x;
 }
 
-// One work item: strip ts (columns from ts * ow) and chunk tc (rows from
-// tc * jc) of the grid described by P, advanced NT steps.
+// One work item: strip ts (columns from ts * ow) and rows [jb, je) of the
+// grid described by P, advanced NT steps.
 template <typename T, int ORDER, int NT, int CLS>
-__device__ __forceinline__ void tb2d_item(const TB2D<T> &P, int ts, int tc) {
+__device__ __forceinline__ void tb2d_item(const TB2D<T> &P, int ts, int jb, int je) {
   constexpr bool SX = (CLS & 1) != 0, SY = (CLS & 2) != 0;  // slabs this class touches
   constexpr int r = ORDER / 2;
   constexpr int W = 2 * r + 1;                  // row window of one level
@@ -96,7 +96,6 @@ __device__ __forceinline__ void tb2d_item(const TB2D<T> &P, int ts, int tc) {
   const long ld = p.ld;
   const int lane = threadIdx.x & 63;
   const int i0 = ts * P.ow;
-  const int jb = tc * P.jc, je = min(jb + P.jc, ny);
   const int i = i0 - G + lane;
   const bool col_in = i >= 0 && i < nx;
   const bool own_lane = lane >= G && lane < G + P.ow && i < nx;
@@ -471,13 +470,14 @@ __global__ __launch_bounds__(64 * WPB) void k_tb2d(TB2D<T> P) {
     ts = (CLS & 1) ? outside(us, P.sx0, P.sx1) : P.sx0 + us;
     tc = (CLS & 2) ? outside(uc, P.cy0, P.cy1) : P.cy0 + uc;
   }
-  tb2d_item<T, ORDER, NT, CLS & 3>(P, ts, tc);
+  const int jb = tc * P.jc;
+  tb2d_item<T, ORDER, NT, CLS & 3>(P, ts, jb, min(jb + P.jc, P.a.ny));
 }
 
-// Many grids in one launch: items[k] = (grid, strip, chunk) of one class,
-// split into 8 contiguous bands (one per XCD); the wavefronts of the blocks
-// an XCD receives stride through its band (one item each when the grid has
-// one wavefront per item, several when it is a persistent grid).
+// Many grids in one launch: items[k] = (grid, strip, first row, end row) of
+// one class, split into 8 contiguous bands (one per XCD); the wavefronts of
+// the blocks an XCD receives stride through its band (one item each when the
+// grid has one wavefront per item, several when it is a persistent grid).
 template <typename T, int ORDER, int NT, int CLS>
 __global__ __launch_bounds__(64 * WPB) void k_tb2d_multi(const TB2D<T> *__restrict__ grids,
                                                          const int4 *__restrict__ items,
@@ -490,7 +490,7 @@ __global__ __launch_bounds__(64 * WPB) void k_tb2d_multi(const TB2D<T> *__restri
   const int wv = threadIdx.x >> 6;
   for (int u = __builtin_amdgcn_readfirstlane((h >> 3) * WPB + wv); u < count; u += nb * WPB) {
     const int4 e = items[start + u];
-    tb2d_item<T, ORDER, NT, CLS>(grids[e.x], e.y, e.z);
+    tb2d_item<T, ORDER, NT, CLS>(grids[e.x], e.y, e.z, e.w);
   }
 }
 
