@@ -29,6 +29,7 @@ import argparse
 import json
 import os
 import sys
+import tempfile
 import time
 from pathlib import Path
 
@@ -44,8 +45,9 @@ def _orc():
 
 
 def host_threads():
-    """Host threads given to this GPU's process: OMP_NUM_THREADS where the box
-    sets it (its CPU share per GPU), else every logical CPU."""
+    """Host threads given to this GPU's process: OMP_NUM_THREADS where the
+    environment sets it (the host's CPU share per GPU), else every logical
+    CPU."""
     try:
         n = int(os.environ.get("OMP_NUM_THREADS", "0"))
     except ValueError:
@@ -520,7 +522,8 @@ def bench_batch(args):
     t = time.perf_counter()
     b = w.make_batch()
     setup = time.perf_counter() - t
-    cache = ROOT / "gpurun_out" / "batch_tune.txt" if args.tune_cache is None else args.tune_cache
+    cache = (Path(tempfile.gettempdir()) / "cpml_batch_tune.txt" if args.tune_cache is None
+             else args.tune_cache)
     Path(cache).parent.mkdir(parents=True, exist_ok=True)
     # the autotuner leaves the batch at rest with the fastest launch shape set
     ty, bpc, tune_ms = b.autotune(trial_steps=min(30, steps), cache_path=str(cache))

@@ -1,6 +1,6 @@
 """Parity of the HIP path with the oracle and with the compiled reference.
 
-Bars (written here):
+Bars (the accuracy requirements of this port, written here):
   * GPU vs oracle on the same inputs: bit-exact for every field, memory
     variable and seismogram sample (2D and 3D, float64 and float32); the
     kernels keep the reference's association and are built without FMA

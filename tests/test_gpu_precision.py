@@ -1,6 +1,6 @@
 """FP32 on the GPU against the reference's double-precision build.
 
-Bar (written here): seismograms within 1e-4
+Bar (the single-precision requirement, written here): seismograms within 1e-4
 relative L2 of the reference's float64 build.  Final wavefields are held to
 the same 1e-4 where the wave is still in the grid (the reduced layered cases,
 600 steps); after the wave has left the grid (the shipped programs, 2000 and

@@ -96,7 +96,7 @@ def test_3d4_reference_halo_reads_matter():
 
 @pytest.mark.parametrize("case", list(CASES))
 def test_reference_f32_build_against_its_f64_build(case):
-    """The north-star bar for single precision is 1e-4 relative L2 against the
+    """The bar for single precision is 1e-4 relative L2 against the
     reference's double-precision build.  The reference's own float32 build
     (the textual variant its authors prescribe, 2D2:131-133) meets it on the
     seismograms of every case; its final wavefields meet it where the wave is
