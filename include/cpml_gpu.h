@@ -183,19 +183,21 @@ int cpml_gpu_set_fused(cpml_solver *h, int on);
  * z chunks of chunk_rows planes (0: 64).  strip_width 0 = every column the
  * ring of steps*(order-1) columns per side leaves (3D: rounded down to a
  * multiple of 8).  Used
- * while energy is off (set_energy cadence 0) on the elastic step without a
- * learned boundary; run() ends a block at every stability check.  Results
+ * while the energy cadence (set_energy) is 0 or a multiple of `steps`, on the
+ * elastic step without a learned boundary; run() ends a block at every
+ * stability check.  Results
  * are identical to the single-step kernels. */
 int cpml_gpu_set_time_block(cpml_solver *h, int steps, int strip_width, int chunk_rows);
 /* No counterpart in the reference (it has one loop structure): times this
  * handle's step with single steps and with every steps-per-launch x chunk
  * depth of cpml_gpu_set_time_block (2D: 16, 24, 40, 64 rows; 3D: 32, 64
  * planes; default strips) over about trial_steps steps, and sets the fastest.
- * The state, seismograms and kernel timers are saved before the trials and
- * restored after them (temporary device copies), so it may be called at any
- * point of a run.  A step without a time-blocked form (energy on, attenuation,
- * the RK4 scheme, a learned boundary, the fused 3D step, 3D order 4) is left
- * as it is (best_ms = -1).  Outputs (may be NULL): steps per launch (0: single
+ * The state, seismograms, energy curve and kernel timers are saved before the
+ * trials and restored after them (temporary device copies), so it may be
+ * called at any point of a run.  With energy on, only steps per launch that
+ * divide the cadence are tried.  A step without a time-blocked form
+ * (attenuation, the RK4 scheme, a learned boundary, the fused 3D step, 3D
+ * order 4) is left as it is (best_ms = -1).  Outputs (may be NULL): steps per launch (0: single
  * steps), chunk depth (0 for single steps), time of trial_steps steps in ms. */
 int cpml_gpu_autotune(cpml_solver *h, int trial_steps, int *steps_per_launch, int *chunk,
                       double *best_ms);

@@ -605,10 +605,11 @@ hipError_t enqueue_boundary(cpml_solver *h) {
   return cpml::abc_enqueue(h->abc, static_cast<float *>(h->abc_corr), view_of(h));
 }
 
-// 2D time blocking is used when set and the step has nothing it does not
-// cover (energy every step, attenuation, the RK4 scheme).
+// Time blocking is used when set and the step has nothing it does not
+// cover: energy at steps inside a block (the cadence must be a multiple of
+// the steps per launch), attenuation, the RK4 scheme.
 bool tb_active(const cpml_solver *h) {
-  if (h->tb <= 0 || h->cadence != 0) return false;
+  if (h->tb <= 0 || (h->cadence != 0 && h->cadence % h->tb != 0)) return false;
   if (h->ndim == 3) return h->order == 2 && !h->abc;
   return !h->ade && h->nsls == 0;
 }
@@ -734,6 +735,12 @@ hipError_t enqueue_tb(cpml_solver *h) {
 }
 template <typename T>
 hipError_t enqueue_tb_tail(cpml_solver *h) {
+  if (h->cadence > 0) {  // energy of the block's last step, the state in memory
+    cpml::Energy<T> e = energy_of<T>(h);
+    e.offset = h->tb - 1;
+    const hipError_t err = cpml::launch_energy<T>(e, h->eblocks, h->stream);
+    if (err != hipSuccess) return err;
+  }
   cpml::Record<T> r = record_of<T>(h);
   r.skip = h->tb - 1;
   return cpml::launch_record<T>(r, h->stream);
@@ -2079,7 +2086,7 @@ int cpml_gpu_autotune(cpml_solver *h, int trial_steps, int *steps_per_launch, in
     int nt, jc;
   };
   std::vector<Cand> cands{{0, 0}};
-  const bool blockable = h->cadence == 0 && !h->ade && h->nsls == 0 && !h->abc && !h->fused &&
+  const bool blockable = !h->ade && h->nsls == 0 && !h->abc && !h->fused &&
                          (h->ndim == 2 || h->order == 2);
   if (blockable) {
     const int nmax = h->ndim == 3 ? cpml::tb3d_max_nt((int)h->esize)
@@ -2087,8 +2094,9 @@ int cpml_gpu_autotune(cpml_solver *h, int trial_steps, int *steps_per_launch, in
     const std::vector<int> depths =
         h->ndim == 3 ? std::vector<int>{32, 64} : std::vector<int>{16, 24, 40, 64};
     for (int nt = 1; nt <= nmax; ++nt)
-      for (int jc : depths)
-        if (chunk_fits(h, jc)) cands.push_back({nt, jc});
+      if (h->cadence == 0 || h->cadence % nt == 0)  // energy only at block ends
+        for (int jc : depths)
+          if (chunk_fits(h, jc)) cands.push_back({nt, jc});
   }
   auto report = [&](int nt, int jc, double ms) {
     if (steps_per_launch) *steps_per_launch = nt;
@@ -2104,7 +2112,7 @@ int cpml_gpu_autotune(cpml_solver *h, int trial_steps, int *steps_per_launch, in
   struct Saved {
     void *dev;
     size_t bytes;
-    int kind, idx;  // 0 field, 1 memory variable, 2 seismogram
+    int kind, idx;  // 0 field, 1 memory variable, 2 seismogram, 3 energy curve
   };
   std::vector<Saved> saved;
   auto release = [&]() {
@@ -2112,6 +2120,7 @@ int cpml_gpu_autotune(cpml_solver *h, int trial_steps, int *steps_per_launch, in
     saved.clear();
   };
   auto where = [&](const Saved &s) -> void * {
+    if (s.kind == 3) return s.idx ? (void *)h->epot : (void *)h->ekin;
     return s.kind == 0 ? fbuf(h, s.idx) : s.kind == 1 ? mbuf(h, s.idx) : h->seis[s.idx];
   };
   std::vector<Saved> want;
@@ -2122,6 +2131,8 @@ int cpml_gpu_autotune(cpml_solver *h, int trial_steps, int *steps_per_launch, in
   if (h->nrec > 0)
     for (int q = 0; q < 3; ++q)
       want.push_back({nullptr, (size_t)h->nstep * h->nrec * h->esize, 2, q});
+  if (h->cadence > 0)
+    for (int q = 0; q < 2; ++q) want.push_back({nullptr, (size_t)h->nstep * sizeof(double), 3, q});
   for (Saved s : want) {
     if (hipMalloc(&s.dev, s.bytes) != hipSuccess) {
       release();

@@ -208,6 +208,7 @@ struct Energy {
   double *partial;  // [nblocks][2]
   double *out_kin, *out_pot;  // [nstep]
   const int *step;
+  int offset;  // the state is that of step *step + offset (time blocks: NT - 1)
 };
 
 // What the learned-boundary code needs of a solver handle: sizes, strides and

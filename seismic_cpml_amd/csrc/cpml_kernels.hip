@@ -421,7 +421,7 @@ __device__ void block_sum2(double &a, double &b) {
 
 template <typename T>
 __global__ __launch_bounds__(EB) void k_energy_partial(Energy<T> p) {
-  const int it = *p.step;
+  const int it = *p.step + p.offset;
   if (it % p.cadence != 0) return;
   const int wx = p.i1 - p.i0 + 1, wy = p.j1 - p.j0 + 1, wz = p.k1 - p.k0 + 1;
   const long n = (long)wx * wy * wz;
@@ -442,7 +442,7 @@ __global__ __launch_bounds__(EB) void k_energy_partial(Energy<T> p) {
 
 template <typename T>
 __global__ __launch_bounds__(EB) void k_energy_final(Energy<T> p, int nblocks) {
-  const int it = *p.step;
+  const int it = *p.step + p.offset;
   if (it % p.cadence != 0) return;
   double kin = 0.0, pot = 0.0;
   for (int b = threadIdx.x; b < nblocks; b += EB) {

@@ -61,20 +61,45 @@ def test_autotune_mid_run_3d_bit_exact():
 
 
 def test_autotune_leaves_a_step_without_time_blocking_alone():
-    """Energy on: the step has no time-blocked form, so nothing is timed."""
-    d = _inputs(4, "f32", 60)
+    """3D 4th order has no time-blocked form, so nothing is timed and the run
+    is the plain one."""
+    from case_inputs import inputs
+
+    d = inputs("3d4_small", "f32", 60)
+    with make_solver("3d4_small", "f32", d, energy=False, graph_steps=16) as s:
+        s.run(1, 20)
+        nt, jc, ms = s.autotune(trial_steps=10)
+        assert (nt, jc, ms) == (0, 0, -1.0)
+        s.run(21, 60)
+        a = s.field("vx")
+    with make_solver("3d4_small", "f32", d, energy=False, graph_steps=16) as s:
+        s.run()
+        np.testing.assert_array_equal(a, s.field("vx"))
+
+
+def test_autotune_with_energy_restores_the_curve():
+    """Energy every 2 steps: only 1 or 2 steps per launch are tried, and the
+    energy curve written before the autotuner ran survives it."""
+    d = _inputs(4, "f32", 120)
     c = CASES[_case(4)]
     saved = c["order"]
     c["order"] = 4
     try:
-        with make_solver(_case(4), "f32", d, energy=True, graph_steps=16) as s:
-            s.run(1, 20)
-            nt, jc, ms = s.autotune(trial_steps=10)
-            assert (nt, jc, ms) == (0, 0, -1.0)
-            s.run(21, 60)
-            a = s.field("vx")
-        with make_solver(_case(4), "f32", d, energy=True, graph_steps=16) as s:
-            s.run()
-            np.testing.assert_array_equal(a, s.field("vx"))
+        outs = []
+        for tune in (False, True):
+            with make_solver(_case(4), "f32", d, energy=False, graph_steps=16) as s:
+                s.set_energy(2, window=d["ewin"])
+                s.run(1, 50)
+                if tune:
+                    nt, jc, ms = s.autotune(trial_steps=10)
+                    assert nt in (0, 1, 2) and ms > 0
+                s.run(51, d["nstep"])
+                outs.append((s.energy(), s.field("vx")))
+        (ka, pa), va = outs[0]
+        (kb, pb), vb = outs[1]
+        np.testing.assert_array_equal(ka, kb)
+        np.testing.assert_array_equal(pa, pb)
+        np.testing.assert_array_equal(va, vb)
+        assert ka[1] > 0 or ka[-1] > 0
     finally:
         c["order"] = saved
