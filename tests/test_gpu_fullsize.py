@@ -15,7 +15,12 @@ that hold at any size, since the oracle cannot run it in seconds:
   by the later steps, stay far below 1e-30 (seen: a handful of 4.5e8 values
   off by 1.4e-45, one of them next to a normal value).  Values above 1e-30
   must double bit for bit; the rest must agree to 2 x 1.2e-38;
-* a zero force leaves every field exactly zero.
+* a zero force leaves every field exactly zero;
+* with the source moved next to a corner (4 cells inside the inner edges of
+  the x, y and z slabs, force x1e6 so that the front is far above underflow
+  when it reaches them), the time-blocked step equals the two-phase kernels
+  on every field, every one of the 18 memory variables and every seismogram
+  sample: the slab classes of the time-blocked launch at full size.
 """
 import numpy as np
 import pytest
@@ -44,11 +49,11 @@ def workload():
     torch.cuda.empty_cache()
 
 
-def _solver(w, scale=1.0, fused=False, time_block=0):
+def _solver(w, scale=1.0, fused=False, time_block=0, src=None):
     s = w.make_solver()
-    if scale != 1.0:
+    if scale != 1.0 or src is not None:
         fx, fy = w.force
-        s.set_source(*w.src, fx * np.float32(scale), fy * np.float32(scale))
+        s.set_source(*(src or w.src), fx * np.float32(scale), fy * np.float32(scale))
     if fused:
         s.set_kchunk(48)
         s.set_tile(16)
@@ -105,3 +110,29 @@ def test_fullsize_zero_force_leaves_zero_fields(workload):
             assert not np.any(z.field(name)), name
     finally:
         z.close()
+
+
+def test_fullsize_corner_source_time_blocked_equals_two_phase(workload):
+    from gpu_runner import MEM3D
+
+    src = (workload.npml + 4,) * 3
+    out = []
+    for tb in (0, 2):
+        s = _solver(workload, scale=1e6, time_block=tb, src=src)
+        try:
+            d = {n: _digest(s.field(n)) for n in FIELDS}
+            touched = 0
+            for m in MEM3D:  # full-grid arrays of 1.8 GB: one at a time
+                a = s.memory(m)
+                d[m] = _digest(a)
+                touched += int(np.any(a))
+                del a
+            out.append((d, s.seismograms(), touched))
+        finally:
+            s.close()
+    (da, sa, ta), (db, sb, tb_) = out
+    assert ta == tb_ and ta >= 9  # the wave reached the slabs' memory variables
+    for k in da:
+        assert da[k] == db[k], k
+    for x, y in zip(sa, sb):
+        np.testing.assert_array_equal(x, y)
