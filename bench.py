@@ -28,6 +28,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import tempfile
 import time
@@ -197,6 +198,49 @@ def cpu_baseline(w, lam, mu, rho, nzs=128, seconds=15.0, threads=None):
     return out
 
 
+REF_BIN = ROOT / "oracle" / "_ref"  # reference programs compiled from their sources
+MPI_BIN = Path("/opt/conda/bin")       # the MPI the 3D reference is built against
+
+
+def cpu_baseline_reference_3d(threads=None):
+    """Time the reference's own 3D program (seismic_CPML_3D_isotropic_MPI_OpenMP,
+    compiled from its sources by oracle/build_ref.py with OpenMP, float32) on a
+    768 x 768 x 64-plane sample: 2 MPI ranks (the program needs an even count)
+    of threads/2 OpenMP threads each, runs of 10 and 40 steps; the difference
+    of their wall times is 30 steps without the set-up.  Its homogeneous model
+    and its energy sum every step, as the program is.  None when the binaries
+    or mpirun are not there."""
+    exes = [REF_BIN / f"ref3d_bench{n}_f32" for n in (10, 40)]
+    mpirun = MPI_BIN / "mpirun"
+    if not mpirun.is_file() or not all(e.is_file() for e in exes):
+        return None
+    threads = threads or host_threads()
+    per_rank = max(1, threads // 2)
+    env = dict(os.environ, OMP_NUM_THREADS=str(per_rank),
+               LD_LIBRARY_PATH=f"{MPI_BIN.parent / 'lib'}:" + os.environ.get("LD_LIBRARY_PATH", ""))
+    walls = []
+    with tempfile.TemporaryDirectory(prefix="cpml_ref3d_") as tmp:
+        for exe in exes:
+            t = time.perf_counter()
+            r = subprocess.run([str(mpirun), "-np", "2", str(exe)], cwd=tmp, env=env,
+                               capture_output=True, text=True, timeout=600)
+            walls.append(time.perf_counter() - t)
+            if r.returncode != 0:
+                print(f"bench.py: reference baseline run failed ({r.returncode}): "
+                      f"{(r.stdout + r.stderr)[-300:]}", file=sys.stderr)
+                return None
+    el = walls[1] - walls[0]
+    if el <= 0:
+        return None
+    cells = 768 * 768 * 64
+    return {"value": round(cells * 30 / el / 1e9, 4), "unit": "Gcell/s", "cores": 2 * per_rank,
+            "kind": "reference", "cpu_model": cpu_model(),
+            "sample": "seismic_CPML_3D_isotropic_MPI_OpenMP itself (amdflang -O3 -fopenmp, "
+                      "float32, its homogeneous model, energy every step) on 768x768x64, "
+                      f"2 MPI ranks x {per_rank} OpenMP threads: 30 steps = the 40-step run "
+                      f"({walls[1]:.1f} s) minus the 10-step run ({walls[0]:.1f} s)"}
+
+
 def _timed_run(s, warmup, nstep):
     import torch
 
@@ -300,7 +344,15 @@ def bench_3d(args, order=2):
     cpu = None
     if not args.no_cpu_baseline:
         lam, mu, rho = w.medium_dev
-        cpu = cpu_baseline(w, lam, mu, rho, seconds=args.cpu_seconds)
+        port = cpu_baseline(w, lam, mu, rho, seconds=args.cpu_seconds)
+        ref = cpu_baseline_reference_3d() if order == 2 else None
+        cpu = port
+        if ref is not None:  # the reference program itself; the port's figures beside it
+            cpu = dict(ref, port_value=port["value"], port_sample=port["sample"],
+                       port_cores=port["cores"])
+            for k in ("value_1thread", "sample_1thread"):
+                if k in port:
+                    cpu[k] = port[k]
     config = {
         "workload": f"3D isotropic elastic C-PML {w.nx}^3, NPML 20, "
                     f"{'2nd' if order == 2 else '4th'} order, layered "

@@ -77,6 +77,19 @@ for _p in ("f64", "f32"):
                                             ghost=1, model=True, ricker=True)
     VARIANTS[f"ref3d_small_{_p}"] = dict(src=SRC_3D, prec=_p, params=_SMALL_3D, ndim=3,
                                          ghost=0, nproc=2)
+# The 3D program timed as the CPU baseline of the 768^3 benchmark: its own
+# homogeneous model on a 768 x 768 x 64-plane sample of the benchmark grid,
+# NPML 20, two MPI ranks (the program requires an even count, 3D:388) of
+# OpenMP threads (its PARALLEL DO loops, 3D:825), energy every step as the
+# program computes it, no display steps; built for 10 and 40 steps, so that
+# the difference of the two wall times is the time of 30 steps without the
+# set-up.
+_BENCH_3D = {"NX": "768", "NY": "768", "NZ": "64", "NPROC": "2", "NPOINTS_PML": "20",
+             "IT_DISPLAY": "100000"}
+for _n in (10, 40):
+    VARIANTS[f"ref3d_bench{_n}_f32"] = dict(src=SRC_3D, prec="f32",
+                                           params=dict(_BENCH_3D, NSTEP=str(_n)), ndim=3,
+                                           ghost=0, nproc=2, dump=False, openmp=True)
 # 2D2 and 2D4 as shipped with their PNM snapshots left on (2D2:716-731,
 # 861-999; 2D4:729-731), 200 steps: images of Vx and Vy at steps 5, 100, 200.
 for _p in ("f64", "f32"):
@@ -363,6 +376,8 @@ def build(names=None, verbose=False) -> list[str]:
                 # -I REF_DIR: 3D-visco:1578 includes the reference's SolvOpt file
                 cmd = ["amdflang", "-O3", f"-I{MPI_INC}", f"-I{REF_DIR}", "-o", str(exe),
                        str(f90), f"-L{MPI_LIB}", "-lmpifort", "-lmpi", f"-Wl,-rpath,{MPI_LIB}"]
+            if v.get("openmp"):
+                cmd.insert(1, "-fopenmp")
             r = subprocess.run(cmd, cwd=tmp, capture_output=True, text=True)
             if r.returncode != 0:
                 raise RuntimeError(f"compile {name} failed:\n{r.stdout}\n{r.stderr}")
