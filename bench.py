@@ -205,12 +205,13 @@ MPI_BIN = Path("/opt/conda/bin")       # the MPI the 3D reference is built again
 def cpu_baseline_reference_3d(threads=None):
     """Time the reference's own 3D program (seismic_CPML_3D_isotropic_MPI_OpenMP,
     compiled from its sources by oracle/build_ref.py with OpenMP, float32) on a
-    768 x 768 x 64-plane sample: 2 MPI ranks (the program needs an even count)
-    of threads/2 OpenMP threads each, runs of 10 and 40 steps; the difference
-    of their wall times is 30 steps without the set-up.  Its homogeneous model
+    768 x 768 x 64-plane sample: 2 MPI ranks (the program needs an even
+    count) of threads/2 OpenMP threads each, runs of 10 and 310 steps; the
+    difference of their wall times is 300 steps without the set-up.  Its homogeneous model
     and its energy sum every step, as the program is.  None when the binaries
     or mpirun are not there."""
-    exes = [REF_BIN / f"ref3d_bench{n}_f32" for n in (10, 40)]
+    lo, hi, nz = 10, 310, 64
+    exes = [REF_BIN / f"ref3d_bench{n}_f32" for n in (lo, hi)]
     mpirun = MPI_BIN / "mpirun"
     if not mpirun.is_file() or not all(e.is_file() for e in exes):
         return None
@@ -232,13 +233,15 @@ def cpu_baseline_reference_3d(threads=None):
     el = walls[1] - walls[0]
     if el <= 0:
         return None
-    cells = 768 * 768 * 64
-    return {"value": round(cells * 30 / el / 1e9, 4), "unit": "Gcell/s", "cores": 2 * per_rank,
+    cells = 768 * 768 * nz
+    return {"value": round(cells * (hi - lo) / el / 1e9, 4), "unit": "Gcell/s",
+            "cores": 2 * per_rank,
             "kind": "reference", "cpu_model": cpu_model(),
             "sample": "seismic_CPML_3D_isotropic_MPI_OpenMP itself (amdflang -O3 -fopenmp, "
-                      "float32, its homogeneous model, energy every step) on 768x768x64, "
-                      f"2 MPI ranks x {per_rank} OpenMP threads: 30 steps = the 40-step run "
-                      f"({walls[1]:.1f} s) minus the 10-step run ({walls[0]:.1f} s)"}
+                      f"float32, its homogeneous model, energy every step) on 768x768x{nz}, "
+                      f"2 MPI ranks x {per_rank} OpenMP threads: {hi - lo} steps = the "
+                      f"{hi}-step run ({walls[1]:.1f} s) minus the {lo}-step run "
+                      f"({walls[0]:.1f} s)"}
 
 
 def _timed_run(s, warmup, nstep):
