@@ -402,6 +402,8 @@ def bench_2d(args):
     s = w.make_solver()
     if args.tile:
         s.set_tile(args.tile)
+    if args.energy:  # the energy curve as the reference programs compute it
+        s.set_energy(args.energy)
     if nt > 0:
         s.set_time_block(nt, args.strip, args.chunk_rows)
     slabs = s.slab_widths()
@@ -432,6 +434,7 @@ def bench_2d(args):
         "tile_rows": args.tile or 4,
         "time_block": {"steps": nt, "strip": args.strip, "chunk_rows": args.chunk_rows}
         if nt else None,
+        "energy_cadence": args.energy,
         "bytes_per_cell_update": round(nb["step"] / w.cells, 3),
         "step_hbm_gbs": round(step_gbs, 1), "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 4),
         "kernel_ms": {k: round(v, 4) for k, v in avg.items()},
@@ -758,6 +761,8 @@ def main():
                     help="leapfrog steps per launch (time blocking; 0: off; -1: the "
                          "workload's default: 2 for 3d and 2d)")
     ap.add_argument("--strip", type=int, default=0, help="2d time blocking: strip width")
+    ap.add_argument("--energy", type=int, default=0,
+                    help="2d: energy curve every N steps (0: off; the reference programs: 1)")
     ap.add_argument("--chunk-rows", type=int, default=0, help="2d time blocking: chunk rows")
     ap.add_argument("--fused", action="store_true",
                     help="3d: one fused kernel per step instead of the stress and velocity "

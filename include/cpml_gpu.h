@@ -183,9 +183,11 @@ int cpml_gpu_set_fused(cpml_solver *h, int on);
  * z chunks of chunk_rows planes (0: 64).  strip_width 0 = every column the
  * ring of steps*(order-1) columns per side leaves (3D: rounded down to a
  * multiple of 8).  Used
- * while the energy cadence (set_energy) is 0 or a multiple of `steps`, on the
- * elastic step without a learned boundary; run() ends a block at every
- * stability check.  Results
+ * on the elastic step without a learned boundary; with the energy curve on
+ * (set_energy), in 2D at any cadence (the kernel sums the energy terms of
+ * every level it computes; the curve equals the single-step one to
+ * rounding), in 3D when the cadence is a multiple of `steps` (the energy of
+ * each block's last step); run() ends a block at every stability check.  Results
  * are identical to the single-step kernels. */
 int cpml_gpu_set_time_block(cpml_solver *h, int steps, int strip_width, int chunk_rows);
 /* No counterpart in the reference (it has one loop structure): times this
@@ -194,8 +196,8 @@ int cpml_gpu_set_time_block(cpml_solver *h, int steps, int strip_width, int chun
  * planes; default strips) over about trial_steps steps, and sets the fastest.
  * The state, seismograms, energy curve and kernel timers are saved before the
  * trials and restored after them (temporary device copies), so it may be
- * called at any point of a run.  With energy on, only steps per launch that
- * divide the cadence are tried.  A step without a time-blocked form
+ * called at any point of a run.  With energy on in 3D, only steps per launch
+ * that divide the cadence are tried.  A step without a time-blocked form
  * (attenuation, the RK4 scheme, a learned boundary, the fused 3D step, 3D
  * order 4) is left as it is (best_ms = -1).  Outputs (may be NULL): steps per launch (0: single
  * steps), chunk depth (0 for single steps), time of trial_steps steps in ms. */

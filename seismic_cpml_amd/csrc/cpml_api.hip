@@ -151,6 +151,8 @@ struct cpml_solver {
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int *tb_rec = nullptr;        // per-strip receiver lists (device)
+  double *tb_epart = nullptr;   // 2D time-blocked energy: per-item sums of every level
+  size_t tb_epart_cap = 0;      // its size in doubles
   int tb_rec_ow = -1;           // strip width they were built for (-1: stale)
   std::vector<int> rec_ix, rec_iy, rec_iz;  // receiver grid indices (0-based), host copy
   void *media[6] = {};  // lambda mu rho, and in 3D: DELTAT*lambda DELTAT*mu DELTAT/rho
@@ -238,7 +240,11 @@ int dev_alloc(cpml_solver *h, void **p, size_t bytes) {
   }
   h->allocs.push_back(*p);
   h->bytes += (double)bytes;
+  // cleared on the handle's stream and waited for: the handle's stream does
+  // not synchronise with the null stream, so a blocking hipMemcpy into the
+  // new buffer right after could otherwise be overwritten by the clear
   HIPCHK(h, hipMemsetAsync(*p, 0, bytes, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
   return CPML_OK;
 }
 
@@ -606,10 +612,12 @@ hipError_t enqueue_boundary(cpml_solver *h) {
 }
 
 // Time blocking is used when set and the step has nothing it does not
-// cover: energy at steps inside a block (the cadence must be a multiple of
-// the steps per launch), attenuation, the RK4 scheme.
+// cover: attenuation, the RK4 scheme, and in 3D energy at steps inside a
+// block (the cadence must be a multiple of the steps per launch there; the
+// 2D kernels sum the energy of every level themselves).
 bool tb_active(const cpml_solver *h) {
-  if (h->tb <= 0 || (h->cadence != 0 && h->cadence % h->tb != 0)) return false;
+  if (h->tb <= 0) return false;
+  if (h->ndim == 3 && h->cadence != 0 && h->cadence % h->tb != 0) return false;
   if (h->ndim == 3) return h->order == 2 && !h->abc;
   return !h->ade && h->nsls == 0;
 }
@@ -664,6 +672,26 @@ int build_tb_receivers(cpml_solver *h) {
   return CPML_OK;
 }
 
+// Items of a 2D time-blocked launch (strips x chunks): the energy slots.
+int tb2d_slots(const cpml_solver *h) {
+  return ((h->nx + h->tb_ow - 1) / h->tb_ow) * ((h->ny + h->tb_jc - 1) / h->tb_jc);
+}
+
+// The per-item energy sums of a 2D time-blocked launch (allocated before any
+// graph capture: enqueue_steps calls this first).
+int ensure_tb_energy(cpml_solver *h) {
+  if (h->ndim != 2 || h->cadence <= 0) return CPML_OK;
+  const size_t need = (size_t)h->tb * tb2d_slots(h) * 2;
+  if (need <= h->tb_epart_cap) return CPML_OK;
+  dev_free(h, *(void **)&h->tb_epart);
+  h->tb_epart_cap = 0;
+  int rc = dev_alloc(h, (void **)&h->tb_epart, need * sizeof(double));
+  if (rc) return rc;
+  h->tb_epart_cap = need;
+  h->graph_dirty = true;
+  return CPML_OK;
+}
+
 template <typename T>
 cpml::TB2D<T> tb_of(cpml_solver *h) {
   cpml::TB2D<T> P{};
@@ -687,6 +715,11 @@ cpml::TB2D<T> tb_of(cpml_solver *h) {
   P.sisx = static_cast<T *>(h->seis[0]);
   P.sisy = static_cast<T *>(h->seis[1]);
   P.nrec = h->nrec;
+  if (h->cadence > 0) {  // energy of every level, summed inside the kernel
+    P.epart = h->tb_epart;
+    for (int q = 0; q < 4; ++q) P.ew[q] = h->ewin[q] - 1;
+    P.eslots = tb2d_slots(h);
+  }
   return P;
 }
 
@@ -735,10 +768,16 @@ hipError_t enqueue_tb(cpml_solver *h) {
 }
 template <typename T>
 hipError_t enqueue_tb_tail(cpml_solver *h) {
-  if (h->cadence > 0) {  // energy of the block's last step, the state in memory
-    cpml::Energy<T> e = energy_of<T>(h);
-    e.offset = h->tb - 1;
-    const hipError_t err = cpml::launch_energy<T>(e, h->eblocks, h->stream);
+  if (h->cadence > 0) {
+    hipError_t err;
+    if (h->ndim == 2) {  // every level, from the sums of the block's items
+      err = cpml::launch_tb2d_energy<T>(energy_of<T>(h), h->tb_epart, tb2d_slots(h), h->tb,
+                                        h->stream);
+    } else {  // the block's last step, from the state in memory
+      cpml::Energy<T> e = energy_of<T>(h);
+      e.offset = h->tb - 1;
+      err = cpml::launch_energy<T>(e, h->eblocks, h->stream);
+    }
     if (err != hipSuccess) return err;
   }
   cpml::Record<T> r = record_of<T>(h);
@@ -835,6 +874,7 @@ int ensure_events(cpml_solver *h, size_t n) {
 int enqueue_steps(cpml_solver *h, int n, int *ev_used) {
   if (tb_active(h)) {
     int rc = build_tb_receivers(h);
+    if (!rc) rc = ensure_tb_energy(h);
     if (rc) return rc;
   }
   const int u = unit_steps(h);
@@ -2094,7 +2134,7 @@ int cpml_gpu_autotune(cpml_solver *h, int trial_steps, int *steps_per_launch, in
     const std::vector<int> depths =
         h->ndim == 3 ? std::vector<int>{32, 64} : std::vector<int>{16, 24, 40, 64};
     for (int nt = 1; nt <= nmax; ++nt)
-      if (h->cadence == 0 || h->cadence % nt == 0)  // energy only at block ends
+      if (h->ndim == 2 || h->cadence == 0 || h->cadence % nt == 0)  // 3D energy: block ends
         for (int jc : depths)
           if (chunk_fits(h, jc)) cands.push_back({nt, jc});
   }

@@ -379,7 +379,10 @@ int balloc(cpml_batch *b, void **p, size_t bytes) {
   }
   b->allocs.push_back(*p);
   b->bytes += (double)bytes;
+  // cleared and waited for (the batch's stream does not synchronise with the
+  // null stream, on which blocking copies into the buffer may follow)
   BCHK(b, hipMemsetAsync(*p, 0, bytes, b->stream));
+  BCHK(b, hipStreamSynchronize(b->stream));
   return CPML_OK;
 }
 

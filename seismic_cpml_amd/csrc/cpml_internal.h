@@ -195,6 +195,12 @@ struct TB2D {
   const int *rec_start, *rec_i, *rec_j, *rec_slot;
   T *sisx, *sisy;  // [nstep][nrec]
   int nrec;
+  // energy of every level (single grid, energy on): each item's sums over
+  // its owned points inside the window ew (0-based i0, i1, j0, j1), written
+  // to epart[(level * eslots + item) * 2 + {kinetic, potential}]
+  double *epart;
+  int ew[4];
+  int eslots;
 };
 
 // Energy over a window, per step, accumulated in double.
@@ -300,6 +306,11 @@ int tb3d_max_nt(int prec);
 // nt steps per launch (2D, plain elastic); clean = items without C-PML code
 template <typename T>
 hipError_t launch_tb2d(int order, int nt, const TB2D<T> &p, hipStream_t s, hipStream_t side);
+// The energy of the NT levels of one time-blocked launch from its items' sums
+// (TB2D::epart), into e.out_kin / e.out_pot at steps *e.step .. + NT - 1.
+template <typename T>
+hipError_t launch_tb2d_energy(const Energy<T> &e, const double *epart, int eslots, int nt,
+                              hipStream_t s);
 // largest nt launch_tb2d has a kernel for, per order and precision
 int tb2d_max_nt(int order, int prec);
 // many grids: items (grid, strip, chunk) of one class (cls bit 0: the x

@@ -29,6 +29,7 @@
 // statement by statement, so results are bit-identical to them and to the
 // reference association.
 #include "cpml_device.h"
+#include "cpml_energy.h"
 #include "cpml_internal.h"
 
 namespace cpml {
@@ -82,9 +83,12 @@ __device__ __forceinline__ void push(T (&w)[N], T x) {
 }
 
 // One work item: strip ts (columns from ts * ow) and rows [jb, je) of the
-// grid described by P, advanced NT steps.
-template <typename T, int ORDER, int NT, int CLS>
-__device__ __forceinline__ void tb2d_item(const TB2D<T> &P, int ts, int jb, int je) {
+// grid described by P, advanced NT steps.  EN: also the energy terms of every
+// level at the item's owned points inside the energy window, summed over the
+// wavefront and written to item slot `slot` of P.epart.
+template <typename T, int ORDER, int NT, int CLS, bool EN = false>
+__device__ __forceinline__ void tb2d_item(const TB2D<T> &P, int ts, int jb, int je,
+                                          int slot = 0) {
   constexpr bool SX = (CLS & 1) != 0, SY = (CLS & 2) != 0;  // slabs this class touches
   constexpr int r = ORDER / 2;
   constexpr int W = 2 * r + 1;                  // row window of one level
@@ -218,6 +222,10 @@ __device__ __forceinline__ void tb2d_item(const TB2D<T> &P, int ts, int jb, int 
 #pragma unroll
   for (int m = 0; m < NL; ++m) cur[m] = rc0;
   const int it0 = *p.step;
+  double ek[NT], ep[NT];  // energy terms per level (EN)
+#pragma unroll
+  for (int m = 0; m < NT; ++m) ek[m] = ep[m] = 0.0;
+  const bool ecol = EN && own_lane && i >= P.ew[0] && i <= P.ew[1];
 
   for (int s = s0; s <= s1; ++s) {
     if (s + 2 <= s1) load(N2, s + 2);
@@ -378,6 +386,11 @@ __device__ __forceinline__ void tb2d_item(const TB2D<T> &P, int ts, int jb, int 
           vxn = T(0);
           vyn = T(0);
         }
+        // energy of step it0 + m - 1 at (i, j): v^m, sigma^m of this row
+        // (the middle of the sigma window) and the row's medium
+        if (EN && ecol && j >= jb && j < je && j >= P.ew[2] && j <= P.ew[3])
+          point_energy2d(vxn, vyn, wxx[r], wyy[r], wxy[r], Mx[0][ip - r], Mx[1][ip - r], rho,
+                         ek[m - 1], ep[m - 1]);
         if (m < NT) {
           push(V[m][0], vxn);
           push(V[m][1], vyn);
@@ -423,6 +436,21 @@ __device__ __forceinline__ void tb2d_item(const TB2D<T> &P, int ts, int jb, int 
     C = N;
     N = N2;
   }
+  if (EN) {  // wavefront sums in a fixed order, lane 0 writes the item's slot
+#pragma unroll
+    for (int m = 0; m < NT; ++m) {
+      double a = ek[m], b = ep[m];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+      }
+      if (lane == 0) {
+        P.epart[((long)m * P.eslots + slot) * 2] = a;
+        P.epart[((long)m * P.eslots + slot) * 2 + 1] = b;
+      }
+    }
+  }
 }
 
 // Items of class CLS of one grid: strips clean along x ([sx0, sx1)) or not
@@ -436,7 +464,7 @@ __device__ __forceinline__ int tb2d_class_items(int nstrips, int nchunks, int bx
   return sx * sy;
 }
 
-template <typename T, int ORDER, int NT, int CLS>
+template <typename T, int ORDER, int NT, int CLS, bool EN>
 __global__ __launch_bounds__(64 * WPB) void k_tb2d(TB2D<T> P) {
   const int wv = threadIdx.x >> 6;
   const int bx = P.sx1 - P.sx0, cy = P.cy1 - P.cy0;
@@ -471,7 +499,39 @@ __global__ __launch_bounds__(64 * WPB) void k_tb2d(TB2D<T> P) {
     tc = (CLS & 2) ? outside(uc, P.cy0, P.cy1) : P.cy0 + uc;
   }
   const int jb = tc * P.jc;
-  tb2d_item<T, ORDER, NT, CLS & 3>(P, ts, jb, min(jb + P.jc, P.a.ny));
+  // energy slot: the item's place in the grid (chunk-major), whatever its class
+  tb2d_item<T, ORDER, NT, CLS & 3, EN>(P, ts, jb, min(jb + P.jc, P.a.ny), tc * P.nstrips + ts);
+}
+
+// Energy of the NT levels of one launch: level b = blockIdx.x, step
+// *e.step + b; the items' sums added in slot order, then as a tree.
+constexpr int EBT = 256;
+template <typename T>
+__global__ __launch_bounds__(EBT) void k_tb2d_energy(Energy<T> e, const double *epart,
+                                                     int eslots) {
+  const int it = *e.step + (int)blockIdx.x;
+  if (it % e.cadence != 0) return;
+  const double *pp = epart + (long)blockIdx.x * eslots * 2;
+  double kin = 0.0, pot = 0.0;
+  for (int k = threadIdx.x; k < eslots; k += EBT) {
+    kin += pp[2 * k];
+    pot += pp[2 * k + 1];
+  }
+  __shared__ double sk[EBT], sp[EBT];
+  sk[threadIdx.x] = kin;
+  sp[threadIdx.x] = pot;
+  __syncthreads();
+  for (int h = EBT / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      sk[threadIdx.x] += sk[threadIdx.x + h];
+      sp[threadIdx.x] += sp[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    e.out_kin[it - 1] = 0.5 * sk[0];  // 2D2:711: kinetic summed, then halved
+    e.out_pot[it - 1] = sp[0];
+  }
 }
 
 // Many grids in one launch: items[k] = (grid, strip, first row, end row) of
@@ -526,11 +586,22 @@ hipError_t tb2d_launch(const TB2D<T> &P0, hipStream_t s, hipStream_t side) {
   const int nclean = (P.sx1 - P.sx0) * (P.cy1 - P.cy0);
   const int nslab = P.nstrips * P.nchunks - nclean;
   const dim3 blk(64 * WPB);
+  hipStream_t t = side ? side : s;
+  if (P.epart) {  // the energy-summing instances
+    if (nslab)
+      hipLaunchKernelGGL((k_tb2d<T, ORDER, NT, 7, true>), dim3((nslab + WPB - 1) / WPB), blk, 0,
+                         t, P);
+    if (nclean)
+      hipLaunchKernelGGL((k_tb2d<T, ORDER, NT, 0, true>), dim3((nclean + WPB - 1) / WPB), blk,
+                         0, s, P);
+    return hipGetLastError();
+  }
   if (nslab)
-    hipLaunchKernelGGL((k_tb2d<T, ORDER, NT, 7>), dim3((nslab + WPB - 1) / WPB), blk, 0,
-                       side ? side : s, P);
+    hipLaunchKernelGGL((k_tb2d<T, ORDER, NT, 7, false>), dim3((nslab + WPB - 1) / WPB), blk, 0,
+                       t, P);
   if (nclean)
-    hipLaunchKernelGGL((k_tb2d<T, ORDER, NT, 0>), dim3((nclean + WPB - 1) / WPB), blk, 0, s, P);
+    hipLaunchKernelGGL((k_tb2d<T, ORDER, NT, 0, false>), dim3((nclean + WPB - 1) / WPB), blk,
+                       0, s, P);
   return hipGetLastError();
 }
 
@@ -567,6 +638,17 @@ hipError_t launch_tb2d(int order, int nt, const TB2D<T> &p, hipStream_t s, hipSt
 template hipError_t launch_tb2d<float>(int, int, const TB2D<float> &, hipStream_t, hipStream_t);
 template hipError_t launch_tb2d<double>(int, int, const TB2D<double> &, hipStream_t,
                                         hipStream_t);
+
+template <typename T>
+hipError_t launch_tb2d_energy(const Energy<T> &e, const double *epart, int eslots, int nt,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(k_tb2d_energy<T>, dim3(nt), dim3(EBT), 0, s, e, epart, eslots);
+  return hipGetLastError();
+}
+template hipError_t launch_tb2d_energy<float>(const Energy<float> &, const double *, int, int,
+                                              hipStream_t);
+template hipError_t launch_tb2d_energy<double>(const Energy<double> &, const double *, int, int,
+                                               hipStream_t);
 
 namespace {
 template <typename T, int ORDER, int NT>

@@ -78,8 +78,8 @@ def test_autotune_leaves_a_step_without_time_blocking_alone():
 
 
 def test_autotune_with_energy_restores_the_curve():
-    """Energy every 2 steps: only 1 or 2 steps per launch are tried, and the
-    energy curve written before the autotuner ran survives it."""
+    """Energy every 2 steps: the energy curve written before the autotuner ran
+    survives it, and the run it leaves equals the plain one."""
     d = _inputs(4, "f32", 120)
     c = CASES[_case(4)]
     saved = c["order"]
@@ -92,13 +92,17 @@ def test_autotune_with_energy_restores_the_curve():
                 s.run(1, 50)
                 if tune:
                     nt, jc, ms = s.autotune(trial_steps=10)
-                    assert nt in (0, 1, 2) and ms > 0
+                    assert nt in (0, 1, 2, 3) and ms > 0
                 s.run(51, d["nstep"])
                 outs.append((s.energy(), s.field("vx")))
         (ka, pa), va = outs[0]
         (kb, pb), vb = outs[1]
-        np.testing.assert_array_equal(ka, kb)
-        np.testing.assert_array_equal(pa, pb)
+        # steps 1..50 were written before the trials and restored bit for
+        # bit; later steps may come from the time-blocked kernels' own sums
+        np.testing.assert_array_equal(ka[:50], kb[:50])
+        np.testing.assert_array_equal(pa[:50], pb[:50])
+        np.testing.assert_allclose(kb, ka, rtol=1e-12, atol=0)
+        np.testing.assert_allclose(pb, pa, rtol=1e-12, atol=0)
         np.testing.assert_array_equal(va, vb)
         assert ka[1] > 0 or ka[-1] > 0
     finally:
